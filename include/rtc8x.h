@@ -1,0 +1,344 @@
+/*
+ * rtc8x.h -- C ABI of the MI355X (gfx950) volumetric path tracer + neural radiance cache.
+ *
+ * This is the drop-in boundary for the hot path of rtcamp8's renderer:
+ *   - the three OptiX ray-generation entry points (renderer/gpu_kernels/optix_pathtracing_kernels.cu:438-448)
+ *     with their launch-parameter blocks (renderer/renderer_shared.h:113-172),
+ *   - the NRC data-path kernels (renderer/gpu_kernels/nrc_set_up_kernels.cu),
+ *   - the NeuralRadianceCache class (renderer/network_interface.h:17-31),
+ *   - the volume upload / majorant (renderer/renderer_scene.cpp:491-545),
+ *   - the sky bake and environment importance sampling (renderer/gpu_kernels/analytic_sky_kernels.cu,
+ *     renderer/gpu_kernels/compute_light_probs.cu).
+ *
+ * Conventions
+ *   - Every entry point returns rtc8x_status; nothing throws across the ABI.  rtc8x_last_error()
+ *     returns a message for the calling thread's last failure.
+ *   - Pointers inside the parameter structs are device pointers unless stated otherwise.  The caller
+ *     owns every buffer it passes in (as in the reference, renderer/renderer_main.cpp:80-230); objects
+ *     created by rtc8x_*_create / rtc8x_grid_build_* are owned by the library until destroyed.
+ *   - Streams are hipStream_t passed as void*; NULL is the default stream.  Work is stream-ordered and
+ *     asynchronous unless the function says it synchronises.
+ *   - One GPU per process; the library uses the device that is current on the calling thread.
+ */
+#ifndef RTC8X_H
+#define RTC8X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTC8X_ABI_VERSION 1
+
+typedef enum rtc8x_status {
+    RTC8X_OK = 0,
+    RTC8X_ERR_INVALID_ARGUMENT = 1,
+    RTC8X_ERR_HIP = 2,
+    RTC8X_ERR_OUT_OF_MEMORY = 3,
+    RTC8X_ERR_NOT_SUPPORTED = 4
+} rtc8x_status;
+
+/* ------------------------------------------------------------------------------------------------
+ * Library, memory, streams and timers.
+ * Stand-ins for the CUDA driver wrapper the reference uses (common/utils/cuda_util.h:175-300:
+ * cudau::Buffer, cudau::Timer) so a host program can drive the path without any other GPU runtime.
+ * ---------------------------------------------------------------------------------------------- */
+int32_t rtc8x_abi_version(void);
+const char* rtc8x_last_error(void);
+/* sizeof of an ABI struct by its name ("rtc8x_grid", ...), 0 if unknown: lets bindings check layouts. */
+size_t rtc8x_sizeof(const char* type_name);
+rtc8x_status rtc8x_device_count(int32_t* count);
+rtc8x_status rtc8x_set_device(int32_t device);
+rtc8x_status rtc8x_device_name(char* buf, size_t buf_len);
+rtc8x_status rtc8x_malloc(void** ptr, size_t bytes);
+rtc8x_status rtc8x_free(void* ptr);
+rtc8x_status rtc8x_memcpy_htod(void* dst, const void* src, size_t bytes);
+rtc8x_status rtc8x_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+rtc8x_status rtc8x_memcpy_dtod(void* dst, const void* src, size_t bytes);
+rtc8x_status rtc8x_memset(void* dst, int32_t value, size_t bytes);
+rtc8x_status rtc8x_device_synchronize(void);
+rtc8x_status rtc8x_stream_create(void** stream);
+rtc8x_status rtc8x_stream_destroy(void* stream);
+rtc8x_status rtc8x_stream_synchronize(void* stream);
+/* HIP event pairs, as cudau::Timer (common/utils/cuda_util.h:257-300). */
+rtc8x_status rtc8x_event_create(void** event);
+rtc8x_status rtc8x_event_destroy(void* event);
+rtc8x_status rtc8x_event_record(void* event, void* stream);
+rtc8x_status rtc8x_event_elapsed_ms(float* ms, void* start, void* end);
+
+/* ------------------------------------------------------------------------------------------------
+ * Sparse density grid in the NanoVDB tree layout (root tiles -> 32^3 upper -> 16^3 lower -> 8^3 leaf;
+ * ext/nanovdb/nanovdb/NanoVDB.h).  Node and voxel offsets follow NanoVDB exactly
+ * (InternalNode::CoordToOffset NanoVDB.h:3217-3227, LeafNode::CoordToOffset :3821-3828,
+ *  RootData::CoordToKey :2699-2706); the in-memory tables are this library's own (32-bit child indices).
+ * ---------------------------------------------------------------------------------------------- */
+#define RTC8X_UPPER_TABLE_SIZE 32768u   /* 32^3 */
+#define RTC8X_LOWER_TABLE_SIZE 4096u    /* 16^3 */
+#define RTC8X_LEAF_SIZE 512u            /* 8^3  */
+#define RTC8X_TILE_BIT 0x80000000u      /* table entry is a tile: low 31 bits index tile_values */
+
+typedef struct rtc8x_root_tile {
+    uint64_t key;        /* NanoVDB 64-bit root key of the upper node's origin */
+    uint32_t entry;      /* upper-node index, or RTC8X_TILE_BIT | tile value index */
+    uint32_t reserved;
+} rtc8x_root_tile;
+
+typedef struct rtc8x_grid {
+    /* Map subset used by Grid::worldToIndexF -> Map::applyInverseMapF (NanoVDB.h:2029-2032, 2384):
+     * index = inv_mat (row-major 3x3) * (world - vec). */
+    float inv_mat[9];
+    float vec[3];
+    /* World-space bounding box used to clip rays (Scene::setVolumeGrid, renderer_scene.cpp:517-519). */
+    float world_bbox_min[3];
+    float world_bbox_max[3];
+    int32_t index_bbox_min[3];    /* inclusive index bbox of the active voxels */
+    int32_t index_bbox_max[3];
+    float max_value;              /* brute-force max of all voxel values (renderer_scene.cpp:499-511) */
+    float background;
+    uint32_t num_root_tiles;
+    uint32_t num_upper;
+    uint32_t num_lower;
+    uint32_t num_leaves;
+    const rtc8x_root_tile* root_tiles;   /* num_root_tiles */
+    const uint32_t* upper_tables;        /* num_upper * 32768 entries */
+    const uint32_t* lower_tables;        /* num_lower * 4096 entries */
+    const float* leaf_values;            /* num_leaves * 512 voxel values */
+    const float* tile_values;            /* tile_values[0] == background */
+    const int32_t* leaf_origins;         /* num_leaves * 3 index-space origins (optional, may be NULL) */
+    /* Local-majorant acceleration (optional, used by RTC8X_TRACKING_LOCAL_MAJORANT):
+     * one float per 8^3 leaf cell over the index bbox: the max voxel value over the 3x3x3 block of
+     * leaves around the cell (the trilinear stencil reaches one leaf further in +x/+y/+z, and a
+     * rounded sample position can sit a rounding error into either neighbour). */
+    int32_t cell_origin[3];              /* index-space origin of cell (0,0,0), a multiple of 8 */
+    int32_t cell_dims[3];
+    const float* cell_majorants;         /* cell_dims[0]*cell_dims[1]*cell_dims[2], x-major */
+} rtc8x_grid;
+
+/* Procedural fBm volume description (the benchmark's synthetic input; the reference loads a .nvdb). */
+typedef struct rtc8x_fbm_desc {
+    uint32_t resolution;        /* N: voxels [0,N)^3 spanning the world cube [0,1]^3 (N % 8 == 0) */
+    uint32_t octaves;           /* 4 */
+    uint64_t seed;
+    float base_frequency;       /* lattice cells across the cube at octave 0 */
+    float lacunarity;           /* 2 */
+    float gain;                 /* 0.5 */
+    int32_t sparse;             /* 0: dense (every leaf stored); 1: thresholded cloud */
+    float active_leaf_fraction; /* sparse only: fraction of the N^3/512 leaf slots kept (0.25) */
+} rtc8x_fbm_desc;
+
+/* Builds the fBm grid on the device.  The returned struct's device tables are owned by the library
+ * and released with rtc8x_grid_destroy.  Synchronises the stream. */
+rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* desc, rtc8x_grid* out, void* stream);
+/* Uploads host tables (same layout as rtc8x_grid, host pointers in `host`) to a library-owned copy,
+ * computing max_value and the cell majorants.  Replaces Scene::allocateVolumeGrid
+ * (renderer/renderer_scene.cpp:491-537). */
+rtc8x_status rtc8x_grid_upload(const rtc8x_grid* host, rtc8x_grid* out);
+rtc8x_status rtc8x_grid_destroy(rtc8x_grid* grid);
+/* Parity probe: for n world-space points, the index coordinate, node offsets and trilinear value
+ * exactly as the tracer computes them.  out_ijk[3n], out_offsets[3n] = (upper, lower, leaf) offsets,
+ * out_value[n]; all device pointers. */
+rtc8x_status rtc8x_grid_probe(const rtc8x_grid* grid, uint32_t n, const float* world_xyz,
+                              int32_t* out_ijk, uint32_t* out_offsets, float* out_value, void* stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Environment light and sun.
+ * ---------------------------------------------------------------------------------------------- */
+typedef enum rtc8x_env_kind {
+    RTC8X_ENV_NONE = 0,
+    RTC8X_ENV_CONSTANT = 1,     /* uniform radiance, uniform-sphere sampling */
+    RTC8X_ENV_LATLONG = 2       /* Y-up lat-long RGBA32F texture, point-filtered
+                                   (ImageBasedEnvironmentalLight, common/common_renderer_types.h:2419-2458) */
+} rtc8x_env_kind;
+
+typedef enum rtc8x_env_sampler {
+    RTC8X_ENV_SAMPLER_HIERARCHICAL = 0,  /* HierarchicalImportanceMap (common_renderer_types.h:434-610) */
+    RTC8X_ENV_SAMPLER_ALIAS = 1          /* Walker alias table over texels (common_renderer_types.h:226-281) */
+} rtc8x_env_sampler;
+
+typedef struct rtc8x_alias_entry {
+    uint32_t second_index;
+    float prob_to_pick_first;
+} rtc8x_alias_entry;
+
+typedef struct rtc8x_env_light {
+    int32_t kind;
+    int32_t sampler;
+    float power_coeff;           /* EnvironmentalLight::powerCoeff */
+    float rotation;              /* EnvironmentalLight::rotation (radians) */
+    float constant_radiance[3];
+    uint32_t width, height;      /* texture size */
+    const float* texels;         /* width*height*4 floats, row-major, y = 0 at the zenith */
+    /* Hierarchical importance map: mip 0 is map_dim_x x map_dim_y (powers of two), mip L is
+     * max(dim >> L, 1); levels are stored back to back in `mips`. */
+    uint32_t map_dim_x, map_dim_y, num_mip_levels, reserved0;
+    const float* mips;
+    float map_integral;
+    /* Two-level alias tables (RegularConstantContinuousDistribution2D with alias tables,
+     * common_renderer_types.h:338-418): a marginal over the `height` rows and one conditional
+     * table per row over `width` texels.  Weights are the importance-map mip-0 values
+     * (luminance * pi/2 * sin(theta)); the tables are built on the host in FP64. */
+    float alias_integral;        /* sum of all weights */
+    const rtc8x_alias_entry* alias_rows;   /* height entries */
+    const rtc8x_alias_entry* alias_cols;   /* height * width entries */
+    const float* alias_weights;  /* width*height texel weights */
+} rtc8x_env_light;
+
+/* Builds the two-level alias tables for `env` from its texels (FP64 on the host; device buffers are
+ * library-owned and released with rtc8x_env_release).  Also builds the hierarchical importance map
+ * (mips) on the device as computeImportanceMap does (compute_light_probs.cu:161-211). */
+rtc8x_status rtc8x_env_build_sampler(rtc8x_env_light* env, void* stream);
+rtc8x_status rtc8x_env_release(rtc8x_env_light* env);
+
+typedef struct rtc8x_sun_light {
+    int32_t enabled;
+    float direction[3];          /* unit vector towards the sun */
+    float irradiance[3];         /* radiance integrated over the disc, W/m^2 per channel */
+    float select_prob;           /* probability that next-event estimation picks the sun */
+} rtc8x_sun_light;
+
+/* ------------------------------------------------------------------------------------------------
+ * Camera: PerspectiveCamera (common/common_renderer_types.h:1019-1034) reduced to what the
+ * ray generation uses (optix_pathtracing_kernels.cu:43-55).
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct rtc8x_camera {
+    float position[3];
+    float orientation[9];        /* row-major, Quaternion::toMatrix3x3 (common/basic_types.h:1620-1628) */
+    float vh;                    /* 2 tan(fovY / 2) */
+    float vw;                    /* aspect * vh */
+} rtc8x_camera;
+
+/* Fills an rtc8x_camera from position, unit quaternion (x, y, z, w), fovY and aspect exactly as
+ * the reference computes the same quantities. */
+rtc8x_status rtc8x_camera_setup(rtc8x_camera* cam, const float position[3], const float quat_xyzw[4],
+                                float fov_y, float aspect);
+
+typedef enum rtc8x_phase_kind {
+    RTC8X_PHASE_SCHLICK = 0,     /* SchlickPhaseFunction (common_renderer_types.h:2063-2097) */
+    RTC8X_PHASE_HENYEY_GREENSTEIN = 1
+} rtc8x_phase_kind;
+
+typedef enum rtc8x_tracking {
+    RTC8X_TRACKING_GLOBAL_MAJORANT = 0,   /* reference: delta/ratio tracking against one majorant */
+    RTC8X_TRACKING_LOCAL_MAJORANT = 1     /* DDA over 8^3 cells with per-cell majorants (empty space skipped) */
+} rtc8x_tracking;
+
+typedef enum rtc8x_entry {
+    RTC8X_ENTRY_PATH_TRACE = 0,              /* __raygen__pathTrace */
+    RTC8X_ENTRY_GENERATE_TRAINING_DATA = 1,  /* __raygen__generateTrainingData */
+    RTC8X_ENTRY_PATH_TRACE_WITH_NRC = 2      /* __raygen__pathTraceWithNRC */
+} rtc8x_entry;
+
+/* Counters accumulated by rtc8x_trace when stats != NULL (uint64 device array). */
+enum {
+    RTC8X_STAT_PATHS = 0,
+    RTC8X_STAT_VERTICES = 1,       /* rays traced through the medium (the reference's pathLength) */
+    RTC8X_STAT_LOOKUPS = 2,        /* trilinear density lookups (8 voxel taps each) */
+    RTC8X_STAT_SCATTER = 3,        /* real collisions */
+    RTC8X_STAT_COUNT = 8
+};
+
+/* RadianceQuery (renderer/renderer_shared.h:46-65): 14 floats.  Note that the reference stores the
+ * polar angle theta in the *_phi fields and the azimuth phi in the *_theta fields because it calls
+ * toPolarYUp(theta*, phi*) as toPolarYUp(&phi, &theta) (optix_pathtracing_kernels.cu:18-25). */
+typedef struct rtc8x_radiance_query {
+    float position[3];
+    float normal_phi, normal_theta;
+    float vout_phi, vout_theta;
+    float roughness;
+    float diffuse_reflectance[3];
+    float specular_reflectance[3];
+} rtc8x_radiance_query;
+
+/* TerminalInfo (renderer_shared.h:67-72), unpacked. */
+typedef struct rtc8x_terminal_info {
+    float throughput[3];
+    uint32_t has_query;          /* bit 0: has query; bits 8..15: path length */
+} rtc8x_terminal_info;
+
+/* TrainingVertexInfo (renderer_shared.h:76-82), unpacked. */
+typedef struct rtc8x_training_vertex_info {
+    float local_throughput[3];
+    uint32_t prev_vertex_index;  /* 0x003FFFFF = none */
+    uint32_t path_length;
+    uint32_t is_unbiased;
+} rtc8x_training_vertex_info;
+
+/* TrainingSuffixTerminalInfo (renderer_shared.h:84-90), unpacked. */
+typedef struct rtc8x_training_suffix_terminal {
+    uint32_t prev_vertex_index;
+    uint32_t has_query;
+    uint32_t path_length;
+    uint32_t is_unbiased;
+} rtc8x_training_suffix_terminal;
+
+#define RTC8X_INVALID_VERTEX_INDEX 0x003FFFFFu
+#define RTC8X_NUM_TRAINING_DATA_PER_FRAME 65536u          /* renderer_shared.h:12 */
+#define RTC8X_TRAIN_BUFFER_SIZE (2u * 65536u)             /* renderer_shared.h:13 */
+
+/* NRC buffers of StaticPipelineLaunchParameters (renderer_shared.h:127-141). */
+typedef struct rtc8x_nrc_buffers {
+    uint32_t max_num_training_suffixes;
+    uint32_t nrc_buffer_index;               /* PerFrame::nrcBufferIndex */
+    uint32_t offset_to_select_unbiased_path; /* PerFrame::offsetToSelectUnbiasedPath */
+    float radiance_scale;                    /* PerFrame::radianceScale */
+    float path_termination_factor;           /* renderer_shared.h:11 (0.01) */
+    uint32_t reserved;
+    uint32_t* num_training_data;             /* [2] */
+    int32_t* train_image_size;               /* [2][2] */
+    uint32_t* target_min_max;                /* [2][2][3] ordered-int RGB */
+    float* target_avg;                       /* [2][3] */
+    rtc8x_radiance_query* inference_queries; /* image size (rounded up to 128) */
+    rtc8x_terminal_info* inference_terminals;/* image size */
+    rtc8x_training_suffix_terminal* train_suffix_terminals; /* max_num_training_suffixes */
+    float* inferred_radiance;                /* image size (rounded up to 128) x 3 */
+    float* per_frame_contribution;           /* image size x 3 */
+    rtc8x_radiance_query* train_queries[2];  /* RTC8X_TRAIN_BUFFER_SIZE each */
+    float* train_targets[2];                 /* RTC8X_TRAIN_BUFFER_SIZE x 3 each */
+    rtc8x_training_vertex_info* train_vertex_info; /* RTC8X_TRAIN_BUFFER_SIZE */
+    uint32_t* data_shufflers;                /* RTC8X_NUM_TRAINING_DATA_PER_FRAME LCG states */
+    uint32_t* scratch;                       /* library work space, see rtc8x_nrc_scratch_bytes */
+} rtc8x_nrc_buffers;
+
+/* Everything one launch reads (StaticPipelineLaunchParameters + PerFramePipelineLaunchParameters,
+ * renderer_shared.h:113-167, restricted to the volume-only path). */
+typedef struct rtc8x_frame_params {
+    uint32_t image_width, image_height;      /* render size (trace size for GENERATE_TRAINING_DATA
+                                                comes from nrc.train_image_size) */
+    rtc8x_camera camera;
+    rtc8x_grid grid;
+    float density_coeff;                     /* PerFrame::densityCoeff */
+    float scattering_albedo;                 /* PerFrame::scatteringAlbedo */
+    float scattering_forwardness;            /* Schlick k or HG g */
+    int32_t phase;                           /* rtc8x_phase_kind */
+    rtc8x_env_light env;
+    rtc8x_sun_light sun;
+    float world_aabb_min[3], world_aabb_max[3]; /* WorldDimInfo (common_renderer_types.h:1124-1128) */
+    float world_radius;
+    uint32_t rr_after_path_length;           /* Russian roulette when pathLength > this (reference: 0) */
+    uint32_t max_path_length;                /* 0 = unbounded (reference) */
+    int32_t tracking;                        /* rtc8x_tracking */
+    uint64_t* rng_states;                    /* PCG32 state per pixel, row-major (rngBuffer) */
+    double* accum;                           /* per-pixel RGB sums, FP64 (accumBuffer) */
+    uint32_t* path_vertices;                 /* optional per-pixel vertex count sums */
+    uint64_t* stats;                         /* optional RTC8X_STAT_COUNT counters */
+    rtc8x_nrc_buffers nrc;
+} rtc8x_frame_params;
+
+/* One launch of an entry point over the image (optixPipeline.launch(stream, plp, w, h, 1),
+ * renderer/renderer_main.cpp:1575-1577, 1622-1624, 1640-1642).  num_samples paths are traced per
+ * pixel in that launch; for PATH_TRACE each is added to accum, for the NRC entry points num_samples
+ * must be 1. */
+rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t entry,
+                         uint32_t num_samples);
+
+/* Seeds rng_states row-major from std::mt19937_64(seed) as initializeScreenRelatedBuffers does
+ * (renderer/renderer_main.cpp:86-93).  Host-side, then uploads; `states` is a device pointer. */
+rtc8x_status rtc8x_seed_rng(uint64_t* states, uint32_t width, uint32_t height, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTC8X_H */

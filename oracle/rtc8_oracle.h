@@ -1,0 +1,60 @@
+/*
+ * rtc8_oracle.h -- CPU oracle for the rtc8x volumetric path (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C restatement of rtcamp8's volume-only path-tracing hot path, used by the test-suite as
+ * the parity checker and by bench.py as the timed CPU baseline.  Nothing in the shipped library may
+ * call into it.  Every function cites the reference file:line it restates (paths relative to the
+ * rtcamp8 source tree).
+ *
+ * FP32 functions use rtc8x_detmath.h so that they are bit-identical to the gfx950 kernels on the
+ * same PCG streams; FP64 functions use libm and serve as the converged-image oracle.
+ */
+#ifndef RTC8_ORACLE_H
+#define RTC8_ORACLE_H
+
+#include "../include/rtc8x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a1: PCG32 XSH-RR with increment 1 (common/common_renderer_types.h:101-125). */
+uint32_t oracle_pcg_next(uint64_t* state);
+float oracle_pcg_float(uint64_t* state);
+void oracle_pcg_fill(uint64_t state, uint32_t n, uint32_t* out_u32, float* out_f32, uint64_t* out_state);
+/* mt19937_64 seeding of the per-pixel states (renderer/renderer_main.cpp:86-93). */
+void oracle_mt19937_64(uint64_t seed, uint32_t n, uint64_t* out);
+void oracle_seed_rng(uint64_t* states, uint32_t width, uint32_t height, uint64_t seed);
+/* a2: LCG (renderer/renderer_shared.h:92-109) and the shuffler seeds (renderer_main.cpp:220-229). */
+uint32_t oracle_lcg_next(uint32_t* state);
+void oracle_lcg_seeds(uint32_t seed, uint32_t n, uint32_t* out);
+
+/* Grid: fBm builder into the NanoVDB tree layout, node offsets, uncached trilinear sampler. */
+int oracle_grid_build_fbm(const rtc8x_fbm_desc* desc, rtc8x_grid* out_host);
+void oracle_grid_free(rtc8x_grid* g);
+float oracle_fbm_value(const rtc8x_fbm_desc* desc, int32_t i, int32_t j, int32_t k);
+float oracle_grid_get_value(const rtc8x_grid* g, int32_t i, int32_t j, int32_t k, uint32_t offsets[3]);
+void oracle_grid_probe(const rtc8x_grid* g, uint32_t n, const float* world_xyz,
+                       int32_t* out_ijk, uint32_t* out_offsets, float* out_value);
+void oracle_compute_cell_majorants(rtc8x_grid* g);  /* fills cell_* (malloc'd) */
+int oracle_ray_clip(const float eye[3], const float dir[3], float tmax, const float bmin[3],
+                    const float bmax[3], float* t0, float* t1);
+
+/* Deterministic math wrappers (for accuracy tests against libm). */
+float oracle_logf(float x);
+void oracle_sincosf(float x, float* s, float* c);
+float oracle_atan2f(float y, float x);
+float oracle_acosf(float x);
+
+/* Tracer.  Rows [y0, y1) of the image; nthreads worker threads over rows.
+ * FP32: bit-exact restatement (accum gets the FP32 contributions summed in FP64).
+ * FP64: same algorithm in double precision (the converged-image oracle). */
+int oracle_trace_f32(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
+                     uint32_t y0, uint32_t y1, uint32_t nthreads);
+int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
+                     uint32_t y0, uint32_t y1, uint32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

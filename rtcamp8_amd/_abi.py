@@ -1,0 +1,260 @@
+"""ctypes mirror of include/rtc8x.h (structs, enums, function prototypes).
+
+Pointers inside the structs are plain integers (``c_void_p``): device addresses for the library,
+host addresses when the same structs are handed to the CPU oracle in the test-suite.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+c_f3 = C.c_float * 3
+c_i3 = C.c_int32 * 3
+
+RTC8X_OK = 0
+RTC8X_ERR_INVALID_ARGUMENT = 1
+RTC8X_ERR_HIP = 2
+RTC8X_ERR_OUT_OF_MEMORY = 3
+RTC8X_ERR_NOT_SUPPORTED = 4
+
+UPPER_TABLE_SIZE = 32768
+LOWER_TABLE_SIZE = 4096
+LEAF_SIZE = 512
+TILE_BIT = 0x80000000
+
+ENV_NONE, ENV_CONSTANT, ENV_LATLONG = 0, 1, 2
+ENV_SAMPLER_HIERARCHICAL, ENV_SAMPLER_ALIAS = 0, 1
+PHASE_SCHLICK, PHASE_HENYEY_GREENSTEIN = 0, 1
+TRACKING_GLOBAL_MAJORANT, TRACKING_LOCAL_MAJORANT = 0, 1
+ENTRY_PATH_TRACE, ENTRY_GENERATE_TRAINING_DATA, ENTRY_PATH_TRACE_WITH_NRC = 0, 1, 2
+STAT_PATHS, STAT_VERTICES, STAT_LOOKUPS, STAT_SCATTER, STAT_COUNT = 0, 1, 2, 3, 8
+
+INVALID_VERTEX_INDEX = 0x003FFFFF
+NUM_TRAINING_DATA_PER_FRAME = 65536
+TRAIN_BUFFER_SIZE = 2 * 65536
+
+
+class RootTile(C.Structure):
+    _fields_ = [("key", C.c_uint64), ("entry", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("inv_mat", C.c_float * 9),
+        ("vec", c_f3),
+        ("world_bbox_min", c_f3),
+        ("world_bbox_max", c_f3),
+        ("index_bbox_min", c_i3),
+        ("index_bbox_max", c_i3),
+        ("max_value", C.c_float),
+        ("background", C.c_float),
+        ("num_root_tiles", C.c_uint32),
+        ("num_upper", C.c_uint32),
+        ("num_lower", C.c_uint32),
+        ("num_leaves", C.c_uint32),
+        ("root_tiles", C.c_void_p),
+        ("upper_tables", C.c_void_p),
+        ("lower_tables", C.c_void_p),
+        ("leaf_values", C.c_void_p),
+        ("tile_values", C.c_void_p),
+        ("leaf_origins", C.c_void_p),
+        ("cell_origin", c_i3),
+        ("cell_dims", c_i3),
+        ("cell_majorants", C.c_void_p),
+    ]
+
+
+class FbmDesc(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_uint32),
+        ("octaves", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("base_frequency", C.c_float),
+        ("lacunarity", C.c_float),
+        ("gain", C.c_float),
+        ("sparse", C.c_int32),
+        ("active_leaf_fraction", C.c_float),
+    ]
+
+
+class AliasEntry(C.Structure):
+    _fields_ = [("second_index", C.c_uint32), ("prob_to_pick_first", C.c_float)]
+
+
+class EnvLight(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("sampler", C.c_int32),
+        ("power_coeff", C.c_float),
+        ("rotation", C.c_float),
+        ("constant_radiance", c_f3),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("texels", C.c_void_p),
+        ("map_dim_x", C.c_uint32),
+        ("map_dim_y", C.c_uint32),
+        ("num_mip_levels", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("mips", C.c_void_p),
+        ("map_integral", C.c_float),
+        ("alias_integral", C.c_float),
+        ("alias_rows", C.c_void_p),
+        ("alias_cols", C.c_void_p),
+        ("alias_weights", C.c_void_p),
+    ]
+
+
+class SunLight(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("direction", c_f3),
+        ("irradiance", c_f3),
+        ("select_prob", C.c_float),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("position", c_f3),
+        ("orientation", C.c_float * 9),
+        ("vh", C.c_float),
+        ("vw", C.c_float),
+    ]
+
+
+class RadianceQuery(C.Structure):
+    _fields_ = [
+        ("position", c_f3),
+        ("normal_phi", C.c_float),
+        ("normal_theta", C.c_float),
+        ("vout_phi", C.c_float),
+        ("vout_theta", C.c_float),
+        ("roughness", C.c_float),
+        ("diffuse_reflectance", c_f3),
+        ("specular_reflectance", c_f3),
+    ]
+
+
+class TerminalInfo(C.Structure):
+    _fields_ = [("throughput", c_f3), ("has_query", C.c_uint32)]
+
+
+class TrainingVertexInfo(C.Structure):
+    _fields_ = [
+        ("local_throughput", c_f3),
+        ("prev_vertex_index", C.c_uint32),
+        ("path_length", C.c_uint32),
+        ("is_unbiased", C.c_uint32),
+    ]
+
+
+class TrainingSuffixTerminal(C.Structure):
+    _fields_ = [
+        ("prev_vertex_index", C.c_uint32),
+        ("has_query", C.c_uint32),
+        ("path_length", C.c_uint32),
+        ("is_unbiased", C.c_uint32),
+    ]
+
+
+class NrcBuffers(C.Structure):
+    _fields_ = [
+        ("max_num_training_suffixes", C.c_uint32),
+        ("nrc_buffer_index", C.c_uint32),
+        ("offset_to_select_unbiased_path", C.c_uint32),
+        ("radiance_scale", C.c_float),
+        ("path_termination_factor", C.c_float),
+        ("reserved", C.c_uint32),
+        ("num_training_data", C.c_void_p),
+        ("train_image_size", C.c_void_p),
+        ("target_min_max", C.c_void_p),
+        ("target_avg", C.c_void_p),
+        ("inference_queries", C.c_void_p),
+        ("inference_terminals", C.c_void_p),
+        ("train_suffix_terminals", C.c_void_p),
+        ("inferred_radiance", C.c_void_p),
+        ("per_frame_contribution", C.c_void_p),
+        ("train_queries", C.c_void_p * 2),
+        ("train_targets", C.c_void_p * 2),
+        ("train_vertex_info", C.c_void_p),
+        ("data_shufflers", C.c_void_p),
+        ("scratch", C.c_void_p),
+    ]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [
+        ("image_width", C.c_uint32),
+        ("image_height", C.c_uint32),
+        ("camera", Camera),
+        ("grid", Grid),
+        ("density_coeff", C.c_float),
+        ("scattering_albedo", C.c_float),
+        ("scattering_forwardness", C.c_float),
+        ("phase", C.c_int32),
+        ("env", EnvLight),
+        ("sun", SunLight),
+        ("world_aabb_min", c_f3),
+        ("world_aabb_max", c_f3),
+        ("world_radius", C.c_float),
+        ("rr_after_path_length", C.c_uint32),
+        ("max_path_length", C.c_uint32),
+        ("tracking", C.c_int32),
+        ("rng_states", C.c_void_p),
+        ("accum", C.c_void_p),
+        ("path_vertices", C.c_void_p),
+        ("stats", C.c_void_p),
+        ("nrc", NrcBuffers),
+    ]
+
+
+STRUCTS = {
+    "rtc8x_root_tile": RootTile,
+    "rtc8x_grid": Grid,
+    "rtc8x_fbm_desc": FbmDesc,
+    "rtc8x_alias_entry": AliasEntry,
+    "rtc8x_env_light": EnvLight,
+    "rtc8x_sun_light": SunLight,
+    "rtc8x_camera": Camera,
+    "rtc8x_radiance_query": RadianceQuery,
+    "rtc8x_terminal_info": TerminalInfo,
+    "rtc8x_training_vertex_info": TrainingVertexInfo,
+    "rtc8x_training_suffix_terminal": TrainingSuffixTerminal,
+    "rtc8x_nrc_buffers": NrcBuffers,
+    "rtc8x_frame_params": FrameParams,
+}
+
+_P = C.c_void_p
+_S = C.c_int  # rtc8x_status
+
+# name -> (restype, argtypes); every symbol declared in include/rtc8x.h
+PROTOTYPES = {
+    "rtc8x_abi_version": (C.c_int32, []),
+    "rtc8x_last_error": (C.c_char_p, []),
+    "rtc8x_sizeof": (C.c_size_t, [C.c_char_p]),
+    "rtc8x_device_count": (_S, [C.POINTER(C.c_int32)]),
+    "rtc8x_set_device": (_S, [C.c_int32]),
+    "rtc8x_device_name": (_S, [C.c_char_p, C.c_size_t]),
+    "rtc8x_malloc": (_S, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "rtc8x_free": (_S, [_P]),
+    "rtc8x_memcpy_htod": (_S, [_P, _P, C.c_size_t]),
+    "rtc8x_memcpy_dtoh": (_S, [_P, _P, C.c_size_t]),
+    "rtc8x_memcpy_dtod": (_S, [_P, _P, C.c_size_t]),
+    "rtc8x_memset": (_S, [_P, C.c_int32, C.c_size_t]),
+    "rtc8x_device_synchronize": (_S, []),
+    "rtc8x_stream_create": (_S, [C.POINTER(C.c_void_p)]),
+    "rtc8x_stream_destroy": (_S, [_P]),
+    "rtc8x_stream_synchronize": (_S, [_P]),
+    "rtc8x_event_create": (_S, [C.POINTER(C.c_void_p)]),
+    "rtc8x_event_destroy": (_S, [_P]),
+    "rtc8x_event_record": (_S, [_P, _P]),
+    "rtc8x_event_elapsed_ms": (_S, [C.POINTER(C.c_float), _P, _P]),
+    "rtc8x_grid_build_fbm": (_S, [C.POINTER(FbmDesc), C.POINTER(Grid), _P]),
+    "rtc8x_grid_upload": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
+    "rtc8x_grid_destroy": (_S, [C.POINTER(Grid)]),
+    "rtc8x_grid_probe": (_S, [C.POINTER(Grid), C.c_uint32, _P, _P, _P, _P, _P]),
+    "rtc8x_env_build_sampler": (_S, [C.POINTER(EnvLight), _P]),
+    "rtc8x_env_release": (_S, [C.POINTER(EnvLight)]),
+    "rtc8x_camera_setup": (_S, [C.POINTER(Camera), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float]),
+    "rtc8x_trace": (_S, [_P, C.POINTER(FrameParams), C.c_int32, C.c_uint32]),
+    "rtc8x_seed_rng": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint64]),
+}

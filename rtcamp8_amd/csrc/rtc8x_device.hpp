@@ -1,0 +1,593 @@
+// rtc8x_device.hpp -- device functions of the volumetric path (gfx950).
+//
+// Every arithmetic expression on the path mirrors the reference expression it replaces, in the same
+// evaluation order, with IEEE single precision, no contraction (-ffp-contract=off) and the
+// deterministic transcendental functions of rtc8x_detmath.h.  This is what makes a gfx950 launch
+// reproduce a scalar host run of the same algorithm bit for bit on the same PCG streams.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rtc8x.h"
+#include "../../include/rtc8x_detmath.h"
+
+namespace rtc8x {
+namespace dev {
+
+struct V3 {
+    float x, y, z;
+};
+struct RGB {
+    float r, g, b;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ RGB rgb(float r, float g, float b) { return RGB{r, g, b}; }
+__device__ __forceinline__ RGB mul(RGB a, RGB b) { return RGB{a.r * b.r, a.g * b.g, a.b * b.b}; }
+__device__ __forceinline__ RGB scale(RGB a, float s) { return RGB{a.r * s, a.g * s, a.b * s}; }
+__device__ __forceinline__ RGB add(RGB a, RGB b) { return RGB{a.r + b.r, a.g + b.g, a.b + b.b}; }
+// RGBSpectrum::luminance (common/basic_types.h:2981-2986)
+__device__ __forceinline__ float luminance(RGB c)
+{
+    return (float)0.2126729 * c.r + (float)0.7151522 * c.g + (float)0.0721750 * c.b;
+}
+__device__ __forceinline__ bool all_nonneg_finite(RGB c)
+{
+    return c.r >= 0.0f && c.g >= 0.0f && c.b >= 0.0f && isfinite(c.r) && isfinite(c.g) && isfinite(c.b);
+}
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// normalize(): multiply by 1/length (common/basic_types.h:817-820, 850-853, 1871-1875)
+__device__ __forceinline__ V3 normalize(V3 v)
+{
+    float l = sqrtf(dot(v, v));
+    float r = 1.0f / l;
+    return V3{v.x * r, v.y * r, v.z * r};
+}
+
+// PCG32 XSH-RR, increment 1 (common/common_renderer_types.h:101-125)
+struct Pcg {
+    uint64_t state;
+    __device__ __forceinline__ uint32_t next()
+    {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + 1ULL;
+        uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xorshifted >> rot) | (xorshifted << ((0u - rot) & 31u));
+    }
+    __device__ __forceinline__ float uni()
+    {
+        uint32_t bits = (next() >> 9) | 0x3f800000u;
+        return __uint_as_float(bits) - 1.0f;
+    }
+};
+
+struct Frame {
+    V3 t, b, n;
+};
+// ReferenceFrame(normal) via makeCoordinateSystem (basic_types.h:857-864)
+__device__ __forceinline__ Frame make_frame(V3 n)
+{
+    Frame f;
+    float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    f.t = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    f.b = V3{b, sign + n.y * n.y * a, -n.y};
+    f.n = n;
+    return f;
+}
+__device__ __forceinline__ V3 to_local(const Frame& f, V3 v) { return V3{dot(f.t, v), dot(f.b, v), dot(f.n, v)}; }
+__device__ __forceinline__ V3 from_local(const Frame& f, V3 v)
+{
+    return V3{dot(V3{f.t.x, f.b.x, f.n.x}, v), dot(V3{f.t.y, f.b.y, f.n.y}, v), dot(V3{f.t.z, f.b.z, f.n.z}, v)};
+}
+
+// ------------------------------------------------------------------ grid access
+// NanoVDB node offsets (NanoVDB.h:2699-2706, 3217-3227, 3821-3828).
+__device__ __forceinline__ uint64_t root_key(int32_t i, int32_t j, int32_t k)
+{
+    return ((uint64_t)((uint32_t)k >> 12)) | ((uint64_t)((uint32_t)j >> 12) << 21) |
+           ((uint64_t)((uint32_t)i >> 12) << 42);
+}
+__device__ __forceinline__ uint32_t upper_offset(int32_t i, int32_t j, int32_t k)
+{
+    return ((((uint32_t)i & 4095u) >> 7) << 10) | ((((uint32_t)j & 4095u) >> 7) << 5) | (((uint32_t)k & 4095u) >> 7);
+}
+__device__ __forceinline__ uint32_t lower_offset(int32_t i, int32_t j, int32_t k)
+{
+    return ((((uint32_t)i & 127u) >> 3) << 8) | ((((uint32_t)j & 127u) >> 3) << 4) | (((uint32_t)k & 127u) >> 3);
+}
+__device__ __forceinline__ uint32_t leaf_offset(int32_t i, int32_t j, int32_t k)
+{
+    return (((uint32_t)i & 7u) << 6) | (((uint32_t)j & 7u) << 3) | ((uint32_t)k & 7u);
+}
+
+// Walks root -> upper -> lower; returns the leaf index, or RTC8X_TILE_BIT | tile value index.
+__device__ __forceinline__ uint32_t find_leaf(const rtc8x_grid& g, int32_t i, int32_t j, int32_t k)
+{
+    uint64_t key = root_key(i, j, k);
+    uint32_t entry = RTC8X_TILE_BIT;
+    for (uint32_t t = 0; t < g.num_root_tiles; ++t) {
+        if (g.root_tiles[t].key == key) {
+            entry = g.root_tiles[t].entry;
+            break;
+        }
+    }
+    if (entry & RTC8X_TILE_BIT) return entry;
+    uint32_t e1 = g.upper_tables[(size_t)entry * RTC8X_UPPER_TABLE_SIZE + upper_offset(i, j, k)];
+    if (e1 & RTC8X_TILE_BIT) return e1;
+    return g.lower_tables[(size_t)e1 * RTC8X_LOWER_TABLE_SIZE + lower_offset(i, j, k)];
+}
+
+// ReadAccessor-style cache of the last leaf (value-identical to an uncached walk).
+struct Accessor {
+    int32_t ox, oy, oz;   // origin of the cached leaf (low 3 bits cleared)
+    uint32_t leaf;
+    __device__ __forceinline__ void reset() { ox = oy = oz = 0x7fffffff; leaf = RTC8X_TILE_BIT; }
+    __device__ __forceinline__ float get(const rtc8x_grid& g, int32_t i, int32_t j, int32_t k)
+    {
+        int32_t bx = i & ~7, by = j & ~7, bz = k & ~7;
+        if (bx != ox || by != oy || bz != oz) {
+            ox = bx;
+            oy = by;
+            oz = bz;
+            leaf = find_leaf(g, i, j, k);
+        }
+        if (leaf & RTC8X_TILE_BIT) return g.tile_values[leaf & ~RTC8X_TILE_BIT];
+        return g.leaf_values[(size_t)leaf * RTC8X_LEAF_SIZE + leaf_offset(i, j, k)];
+    }
+};
+
+// Map::applyInverseMapF with matMult (NanoVDB.h:1409-1414, 2029-2032)
+__device__ __forceinline__ V3 world_to_index(const rtc8x_grid& g, float px, float py, float pz)
+{
+    float x = px - g.vec[0], y = py - g.vec[1], z = pz - g.vec[2];
+    const float* m = g.inv_mat;
+    return V3{fmaf(x, m[0], fmaf(y, m[1], z * m[2])), fmaf(x, m[3], fmaf(y, m[4], z * m[5])),
+              fmaf(x, m[6], fmaf(y, m[7], z * m[8]))};
+}
+
+// SampleFromVoxels<Acc,1,false>::operator() (util/SampleFromVoxels.h:419-460) with Floor (:58-65)
+__device__ __forceinline__ float trilinear(const rtc8x_grid& g, Accessor& acc, V3 idx)
+{
+    float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
+    float x = idx.x - fx, y = idx.y - fy, z = idx.z - fz;
+    int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
+    float v000 = acc.get(g, i, j, k);
+    float v001 = acc.get(g, i, j, k + 1);
+    float vy = v000 + z * (v001 - v000);
+    float v011 = acc.get(g, i, j + 1, k + 1);
+    float v010 = acc.get(g, i, j + 1, k);
+    float vy1 = v010 + z * (v011 - v010);
+    float vx = vy + y * (vy1 - vy);
+    float v110 = acc.get(g, i + 1, j + 1, k);
+    float v111 = acc.get(g, i + 1, j + 1, k + 1);
+    vy1 = v110 + z * (v111 - v110);
+    float v101 = acc.get(g, i + 1, j, k + 1);
+    float v100 = acc.get(g, i + 1, j, k);
+    vy = v100 + z * (v101 - v100);
+    float vx1 = vy + y * (vy1 - vy);
+    return vx + x * (vx1 - vx);
+}
+
+struct Medium {
+    float coeff;
+    float majorant;
+    uint32_t lookups;
+};
+
+__device__ __forceinline__ float sample_density(const rtc8x_grid& g, Medium& m, V3 o, V3 d, float t)
+{
+    Accessor acc;
+    acc.reset();
+    V3 idx = world_to_index(g, fmaf(t, d.x, o.x), fmaf(t, d.y, o.y), fmaf(t, d.z, o.z));
+    m.lookups++;
+    return m.coeff * trilinear(g, acc, idx);
+}
+
+// nanovdb::Ray::clip against the float world bbox (util/Ray.h:63-73, 427-455, 512-520)
+__device__ __forceinline__ bool clip(const rtc8x_grid& g, V3 o, V3 d, float tmax, float& t0o, float& t1o)
+{
+    float inv[3] = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float e[3] = {o.x, o.y, o.z};
+    float t0 = 0.0f, t1 = tmax;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float lo = g.world_bbox_min[a], hi = g.world_bbox_max[a];
+        if (lo >= hi) return false;
+        lo = (lo - e[a]) * inv[a];
+        hi = (hi - e[a]) * inv[a];
+        if (lo > hi) {
+            float t = lo;
+            lo = hi;
+            hi = t;
+        }
+        if (lo > t0) t0 = lo;
+        if (hi < t1) t1 = hi;
+        if (t0 > t1) return false;
+    }
+    t0o = t0;
+    t1o = t1;
+    return true;
+}
+
+// -------------------------------------------------------------- tracking
+// Global majorant, reference algorithm (optix_pathtracing_kernels.cu:110-125)
+__device__ __forceinline__ bool track_global(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0,
+                                             float t1, float& thit)
+{
+    float t = fmaxf(0.0f, t0);
+    for (;;) {
+        t += -rtc8x_logf(1.0f - rng.uni()) / m.majorant;
+        if (t > t1) return false;
+        float sigma = sample_density(g, m, o, d, t);
+        if (rng.uni() < sigma / m.majorant) {
+            thit = t;
+            return true;
+        }
+    }
+}
+
+// Ratio tracking, reference algorithm (renderer_kernel_common.h:371-382)
+__device__ __forceinline__ float ratio_global(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0,
+                                              float t1)
+{
+    float vis = 1.0f;
+    float t = fmaxf(0.0f, t0);
+    for (;;) {
+        t += -rtc8x_logf(1.0f - rng.uni()) / m.majorant;
+        if (t > t1) break;
+        float sigma = sample_density(g, m, o, d, t);
+        vis *= 1.0f - sigma / m.majorant;
+    }
+    return vis;
+}
+
+// 3D DDA over the 8^3 majorant cells in index space; boundaries recomputed from the cell index.
+struct Dda {
+    int32_t c[3], step[3];
+    float io[3], inv[3], tnext[3];
+    __device__ __forceinline__ float boundary(const rtc8x_grid& g, int a) const
+    {
+        int32_t b = g.cell_origin[a] + 8 * (c[a] + (step[a] > 0 ? 1 : 0));
+        return ((float)b - io[a]) * inv[a];
+    }
+    __device__ __forceinline__ void init(const rtc8x_grid& g, V3 o, V3 d, float t)
+    {
+        const float* mm = g.inv_mat;
+        float x = o.x - g.vec[0], y = o.y - g.vec[1], z = o.z - g.vec[2];
+        io[0] = fmaf(x, mm[0], fmaf(y, mm[1], z * mm[2]));
+        io[1] = fmaf(x, mm[3], fmaf(y, mm[4], z * mm[5]));
+        io[2] = fmaf(x, mm[6], fmaf(y, mm[7], z * mm[8]));
+        float id[3];
+        id[0] = fmaf(d.x, mm[0], fmaf(d.y, mm[1], d.z * mm[2]));
+        id[1] = fmaf(d.x, mm[3], fmaf(d.y, mm[4], d.z * mm[5]));
+        id[2] = fmaf(d.x, mm[6], fmaf(d.y, mm[7], d.z * mm[8]));
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float pa = fmaf(t, id[a], io[a]);
+            float cf = floorf((pa - (float)g.cell_origin[a]) * 0.125f);
+            int32_t cc = cf < 0.0f ? 0 : (cf > (float)(g.cell_dims[a] - 1) ? g.cell_dims[a] - 1 : (int32_t)cf);
+            c[a] = cc;
+            step[a] = id[a] > 0.0f ? 1 : (id[a] < 0.0f ? -1 : 0);
+            inv[a] = 1.0f / id[a];
+            tnext[a] = step[a] != 0 ? boundary(g, a) : __builtin_huge_valf();
+        }
+    }
+    __device__ __forceinline__ int axis() const
+    {
+        int a = 0;
+        if (tnext[1] < tnext[a]) a = 1;
+        if (tnext[2] < tnext[a]) a = 2;
+        return a;
+    }
+};
+
+__device__ __forceinline__ float cell_majorant(const rtc8x_grid& g, const Medium& m, const Dda& s)
+{
+    size_t idx = ((size_t)s.c[0] * (size_t)g.cell_dims[1] + (size_t)s.c[1]) * (size_t)g.cell_dims[2] + (size_t)s.c[2];
+    return m.coeff * g.cell_majorants[idx];
+}
+
+__device__ __forceinline__ bool track_local(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0, float t1,
+                                            float& thit)
+{
+    float t = fmaxf(0.0f, t0);
+    Dda s;
+    s.init(g, o, d, t);
+    float tau = -rtc8x_logf(1.0f - rng.uni());
+    for (;;) {
+        int a = s.axis();
+        float tend = s.tnext[a] < t1 ? s.tnext[a] : t1;
+        float mu = cell_majorant(g, m, s);
+        if (mu > 0.0f) {
+            for (;;) {
+                float tc = t + tau / mu;
+                if (!(tc < tend)) {
+                    tau = tau - mu * (tend - t);
+                    if (tau < 0.0f) tau = 0.0f;
+                    break;
+                }
+                t = tc;
+                float sigma = sample_density(g, m, o, d, t);
+                if (rng.uni() < sigma / mu) {
+                    thit = t;
+                    return true;
+                }
+                tau = -rtc8x_logf(1.0f - rng.uni());
+            }
+        }
+        if (!(s.tnext[a] < t1)) return false;
+        t = tend;
+        s.c[a] += s.step[a];
+        if (s.c[a] < 0 || s.c[a] >= g.cell_dims[a]) return false;
+        s.tnext[a] = s.boundary(g, a);
+    }
+}
+
+__device__ __forceinline__ float ratio_local(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0, float t1)
+{
+    float vis = 1.0f;
+    float t = fmaxf(0.0f, t0);
+    Dda s;
+    s.init(g, o, d, t);
+    float tau = -rtc8x_logf(1.0f - rng.uni());
+    for (;;) {
+        int a = s.axis();
+        float tend = s.tnext[a] < t1 ? s.tnext[a] : t1;
+        float mu = cell_majorant(g, m, s);
+        if (mu > 0.0f) {
+            for (;;) {
+                float tc = t + tau / mu;
+                if (!(tc < tend)) {
+                    tau = tau - mu * (tend - t);
+                    if (tau < 0.0f) tau = 0.0f;
+                    break;
+                }
+                t = tc;
+                float sigma = sample_density(g, m, o, d, t);
+                vis *= 1.0f - sigma / mu;
+                tau = -rtc8x_logf(1.0f - rng.uni());
+            }
+        }
+        if (!(s.tnext[a] < t1)) break;
+        t = tend;
+        s.c[a] += s.step[a];
+        if (s.c[a] < 0 || s.c[a] >= g.cell_dims[a]) break;
+        s.tnext[a] = s.boundary(g, a);
+    }
+    return vis;
+}
+
+template <int TRACKING>
+__device__ __forceinline__ bool free_flight(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float tmax,
+                                            float& thit)
+{
+    float t0, t1;
+    if (!(m.majorant > 0.0f)) return false;
+    if (!clip(g, o, d, tmax, t0, t1)) return false;
+    if (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) return track_local(g, m, rng, o, d, t0, t1, thit);
+    return track_global(g, m, rng, o, d, t0, t1, thit);
+}
+
+template <int TRACKING>
+__device__ __forceinline__ float transmittance(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float tmax)
+{
+    float t0, t1;
+    if (!(m.majorant > 0.0f)) return 1.0f;
+    if (!clip(g, o, d, tmax, t0, t1)) return 1.0f;
+    if (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) return ratio_local(g, m, rng, o, d, t0, t1);
+    return ratio_global(g, m, rng, o, d, t0, t1);
+}
+
+// -------------------------------------------------------------- phase functions
+// SchlickPhaseFunction (common_renderer_types.h:2063-2097) and Henyey-Greenstein.
+__device__ __forceinline__ float phase_value(int kind, float k, float cos_t)
+{
+    if (kind == RTC8X_PHASE_SCHLICK) {
+        float dterm = 1.0f - k * cos_t;
+        return (1.0f - k * k) / (4.0f * RTC8X_PI_F * (dterm * dterm));
+    }
+    float denom = 1.0f + k * k - 2.0f * k * cos_t;
+    return (1.0f - k * k) / (4.0f * RTC8X_PI_F * (denom * sqrtf(denom)));
+}
+
+__device__ __forceinline__ float phase_sample(int kind, float k, float u0, float u1, V3& dir_local)
+{
+    float cos_t;
+    if (kind == RTC8X_PHASE_SCHLICK) {
+        cos_t = (2.0f * u1 + k - 1.0f) / (2.0f * k * u1 - k + 1.0f);
+    } else {
+        if ((k < 0.0f ? -k : k) < (float)1e-3) {
+            cos_t = 1.0f - 2.0f * u1;
+        } else {
+            float sq = (1.0f - k * k) / (1.0f - k + 2.0f * k * u1);
+            cos_t = (1.0f + k * k - sq * sq) / (2.0f * k);
+        }
+    }
+    cos_t = cos_t < -1.0f ? -1.0f : (cos_t > 1.0f ? 1.0f : cos_t);
+    float phi = 2.0f * RTC8X_PI_F * u0;
+    float value = phase_value(kind, k, cos_t);
+    float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    float sp, cp;
+    rtc8x_sincosf(phi, &sp, &cp);
+    dir_local = V3{cp * sin_t, sp * sin_t, -cos_t};
+    return value;
+}
+
+// -------------------------------------------------------------- environment light
+__device__ __forceinline__ RGB env_texel(const rtc8x_env_light& e, float u, float v)
+{
+    uint32_t x = u >= 0.0f ? (uint32_t)(u * (float)e.width) : 0u;
+    uint32_t y = v >= 0.0f ? (uint32_t)(v * (float)e.height) : 0u;
+    if (x >= e.width) x = e.width - 1;
+    if (y >= e.height) y = e.height - 1;
+    const float4 t = *reinterpret_cast<const float4*>(e.texels + ((size_t)y * e.width + x) * 4);
+    return RGB{t.x, t.y, t.z};
+}
+
+__device__ __forceinline__ float env_texel_prob(const rtc8x_env_light& e, uint32_t px, uint32_t py)
+{
+    if (e.sampler == RTC8X_ENV_SAMPLER_ALIAS) {
+        if (px >= e.width) px = e.width - 1;
+        if (py >= e.height) py = e.height - 1;
+        return e.alias_weights[(size_t)py * e.width + px] / e.alias_integral;
+    }
+    if (px >= e.map_dim_x) px = e.map_dim_x - 1;
+    if (py >= e.map_dim_y) py = e.map_dim_y - 1;
+    return e.mips[(size_t)py * e.map_dim_x + px] / e.map_integral;
+}
+
+// EnvironmentalLight::evaluate + ImageBasedEnvironmentalLight::evaluate (common_renderer_types.h:1002-1013, 2442-2458)
+__device__ __forceinline__ RGB env_evaluate(const rtc8x_env_light& e, V3 dir, float& pdf)
+{
+    if (e.kind == RTC8X_ENV_CONSTANT) {
+        pdf = 1.0f / (4.0f * RTC8X_PI_F);
+        return scale(RGB{e.constant_radiance[0], e.constant_radiance[1], e.constant_radiance[2]}, e.power_coeff);
+    }
+    const float two_pi = 2.0f * RTC8X_PI_F;
+    float y = dir.y < -1.0f ? -1.0f : (dir.y > 1.0f ? 1.0f : dir.y);
+    float theta = rtc8x_acosf(y);
+    float pos_phi = rtc8x_fmodf(rtc8x_atan2f(-dir.x, dir.z) + two_pi, two_pi);
+    float phi = pos_phi + e.rotation;
+    phi = phi - floorf(phi / two_pi) * two_pi;
+    float u = phi / two_pi, v = theta / RTC8X_PI_F;
+    RGB tex = env_texel(e, u, v);
+    uint32_t px = (uint32_t)((float)e.width * u), py = (uint32_t)((float)e.height * v);
+    float prob = env_texel_prob(e, px, py);
+    float uv_pd = (float)(e.width * e.height) * prob;
+    float sp, cp;
+    rtc8x_sincosf(theta, &sp, &cp);
+    pdf = uv_pd / (2.0f * (RTC8X_PI_F * RTC8X_PI_F) * sp);
+    if (!isfinite(pdf)) {
+        pdf = 0.0f;
+        return RGB{0.0f, 0.0f, 0.0f};
+    }
+    return scale(tex, e.power_coeff);
+}
+
+// HierarchicalImportanceMap::sample (common_renderer_types.h:479-548) on explicit mip arrays.
+__device__ __forceinline__ void map_sample(const rtc8x_env_light& e, float u0, float u1, uint32_t& ox, uint32_t& oy,
+                                           float& ru0, float& ru1, float& prob)
+{
+    uint32_t ix = 0, iy = 0;
+    uint32_t nlev = e.num_mip_levels;
+    uint32_t cdx = 2, cdy = e.map_dim_x > e.map_dim_y ? 1 : 2;
+    float pr = 1.0f;
+    // offset of level lev = sum of the sizes of levels 0..lev-1
+    size_t off_top = 0;
+    for (uint32_t l = 0; l + 2 < nlev; ++l) {
+        uint32_t dx = max(e.map_dim_x >> l, 1u), dy = max(e.map_dim_y >> l, 1u);
+        off_top += (size_t)dx * dy;
+    }
+    size_t off = off_top;
+    for (int32_t lev = (int32_t)nlev - 2; lev >= 0; --lev) {
+        ix *= 2;
+        iy *= 2;
+        uint32_t dx = max(e.map_dim_x >> lev, 1u);
+        const float* mip = e.mips + off;
+        float vul = mip[(size_t)iy * dx + ix];
+        float vur = (ix + 1 < cdx) ? mip[(size_t)iy * dx + ix + 1] : 0.0f;
+        float vll = (iy + 1 < cdy) ? mip[(size_t)(iy + 1) * dx + ix] : 0.0f;
+        float vlr = (ix + 1 < cdx && iy + 1 < cdy) ? mip[(size_t)(iy + 1) * dx + ix + 1] : 0.0f;
+        float left = vll + vul, right = vlr + vur;
+        u0 *= (left + right);
+        pr *= 1.0f / (left + right);
+        float upper, lower;
+        if (u0 < left) {
+            u0 = (u0 - 0.0f) / left;
+            upper = vul;
+            lower = vll;
+        } else {
+            ix += 1;
+            u0 = (u0 - left) / right;
+            upper = vur;
+            lower = vlr;
+        }
+        u1 *= (upper + lower);
+        if (u1 < upper) {
+            pr *= upper;
+            u1 = (u1 - 0.0f) / upper;
+        } else {
+            iy += 1;
+            pr *= lower;
+            u1 = (u1 - upper) / lower;
+        }
+        cdx *= 2;
+        cdy *= 2;
+        if (lev > 0) {
+            uint32_t pdx = max(e.map_dim_x >> (lev - 1), 1u), pdy = max(e.map_dim_y >> (lev - 1), 1u);
+            off -= (size_t)pdx * pdy;
+        }
+    }
+    ox = ix;
+    oy = iy;
+    ru0 = u0;
+    ru1 = u1;
+    prob = pr;
+}
+
+// DiscreteDistribution1DTemplate<.., true>::sample (common_renderer_types.h:243-259)
+__device__ __forceinline__ uint32_t alias_pick(const rtc8x_alias_entry* table, uint32_t n, float u, float& remapped)
+{
+    uint32_t idx = (uint32_t)(u * (float)n);
+    if (idx > n - 1) idx = n - 1;
+    float ur = u * (float)n - (float)idx;
+    rtc8x_alias_entry en = table[idx];
+    float pf = en.prob_to_pick_first;
+    if (ur < pf) {
+        remapped = ur / pf;
+        return idx;
+    }
+    remapped = (ur - pf) / (1.0f - pf);
+    return en.second_index;
+}
+
+// EnvironmentalLight::sample + ImageBasedEnvironmentalLight::sample (common_renderer_types.h:988-1000, 2419-2440)
+__device__ __forceinline__ RGB env_sample(const rtc8x_env_light& e, float u0, float u1, V3& dir, float& pdf)
+{
+    if (e.kind == RTC8X_ENV_CONSTANT) {
+        float z = 1.0f - 2.0f * u0;
+        float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float phi = 2.0f * RTC8X_PI_F * u1;
+        float sp, cp;
+        rtc8x_sincosf(phi, &sp, &cp);
+        dir = V3{r * cp, r * sp, z};
+        pdf = 1.0f / (4.0f * RTC8X_PI_F);
+        return scale(RGB{e.constant_radiance[0], e.constant_radiance[1], e.constant_radiance[2]}, e.power_coeff);
+    }
+    uint32_t px, py;
+    float ru0, ru1, prob;
+    if (e.sampler == RTC8X_ENV_SAMPLER_ALIAS) {
+        py = alias_pick(e.alias_rows, e.height, u1, ru1);
+        px = alias_pick(e.alias_cols + (size_t)py * e.width, e.width, u0, ru0);
+        prob = e.alias_weights[(size_t)py * e.width + px] / e.alias_integral;
+    } else {
+        map_sample(e, u0, u1, px, py, ru0, ru1, prob);
+    }
+    float u = ((float)px + ru0) / (float)e.width;
+    float v = ((float)py + ru1) / (float)e.height;
+    float uv_pd = (float)(e.width * e.height) * prob;
+    const float two_pi = 2.0f * RTC8X_PI_F;
+    float phi = two_pi * u;
+    float theta = RTC8X_PI_F * v;
+    float st, ct;
+    rtc8x_sincosf(theta, &st, &ct);
+    pdf = uv_pd / (2.0f * (RTC8X_PI_F * RTC8X_PI_F) * st);
+    if (!isfinite(pdf)) {
+        pdf = 0.0f;
+        return RGB{0.0f, 0.0f, 0.0f};
+    }
+    RGB radiance = scale(env_texel(e, u, v), e.power_coeff);
+    float pos_phi = phi - e.rotation;
+    pos_phi = pos_phi - floorf(pos_phi / two_pi) * two_pi;
+    float sp, cp;
+    rtc8x_sincosf(pos_phi, &sp, &cp);
+    dir = V3{-sp * st, ct, cp * st};  // Vector3D::fromPolarYUp (basic_types.h:900-907)
+    return radiance;
+}
+
+}  // namespace dev
+}  // namespace rtc8x

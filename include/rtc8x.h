@@ -199,6 +199,47 @@ typedef struct rtc8x_sun_light {
 } rtc8x_sun_light;
 
 /* ------------------------------------------------------------------------------------------------
+ * Hosek-Wilkie 2013 sky bake (renderer/gpu_kernels/analytic_sky_kernels.cu): 16 spectral bands
+ * (CIE 1931 2-degree observer binned over 360-830 nm), each with its own model state, summed to
+ * XYZ, converted to linear sRGB, saturation adjusted, solar disc inside solar_radius, and a sea
+ * colour below the horizon.
+ * ---------------------------------------------------------------------------------------------- */
+#define RTC8X_SKY_BANDS 16
+
+/* ArHosekSkyModelState in float (common/common_renderer_types.h:955-966). */
+typedef struct rtc8x_hosek_band {
+    float configs[11][9];
+    float radiances[11];
+    float turbidity;
+    float solar_radius;
+    float emission_correction_factor_sky[11];
+    float emission_correction_factor_sun[11];
+    float albedo;
+    float elevation;
+} rtc8x_hosek_band;
+
+typedef struct rtc8x_hosek_sky {
+    const rtc8x_hosek_band* bands;      /* device, RTC8X_SKY_BANDS entries */
+    const float* solar_rows;            /* device, 11 wavelengths x 2 turbidity indices x 180 coefficients */
+    const float* limb_darkening;        /* device, 11 x 6 */
+    int32_t solar_turbidity_base;       /* turbidity index of solar_rows[.][0] (= int(turbidity) - 1) */
+    float cmf_x[RTC8X_SKY_BANDS];       /* ArHosekSkyModelCMFSet (common_renderer_types.h:968-975) */
+    float cmf_y[RTC8X_SKY_BANDS];
+    float cmf_z[RTC8X_SKY_BANDS];
+    float center_wavelengths[RTC8X_SKY_BANDS];
+    float integral_cmf;
+} rtc8x_hosek_sky;
+
+/* generateArHosekSkyEnvironmentalTexture (analytic_sky_kernels.cu:220-276): writes width x height
+ * RGBA32F texels (alpha 1) of the lat-long sky for the given unit sun direction. */
+rtc8x_status rtc8x_sky_bake(void* stream, const rtc8x_hosek_sky* sky, const float sun_dir[3], uint32_t width,
+                            uint32_t height, float* out_rgba);
+/* The same per-texel evaluation at n explicit directions (phi[i], theta[i]) (Y-up polar angles);
+ * out_rgb has 3n floats.  Used to check the bake against FP64 ground truth. */
+rtc8x_status rtc8x_sky_eval(void* stream, const rtc8x_hosek_sky* sky, const float sun_dir[3], uint32_t n,
+                            const float* phi, const float* theta, float* out_rgb);
+
+/* ------------------------------------------------------------------------------------------------
  * Camera: PerspectiveCamera (common/common_renderer_types.h:1019-1034) reduced to what the
  * ray generation uses (optix_pathtracing_kernels.cu:43-55).
  * ---------------------------------------------------------------------------------------------- */

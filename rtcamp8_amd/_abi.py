@@ -207,7 +207,39 @@ class FrameParams(C.Structure):
     ]
 
 
+class HosekBand(C.Structure):
+    _fields_ = [
+        ("configs", (C.c_float * 9) * 11),
+        ("radiances", C.c_float * 11),
+        ("turbidity", C.c_float),
+        ("solar_radius", C.c_float),
+        ("emission_correction_factor_sky", C.c_float * 11),
+        ("emission_correction_factor_sun", C.c_float * 11),
+        ("albedo", C.c_float),
+        ("elevation", C.c_float),
+    ]
+
+
+SKY_BANDS = 16
+
+
+class HosekSky(C.Structure):
+    _fields_ = [
+        ("bands", C.c_void_p),
+        ("solar_rows", C.c_void_p),
+        ("limb_darkening", C.c_void_p),
+        ("solar_turbidity_base", C.c_int32),
+        ("cmf_x", C.c_float * SKY_BANDS),
+        ("cmf_y", C.c_float * SKY_BANDS),
+        ("cmf_z", C.c_float * SKY_BANDS),
+        ("center_wavelengths", C.c_float * SKY_BANDS),
+        ("integral_cmf", C.c_float),
+    ]
+
+
 STRUCTS = {
+    "rtc8x_hosek_band": HosekBand,
+    "rtc8x_hosek_sky": HosekSky,
     "rtc8x_root_tile": RootTile,
     "rtc8x_grid": Grid,
     "rtc8x_fbm_desc": FbmDesc,
@@ -257,4 +289,6 @@ PROTOTYPES = {
     "rtc8x_camera_setup": (_S, [C.POINTER(Camera), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float]),
     "rtc8x_trace": (_S, [_P, C.POINTER(FrameParams), C.c_int32, C.c_uint32]),
     "rtc8x_seed_rng": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "rtc8x_sky_bake": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, _P]),
+    "rtc8x_sky_eval": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, _P, _P, _P]),
 }

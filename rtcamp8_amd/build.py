@@ -23,6 +23,7 @@ SOURCES = [
     "rtc8x_api.hip",
     "rtc8x_grid.hip",
     "rtc8x_env.hip",
+    "rtc8x_sky.hip",
     "rtc8x_trace.hip",
 ]
 

@@ -89,7 +89,8 @@ CONFIGS = {
         fbm=FbmConfig(resolution=512, seed=0x5EED0002, sparse=True, active_leaf_fraction=0.25),
         density_coeff=50.0, albedo=0.999, phase=_abi.PHASE_HENYEY_GREENSTEIN, forwardness=0.85,
         width=1920, height=1080, spp=16, max_path_length=256, rr_after_path_length=8,
-        env_kind="hosek", env_sampler=_abi.ENV_SAMPLER_ALIAS, sun_enabled=False, fov_y_deg=30.0),
+        env_kind="hosek", env_sampler=_abi.ENV_SAMPLER_ALIAS, sun_enabled=False,
+        camera_position=(1.27, 0.95, 2.0), fov_y_deg=32.0),
 }
 
 

@@ -197,3 +197,66 @@ def trace_parity_case(config: str, width: int, height: int, spp: int, tracking: 
         "paths": counters["paths"], "vertices": counters["vertices"], "lookups": counters["lookups"],
         "cpu_stats": stats,
     }
+
+
+def nanovdb_fixture():
+    """The NanoVDB ground truth (tests/golden/nanovdb_probe.npz) with the probe grid's voxel values
+    regenerated from the probe's hash (value = 0.05 + (mix(l*512+v ^ 0xA5A5A5A5) >> 8) / 2^24)."""
+    fx = dict(np.load(ROOT / "tests" / "golden" / "nanovdb_probe.npz"))
+    nl = fx["leaf_origins"].shape[0]
+    x = (np.arange(nl * 512, dtype=np.uint64) ^ np.uint64(0xA5A5A5A5)).astype(np.uint32)
+    x ^= x >> np.uint32(16)
+    x = (x.astype(np.uint64) * 0x7FEB352D & 0xFFFFFFFF).astype(np.uint32)
+    x ^= x >> np.uint32(15)
+    x = (x.astype(np.uint64) * 0x846CA68B & 0xFFFFFFFF).astype(np.uint32)
+    x ^= x >> np.uint32(16)
+    vals = np.float32(0.05) + (x >> np.uint32(8)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+    fx["leaf_values"] = vals.reshape(nl, 512)
+    assert np.array_equal(fx["leaf_values"][0], fx["leaf0_values"]), "probe value formula drifted"
+    return fx
+
+
+def tables_from_leaves(origins: np.ndarray, values: np.ndarray, voxel_size: float, world_bbox) -> HostGrid:
+    """Builds the library's NanoVDB-layout tables for arbitrary leaves (several root tiles allowed)."""
+    origins = np.asarray(origins, dtype=np.int32).reshape(-1, 3)
+    u = origins.astype(np.int64) & 0xFFFFFFFF
+    keys = (u[:, 2] >> 12) | ((u[:, 1] >> 12) << 21) | ((u[:, 0] >> 12) << 42)
+    uniq_keys = sorted(set(int(k) for k in keys))
+    tile_of = {k: i for i, k in enumerate(uniq_keys)}
+    uoff = (((u[:, 0] & 4095) >> 7) << 10) | (((u[:, 1] & 4095) >> 7) << 5) | ((u[:, 2] & 4095) >> 7)
+    loff = (((u[:, 0] & 127) >> 3) << 8) | (((u[:, 1] & 127) >> 3) << 4) | ((u[:, 2] & 127) >> 3)
+    upper = np.full(len(uniq_keys) * UPPER, TILE, dtype=np.uint32)
+    lower_of = {}
+    for l in range(len(origins)):
+        t = tile_of[int(keys[l])]
+        slot = t * UPPER + int(uoff[l])
+        if slot not in lower_of:
+            lower_of[slot] = len(lower_of)
+            upper[slot] = lower_of[slot]
+    lower = np.full(max(len(lower_of), 1) * LOWER, TILE, dtype=np.uint32)
+    for l in range(len(origins)):
+        t = tile_of[int(keys[l])]
+        li = lower_of[t * UPPER + int(uoff[l])]
+        lower[li * LOWER + int(loff[l])] = l
+    root = np.zeros(len(uniq_keys), dtype=np.dtype([("key", "<u8"), ("entry", "<u4"), ("reserved", "<u4")]))
+    root["key"] = np.array(uniq_keys, dtype=np.uint64)
+    root["entry"] = np.arange(len(uniq_keys), dtype=np.uint32)
+    g = _abi.Grid()
+    inv = np.float32(1.0 / voxel_size)
+    g.inv_mat[:] = [inv, 0, 0, 0, inv, 0, 0, 0, inv]
+    g.vec[:] = [0.0, 0.0, 0.0]
+    g.world_bbox_min[:] = list(np.asarray(world_bbox[:3], dtype=np.float32))
+    g.world_bbox_max[:] = list(np.asarray(world_bbox[3:], dtype=np.float32))
+    g.index_bbox_min[:] = origins.min(axis=0).tolist()
+    g.index_bbox_max[:] = (origins.max(axis=0) + 7).tolist()
+    g.max_value = float(np.max(values))
+    g.num_root_tiles, g.num_upper, g.num_lower, g.num_leaves = len(uniq_keys), len(uniq_keys), len(lower_of), len(origins)
+    arrays = {"root": root, "upper": upper, "lower": lower,
+              "values": np.ascontiguousarray(values, dtype=np.float32).reshape(-1),
+              "tiles": np.zeros(1, np.float32), "origins": origins.reshape(-1).copy(), "cells": None}
+    return host_grid_from_arrays(g, arrays)
+
+
+UPPER = _abi.UPPER_TABLE_SIZE
+LOWER = _abi.LOWER_TABLE_SIZE
+TILE = _abi.TILE_BIT

@@ -104,3 +104,29 @@ def test_schlick_reference_mode_bit_exact():
                             phase=_abi.PHASE_SCHLICK, forwardness=0.3, max_path_length=0, sun_enabled=False)
     assert res["rng_equal"] and res["vertices_equal"]
     assert np.array_equal(res["gpu_accum"].view(np.uint64), res["cpu_accum"].view(np.uint64))
+
+
+def test_uploaded_grid_matches_nanovdb_fixture():
+    """rtc8x_grid_upload + rtc8x_grid_probe reproduce NanoVDB's own lookups (several root tiles)."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
+
+    from oracle_lib import nanovdb_fixture, tables_from_leaves
+
+    fx = nanovdb_fixture()
+    hg = tables_from_leaves(fx["leaf_origins"], fx["leaf_values"], float(fx["voxel_size"]), fx["world_bbox"])
+    dg = _abi.Grid()
+    call("rtc8x_grid_upload", C.byref(hg.grid), C.byref(dg))
+    try:
+        assert dg.max_value == float(np.max(fx["leaf_values"]))
+        n = fx["p_world"].shape[0]
+        dpts = DeviceBuffer.from_array(np.ascontiguousarray(fx["p_world"], dtype=np.float32))
+        dijk, doff, dval = DeviceBuffer(12 * n), DeviceBuffer(12 * n), DeviceBuffer(4 * n)
+        call("rtc8x_grid_probe", C.byref(dg), n, C.c_void_p(dpts.ptr), C.c_void_p(dijk.ptr), C.c_void_p(doff.ptr),
+             C.c_void_p(dval.ptr), None)
+        assert np.array_equal(dijk.download(np.int32, 3 * n).reshape(n, 3), fx["p_ijk"])
+        assert np.array_equal(doff.download(np.uint32, 3 * n).reshape(n, 3), fx["p_offsets"])
+        assert np.array_equal(dval.download(np.float32, n).view(np.uint32), fx["p_value"].view(np.uint32))
+    finally:
+        call("rtc8x_grid_destroy", C.byref(dg))

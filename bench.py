@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="sparse512")
-    ap.add_argument("--tracking", choices=["local", "global"], default="local")
+    ap.add_argument("--tracking", choices=["local", "global"], default="global")
     ap.add_argument("--cpu-rows", type=int, default=6, help="image rows of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()

@@ -126,6 +126,7 @@ typedef struct rtc8x_fbm_desc {
     float gain;                 /* 0.5 */
     int32_t sparse;             /* 0: dense (every leaf stored); 1: thresholded cloud */
     float active_leaf_fraction; /* sparse only: fraction of the N^3/512 leaf slots kept (0.25) */
+    float contrast;             /* density = min(contrast * (g - threshold) / (g_max - threshold), 1) */
 } rtc8x_fbm_desc;
 
 /* Builds the fBm grid on the device.  The returned struct's device tables are owned by the library

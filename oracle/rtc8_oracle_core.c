@@ -296,6 +296,8 @@ int oracle_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid* g)
                 float dv = gv - sub;
                 if (dv < 0.0f) dv = 0.0f;
                 dv = dv * inv_range;
+                dv = dv * d->contrast;
+                if (dv > 1.0f) dv = 1.0f;
                 values[(size_t)e * 512 + (size_t)v] = dv;
                 if (dv > vmax) vmax = dv;
             }

@@ -73,6 +73,7 @@ class FbmDesc(C.Structure):
         ("gain", C.c_float),
         ("sparse", C.c_int32),
         ("active_leaf_fraction", C.c_float),
+        ("contrast", C.c_float),
     ]
 
 

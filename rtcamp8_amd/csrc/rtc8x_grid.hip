@@ -112,6 +112,8 @@ __global__ __launch_bounds__(512) void fill_leaves_kernel(const rtc8x_fbm_desc d
     float dv = gv - sub;
     if (dv < 0.0f) dv = 0.0f;
     dv = dv * inv_range;
+    dv = dv * d.contrast;
+    if (dv > 1.0f) dv = 1.0f;
     values[(size_t)leaf * 512 + v] = dv;
 }
 
@@ -223,6 +225,7 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
                   "rtc8x_grid_build_fbm: non-finite parameters");
     RTC8X_REQUIRE(!d->sparse || (d->active_leaf_fraction > 0.0f && d->active_leaf_fraction <= 1.0f),
                   "rtc8x_grid_build_fbm: active_leaf_fraction must be in (0, 1]");
+    RTC8X_REQUIRE(std::isfinite(d->contrast) && d->contrast >= 1.0f, "rtc8x_grid_build_fbm: contrast must be >= 1");
     std::memset(out, 0, sizeof(*out));
     hipStream_t s = (hipStream_t)stream;
     const uint32_t nl = n / 8, num_slots = nl * nl * nl;
@@ -298,7 +301,10 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
             }
             float dv = slot_max[(lx * nl + ly) * nl + lz] - sub;
             if (dv < 0.0f) dv = 0.0f;
-            leaf_max.push_back(dv * inv_range);
+            dv = dv * inv_range;
+            dv = dv * d->contrast;
+            if (dv > 1.0f) dv = 1.0f;
+            leaf_max.push_back(dv);
         }
     }
     RTC8X_REQUIRE(num_leaves > 0, "rtc8x_grid_build_fbm: no active leaves");

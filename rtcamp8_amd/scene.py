@@ -36,10 +36,11 @@ class FbmConfig:
     gain: float = 0.5
     sparse: bool = False
     active_leaf_fraction: float = 0.25
+    contrast: float = 1.0
 
     def desc(self) -> _abi.FbmDesc:
         return _abi.FbmDesc(self.resolution, self.octaves, self.seed, self.base_frequency, self.lacunarity, self.gain,
-                            1 if self.sparse else 0, self.active_leaf_fraction)
+                            1 if self.sparse else 0, self.active_leaf_fraction, self.contrast)
 
 
 @dataclass(frozen=True)
@@ -86,7 +87,7 @@ CONFIGS = {
         env_kind="constant", env_radiance=(1.0, 1.0, 1.0), sun_enabled=True),
     "sparse512": SceneConfig(
         name="sparse512",
-        fbm=FbmConfig(resolution=512, seed=0x5EED0002, sparse=True, active_leaf_fraction=0.25),
+        fbm=FbmConfig(resolution=512, seed=0x5EED0002, sparse=True, active_leaf_fraction=0.25, contrast=4.0),
         density_coeff=50.0, albedo=0.999, phase=_abi.PHASE_HENYEY_GREENSTEIN, forwardness=0.85,
         width=1920, height=1080, spp=16, max_path_length=256, rr_after_path_length=8,
         env_kind="hosek", env_sampler=_abi.ENV_SAMPLER_ALIAS, sun_enabled=False,

@@ -329,7 +329,7 @@ typedef struct rtc8x_nrc_buffers {
     uint32_t reserved;
     uint32_t* num_training_data;             /* [2] */
     int32_t* train_image_size;               /* [2][2] */
-    uint32_t* target_min_max;                /* [2][2][3] ordered-int RGB */
+    float* target_min_max;                   /* [2][2][3]: per buffer index, min RGB then max RGB */
     float* target_avg;                       /* [2][3] */
     rtc8x_radiance_query* inference_queries; /* image size (rounded up to 128) */
     rtc8x_terminal_info* inference_terminals;/* image size */
@@ -374,6 +374,87 @@ typedef struct rtc8x_frame_params {
  * must be 1. */
 rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t entry,
                          uint32_t num_samples);
+
+/* ------------------------------------------------------------------------------------------------
+ * NRC data-path kernels (renderer/gpu_kernels/nrc_set_up_kernels.cu).  They read the same
+ * rtc8x_frame_params as rtc8x_trace (buffers in params->nrc, image size, accum).
+ * ---------------------------------------------------------------------------------------------- */
+/* prepareNRC (:4-44): sets train_image_size[buf] (image/4 for a new sequence, else the previous size
+ * scaled by sqrt(65536 / previous record count), clamped to the image), zeroes num_training_data[buf],
+ * resets the target statistics and clears the training-suffix terminals. */
+rtc8x_status rtc8x_nrc_prepare(void* stream, const rtc8x_frame_params* params, int32_t is_new_sequence);
+/* propagateRadianceValues (:46-86): completes training targets along each suffix's vertex chain. */
+rtc8x_status rtc8x_nrc_propagate(void* stream, const rtc8x_frame_params* params);
+/* shuffleTrainingData (:88-174): scatters 65536 records from train_queries/targets[0] into [1] through
+ * the LCG permutation, zeroes invalid records, scales and clamps targets, and writes the target
+ * min / max / mean (reduced in a fixed order, so the result is deterministic). */
+rtc8x_status rtc8x_nrc_shuffle(void* stream, const rtc8x_frame_params* params);
+/* accumulateInferredRadianceValues (:176-219): accum += direct + throughput * max(NRC, 0) / scale *
+ * (rho_d + rho_s), in FP64. */
+rtc8x_status rtc8x_nrc_accumulate(void* stream, const rtc8x_frame_params* params);
+/* Bytes of params->nrc.scratch the library needs for an image of width x height. */
+size_t rtc8x_nrc_scratch_bytes(uint32_t width, uint32_t height);
+
+/* ------------------------------------------------------------------------------------------------
+ * Neural radiance cache network: replaces class NeuralRadianceCache (renderer/network_interface.h:17-31,
+ * network_interface.cu:50-159) and the tiny-cuda-nn objects behind it.
+ *
+ * Input: n RadianceQuery records (14 floats each, AoS).  Encoding (64 dims): TriangleWave with 12
+ * frequencies on the 3 position dims (36), OneBlob with 4 bins on the 5 angle/roughness dims (20;
+ * polar angles scaled by 1/pi, azimuths by 1/(2 pi)), identity on the 6 reflectance dims, then a
+ * constant 1 (the first layer's bias) and one zero pad.  Network: num_hidden_layers ReLU layers of
+ * width 64 and a linear layer to RGB, no biases (FullyFusedMLP).  Loss: relative L2 over luminance,
+ * sum_c (p_c - t_c)^2 / (lum(p)^2 + 0.01) with the normaliser held constant.  Optimiser: Adam with L2
+ * regularisation, wrapped by an exponential moving average of the weights that inference uses.
+ * Weights and activations are FP16 or BF16 with FP32 accumulation (MFMA); master weights, moments and
+ * the EMA are FP32.  The per-sample loss gradient is used unscaled in the backward pass (a loss scale
+ * of 3n) and divided out in the optimiser step.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct rtc8x_nrc rtc8x_nrc;
+
+typedef enum rtc8x_nrc_precision {
+    RTC8X_NRC_FP16 = 0,
+    RTC8X_NRC_BF16 = 1
+} rtc8x_nrc_precision;
+
+typedef struct rtc8x_nrc_desc {
+    uint32_t num_hidden_layers;  /* renderer_scene.cpp:50 uses 2; the standalone benchmark 5 (1..8) */
+    float learning_rate;         /* 1e-2 */
+    float beta1, beta2;          /* 0.9, 0.99 */
+    float epsilon;               /* 1e-8 (TriangleWave configuration, network_interface.cu:96) */
+    float l2_reg;                /* 1e-6 */
+    float ema_decay;             /* 0.99 */
+    int32_t precision;           /* rtc8x_nrc_precision */
+    uint64_t seed;               /* weight initialisation (Xavier-uniform from PCG32) */
+} rtc8x_nrc_desc;
+
+#define RTC8X_NRC_INPUT_DIMS 14
+#define RTC8X_NRC_ENCODED_DIMS 64
+#define RTC8X_NRC_WIDTH 64
+#define RTC8X_NRC_OUTPUT_DIMS 3
+#define RTC8X_NRC_OUTPUT_PAD 16
+
+/* initialize() (network_interface.cu:50-134) */
+rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* desc, rtc8x_nrc** out);
+/* finalize() (network_interface.cu:136-141) */
+rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* nrc);
+/* infer() (network_interface.cu:143-149): out[3n] from in[14n], EMA weights. */
+rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* nrc, void* stream, const float* in, uint32_t n, float* out);
+/* train() (network_interface.cu:151-159): one optimiser step on n samples (n <= 65536); if loss_on_cpu
+ * is not NULL the mean loss is copied to it (synchronises the stream). */
+rtc8x_status rtc8x_nrc_train(rtc8x_nrc* nrc, void* stream, const float* in, const float* target, uint32_t n,
+                             float* loss_on_cpu);
+/* Number of weights (layer matrices [out][in], the output layer padded to 16 rows), and host copies of
+ * them: which = 0 master weights, 1 the (bias-corrected) EMA used for inference. */
+uint32_t rtc8x_nrc_num_params(const rtc8x_nrc* nrc);
+rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* nrc, int32_t which, float* host, size_t count);
+/* Replaces the master weights (and resets optimiser state and EMA) from a host array. */
+rtc8x_status rtc8x_nrc_set_weights(rtc8x_nrc* nrc, const float* host, size_t count);
+/* Parity probes: the 64-dim encoding of n queries (out[64n]), and the summed per-sample loss gradient
+ * with respect to the weights for one batch without an update (grad[num_params], device). */
+rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t n, float* out);
+rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* nrc, void* stream, const float* in, const float* target, uint32_t n,
+                                float* grad);
 
 /* Seeds rng_states row-major from std::mt19937_64(seed) as initializeScreenRelatedBuffers does
  * (renderer/renderer_main.cpp:86-93).  Host-side, then uploads; `states` is a device pointer. */

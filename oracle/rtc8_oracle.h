@@ -54,6 +54,27 @@ int oracle_trace_f32(const rtc8x_frame_params* p, int32_t entry, uint32_t num_sa
 int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
 
+
+/* NRC data path (renderer/gpu_kernels/nrc_set_up_kernels.cu) on host copies of params->nrc. */
+void oracle_nrc_prepare(const rtc8x_frame_params* p, int32_t is_new_sequence);
+void oracle_nrc_propagate(const rtc8x_frame_params* p);
+void oracle_nrc_shuffle(const rtc8x_frame_params* p);
+void oracle_nrc_accumulate(const rtc8x_frame_params* p);
+
+/* NRC network (FP64 forward / gradient, FP32 Adam). round_mode: 0 none, 1 FP16, 2 BF16. */
+uint32_t oracle_nrc_num_params(uint32_t L);
+void oracle_nrc_encode(const rtc8x_radiance_query* q, uint32_t n, double* out);
+void oracle_nrc_forward(const float* weights, uint32_t L, const rtc8x_radiance_query* q, uint32_t n, double* out3,
+                        int round_mode);
+double oracle_nrc_loss(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target, uint32_t n,
+                       int round_mode);
+double oracle_nrc_gradient(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target,
+                           uint32_t n, double* grad, int round_mode);
+void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_sum, uint32_t P, uint32_t n,
+                     float lr, float beta1, float beta2, float eps, float l2, float decay, uint64_t step);
+float oracle_round_fp16(float x);
+float oracle_round_bf16(float x);
+
 #ifdef __cplusplus
 }
 #endif

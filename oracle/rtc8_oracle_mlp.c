@@ -1,0 +1,205 @@
+/*
+ * rtc8_oracle_mlp.c -- CPU oracle (TEST INFRASTRUCTURE ONLY) of the neural radiance cache network:
+ * input encoding, forward pass, relative-L2-luminance loss and its weight gradient in FP64, and the
+ * Adam + EMA step in FP32.  The network replaces tiny-cuda-nn behind renderer/network_interface.cu,
+ * whose exact numerics are not available (the tiny-cuda-nn submodule is not vendored), so this
+ * restates the published methods: FullyFusedMLP without biases, RelativeL2Luminance (Mueller et al.
+ * 2021, normaliser held constant), Adam (Kingma & Ba 2015) with L2 regularisation, weight EMA.
+ *
+ * `round_mode` emulates the library's storage precision (0: none, 1: FP16, 2: BF16) on weights,
+ * activations, output gradients and pre-activation gradients, so that the only remaining difference
+ * to the GPU is the accumulation order of the FP32 MFMA dot products.
+ */
+#include "rtc8_oracle.h"
+#include "../include/rtc8x_detmath.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define W 64
+#define LAYER (W * W)
+
+uint32_t oracle_nrc_num_params(uint32_t L) { return L * LAYER + RTC8X_NRC_OUTPUT_PAD * W; }
+
+/* IEEE binary16, round to nearest even (overflow to inf), back to float */
+static float round_fp16(float x)
+{
+    uint32_t u = rtc8x_f2u(x);
+    uint32_t sign = u & 0x80000000u;
+    uint32_t ax = u & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return x;                         /* inf / nan */
+    float a = rtc8x_u2f(ax);
+    if (a >= 65520.0f) return rtc8x_u2f(sign | 0x7f800000u); /* rounds to inf */
+    if (a < 6.103515625e-05f) {                              /* subnormal half: quantum 2^-24 */
+        float q = a * 16777216.0f;                           /* exact scaling */
+        float r = nearbyintf(q);                             /* round half to even (default mode) */
+        return rtc8x_u2f(sign | rtc8x_f2u(r / 16777216.0f));
+    }
+    uint32_t lsb = (ax >> 13) & 1u;
+    uint32_t rounded = (ax + 0xfffu + lsb) & ~0x1fffu;       /* keep 10 mantissa bits, RNE */
+    return rtc8x_u2f(sign | rounded);
+}
+
+static float round_bf16(float x)
+{
+    uint32_t u = rtc8x_f2u(x);
+    if ((u & 0x7fffffffu) >= 0x7f800000u) return x;
+    uint32_t lsb = (u >> 16) & 1u;
+    u = (u + 0x7fffu + lsb) & 0xffff0000u;
+    return rtc8x_u2f(u);
+}
+
+static double rnd(double x, int mode)
+{
+    if (mode == 1) return (double)round_fp16((float)x);
+    if (mode == 2) return (double)round_bf16((float)x);
+    return x;
+}
+
+/* The library's 64-feature input encoding (include/rtc8x.h, "Neural radiance cache network"). */
+void oracle_nrc_encode(const rtc8x_radiance_query* q, uint32_t n, double* out)
+{
+    const double pi = 3.14159265358979323846;
+    for (uint32_t s = 0; s < n; ++s) {
+        double* e = out + (size_t)s * 64;
+        for (int d = 0; d < 3; ++d)
+            for (int f = 0; f < 12; ++f) {
+                double t = (double)q[s].position[d] * ldexp(1.0, f);
+                t = t - floor(t);
+                e[12 * d + f] = fabs(2.0 * t - 1.0);
+            }
+        double a[5] = {q[s].normal_phi / pi, q[s].normal_theta / (2 * pi), q[s].vout_phi / pi, q[s].vout_theta / (2 * pi),
+                       q[s].roughness};
+        for (int j = 0; j < 5; ++j)
+            for (int b = 0; b < 4; ++b) {
+                double dd = a[j] - (b + 0.5) * 0.25;
+                e[36 + 4 * j + b] = exp(-8.0 * dd * dd);
+            }
+        for (int c = 0; c < 3; ++c) {
+            e[56 + c] = q[s].diffuse_reflectance[c];
+            e[59 + c] = q[s].specular_reflectance[c];
+        }
+        e[62] = 1.0;
+        e[63] = 0.0;
+    }
+}
+
+static double luminance(const double* p) { return 0.2126729 * p[0] + 0.7151522 * p[1] + 0.0721750 * p[2]; }
+
+/* Forward + (optionally) backward for n samples.  Returns the summed per-sample loss
+ * sum_c (p_c - t_c)^2 / (lum(p)^2 + 0.01); grad (if not NULL) receives the summed per-sample gradient
+ * of that loss with respect to the weights (the library's unscaled backward). */
+static double run(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target, uint32_t n,
+                  double* out3, double* grad, int mode)
+{
+    uint32_t P = oracle_nrc_num_params(L);
+    double* w = (double*)malloc(sizeof(double) * P);
+    for (uint32_t i = 0; i < P; ++i) w[i] = rnd((double)weights[i], mode);
+    double* h = (double*)malloc(sizeof(double) * 64 * (L + 1));
+    double* dz = (double*)malloc(sizeof(double) * 64);
+    double* dh = (double*)malloc(sizeof(double) * 64);
+    double enc[64];
+    double total = 0.0;
+    if (grad) memset(grad, 0, sizeof(double) * P);
+    for (uint32_t s = 0; s < n; ++s) {
+        oracle_nrc_encode(&q[s], 1, enc);
+        for (int j = 0; j < 64; ++j) h[j] = rnd(enc[j], mode);
+        for (uint32_t l = 0; l < L; ++l) {
+            const double* wl = w + (size_t)l * LAYER;
+            const double* hin = h + 64 * l;
+            double* hout = h + 64 * (l + 1);
+            for (int o = 0; o < W; ++o) {
+                double z = 0.0;
+                for (int i = 0; i < W; ++i) z += wl[o * W + i] * hin[i];
+                hout[o] = rnd(z > 0.0 ? z : 0.0, mode);
+            }
+        }
+        double p[3];
+        const double* wo = w + (size_t)L * LAYER;
+        for (int o = 0; o < 3; ++o) {
+            double z = 0.0;
+            for (int i = 0; i < W; ++i) z += wo[o * W + i] * h[64 * L + i];
+            p[o] = z;
+        }
+        if (out3)
+            for (int o = 0; o < 3; ++o) out3[3 * (size_t)s + o] = p[o];
+        if (!target) continue;
+        double lum = luminance(p), denom = lum * lum + 0.01;
+        double g[3];
+        for (int o = 0; o < 3; ++o) {
+            double d = p[o] - (double)target[3 * (size_t)s + o];
+            total += d * d / denom;
+            g[o] = rnd(2.0 * d / denom, mode);
+        }
+        if (!grad) continue;
+        /* output layer */
+        for (int o = 0; o < 3; ++o)
+            for (int i = 0; i < W; ++i) grad[(size_t)L * LAYER + o * W + i] += g[o] * h[64 * L + i];
+        for (int i = 0; i < W; ++i) {
+            double acc = 0.0;
+            for (int o = 0; o < 3; ++o) acc += wo[o * W + i] * g[o];
+            dh[i] = acc;
+        }
+        for (int l = (int)L - 1; l >= 0; --l) {
+            const double* hout = h + 64 * (l + 1);
+            for (int o = 0; o < W; ++o) dz[o] = rnd(hout[o] > 0.0 ? dh[o] : 0.0, mode);
+            const double* hin = h + 64 * l;
+            for (int o = 0; o < W; ++o)
+                for (int i = 0; i < W; ++i) grad[(size_t)l * LAYER + o * W + i] += dz[o] * hin[i];
+            if (l == 0) break;
+            const double* wl = w + (size_t)l * LAYER;
+            for (int i = 0; i < W; ++i) {
+                double acc = 0.0;
+                for (int o = 0; o < W; ++o) acc += wl[o * W + i] * dz[o];
+                dh[i] = acc;
+            }
+        }
+    }
+    free(w);
+    free(h);
+    free(dz);
+    free(dh);
+    return total;
+}
+
+void oracle_nrc_forward(const float* weights, uint32_t L, const rtc8x_radiance_query* q, uint32_t n, double* out3,
+                        int round_mode)
+{
+    run(weights, L, q, NULL, n, out3, NULL, round_mode);
+}
+
+double oracle_nrc_loss(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target, uint32_t n,
+                       int round_mode)
+{
+    return run(weights, L, q, target, n, NULL, NULL, round_mode);
+}
+
+double oracle_nrc_gradient(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target,
+                           uint32_t n, double* grad, int round_mode)
+{
+    return run(weights, L, q, target, n, NULL, grad, round_mode);
+}
+
+/* One Adam + EMA step in FP32, in the library's expression order, from a summed gradient. */
+void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_sum, uint32_t P, uint32_t n,
+                     float lr, float beta1, float beta2, float eps, float l2, float decay, uint64_t step)
+{
+    float inv_scale = (float)(1.0 / (3.0 * (double)n));
+    float bc1 = (float)(1.0 - pow((double)beta1, (double)step));
+    float bc2 = (float)(1.0 - pow((double)beta2, (double)step));
+    for (uint32_t i = 0; i < P; ++i) {
+        float w0 = master[i];
+        float g = grad_sum[i] * inv_scale + l2 * w0;
+        float mi = beta1 * m[i] + (1.0f - beta1) * g;
+        float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        float w1 = w0 - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+        master[i] = w1;
+        ema[i] = decay * ema[i] + (1.0f - decay) * w1;
+    }
+}
+
+float oracle_round_fp16(float x) { return round_fp16(x); }
+float oracle_round_bf16(float x) { return round_bf16(x); }

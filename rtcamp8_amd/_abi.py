@@ -223,6 +223,22 @@ class HosekBand(C.Structure):
 
 SKY_BANDS = 16
 
+NRC_FP16, NRC_BF16 = 0, 1
+
+
+class NrcDesc(C.Structure):
+    _fields_ = [
+        ("num_hidden_layers", C.c_uint32),
+        ("learning_rate", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("epsilon", C.c_float),
+        ("l2_reg", C.c_float),
+        ("ema_decay", C.c_float),
+        ("precision", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
 
 class HosekSky(C.Structure):
     _fields_ = [
@@ -239,6 +255,7 @@ class HosekSky(C.Structure):
 
 
 STRUCTS = {
+    "rtc8x_nrc_desc": NrcDesc,
     "rtc8x_hosek_band": HosekBand,
     "rtc8x_hosek_sky": HosekSky,
     "rtc8x_root_tile": RootTile,
@@ -292,4 +309,18 @@ PROTOTYPES = {
     "rtc8x_seed_rng": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint64]),
     "rtc8x_sky_bake": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, _P]),
     "rtc8x_sky_eval": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, _P, _P, _P]),
+    "rtc8x_nrc_prepare": (_S, [_P, C.POINTER(FrameParams), C.c_int32]),
+    "rtc8x_nrc_propagate": (_S, [_P, C.POINTER(FrameParams)]),
+    "rtc8x_nrc_shuffle": (_S, [_P, C.POINTER(FrameParams)]),
+    "rtc8x_nrc_accumulate": (_S, [_P, C.POINTER(FrameParams)]),
+    "rtc8x_nrc_scratch_bytes": (C.c_size_t, [C.c_uint32, C.c_uint32]),
+    "rtc8x_nrc_create": (_S, [C.POINTER(NrcDesc), C.POINTER(C.c_void_p)]),
+    "rtc8x_nrc_destroy": (_S, [_P]),
+    "rtc8x_nrc_infer": (_S, [_P, _P, _P, C.c_uint32, _P]),
+    "rtc8x_nrc_train": (_S, [_P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
+    "rtc8x_nrc_num_params": (C.c_uint32, [_P]),
+    "rtc8x_nrc_get_weights": (_S, [_P, C.c_int32, _P, C.c_size_t]),
+    "rtc8x_nrc_set_weights": (_S, [_P, _P, C.c_size_t]),
+    "rtc8x_nrc_encode": (_S, [_P, _P, C.c_uint32, _P]),
+    "rtc8x_nrc_gradient": (_S, [_P, _P, _P, _P, C.c_uint32, _P]),
 }

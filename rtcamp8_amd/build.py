@@ -25,6 +25,8 @@ SOURCES = [
     "rtc8x_env.hip",
     "rtc8x_sky.hip",
     "rtc8x_trace.hip",
+    "rtc8x_nrc_data.hip",
+    "rtc8x_nrc_mlp.hip",
 ]
 
 HIPCC_FLAGS = [

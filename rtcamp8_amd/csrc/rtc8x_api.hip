@@ -230,6 +230,7 @@ extern "C" size_t rtc8x_sizeof(const char* type_name)
         {"rtc8x_training_suffix_terminal", sizeof(rtc8x_training_suffix_terminal)},
         {"rtc8x_nrc_buffers", sizeof(rtc8x_nrc_buffers)},
         {"rtc8x_frame_params", sizeof(rtc8x_frame_params)},
+        {"rtc8x_nrc_desc", sizeof(rtc8x_nrc_desc)},
         {"rtc8x_hosek_band", sizeof(rtc8x_hosek_band)},
         {"rtc8x_hosek_sky", sizeof(rtc8x_hosek_sky)},
     };

@@ -1,0 +1,600 @@
+// rtc8x_nrc_mlp.hip -- the neural radiance cache network on gfx950 MFMA (replaces tiny-cuda-nn's
+// NetworkWithInputEncoding + FullyFusedMLP + RelativeL2Luminance + EMA(Adam) behind
+// renderer/network_interface.cu).
+//
+// Forward (inference and training): a 256-lane workgroup = 4 wavefronts x 64 samples.  Each lane
+// encodes one query into 64 features and writes them to its wavefront's LDS tile; every layer is then
+// 64x64x64 per wavefront = 4 (samples) x 4 (outputs) x 2 (K) v_mfma_f32_16x16x32 {f16,bf16}
+// instructions with FP32 accumulation.  A operands (activations) come from LDS, B operands (weights,
+// [out][in] rows) straight from global memory, where the ~45 KB of weights stay L1/L2-resident for
+// the whole launch.  ReLU and the down-conversion are fused into the write-back to LDS.
+//
+// Backward: a 256-lane workgroup owns 256 samples.  Per layer, from the top: dW = dZ^T H (K = 256
+// samples, operands feature-major in LDS) and dH = dZ W (K = layer width, transposed weights from
+// global), with the ReLU mask applied on the way down.  Each workgroup writes its dW to its own slot;
+// the slots are summed in a fixed order, so a training step is deterministic.
+//
+// Optimiser: FP32 master weights, Adam moments and the weight EMA; one lane per weight.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rtc8x_common.hpp"
+#include "rtc8x_device.hpp"
+
+namespace rtc8x {
+namespace nrc {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Vec8;
+template <>
+struct Vec8<_Float16> {
+    using type = h8;
+};
+template <>
+struct Vec8<__bf16> {
+    using type = b8;
+};
+
+__device__ __forceinline__ f4 mfma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+constexpr uint32_t kWidth = 64;
+constexpr uint32_t kLayer = kWidth * kWidth;             // 4096 weights per hidden layer
+constexpr uint32_t kOutPad = RTC8X_NRC_OUTPUT_PAD;       // output rows padded to one 16-row tile
+constexpr uint32_t kOutT = 32;                           // transposed output layer: [64 in][32]
+constexpr uint32_t kActStride = 72;                      // LDS row stride of a 64-feature row (halves)
+constexpr uint32_t kColStride = 264;                     // LDS row stride of a 256-sample column
+constexpr uint32_t kMaxTrain = 65536;
+constexpr uint32_t kMaxLayers = 8;
+
+__host__ __device__ constexpr uint32_t num_params(uint32_t L) { return L * kLayer + kOutPad * kWidth; }
+__host__ __device__ constexpr uint32_t num_params_t(uint32_t L) { return L * kLayer + kWidth * kOutT; }
+
+// Input encoding of one RadianceQuery (renderer/renderer_shared.h:46-55), 64 features.
+__device__ __forceinline__ void encode(const rtc8x_radiance_query& q, float* e)
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float sc = 1.0f;
+#pragma unroll
+        for (int f = 0; f < 12; ++f) {
+            float t = q.position[d] * sc;
+            t = t - floorf(t);
+            e[12 * d + f] = fabsf(2.0f * t - 1.0f);
+            sc = sc * 2.0f;
+        }
+    }
+    const float inv_pi = 1.0f / RTC8X_PI_F, inv_2pi = 1.0f / (2.0f * RTC8X_PI_F);
+    // The *_phi fields hold polar angles and the *_theta fields azimuths (see rtc8x_radiance_query).
+    const float a[5] = {q.normal_phi * inv_pi, q.normal_theta * inv_2pi, q.vout_phi * inv_pi, q.vout_theta * inv_2pi,
+                        q.roughness};
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float d = a[j] - ((float)b + 0.5f) * 0.25f;
+            e[36 + 4 * j + b] = expf(-8.0f * d * d);  // Gaussian one-blob kernel, sigma = 1/4 bins
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        e[56 + c] = q.diffuse_reflectance[c];
+        e[59 + c] = q.specular_reflectance[c];
+    }
+    e[62] = 1.0f;
+    e[63] = 0.0f;
+}
+
+template <typename T, bool TRAIN>
+__global__ __launch_bounds__(256) void forward_kernel(const T* __restrict__ w, uint32_t L,
+                                                      const rtc8x_radiance_query* __restrict__ in, uint32_t n,
+                                                      float* __restrict__ out, const float* __restrict__ target,
+                                                      T* __restrict__ hbuf, T* __restrict__ dout,
+                                                      float* __restrict__ loss_part)
+{
+    using V = typename Vec8<T>::type;
+    __shared__ __align__(16) T act[4][64 * kActStride];
+    __shared__ float pred[4][64][3];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t s0 = (blockIdx.x * 4 + wave) * 64;
+    const uint32_t s = s0 + lane;
+    T* A = act[wave];
+    {
+        float e[64];
+        if (s < n) {
+            encode(in[s], e);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 64; ++j) e[j] = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 64; j += 8) {
+            V v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (T)e[j + k];
+            *reinterpret_cast<V*>(&A[lane * kActStride + j]) = v;
+            if (TRAIN && s < n) *reinterpret_cast<V*>(&hbuf[(size_t)s * 64 + j]) = v;
+        }
+    }
+    const uint32_t r16 = lane & 15, kq = 8 * (lane >> 4);
+    for (uint32_t l = 0; l <= L; ++l) {
+        const bool last = l == L;
+        const T* W = w + (size_t)l * kLayer;
+        const int ntiles = last ? 1 : 4;
+        f4 acc[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            V a[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                a[mt] = *reinterpret_cast<const V*>(&A[(mt * 16 + r16) * kActStride + ks * 32 + kq]);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if (nt < ntiles) {
+                    const V b = *reinterpret_cast<const V*>(&W[(nt * 16 + r16) * kWidth + ks * 32 + kq]);
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(a[mt], b, acc[mt][nt]);
+                }
+            }
+        }
+        if (!last) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        A[(mt * 16 + 4 * (lane >> 4) + i) * kActStride + nt * 16 + r16] = (T)fmaxf(acc[mt][nt][i], 0.0f);
+            if (TRAIN && s < n) {
+                T* dst = hbuf + ((size_t)(l + 1) * n + s) * 64;
+#pragma unroll
+                for (int j = 0; j < 64; j += 8)
+                    *reinterpret_cast<V*>(&dst[j]) = *reinterpret_cast<const V*>(&A[lane * kActStride + j]);
+            }
+        } else {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (r16 < 3) pred[wave][mt * 16 + 4 * (lane >> 4) + i][r16] = acc[mt][0][i];
+        }
+    }
+    if (!TRAIN) {
+        if (s < n) {
+            out[3 * (size_t)s] = pred[wave][lane][0];
+            out[3 * (size_t)s + 1] = pred[wave][lane][1];
+            out[3 * (size_t)s + 2] = pred[wave][lane][2];
+        }
+        return;
+    }
+    // relative L2 over luminance; per-sample gradient 2 (p - t) / (lum(p)^2 + 0.01)
+    float loss = 0.0f;
+    if (s < n) {
+        const float p0 = pred[wave][lane][0], p1 = pred[wave][lane][1], p2 = pred[wave][lane][2];
+        const float lum = (float)0.2126729 * p0 + (float)0.7151522 * p1 + (float)0.0721750 * p2;
+        const float denom = lum * lum + 0.01f;
+        const float d0 = p0 - target[3 * (size_t)s], d1 = p1 - target[3 * (size_t)s + 1], d2 = p2 - target[3 * (size_t)s + 2];
+        loss = (d0 * d0 + d1 * d1 + d2 * d2) / denom;
+        T* g = dout + (size_t)s * kOutPad;
+        g[0] = (T)(2.0f * d0 / denom);
+        g[1] = (T)(2.0f * d1 / denom);
+        g[2] = (T)(2.0f * d2 / denom);
+#pragma unroll
+        for (int c = 3; c < (int)kOutPad; ++c) g[c] = (T)0.0f;
+    }
+    for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o, 64);
+    if (lane == 0) loss_part[blockIdx.x * 4 + wave] = loss;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt, uint32_t L, uint32_t n,
+                                                       const T* __restrict__ hbuf, const T* __restrict__ dout,
+                                                       float* __restrict__ gpart)
+{
+    using V = typename Vec8<T>::type;
+    __shared__ __align__(16) T dzs[256 * kActStride];   // dZ, sample-major [sample][out]
+    __shared__ __align__(16) T dzf[64 * kColStride];    // dZ, feature-major [out][sample]
+    __shared__ __align__(16) T hf[64 * kColStride];     // layer input H, feature-major [in][sample]
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t r16 = lane & 15, kq = 8 * (lane >> 4);
+    const uint32_t b0 = blockIdx.x * 256;
+    const uint32_t P = num_params(L);
+    float* gblk = gpart + (size_t)blockIdx.x * P;
+    for (uint32_t idx = tid; idx < 256 * 64; idx += 256) {
+        const uint32_t s = idx >> 6, c = idx & 63;
+        const T v = (c < kOutPad && b0 + s < n) ? dout[(size_t)(b0 + s) * kOutPad + c] : (T)0.0f;
+        dzs[s * kActStride + c] = v;
+        dzf[c * kColStride + s] = v;
+    }
+    __syncthreads();
+    for (int l = (int)L; l >= 0; --l) {
+        const bool top = (uint32_t)l == L;
+        for (uint32_t idx = tid; idx < 256 * 64; idx += 256) {
+            const uint32_t s = idx >> 6, c = idx & 63;
+            hf[c * kColStride + s] = (b0 + s < n) ? hbuf[((size_t)l * n + b0 + s) * 64 + c] : (T)0.0f;
+        }
+        __syncthreads();
+        // dW_l[out][in] = sum over the block's samples of dZ[s][out] H[s][in]
+        const int otiles = top ? 1 : 4;
+        const uint32_t goff = (uint32_t)l * kLayer;
+        for (int t = (int)wave; t < otiles * 4; t += 4) {
+            const int mt = t >> 2, nt = t & 3;
+            f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks) {
+                const V a = *reinterpret_cast<const V*>(&dzf[(mt * 16 + r16) * kColStride + ks * 32 + kq]);
+                const V b = *reinterpret_cast<const V*>(&hf[(nt * 16 + r16) * kColStride + ks * 32 + kq]);
+                acc = mfma(a, b, acc);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gblk[goff + (mt * 16 + 4 * (lane >> 4) + i) * kWidth + nt * 16 + r16] = acc[i];
+        }
+        if (l == 0) break;
+        // dH_l = dZ_l W_l for this wavefront's 64 samples (K = 16, padded to one 32-step, for the top layer)
+        const T* WT = wt + (size_t)l * kLayer;
+        const uint32_t ldk = top ? kOutT : kWidth;
+        const int ksteps = top ? 1 : 2;
+        f4 acc[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < ksteps; ++ks) {
+            V a[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                a[mt] = *reinterpret_cast<const V*>(&dzs[(wave * 64 + mt * 16 + r16) * kActStride + ks * 32 + kq]);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const V b = *reinterpret_cast<const V*>(&WT[(nt * 16 + r16) * ldk + ks * 32 + kq]);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(a[mt], b, acc[mt][nt]);
+            }
+        }
+        __syncthreads();
+        // ReLU mask: H_l is the post-ReLU output of layer l-1, so the derivative is [H_l > 0]
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t s = wave * 64 + mt * 16 + 4 * (lane >> 4) + i, c = nt * 16 + r16;
+                    const float hv = (float)hf[c * kColStride + s];
+                    const T v = (T)(hv > 0.0f ? acc[mt][nt][i] : 0.0f);
+                    dzs[s * kActStride + c] = v;
+                    dzf[c * kColStride + s] = v;
+                }
+        __syncthreads();
+    }
+}
+
+// Sum of the per-workgroup weight gradients in workgroup order; flags non-finite sums.
+__global__ void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P,
+                                   float* __restrict__ grad, uint32_t* __restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float g = 0.0f;
+    for (uint32_t b = 0; b < nblocks; ++b) g += gpart[(size_t)b * P + i];
+    grad[i] = g;
+    if (!isfinite(g)) atomicOr(bad, 1u);
+}
+
+struct AdamArgs {
+    float lr, beta1, beta2, eps, l2, decay;
+    float inv_scale;      // 1 / (3 n): the loss scale of the backward pass
+    float bc1, bc2;       // 1 - beta^t
+    float ema_bc;         // 1 - decay^t
+    uint32_t P, L;
+};
+
+template <typename T>
+__global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
+                            float* __restrict__ ema, const float* __restrict__ grad, uint32_t* __restrict__ bad,
+                            T* __restrict__ w, T* __restrict__ wt, T* __restrict__ winf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.P) return;
+    if (*bad) return;  // a non-finite gradient skips the whole step
+    const float w0 = master[i];
+    const float g = grad[i] * a.inv_scale + a.l2 * w0;
+    const float mi = a.beta1 * m[i] + (1.0f - a.beta1) * g;
+    const float vi = a.beta2 * v[i] + (1.0f - a.beta2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float w1 = w0 - a.lr * (mi / a.bc1) / (sqrtf(vi / a.bc2) + a.eps);
+    master[i] = w1;
+    const float e = a.decay * ema[i] + (1.0f - a.decay) * w1;
+    ema[i] = e;
+    w[i] = (T)w1;
+    winf[i] = (T)(e / a.ema_bc);
+    const uint32_t l = i / kLayer, r = i % kLayer, out = r / kWidth, in = r % kWidth;
+    if (l < a.L)
+        wt[(size_t)l * kLayer + in * kWidth + out] = (T)w1;
+    else
+        wt[(size_t)a.L * kLayer + in * kOutT + out] = (T)w1;
+}
+
+template <typename T>
+__global__ void convert_kernel(const float* __restrict__ src, uint32_t P, uint32_t L, T* __restrict__ w, T* __restrict__ wt,
+                               T* __restrict__ winf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const T x = (T)src[i];
+    w[i] = x;
+    winf[i] = x;
+    const uint32_t l = i / kLayer, r = i % kLayer, out = r / kWidth, in = r % kWidth;
+    if (l < L)
+        wt[(size_t)l * kLayer + in * kWidth + out] = x;
+    else
+        wt[(size_t)L * kLayer + in * kOutT + out] = x;
+}
+
+__global__ void encode_kernel(const rtc8x_radiance_query* __restrict__ in, uint32_t n, float* __restrict__ out)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float e[64];
+    encode(in[s], e);
+    for (int j = 0; j < 64; ++j) out[(size_t)s * 64 + j] = e[j];
+}
+
+}  // namespace nrc
+}  // namespace rtc8x
+
+using namespace rtc8x;
+
+struct rtc8x_nrc {
+    rtc8x_nrc_desc desc;
+    uint32_t L = 0, P = 0, PT = 0;
+    size_t tsize = 2;
+    uint64_t step = 0;
+    float *master = nullptr, *m = nullptr, *v = nullptr, *ema = nullptr, *grad = nullptr, *gpart = nullptr;
+    float* loss_part = nullptr;
+    uint32_t* bad = nullptr;
+    void *w = nullptr, *wt = nullptr, *winf = nullptr, *hbuf = nullptr, *dout = nullptr;
+};
+
+namespace {
+
+template <typename T>
+hipError_t launch_forward(rtc8x_nrc* h, bool train, const void* w, hipStream_t s, const float* in, uint32_t n, float* out,
+                          const float* target)
+{
+    const uint32_t blocks = (n + 255) / 256;
+    if (train)
+        hipLaunchKernelGGL((nrc::forward_kernel<T, true>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
+                           (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part);
+    else
+        hipLaunchKernelGGL((nrc::forward_kernel<T, false>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
+                           (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr, (float*)nullptr);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_backward(rtc8x_nrc* h, hipStream_t s, uint32_t n)
+{
+    const uint32_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL((nrc::backward_kernel<T>), dim3(blocks), dim3(256), 0, s, (const T*)h->wt, h->L, n,
+                       (const T*)h->hbuf, (const T*)h->dout, h->gpart);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_adam(rtc8x_nrc* h, hipStream_t s, const nrc::AdamArgs& a)
+{
+    hipLaunchKernelGGL((nrc::adam_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, a, h->master, h->m, h->v, h->ema,
+                       h->grad, h->bad, (T*)h->w, (T*)h->wt, (T*)h->winf);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_convert(rtc8x_nrc* h, hipStream_t s)
+{
+    hipLaunchKernelGGL((nrc::convert_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, h->master, h->P, h->L, (T*)h->w,
+                       (T*)h->wt, (T*)h->winf);
+    return hipGetLastError();
+}
+
+bool is_bf16(const rtc8x_nrc* h) { return h->desc.precision == RTC8X_NRC_BF16; }
+
+rtc8x_status reset_from_master(rtc8x_nrc* h)
+{
+    RTC8X_HIP(hipMemset(h->m, 0, sizeof(float) * h->P));
+    RTC8X_HIP(hipMemset(h->v, 0, sizeof(float) * h->P));
+    RTC8X_HIP(hipMemset(h->ema, 0, sizeof(float) * h->P));
+    RTC8X_HIP(hipMemset(h->wt, 0, h->tsize * h->PT));
+    RTC8X_HIP(is_bf16(h) ? launch_convert<__bf16>(h, nullptr) : launch_convert<_Float16>(h, nullptr));
+    RTC8X_HIP(hipDeviceSynchronize());
+    h->step = 0;
+    return RTC8X_OK;
+}
+
+// forward (train) + backward + ordered reduction into h->grad; returns the loss sum in *loss_sum when set
+rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const float* target, uint32_t n)
+{
+    const bool bf = is_bf16(h);
+    RTC8X_HIP(bf ? launch_forward<__bf16>(h, true, h->w, s, in, n, nullptr, target)
+                 : launch_forward<_Float16>(h, true, h->w, s, in, n, nullptr, target));
+    RTC8X_HIP(bf ? launch_backward<__bf16>(h, s, n) : launch_backward<_Float16>(h, s, n));
+    RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->gpart, (n + 255) / 256, h->P,
+                       h->grad, h->bad);
+    RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}
+
+}  // namespace
+
+extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
+{
+    if (!h) return RTC8X_OK;
+    void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout};
+    for (void* p : ptrs)
+        if (p) RTC8X_HIP(hipFree(p));
+    delete h;
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** out)
+{
+    RTC8X_REQUIRE(d && out, "rtc8x_nrc_create: NULL argument");
+    RTC8X_REQUIRE(d->num_hidden_layers >= 1 && d->num_hidden_layers <= nrc::kMaxLayers,
+                  "rtc8x_nrc_create: num_hidden_layers must be in [1, %u]", nrc::kMaxLayers);
+    RTC8X_REQUIRE(d->precision == RTC8X_NRC_FP16 || d->precision == RTC8X_NRC_BF16, "rtc8x_nrc_create: bad precision");
+    RTC8X_REQUIRE(d->learning_rate > 0.0f && d->beta1 >= 0.0f && d->beta1 < 1.0f && d->beta2 >= 0.0f && d->beta2 < 1.0f &&
+                      d->epsilon > 0.0f && d->ema_decay >= 0.0f && d->ema_decay < 1.0f && d->l2_reg >= 0.0f,
+                  "rtc8x_nrc_create: bad optimiser parameters");
+    *out = nullptr;
+    rtc8x_nrc* h = new rtc8x_nrc();
+    h->desc = *d;
+    h->L = d->num_hidden_layers;
+    h->P = nrc::num_params(h->L);
+    h->PT = nrc::num_params_t(h->L);
+    const size_t P = h->P;
+    const size_t max_blocks = nrc::kMaxTrain / 256;
+    auto alloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
+    bool ok = alloc((void**)&h->master, 4 * P) && alloc((void**)&h->m, 4 * P) && alloc((void**)&h->v, 4 * P) &&
+              alloc((void**)&h->ema, 4 * P) && alloc((void**)&h->grad, 4 * P) && alloc((void**)&h->gpart, 4 * P * max_blocks) &&
+              alloc((void**)&h->loss_part, 4 * 4 * max_blocks) && alloc((void**)&h->bad, 4) && alloc(&h->w, h->tsize * P) &&
+              alloc(&h->wt, h->tsize * h->PT) && alloc(&h->winf, h->tsize * P) &&
+              alloc(&h->hbuf, h->tsize * 64 * (size_t)(h->L + 1) * nrc::kMaxTrain) &&
+              alloc(&h->dout, h->tsize * nrc::kOutPad * (size_t)nrc::kMaxTrain);
+    if (!ok) {
+        rtc8x_nrc_destroy(h);
+        return rtc8x::fail(RTC8X_ERR_OUT_OF_MEMORY, "rtc8x_nrc_create: out of device memory");
+    }
+    // Xavier-uniform initialisation from a PCG32 stream (tiny-cuda-nn's initialiser is not available).
+    std::vector<float> init(P, 0.0f);
+    uint64_t state = d->seed;
+    auto uni = [&]() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + 1442695040888963407ULL;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u), rot = (uint32_t)(old >> 59u);
+        uint32_t r = (xs >> rot) | (xs << ((0u - rot) & 31u));
+        return (double)(r >> 8) * (1.0 / 16777216.0);
+    };
+    for (uint32_t l = 0; l <= h->L; ++l) {
+        const bool last = l == h->L;
+        const uint32_t rows = last ? RTC8X_NRC_OUTPUT_DIMS : nrc::kWidth;
+        const double lim = std::sqrt(6.0 / (64.0 + (double)rows));
+        for (uint32_t r = 0; r < rows; ++r)
+            for (uint32_t c = 0; c < nrc::kWidth; ++c) init[(size_t)l * nrc::kLayer + r * nrc::kWidth + c] = (float)((2.0 * uni() - 1.0) * lim);
+    }
+    hipError_t e = hipMemcpy(h->master, init.data(), 4 * P, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rtc8x_nrc_destroy(h);
+        return rtc8x::fail(RTC8X_ERR_HIP, "rtc8x_nrc_create: %s", hipGetErrorString(e));
+    }
+    rtc8x_status st = reset_from_master(h);
+    if (st != RTC8X_OK) {
+        rtc8x_nrc_destroy(h);
+        return st;
+    }
+    *out = h;
+    return RTC8X_OK;
+}
+
+extern "C" uint32_t rtc8x_nrc_num_params(const rtc8x_nrc* h) { return h ? h->P : 0; }
+
+extern "C" rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* h, int32_t which, float* host, size_t count)
+{
+    RTC8X_REQUIRE(h && host && count == h->P, "rtc8x_nrc_get_weights: bad arguments");
+    if (which == 0 || h->step == 0) {
+        RTC8X_HIP(hipMemcpy(host, h->master, 4 * h->P, hipMemcpyDeviceToHost));
+        return RTC8X_OK;
+    }
+    RTC8X_HIP(hipMemcpy(host, h->ema, 4 * h->P, hipMemcpyDeviceToHost));
+    const double bc = 1.0 - std::pow((double)h->desc.ema_decay, (double)h->step);
+    const float fbc = (float)bc;
+    for (size_t i = 0; i < count; ++i) host[i] = host[i] / fbc;
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_nrc_set_weights(rtc8x_nrc* h, const float* host, size_t count)
+{
+    RTC8X_REQUIRE(h && host && count == h->P, "rtc8x_nrc_set_weights: bad arguments");
+    RTC8X_HIP(hipMemcpy(h->master, host, 4 * h->P, hipMemcpyHostToDevice));
+    return reset_from_master(h);
+}
+
+extern "C" rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* h, void* stream, const float* in, uint32_t n, float* out)
+{
+    RTC8X_REQUIRE(h && in && out, "rtc8x_nrc_infer: NULL argument");
+    if (n == 0) return RTC8X_OK;
+    hipStream_t s = (hipStream_t)stream;
+    RTC8X_HIP(is_bf16(h) ? launch_forward<__bf16>(h, false, h->winf, s, in, n, out, nullptr)
+                         : launch_forward<_Float16>(h, false, h->winf, s, in, n, out, nullptr));
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* h, void* stream, const float* in, const float* target, uint32_t n,
+                                           float* grad)
+{
+    RTC8X_REQUIRE(h && in && target && grad, "rtc8x_nrc_gradient: NULL argument");
+    RTC8X_REQUIRE(n > 0 && n <= nrc::kMaxTrain, "rtc8x_nrc_gradient: n must be in [1, %u]", nrc::kMaxTrain);
+    hipStream_t s = (hipStream_t)stream;
+    rtc8x_status st = gradient_pass(h, s, in, target, n);
+    if (st != RTC8X_OK) return st;
+    RTC8X_HIP(hipMemcpyAsync(grad, h->grad, 4 * h->P, hipMemcpyDeviceToDevice, s));
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_nrc_train(rtc8x_nrc* h, void* stream, const float* in, const float* target, uint32_t n,
+                                        float* loss_on_cpu)
+{
+    RTC8X_REQUIRE(h && in && target, "rtc8x_nrc_train: NULL argument");
+    RTC8X_REQUIRE(n > 0 && n <= nrc::kMaxTrain, "rtc8x_nrc_train: n must be in [1, %u]", nrc::kMaxTrain);
+    hipStream_t s = (hipStream_t)stream;
+    rtc8x_status st = gradient_pass(h, s, in, target, n);
+    if (st != RTC8X_OK) return st;
+    h->step += 1;
+    const rtc8x_nrc_desc& d = h->desc;
+    nrc::AdamArgs a;
+    a.lr = d.learning_rate;
+    a.beta1 = d.beta1;
+    a.beta2 = d.beta2;
+    a.eps = d.epsilon;
+    a.l2 = d.l2_reg;
+    a.decay = d.ema_decay;
+    a.inv_scale = (float)(1.0 / (3.0 * (double)n));
+    a.bc1 = (float)(1.0 - std::pow((double)d.beta1, (double)h->step));
+    a.bc2 = (float)(1.0 - std::pow((double)d.beta2, (double)h->step));
+    a.ema_bc = (float)(1.0 - std::pow((double)d.ema_decay, (double)h->step));
+    a.P = h->P;
+    a.L = h->L;
+    RTC8X_HIP(is_bf16(h) ? launch_adam<__bf16>(h, s, a) : launch_adam<_Float16>(h, s, a));
+    if (loss_on_cpu) {
+        const uint32_t nw = 4 * ((n + 255) / 256);
+        std::vector<float> parts(nw);
+        RTC8X_HIP(hipMemcpyAsync(parts.data(), h->loss_part, 4 * nw, hipMemcpyDeviceToHost, s));
+        RTC8X_HIP(hipStreamSynchronize(s));
+        double sum = 0.0;
+        for (float p : parts) sum += p;
+        *loss_on_cpu = (float)(sum / (3.0 * (double)n));
+    }
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t n, float* out)
+{
+    RTC8X_REQUIRE(in && out, "rtc8x_nrc_encode: NULL argument");
+    if (n == 0) return RTC8X_OK;
+    hipLaunchKernelGGL(nrc::encode_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       (const rtc8x_radiance_query*)in, n, out);
+    RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}

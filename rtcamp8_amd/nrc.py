@@ -1,0 +1,259 @@
+"""Neural radiance cache: the host-side mirror of the reference's NeuralRadianceCache class
+(renderer/network_interface.h:17-31) and of the NRC branch of its frame loop
+(renderer/renderer_main.cpp:1554-1638), driving the gfx950 library through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._lib import call, lib
+from .device import DeviceBuffer
+from .scene import DeviceScene
+
+TRAIN_BATCH = _abi.NUM_TRAINING_DATA_PER_FRAME // 4  # renderer_main.cpp:1595: 4 steps of 16384 per iteration
+
+
+class Mt19937:
+    """std::mt19937 (32-bit Mersenne Twister), used by the reference as perFrameRng (renderer_main.cpp:1500)."""
+
+    def __init__(self, seed: int):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+        self.idx = 624
+
+    def __call__(self) -> int:
+        if self.idx >= 624:
+            for i in range(624):
+                y = (self.mt[i] & 0x80000000) | (self.mt[(i + 1) % 624] & 0x7FFFFFFF)
+                v = self.mt[(i + 397) % 624] ^ (y >> 1)
+                if y & 1:
+                    v ^= 0x9908B0DF
+                self.mt[i] = v
+            self.idx = 0
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y & 0xFFFFFFFF
+
+
+class NeuralRadianceCache:
+    """initialize / finalize / infer / train, as network_interface.h declares them."""
+
+    def __init__(self):
+        self.handle = None
+
+    def initialize(self, num_hidden_layers: int = 2, learning_rate: float = 1e-2, precision: int = _abi.NRC_FP16,
+                   seed: int = 0x9E3779B97F4A7C15, ema_decay: float = 0.99) -> None:
+        d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, 1e-8, 1e-6, ema_decay, precision, seed)
+        h = C.c_void_p()
+        call("rtc8x_nrc_create", C.byref(d), C.byref(h))
+        self.handle = h.value
+        self.num_hidden_layers = num_hidden_layers
+
+    def finalize(self) -> None:
+        if self.handle:
+            call("rtc8x_nrc_destroy", C.c_void_p(self.handle))
+            self.handle = None
+
+    def infer(self, stream, input_ptr: int, num: int, output_ptr: int) -> None:
+        call("rtc8x_nrc_infer", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr), int(num),
+             C.c_void_p(output_ptr))
+
+    def train(self, stream, input_ptr: int, target_ptr: int, num: int, want_loss: bool = False):
+        loss = C.c_float(0.0)
+        call("rtc8x_nrc_train", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr),
+             C.c_void_p(target_ptr), int(num), C.byref(loss) if want_loss else None)
+        return float(loss.value) if want_loss else None
+
+    def num_params(self) -> int:
+        return int(lib().rtc8x_nrc_num_params(C.c_void_p(self.handle)))
+
+    def get_weights(self, which: int = 0) -> np.ndarray:
+        w = np.empty(self.num_params(), dtype=np.float32)
+        call("rtc8x_nrc_get_weights", C.c_void_p(self.handle), which, w.ctypes.data_as(C.c_void_p), w.size)
+        return w
+
+    def set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        call("rtc8x_nrc_set_weights", C.c_void_p(self.handle), w.ctypes.data_as(C.c_void_p), w.size)
+
+    def gradient(self, stream, input_ptr: int, target_ptr: int, num: int) -> np.ndarray:
+        g = DeviceBuffer(4 * self.num_params())
+        call("rtc8x_nrc_gradient", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr),
+             C.c_void_p(target_ptr), int(num), C.c_void_p(g.ptr))
+        out = g.download(np.float32, self.num_params())
+        g.free()
+        return out
+
+
+def round_up(x: int, m: int) -> int:
+    return (x + m - 1) // m * m
+
+
+class NrcBuffers:
+    """Screen-size and training buffers of the NRC path (renderer_main.cpp:80-129, 197-230)."""
+
+    def __init__(self, width: int, height: int, shuffler_seed: int = 471313181):
+        n = width * height
+        nb_inf = round_up(n, 128)
+        self.width, self.height = width, height
+        self.num_training_data = DeviceBuffer.zeros(8)
+        self.train_image_size = DeviceBuffer.zeros(16)
+        self.target_min_max = DeviceBuffer.zeros(4 * 12)
+        self.target_avg = DeviceBuffer.zeros(4 * 6)
+        self.inference_queries = DeviceBuffer.zeros(56 * nb_inf)
+        self.inference_terminals = DeviceBuffer.zeros(16 * n)
+        self.train_suffix_terminals = DeviceBuffer.zeros(16 * n)
+        self.inferred_radiance = DeviceBuffer.zeros(12 * nb_inf)
+        self.per_frame_contribution = DeviceBuffer.zeros(12 * n)
+        self.train_queries = [DeviceBuffer.zeros(56 * _abi.TRAIN_BUFFER_SIZE) for _ in range(2)]
+        self.train_targets = [DeviceBuffer.zeros(12 * _abi.TRAIN_BUFFER_SIZE) for _ in range(2)]
+        self.train_vertex_info = DeviceBuffer.zeros(24 * _abi.TRAIN_BUFFER_SIZE)
+        # LCG states: the i-th slot holds the state after i + 1 steps from the seed (renderer_main.cpp:220-229)
+        s = shuffler_seed
+        states = np.empty(_abi.NUM_TRAINING_DATA_PER_FRAME, dtype=np.uint32)
+        for i in range(states.size):
+            s = ((s * 1103515245 + 12345) & 0xFFFFFFFF) % (1 << 31)
+            states[i] = s
+        self.data_shufflers = DeviceBuffer.from_array(states)
+        self.scratch = DeviceBuffer.zeros(lib().rtc8x_nrc_scratch_bytes(width, height))
+
+    def fill(self, nrc: _abi.NrcBuffers) -> None:
+        nrc.max_num_training_suffixes = self.width * self.height
+        nrc.num_training_data = self.num_training_data.ptr
+        nrc.train_image_size = self.train_image_size.ptr
+        nrc.target_min_max = self.target_min_max.ptr
+        nrc.target_avg = self.target_avg.ptr
+        nrc.inference_queries = self.inference_queries.ptr
+        nrc.inference_terminals = self.inference_terminals.ptr
+        nrc.train_suffix_terminals = self.train_suffix_terminals.ptr
+        nrc.inferred_radiance = self.inferred_radiance.ptr
+        nrc.per_frame_contribution = self.per_frame_contribution.ptr
+        nrc.train_queries[0], nrc.train_queries[1] = self.train_queries[0].ptr, self.train_queries[1].ptr
+        nrc.train_targets[0], nrc.train_targets[1] = self.train_targets[0].ptr, self.train_targets[1].ptr
+        nrc.train_vertex_info = self.train_vertex_info.ptr
+        nrc.data_shufflers = self.data_shufflers.ptr
+        nrc.scratch = self.scratch.ptr
+
+    def free(self) -> None:
+        for v in list(self.__dict__.values()):
+            if isinstance(v, DeviceBuffer):
+                v.free()
+            elif isinstance(v, list):
+                for b in v:
+                    if isinstance(b, DeviceBuffer):
+                        b.free()
+
+
+@dataclass
+class NrcStats:
+    train_iterations: int = 0
+    train_steps: int = 0
+    inference_queries: int = 0
+    last_loss: float | None = None
+
+
+class NrcRenderer:
+    """Frame loop of the NRC-accelerated tracer (renderer_main.cpp:1554-1638): per frame, training
+    iterations (8 on the first frame, then 1) of prepare -> trace training paths -> infer terminal
+    queries -> propagate -> shuffle -> 4 x train(16384); then per sample trace with NRC -> infer ->
+    accumulate.  The plain tracer (rtcamp8_amd.tracer) stays the unbiased reference."""
+
+    def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
+                 precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121):
+        cfg = scene.cfg
+        self.scene = scene
+        self.stream = stream
+        self.width, self.height = cfg.width, cfg.height
+        n = cfg.width * cfg.height
+        self.rng = DeviceBuffer(8 * n)
+        self.accum = DeviceBuffer(24 * n)
+        self.stats_buf = DeviceBuffer.zeros(8 * _abi.STAT_COUNT)
+        self.buffers = NrcBuffers(cfg.width, cfg.height)
+        self.cache = NeuralRadianceCache()
+        self.cache.initialize(num_hidden_layers, learning_rate, precision)
+        self.frame_rng = Mt19937(frame_rng_seed)
+        self.buf_idx = 1  # the first iteration toggles to 0 (renderer_main.cpp:1558-1559)
+        self.radiance_scale = radiance_scale
+        self.stats = NrcStats()
+        self.samples = 0
+        self.frame = 0
+        call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(cfg.rng_seed))
+        self.accum.fill(0)
+
+    def params(self) -> _abi.FrameParams:
+        p = self.scene.params()
+        p.rng_states = self.rng.ptr
+        p.accum = self.accum.ptr
+        p.stats = self.stats_buf.ptr
+        self.buffers.fill(p.nrc)
+        p.nrc.nrc_buffer_index = self.buf_idx
+        p.nrc.radiance_scale = self.radiance_scale
+        p.nrc.path_termination_factor = 0.01
+        return p
+
+    def train_iteration(self, is_new_sequence: bool, want_loss: bool = False):
+        s = C.c_void_p(self.stream)
+        self.buf_idx = (self.buf_idx + 1) % 2
+        p = self.params()
+        p.nrc.offset_to_select_unbiased_path = self.frame_rng()
+        call("rtc8x_nrc_prepare", s, C.byref(p), 1 if is_new_sequence else 0)
+        call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_GENERATE_TRAINING_DATA, 1)
+        ts = self.buffers.train_image_size.download(np.int32, 4)
+        tw, th = int(ts[2 * self.buf_idx]), int(ts[2 * self.buf_idx + 1])
+        self.cache.infer(self.stream, self.buffers.inference_queries.ptr, round_up(tw * th, 128),
+                         self.buffers.inferred_radiance.ptr)
+        call("rtc8x_nrc_propagate", s, C.byref(p))
+        call("rtc8x_nrc_shuffle", s, C.byref(p))
+        loss = None
+        for step in range(4):
+            off = step * TRAIN_BATCH
+            loss = self.cache.train(self.stream, self.buffers.train_queries[1].ptr + 56 * off,
+                                    self.buffers.train_targets[1].ptr + 12 * off, TRAIN_BATCH,
+                                    want_loss=want_loss and step == 3)
+        self.stats.train_iterations += 1
+        self.stats.train_steps += 4
+        self.stats.last_loss = loss
+        return loss
+
+    def render_sample(self) -> None:
+        s = C.c_void_p(self.stream)
+        p = self.params()
+        call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
+        nq = round_up(self.width * self.height, 128)
+        self.cache.infer(self.stream, self.buffers.inference_queries.ptr, nq, self.buffers.inferred_radiance.ptr)
+        call("rtc8x_nrc_accumulate", s, C.byref(p))
+        self.stats.inference_queries += nq
+        self.samples += 1
+
+    def render_frame(self, spp: int, train_iterations: int | None = None) -> None:
+        iters = train_iterations if train_iterations is not None else (8 if self.frame == 0 else 1)
+        for _ in range(iters):
+            self.train_iteration(self.frame == 0)
+        for _ in range(spp):
+            self.render_sample()
+        self.frame += 1
+
+    def image(self) -> np.ndarray:
+        a = self.accum.download(np.float64, 3 * self.width * self.height).reshape(self.height, self.width, 3)
+        return a / max(self.samples, 1)
+
+    def counters(self) -> dict:
+        s = self.stats_buf.download(np.uint64, _abi.STAT_COUNT)
+        return {"paths": int(s[_abi.STAT_PATHS]), "vertices": int(s[_abi.STAT_VERTICES]),
+                "lookups": int(s[_abi.STAT_LOOKUPS])}
+
+    def close(self) -> None:
+        self.cache.finalize()
+        self.buffers.free()
+        for b in (self.rng, self.accum, self.stats_buf):
+            b.free()

@@ -1,0 +1,263 @@
+"""GPU parity of the neural radiance cache: the MFMA network against the FP64 oracle (FP16 / BF16
+storage emulated there, so the remaining difference is FP32 accumulation order), the optimiser step
+bit for bit, training convergence, and the NRC data path and trace entry points bit for bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle_lib import HostNrc, download_env, download_grid, oracle, oracle_seed, ptr
+from test_oracle_nrc import random_queries, random_weights
+
+pytestmark = pytest.mark.gpu
+
+
+def _cache(L, precision):
+    from rtcamp8_amd.nrc import NeuralRadianceCache
+
+    c = NeuralRadianceCache()
+    c.initialize(num_hidden_layers=L, learning_rate=1e-2, precision=precision)
+    return c
+
+
+def test_encoding_matches_oracle():
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
+
+    n = 3000
+    q = random_queries(n, seed=9)
+    dq, de = DeviceBuffer.from_array(q), DeviceBuffer(4 * 64 * n)
+    call("rtc8x_nrc_encode", None, C.c_void_p(dq.ptr), n, C.c_void_p(de.ptr))
+    got = de.download(np.float32, 64 * n).astype(np.float64)
+    ref = np.zeros(64 * n)
+    oracle().oracle_nrc_encode(ptr(q), n, ptr(ref))
+    assert np.max(np.abs(got - ref)) < 2e-6
+
+
+@pytest.mark.parametrize("L,precision,mode,tol", [(2, 0, 1, 4e-3), (5, 0, 1, 4e-3), (2, 1, 2, 2e-2), (5, 1, 2, 2e-2)])
+def test_inference_matches_oracle(L, precision, mode, tol):
+    from rtcamp8_amd.device import DeviceBuffer
+
+    n = 1000  # not a multiple of the 256-query workgroup: exercises the tail
+    q = random_queries(n, seed=10)
+    w = random_weights(L, seed=11, scale=1.2)
+    c = _cache(L, precision)
+    c.set_weights(w)
+    dq, do = DeviceBuffer.from_array(q), DeviceBuffer(12 * n)
+    c.infer(None, dq.ptr, n, do.ptr)
+    got = do.download(np.float32, 3 * n).astype(np.float64)
+    ref = np.zeros(3 * n)
+    oracle().oracle_nrc_forward(ptr(w), L, ptr(q), n, ptr(ref), mode)
+    rms = np.sqrt(np.mean(ref ** 2))
+    assert rms > 1e-3
+    assert np.max(np.abs(got - ref)) < tol * rms, np.max(np.abs(got - ref)) / rms
+    c.finalize()
+
+
+@pytest.mark.parametrize("L,precision,mode,tol", [(2, 0, 1, 5e-3), (5, 0, 1, 5e-3), (3, 1, 2, 3e-2)])
+def test_gradient_matches_oracle(L, precision, mode, tol):
+    from rtcamp8_amd.device import DeviceBuffer
+
+    n = 700
+    q = random_queries(n, seed=12)
+    t = np.random.default_rng(13).random((n, 3)).astype(np.float32)
+    w = random_weights(L, seed=14, scale=1.2)
+    c = _cache(L, precision)
+    c.set_weights(w)
+    dq, dt = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t)
+    got = c.gradient(None, dq.ptr, dt.ptr, n).astype(np.float64)
+    ref = np.zeros(w.size)
+    oracle().oracle_nrc_gradient(ptr(w), L, ptr(q), ptr(t), n, ptr(ref), mode)
+    for layer in range(L + 1):  # every layer on its own, so a small layer cannot hide in the total
+        sl = slice(layer * 4096, (layer + 1) * 4096 if layer < L else layer * 4096 + 3 * 64)
+        err = np.linalg.norm(got[sl] - ref[sl]) / np.linalg.norm(ref[sl])
+        assert err < tol, (layer, err)
+    exact = np.zeros(w.size)
+    oracle().oracle_nrc_gradient(ptr(w), L, ptr(q), ptr(t), n, ptr(exact), 0)
+    assert np.linalg.norm(got - exact) / np.linalg.norm(exact) < 10 * tol
+    assert np.all(got[L * 4096 + 3 * 64:] == 0)
+    c.finalize()
+
+
+def test_adam_step_bit_exact():
+    from rtcamp8_amd.device import DeviceBuffer
+
+    L, n = 2, 2048
+    q = random_queries(n, seed=15)
+    t = np.random.default_rng(16).random((n, 3)).astype(np.float32)
+    w = random_weights(L, seed=17, scale=1.0)
+    c = _cache(L, 0)
+    c.set_weights(w)
+    dq, dt = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t)
+    g = c.gradient(None, dq.ptr, dt.ptr, n)
+    c.train(None, dq.ptr, dt.ptr, n)
+    master = c.get_weights(0)
+    ema = c.get_weights(1)
+    om, m, v, oe = w.copy(), np.zeros_like(w), np.zeros_like(w), np.zeros_like(w)
+    oracle().oracle_nrc_adam(ptr(om), ptr(m), ptr(v), ptr(oe), ptr(g), w.size, n, 1e-2, 0.9, 0.99, 1e-8, 1e-6, 0.99, 1)
+    assert np.array_equal(master.view(np.uint32), om.view(np.uint32))
+    # the debiasing divisor is 1 - decay^t in double from the FP32 decay, rounded to FP32
+    debias = np.float32(1.0 - float(np.float32(0.99)))
+    assert np.array_equal(ema.view(np.uint32), (oe / debias).view(np.uint32))
+    # the FP16 training copies written by the optimiser step: the gradient at the new weights matches
+    # the oracle's at the new master weights
+    g2 = c.gradient(None, dq.ptr, dt.ptr, n).astype(np.float64)
+    ref = np.zeros(w.size)
+    oracle().oracle_nrc_gradient(ptr(master), L, ptr(q), ptr(t), n, ptr(ref), 1)
+    assert np.linalg.norm(g2 - ref) / np.linalg.norm(ref) < 5e-3
+    c.finalize()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_training_reduces_loss(precision):
+    """Online training on a smooth synthetic radiance field drops the relative L2 loss by > 10x.  Every
+    triangle wave tri(2^k x) is symmetric under x -> 1 - x on [0, 1], so the field is chosen with that
+    symmetry in each position coordinate (a field without it is not representable by the encoding)."""
+    from rtcamp8_amd.device import DeviceBuffer
+
+    L, n = 3, 16384
+    c = _cache(L, precision)
+    losses = []
+    for step in range(300):
+        q = random_queries(n, seed=100 + step)
+        x = q[:, 0:3].astype(np.float64)
+        t = np.stack([0.5 + 0.4 * np.cos(2 * np.pi * x[:, 0]),
+                      0.4 + 0.3 * np.sin(np.pi * x[:, 1]) * np.sin(np.pi * x[:, 2]),
+                      0.3 + 0.2 * np.cos(4 * np.pi * x[:, 2]) + 0.1 * q[:, 3] / np.pi], axis=1).astype(np.float32)
+        dq, dt = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t)
+        loss = c.train(None, dq.ptr, dt.ptr, n, want_loss=(step % 25 == 0 or step == 299))
+        if loss is not None:
+            losses.append(loss)
+        dq.free()
+        dt.free()
+    assert np.isfinite(losses).all()
+    assert losses[-1] < losses[0] / 10, losses
+    c.finalize()
+
+
+def _nrc_case(width=96, height=64, tracking=0):
+    from rtcamp8_amd import scene as sc
+    from rtcamp8_amd.nrc import NrcRenderer
+
+    cfg = sc.CONFIGS["dense128"].with_(width=width, height=height, tracking=tracking,
+                                       fbm=sc.FbmConfig(resolution=64, seed=0x5EED0001))
+    ds = sc.DeviceScene(cfg)
+    r = NrcRenderer(ds, num_hidden_layers=2)
+    return cfg, ds, r
+
+
+def _host_params(cfg, ds, r, host_nrc, rng, accum):
+    from rtcamp8_amd import scene as sc
+
+    hg = download_grid(ds.grid)
+    he, keep = download_env(ds.env)
+    p = sc.fill_params(cfg, hg.grid, he, ds.camera)
+    host_nrc.fill(p.nrc)
+    dp = r.params()
+    p.nrc.nrc_buffer_index = dp.nrc.nrc_buffer_index
+    p.nrc.offset_to_select_unbiased_path = dp.nrc.offset_to_select_unbiased_path
+    p.nrc.radiance_scale = dp.nrc.radiance_scale
+    p.nrc.path_termination_factor = dp.nrc.path_termination_factor
+    p.rng_states, p.accum = ptr(rng), ptr(accum)
+    return p, (hg, he, keep)
+
+
+@pytest.mark.parametrize("tracking", [0, 1])
+def test_training_data_path_bit_exact(tracking):
+    """prepare -> GENERATE_TRAINING_DATA trace -> propagate -> shuffle, GPU vs oracle on the same inputs."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd._lib import call
+
+    cfg, ds, r = _nrc_case(tracking=tracking)
+    b = r.buffers
+    r.buf_idx = 0
+    p = r.params()
+    p.nrc.offset_to_select_unbiased_path = 5
+    # snapshot of the inputs, then the GPU steps
+    before = HostNrc(b)
+    rng0 = r.rng.download(np.uint64, cfg.width * cfg.height)
+    call("rtc8x_nrc_prepare", None, C.byref(p), 1)
+    call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_GENERATE_TRAINING_DATA, 1)
+    ts = b.train_image_size.download(np.int32, 4)
+    n_term = int(ts[0] * ts[1])
+    inferred = np.random.default_rng(19).random(b.inferred_radiance.nbytes // 4).astype(np.float32) * 2 - 0.2
+    b.inferred_radiance.upload(inferred)
+    after_trace = HostNrc(b)
+    call("rtc8x_nrc_propagate", None, C.byref(p))
+    after_prop = HostNrc(b)
+    call("rtc8x_nrc_shuffle", None, C.byref(p))
+    after_shuf = HostNrc(b)
+    gpu_rng = r.rng.download(np.uint64, cfg.width * cfg.height)
+
+    # oracle on host copies
+    rng = rng0.copy()
+    accum = np.zeros(3 * cfg.width * cfg.height)
+    hp, keep = _host_params(cfg, ds, r, before, rng, accum)
+    hp.nrc.nrc_buffer_index = 0
+    hp.nrc.offset_to_select_unbiased_path = 5
+    oracle().oracle_nrc_prepare(C.byref(hp), 1)
+    assert oracle().oracle_trace_f32(C.byref(hp), _abi.ENTRY_GENERATE_TRAINING_DATA, 1, 0, 10 ** 6, 4) == 0
+    a, g = before.a, after_trace.a
+    num = int(a["num_training_data"][0])
+    assert num == int(g["num_training_data"][0]) and num > 100
+    assert np.array_equal(a["train_image_size"], g["train_image_size"])
+    assert np.array_equal(rng, gpu_rng)
+    k = min(num, _abi.TRAIN_BUFFER_SIZE)
+    assert np.array_equal(a["train_queries0"][:14 * k].view(np.uint32), g["train_queries0"][:14 * k].view(np.uint32))
+    assert np.array_equal(a["train_targets0"][:3 * k].view(np.uint32), g["train_targets0"][:3 * k].view(np.uint32))
+    assert np.array_equal(a["train_vertex_info"][:6 * k], g["train_vertex_info"][:6 * k])
+    assert np.array_equal(a["train_suffix_terminals"][:4 * n_term], g["train_suffix_terminals"][:4 * n_term])
+    has_q = a["train_suffix_terminals"][:4 * n_term].reshape(-1, 4)[:, 1] == 1
+    qi = a["inference_queries"].reshape(-1, 14)[:n_term][has_q]
+    qg = g["inference_queries"].reshape(-1, 14)[:n_term][has_q]
+    assert np.array_equal(qi.view(np.uint32), qg.view(np.uint32))
+    # propagate and shuffle from the GPU's post-trace state
+    a2 = HostNrc(b)
+    a2.a = {k2: v.copy() for k2, v in after_trace.a.items()}
+    hp2, keep2 = _host_params(cfg, ds, r, a2, rng, accum)
+    hp2.nrc.nrc_buffer_index = 0
+    oracle().oracle_nrc_propagate(C.byref(hp2))
+    assert np.array_equal(a2.a["train_targets0"].view(np.uint32), after_prop.a["train_targets0"].view(np.uint32))
+    oracle().oracle_nrc_shuffle(C.byref(hp2))
+    for key in ("train_queries1", "train_targets1", "data_shufflers", "target_min_max", "target_avg"):
+        assert np.array_equal(a2.a[key].view(np.uint32), after_shuf.a[key].view(np.uint32)), key
+    r.close()
+    ds.close()
+
+
+def test_render_with_nrc_bit_exact():
+    """PATH_TRACE_WITH_NRC trace -> (synthetic) inference -> accumulate, GPU vs oracle."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd._lib import call
+
+    cfg, ds, r = _nrc_case()
+    b = r.buffers
+    p = r.params()
+    rng0 = r.rng.download(np.uint64, cfg.width * cfg.height)
+    before = HostNrc(b)
+    call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
+    inferred = np.random.default_rng(20).random(b.inferred_radiance.nbytes // 4).astype(np.float32) * 3 - 0.5
+    b.inferred_radiance.upload(inferred)
+    call("rtc8x_nrc_accumulate", None, C.byref(p))
+    after = HostNrc(b)
+    gpu_accum = r.accum.download(np.float64, 3 * cfg.width * cfg.height)
+    gpu_rng = r.rng.download(np.uint64, cfg.width * cfg.height)
+
+    rng = rng0.copy()
+    accum = np.zeros(3 * cfg.width * cfg.height)
+    hp, keep = _host_params(cfg, ds, r, before, rng, accum)
+    assert oracle().oracle_trace_f32(C.byref(hp), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1, 0, 10 ** 6, 4) == 0
+    before.a["inferred_radiance"][:] = inferred
+    oracle().oracle_nrc_accumulate(C.byref(hp))
+    n = cfg.width * cfg.height
+    a, g = before.a, after.a
+    assert np.array_equal(rng, gpu_rng)
+    assert np.array_equal(a["inference_terminals"].view(np.uint32), g["inference_terminals"].view(np.uint32))
+    assert np.array_equal(a["per_frame_contribution"].view(np.uint32), g["per_frame_contribution"].view(np.uint32))
+    has_q = (a["inference_terminals"].reshape(-1, 4)[:, 3] & 1) == 1
+    assert has_q.sum() > 10
+    assert np.array_equal(a["inference_queries"].reshape(-1, 14)[:n][has_q].view(np.uint32),
+                          g["inference_queries"].reshape(-1, 14)[:n][has_q].view(np.uint32))
+    assert np.array_equal(accum.view(np.uint64), gpu_accum.view(np.uint64))
+    r.close()
+    ds.close()

@@ -5,9 +5,11 @@ albedo 0.999, Henyey-Greenstein g = 0.85, Hosek-Wilkie sky (turbidity 3, sun ele
 with alias tables, 1920x1080, up to 256 bounces, Russian roulette after bounce 8.  One step = one sample
 per pixel over the full image (one launch); the default K = 16 steps is the workload's 16 spp.
 
-Prints one JSON line: value = path vertices per second (rays traced through the medium per second,
-all inputs resident in HBM), with the trace kernel's roofline (HIP-event kernel time, 32 algorithmic
-bytes per trilinear density lookup) and the CPU oracle timed on the host's cores as a baseline.
+Prints one JSON line: value = path vertices per second of the plain tracer (rays traced through the
+medium per second, all inputs resident in HBM), with the trace kernel's roofline (HIP-event kernel time,
+32 algorithmic bytes per trilinear density lookup) and the CPU oracle timed on the host's cores as a
+baseline.  Two NRC legs ride along: the standalone network step (2^16 training samples + 2^21 inference
+queries, 5 hidden layers, FP16) giving nrc_queries_per_s, and NRC-integrated rendering of the same scene.
 """
 from __future__ import annotations
 
@@ -27,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "path vertices/sec + NRC queries/sec; wall time to equal RMSE vs FP64 oracle"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LOOKUP = 32  # 8 FP32 voxel taps per trilinear lookup
+FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 
 
 def parse():
@@ -36,8 +39,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="sparse512")
     ap.add_argument("--tracking", choices=["local", "global"], default="global")
-    ap.add_argument("--cpu-rows", type=int, default=6, help="image rows of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nrc-steps", type=int, default=8, help="steps of each NRC leg (0 skips them)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -52,8 +58,9 @@ def traffic_from_profiles(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(scene, args, rows: int):
-    """The CPU oracle (FP32 restatement of the same algorithm) on `rows` image rows, 1 spp, all cores."""
+def cpu_baseline(scene, target_s: float):
+    """The CPU oracle (FP32 restatement of the same algorithm) on all host cores, over full-frame passes
+    at 1 spp until about `target_s` seconds of work."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import download_env, download_grid, oracle, oracle_seed, ptr  # oracle: checker / baseline only
 
@@ -70,16 +77,149 @@ def cpu_baseline(scene, args, rows: int):
     stats = np.zeros(_abi.STAT_COUNT, dtype=np.uint64)
     p.rng_states, p.accum, p.path_vertices, p.stats = ptr(rng), ptr(accum), None, ptr(stats)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    y0 = h // 2 - rows // 2
-    t0 = time.perf_counter()
-    rc = oracle().oracle_trace_f32(C.byref(p), _abi.ENTRY_PATH_TRACE, 1, y0, y0 + rows, threads)
-    dt = time.perf_counter() - t0
-    assert rc == 0
+    # full-frame passes at 1 spp each (the RNG states advance between passes), the last one cut to a
+    # centred band of rows so the sample lands near target_s
+    dt, passes, rows_done = 0.0, 0, 0
+    rows = h
+    while dt < target_s and passes < 64:
+        y0 = (h - rows) // 2
+        t0 = time.perf_counter()
+        assert oracle().oracle_trace_f32(C.byref(p), _abi.ENTRY_PATH_TRACE, 1, y0, y0 + rows, threads) == 0
+        dt += time.perf_counter() - t0
+        passes += 1
+        rows_done += rows
+        rate = rows_done / max(dt, 1e-3)
+        rows = int(min(h, max(16, rate * (target_s - dt))))
     vertices = int(stats[_abi.STAT_VERTICES])
     return {"value": vertices / dt, "unit": "path vertices/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} rows x {w} px at 1 spp through the image centre ({rows * w} paths, {vertices} vertices, "
-                      f"{dt:.1f} s), FP32 oracle",
+            "sample": f"{passes} passes over {rows_done} image rows of {w} px ({rows_done * w} paths at 1 spp each), "
+                      f"{vertices} vertices in {dt:.1f} s, FP32 oracle",
             "seconds": dt}
+
+
+def random_leaf_queries(scene, n: int, seed: int) -> np.ndarray:
+    """NRC queries uniform over the volume's active leaves: position = normalised index-space point inside
+    a random active leaf, directions uniform, albedo as the medium's."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd.device import download_ptr
+
+    g = scene.grid
+    origins = download_ptr(g.leaf_origins, np.int32, 3 * g.num_leaves).reshape(-1, 3)
+    lo = np.array(g.index_bbox_min[:], dtype=np.float64)
+    ext = np.array(g.index_bbox_max[:], dtype=np.float64) + 1.0 - lo
+    rng = np.random.default_rng(seed)
+    leaf = origins[rng.integers(0, origins.shape[0], n)]
+    pos = (leaf + rng.random((n, 3)) * 8.0 - lo) / ext
+    q = np.zeros((n, 14), dtype=np.float32)
+    q[:, 0:3] = np.clip(pos, 0.0, 1.0)
+    q[:, 3] = np.arccos(1.0 - 2.0 * rng.random(n))
+    q[:, 4] = rng.random(n) * 2.0 * np.pi
+    q[:, 5:7] = q[:, 3:5]
+    q[:, 8:11] = scene.cfg.albedo
+    return q
+
+
+def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 21, n_train: int = 1 << 16):
+    """Standalone network step: train on 2^16 samples, then infer 2^21 queries (5 hidden layers of 64,
+    FP16 storage, FP32 accumulation)."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd.device import DeviceBuffer, EventTimer
+    from rtcamp8_amd.nrc import NeuralRadianceCache
+
+    q = random_leaf_queries(scene, n_infer, seed=1234)
+    t = (0.2 + np.random.default_rng(99).random((n_train, 3))).astype(np.float32)
+    dq, dt, dout = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t), DeviceBuffer(12 * n_infer)
+    cache = NeuralRadianceCache()
+    cache.initialize(num_hidden_layers=hidden_layers, learning_rate=1e-2, precision=_abi.NRC_FP16)
+    for _ in range(2):
+        cache.train(None, dq.ptr, dt.ptr, n_train)
+        cache.infer(None, dq.ptr, n_infer, dout.ptr)
+    tt, ti = [EventTimer() for _ in range(steps)], [EventTimer() for _ in range(steps)]
+    for i in range(steps):
+        tt[i].start(None)
+        cache.train(None, dq.ptr, dt.ptr, n_train)
+        tt[i].stop(None)
+        ti[i].start(None)
+        cache.infer(None, dq.ptr, n_infer, dout.ptr)
+        ti[i].stop(None)
+    train_ms = float(np.mean([x.elapsed_ms() for x in tt]))
+    infer_ms = float(np.mean([x.elapsed_ms() for x in ti]))
+    out = dout.download(np.float32, 3 * n_infer)
+    ok = bool(np.all(np.isfinite(out)))
+    for x in tt + ti:
+        x.destroy()
+    cache.finalize()
+    for b in (dq, dt, dout):
+        b.free()
+    fwd_flop = 2.0 * (hidden_layers * 64 * 64 + 64 * 3)
+    train_flop = 3.0 * fwd_flop  # forward + backward (input and weight gradients)
+    return {
+        "hidden_layers": hidden_layers, "precision": "fp16 storage, fp32 accumulate",
+        "infer_queries": n_infer, "train_samples": n_train,
+        "infer_ms": infer_ms, "train_ms": train_ms,
+        "queries_per_s": n_infer / (infer_ms * 1e-3),
+        "train_samples_per_s": n_train / (train_ms * 1e-3),
+        "infer_tflops": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12,
+        "train_tflops": n_train * train_flop / (train_ms * 1e-3) / 1e12,
+        "mfma_peak_tflops": FP16_PEAK_TFLOPS,
+        "infer_frac": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12 / FP16_PEAK_TFLOPS,
+        "finite": ok,
+    }
+
+
+def nrc_render_leg(scene, steps: int, stream):
+    """NRC-integrated rendering of the same scene: per step one training iteration and one sample per
+    pixel traced into the cache (2 hidden layers, FP16), after the first frame's 8 training iterations."""
+    from rtcamp8_amd.device import EventTimer
+    from rtcamp8_amd.nrc import NrcRenderer
+
+    r = NrcRenderer(scene, num_hidden_layers=2, stream=stream.handle)
+    r.render_frame(spp=1)
+    stream.synchronize()
+    r.stats_buf.fill(0)
+    q0 = r.stats.inference_queries
+    ev = EventTimer()
+    ev.start(stream.handle)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render_frame(spp=1, train_iterations=1)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    ev.stop(stream.handle)
+    gpu_ms = ev.elapsed_ms()
+    ev.destroy()
+    c = r.counters()
+    img = r.image()
+    res = {
+        "ms_per_spp": wall * 1e3 / steps, "gpu_ms_per_spp": gpu_ms / steps,
+        "spp": steps, "hidden_layers": 2,
+        "inference_queries_per_s": (r.stats.inference_queries - q0) / wall,
+        "path_vertices_per_s": c["vertices"] / wall,
+        "vertices_per_path": c["vertices"] / max(c["paths"], 1),
+        "finite": bool(np.all(np.isfinite(img))),
+        "last_loss": r.stats.last_loss,
+    }
+    r.close()
+    return res
+
+
+def calibrate_copy(mib: int, steps: int) -> int:
+    """HBM counter calibration: `steps` launches of the plain copy kernel over `mib` MiB."""
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer, EventTimer
+
+    nbytes = mib << 20
+    a, b = DeviceBuffer.zeros(nbytes), DeviceBuffer(nbytes)
+    call("rtc8x_copy_async", C.c_void_p(b.ptr), C.c_void_p(a.ptr), nbytes, None)
+    timers = [EventTimer() for _ in range(steps)]
+    for t in timers:
+        t.start(None)
+        call("rtc8x_copy_async", C.c_void_p(b.ptr), C.c_void_p(a.ptr), nbytes, None)
+        t.stop(None)
+    ms = float(np.mean([t.elapsed_ms() for t in timers]))
+    print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": steps + 1, "ms_avg": ms,
+                      "read_plus_write_gbs": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -87,6 +227,8 @@ def main() -> int:
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); run it with --gpus 1", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy, args.steps)
     from rtcamp8_amd import _abi
     from rtcamp8_amd import scene as sc
     from rtcamp8_amd.device import EventTimer, Stream, device_name, synchronize
@@ -172,9 +314,19 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if args.nrc_steps > 0:
+        print("[bench] NRC network leg ...", file=sys.stderr, flush=True)
+        mlp = nrc_mlp_leg(scene, args.nrc_steps)
+        print(f"[bench] NRC network: infer {mlp['infer_ms']:.2f} ms / 2^21, train {mlp['train_ms']:.2f} ms / 2^16",
+              file=sys.stderr, flush=True)
+        print("[bench] NRC render leg ...", file=sys.stderr, flush=True)
+        rend = nrc_render_leg(scene, args.nrc_steps, stream)
+        print(f"[bench] NRC render: {rend['ms_per_spp']:.2f} ms/spp", file=sys.stderr, flush=True)
+        result["nrc_queries_per_s"] = mlp["queries_per_s"]
+        result["nrc"] = {"network": mlp, "render": rend}
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(scene, args, args.cpu_rows)
+        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         print(f"[bench] CPU baseline {result['cpu_baseline']['value']:.3g} vertices/s "
               f"({result['cpu_baseline']['seconds']:.1f} s)", file=sys.stderr, flush=True)
     print(json.dumps(result), flush=True)

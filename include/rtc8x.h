@@ -58,6 +58,9 @@ rtc8x_status rtc8x_memcpy_htod(void* dst, const void* src, size_t bytes);
 rtc8x_status rtc8x_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 rtc8x_status rtc8x_memcpy_dtod(void* dst, const void* src, size_t bytes);
 rtc8x_status rtc8x_memset(void* dst, int32_t value, size_t bytes);
+/* Device-to-device copy by a plain 16-byte-per-lane kernel on `stream` (dst, src and bytes 16-byte
+ * aligned).  Reads and writes exactly `bytes` each, so its HBM counters calibrate FETCH/WRITE sizes. */
+rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* stream);
 rtc8x_status rtc8x_device_synchronize(void);
 rtc8x_status rtc8x_stream_create(void** stream);
 rtc8x_status rtc8x_stream_destroy(void* stream);

@@ -58,9 +58,6 @@ rtc8x_status rtc8x_memcpy_htod(void* dst, const void* src, size_t bytes);
 rtc8x_status rtc8x_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 rtc8x_status rtc8x_memcpy_dtod(void* dst, const void* src, size_t bytes);
 rtc8x_status rtc8x_memset(void* dst, int32_t value, size_t bytes);
-/* Device-to-device copy by a plain 16-byte-per-lane kernel on `stream` (dst, src and bytes 16-byte
- * aligned).  Reads and writes exactly `bytes` each, so its HBM counters calibrate FETCH/WRITE sizes. */
-rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* stream);
 rtc8x_status rtc8x_device_synchronize(void);
 rtc8x_status rtc8x_stream_create(void** stream);
 rtc8x_status rtc8x_stream_destroy(void* stream);
@@ -70,6 +67,18 @@ rtc8x_status rtc8x_event_create(void** event);
 rtc8x_status rtc8x_event_destroy(void* event);
 rtc8x_status rtc8x_event_record(void* event, void* stream);
 rtc8x_status rtc8x_event_elapsed_ms(float* ms, void* start, void* end);
+
+/* Measurement kernels (no counterpart in the reference).
+ * rtc8x_copy_async: device-to-device copy by a plain 16-byte-per-lane kernel on `stream` (dst, src and
+ *   bytes 16-byte aligned).  It reads and writes exactly `bytes` each, which calibrates HBM byte counters.
+ * rtc8x_gather_probe: the memory-latency roofline of density lookups.  `threads` lanes (a multiple of
+ *   256) each perform `steps` serially dependent lookups; one lookup reads the 8 taps of a 2x2x2 stencil
+ *   at a random position of a random leaf of `leaf_values` (num_leaves x 512 floats), as a trilinear
+ *   lookup does.  Occupancy is pinned to `waves_per_simd` (3..8) waves per SIMD through an LDS
+ *   reservation.  out[threads] receives a per-lane checksum. */
+rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* stream);
+rtc8x_status rtc8x_gather_probe(void* stream, const float* leaf_values, uint32_t num_leaves, uint32_t threads,
+                                uint32_t steps, uint32_t waves_per_simd, uint32_t seed, float* out);
 
 /* ------------------------------------------------------------------------------------------------
  * Sparse density grid in the NanoVDB tree layout (root tiles -> 32^3 upper -> 16^3 lower -> 8^3 leaf;
@@ -117,6 +126,14 @@ typedef struct rtc8x_grid {
     int32_t cell_origin[3];              /* index-space origin of cell (0,0,0), a multiple of 8 */
     int32_t cell_dims[3];
     const float* cell_majorants;         /* cell_dims[0]*cell_dims[1]*cell_dims[2], x-major */
+    /* Dense leaf map (library-built, optional): for every 8^3 block of index space in
+     * [leaf_map_origin, leaf_map_origin + 8 * leaf_map_dims) the result of the root -> upper -> lower
+     * walk (leaf index, or RTC8X_TILE_BIT | tile value index), x-major.  It is a cache of the tree:
+     * lookups outside it walk the tree and get the same values.  NULL when the index bbox needs more
+     * than 2^26 blocks. */
+    int32_t leaf_map_origin[3];          /* a multiple of 8 */
+    int32_t leaf_map_dims[3];
+    const uint32_t* leaf_map;
 } rtc8x_grid;
 
 /* Procedural fBm volume description (the benchmark's synthetic input; the reference loads a .nvdb). */

@@ -27,6 +27,7 @@ SOURCES = [
     "rtc8x_trace.hip",
     "rtc8x_nrc_data.hip",
     "rtc8x_nrc_mlp.hip",
+    "rtc8x_measure.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -2,7 +2,6 @@
 // helper and the per-pixel RNG seeding.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -13,12 +12,6 @@
 #include "rtc8x_common.hpp"
 
 namespace rtc8x {
-
-__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
-{
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
-}
 
 static thread_local std::string g_last_error;
 
@@ -116,19 +109,6 @@ rtc8x_status rtc8x_memset(void* dst, int32_t value, size_t bytes)
     if (bytes == 0) return RTC8X_OK;
     RTC8X_REQUIRE(dst != nullptr, "rtc8x_memset: NULL pointer");
     RTC8X_HIP(hipMemset(dst, value, bytes));
-    return RTC8X_OK;
-}
-
-rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* stream)
-{
-    if (bytes == 0) return RTC8X_OK;
-    RTC8X_REQUIRE(dst && src, "rtc8x_copy_async: NULL pointer");
-    RTC8X_REQUIRE(((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0, "rtc8x_copy_async: needs 16-byte alignment");
-    const size_t n = bytes / 16;
-    const size_t blocks = std::min<size_t>((n + 255) / 256, 256 * 32);
-    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (const uint4*)src,
-                       (uint4*)dst, n);
-    RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
 

@@ -121,24 +121,23 @@ __device__ __forceinline__ uint32_t find_leaf(const rtc8x_grid& g, int32_t i, in
     return g.lower_tables[(size_t)e1 * RTC8X_LOWER_TABLE_SIZE + lower_offset(i, j, k)];
 }
 
-// ReadAccessor-style cache of the last leaf (value-identical to an uncached walk).
-struct Accessor {
-    int32_t ox, oy, oz;   // origin of the cached leaf (low 3 bits cleared)
-    uint32_t leaf;
-    __device__ __forceinline__ void reset() { ox = oy = oz = 0x7fffffff; leaf = RTC8X_TILE_BIT; }
-    __device__ __forceinline__ float get(const rtc8x_grid& g, int32_t i, int32_t j, int32_t k)
-    {
-        int32_t bx = i & ~7, by = j & ~7, bz = k & ~7;
-        if (bx != ox || by != oy || bz != oz) {
-            ox = bx;
-            oy = by;
-            oz = bz;
-            leaf = find_leaf(g, i, j, k);
-        }
-        if (leaf & RTC8X_TILE_BIT) return g.tile_values[leaf & ~RTC8X_TILE_BIT];
-        return g.leaf_values[(size_t)leaf * RTC8X_LEAF_SIZE + leaf_offset(i, j, k)];
-    }
-};
+// Tree walk through the dense leaf map when the block is inside it (the map caches find_leaf).
+__device__ __forceinline__ uint32_t block_entry(const rtc8x_grid& g, int32_t i, int32_t j, int32_t k)
+{
+    const int32_t bx = (i >> 3) - (g.leaf_map_origin[0] >> 3), by = (j >> 3) - (g.leaf_map_origin[1] >> 3),
+                  bz = (k >> 3) - (g.leaf_map_origin[2] >> 3);
+    if (g.leaf_map && (uint32_t)bx < (uint32_t)g.leaf_map_dims[0] && (uint32_t)by < (uint32_t)g.leaf_map_dims[1] &&
+        (uint32_t)bz < (uint32_t)g.leaf_map_dims[2])
+        return g.leaf_map[((size_t)bx * (uint32_t)g.leaf_map_dims[1] + (uint32_t)by) * (uint32_t)g.leaf_map_dims[2] +
+                          (uint32_t)bz];
+    return find_leaf(g, i, j, k);
+}
+
+__device__ __forceinline__ const float* voxel_ptr(const rtc8x_grid& g, uint32_t e, int32_t i, int32_t j, int32_t k)
+{
+    return (e & RTC8X_TILE_BIT) ? g.tile_values + (e & ~RTC8X_TILE_BIT)
+                                : g.leaf_values + (size_t)e * RTC8X_LEAF_SIZE + leaf_offset(i, j, k);
+}
 
 // Map::applyInverseMapF with matMult (NanoVDB.h:1409-1414, 2029-2032)
 __device__ __forceinline__ V3 world_to_index(const rtc8x_grid& g, float px, float py, float pz)
@@ -149,24 +148,56 @@ __device__ __forceinline__ V3 world_to_index(const rtc8x_grid& g, float px, floa
               fmaf(x, m[6], fmaf(y, m[7], z * m[8]))};
 }
 
-// SampleFromVoxels<Acc,1,false>::operator() (util/SampleFromVoxels.h:419-460) with Floor (:58-65)
-__device__ __forceinline__ float trilinear(const rtc8x_grid& g, Accessor& acc, V3 idx)
+// SampleFromVoxels<Acc,1,false>::operator() (util/SampleFromVoxels.h:419-460) with Floor (:58-65).
+// The 8 taps are fetched in two dependent rounds -- the 8 leaf-map entries, then the 8 voxel values,
+// each round issued together -- and combined with the reference's lerps in its order.
+__device__ __forceinline__ float trilinear(const rtc8x_grid& g, V3 idx)
 {
     float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
     float x = idx.x - fx, y = idx.y - fy, z = idx.z - fz;
     int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
-    float v000 = acc.get(g, i, j, k);
-    float v001 = acc.get(g, i, j, k + 1);
+    const int32_t ob0 = g.leaf_map_origin[0] >> 3, ob1 = g.leaf_map_origin[1] >> 3, ob2 = g.leaf_map_origin[2] >> 3;
+    const int32_t bx0 = (i >> 3) - ob0, bx1 = ((i + 1) >> 3) - ob0;
+    const int32_t by0 = (j >> 3) - ob1, by1 = ((j + 1) >> 3) - ob1;
+    const int32_t bz0 = (k >> 3) - ob2, bz1 = ((k + 1) >> 3) - ob2;
+    uint32_t e000, e001, e010, e011, e100, e101, e110, e111;
+    if (g.leaf_map && bx0 >= 0 && by0 >= 0 && bz0 >= 0 && bx1 < g.leaf_map_dims[0] && by1 < g.leaf_map_dims[1] &&
+        bz1 < g.leaf_map_dims[2]) {
+        const size_t d1 = (uint32_t)g.leaf_map_dims[1], d2 = (uint32_t)g.leaf_map_dims[2];
+        const uint32_t* m00 = g.leaf_map + ((size_t)bx0 * d1 + by0) * d2;
+        const uint32_t* m01 = g.leaf_map + ((size_t)bx0 * d1 + by1) * d2;
+        const uint32_t* m10 = g.leaf_map + ((size_t)bx1 * d1 + by0) * d2;
+        const uint32_t* m11 = g.leaf_map + ((size_t)bx1 * d1 + by1) * d2;
+        e000 = m00[bz0];
+        e001 = m00[bz1];
+        e010 = m01[bz0];
+        e011 = m01[bz1];
+        e100 = m10[bz0];
+        e101 = m10[bz1];
+        e110 = m11[bz0];
+        e111 = m11[bz1];
+    } else {
+        e000 = find_leaf(g, i, j, k);
+        e001 = find_leaf(g, i, j, k + 1);
+        e010 = find_leaf(g, i, j + 1, k);
+        e011 = find_leaf(g, i, j + 1, k + 1);
+        e100 = find_leaf(g, i + 1, j, k);
+        e101 = find_leaf(g, i + 1, j, k + 1);
+        e110 = find_leaf(g, i + 1, j + 1, k);
+        e111 = find_leaf(g, i + 1, j + 1, k + 1);
+    }
+    const float v000 = *voxel_ptr(g, e000, i, j, k);
+    const float v001 = *voxel_ptr(g, e001, i, j, k + 1);
+    const float v011 = *voxel_ptr(g, e011, i, j + 1, k + 1);
+    const float v010 = *voxel_ptr(g, e010, i, j + 1, k);
+    const float v110 = *voxel_ptr(g, e110, i + 1, j + 1, k);
+    const float v111 = *voxel_ptr(g, e111, i + 1, j + 1, k + 1);
+    const float v101 = *voxel_ptr(g, e101, i + 1, j, k + 1);
+    const float v100 = *voxel_ptr(g, e100, i + 1, j, k);
     float vy = v000 + z * (v001 - v000);
-    float v011 = acc.get(g, i, j + 1, k + 1);
-    float v010 = acc.get(g, i, j + 1, k);
     float vy1 = v010 + z * (v011 - v010);
     float vx = vy + y * (vy1 - vy);
-    float v110 = acc.get(g, i + 1, j + 1, k);
-    float v111 = acc.get(g, i + 1, j + 1, k + 1);
     vy1 = v110 + z * (v111 - v110);
-    float v101 = acc.get(g, i + 1, j, k + 1);
-    float v100 = acc.get(g, i + 1, j, k);
     vy = v100 + z * (v101 - v100);
     float vx1 = vy + y * (vy1 - vy);
     return vx + x * (vx1 - vx);
@@ -180,11 +211,9 @@ struct Medium {
 
 __device__ __forceinline__ float sample_density(const rtc8x_grid& g, Medium& m, V3 o, V3 d, float t)
 {
-    Accessor acc;
-    acc.reset();
     V3 idx = world_to_index(g, fmaf(t, d.x, o.x), fmaf(t, d.y, o.y), fmaf(t, d.z, o.z));
     m.lookups++;
-    return m.coeff * trilinear(g, acc, idx);
+    return m.coeff * trilinear(g, idx);
 }
 
 // nanovdb::Ray::clip against the float world bbox (util/Ray.h:63-73, 427-455, 512-520)

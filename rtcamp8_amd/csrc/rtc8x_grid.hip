@@ -9,6 +9,7 @@
 //   leaf_values     num_leaves x 512 x f32    (2 KiB per leaf, NanoVDB voxel order x-major)
 //   leaf_origins    num_leaves x 3 x i32
 //   cell_majorants  one f32 per 8^3 cell of the index bbox
+//   leaf_map        one u32 per 8^3 block of the index bbox plus a one-block margin (the tree walk's result)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -117,15 +118,27 @@ __global__ __launch_bounds__(512) void fill_leaves_kernel(const rtc8x_fbm_desc d
     values[(size_t)leaf * 512 + v] = dv;
 }
 
+// One lane per block of the dense leaf map: the tree walk's result for that block.
+__global__ void leaf_map_kernel(const rtc8x_grid g, uint32_t* __restrict__ map)
+{
+    const size_t n = (size_t)g.leaf_map_dims[0] * g.leaf_map_dims[1] * g.leaf_map_dims[2];
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t bz = (uint32_t)(q % (uint32_t)g.leaf_map_dims[2]);
+    const size_t r = q / (uint32_t)g.leaf_map_dims[2];
+    const uint32_t by = (uint32_t)(r % (uint32_t)g.leaf_map_dims[1]);
+    const uint32_t bx = (uint32_t)(r / (uint32_t)g.leaf_map_dims[1]);
+    map[q] = find_leaf(g, g.leaf_map_origin[0] + 8 * (int32_t)bx, g.leaf_map_origin[1] + 8 * (int32_t)by,
+                       g.leaf_map_origin[2] + 8 * (int32_t)bz);
+}
+
 __global__ void probe_kernel(const rtc8x_grid g, uint32_t n, const float* __restrict__ xyz, int32_t* __restrict__ out_ijk,
                              uint32_t* __restrict__ out_off, float* __restrict__ out_val)
 {
     uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     V3 idx = world_to_index(g, xyz[3 * q], xyz[3 * q + 1], xyz[3 * q + 2]);
-    Accessor acc;
-    acc.reset();
-    float v = trilinear(g, acc, idx);
+    float v = trilinear(g, idx);
     int32_t i = (int32_t)floorf(idx.x), j = (int32_t)floorf(idx.y), k = (int32_t)floorf(idx.z);
     out_ijk[3 * q] = i;
     out_ijk[3 * q + 1] = j;
@@ -199,6 +212,33 @@ rtc8x_status upload_vec(const std::vector<T>& v, T** dst)
         if (st_ != RTC8X_OK) return st_;       \
     } while (0)
 
+// Dense leaf map over the index bbox's blocks plus one block on every side (a trilinear stencil at a
+// position inside the world bbox reaches at most one voxel past the index bbox), built on the device
+// from the uploaded tables.  Skipped (leaf_map = NULL) beyond 2^26 blocks.
+rtc8x_status build_leaf_map(rtc8x_grid& g, hipStream_t s)
+{
+    g.leaf_map = nullptr;
+    size_t n = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int32_t lo = (int32_t)std::floor((float)g.index_bbox_min[a] / 8.0f) - 1;
+        const int32_t hi = (int32_t)std::floor((float)g.index_bbox_max[a] / 8.0f) + 1;
+        g.leaf_map_origin[a] = lo * 8;
+        g.leaf_map_dims[a] = hi - lo + 1;
+        n *= (size_t)g.leaf_map_dims[a];
+    }
+    if (n > ((size_t)1 << 26)) {
+        for (int a = 0; a < 3; ++a) g.leaf_map_dims[a] = 0;
+        return RTC8X_OK;
+    }
+    uint32_t* d_map = nullptr;
+    RTC8X_HIP(hipMalloc((void**)&d_map, n * sizeof(uint32_t)));
+    hipLaunchKernelGGL(dev::leaf_map_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, d_map);
+    RTC8X_HIP(hipGetLastError());
+    RTC8X_HIP(hipStreamSynchronize(s));
+    g.leaf_map = d_map;
+    return RTC8X_OK;
+}
+
 }  // namespace
 }  // namespace rtc8x
 
@@ -208,7 +248,7 @@ extern "C" rtc8x_status rtc8x_grid_destroy(rtc8x_grid* g)
 {
     RTC8X_REQUIRE(g != nullptr, "rtc8x_grid_destroy: grid is NULL");
     const void* ptrs[] = {g->root_tiles, g->upper_tables, g->lower_tables, g->leaf_values,
-                          g->tile_values, g->leaf_origins, g->cell_majorants};
+                          g->tile_values, g->leaf_origins, g->cell_majorants, g->leaf_map};
     for (const void* p : ptrs)
         if (p) RTC8X_HIP(hipFree(const_cast<void*>(p)));
     std::memset(g, 0, sizeof(*g));
@@ -362,6 +402,7 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
     hipLaunchKernelGGL(dev::fill_leaves_kernel, dim3(num_leaves), dim3(512), 0, s, *d, d_origins, sub, inv_range, d_values);
     RTC8X_HIP(hipGetLastError());
     RTC8X_HIP(hipStreamSynchronize(s));
+    RTC8X_TRY(build_leaf_map(g, s));
     *out = g;
     return RTC8X_OK;
 }
@@ -460,6 +501,8 @@ extern "C" rtc8x_status rtc8x_grid_upload(const rtc8x_grid* h, rtc8x_grid* out)
     g.tile_values = d_tiles;
     g.leaf_origins = d_origins;
     g.cell_majorants = d_cm;
+    g.leaf_map = nullptr;
+    if (h->num_root_tiles > 0 && h->num_leaves > 0) RTC8X_TRY(build_leaf_map(g, nullptr));
     *out = g;
     return RTC8X_OK;
 }

@@ -2,8 +2,8 @@
 
 Workload (sparse512): sparse 512^3 NanoVDB-layout fBm cloud (25 % of 8^3 leaves active, sigma_t max 50),
 albedo 0.999, Henyey-Greenstein g = 0.85, Hosek-Wilkie sky (turbidity 3, sun elevation 30 deg) sampled
-with alias tables, 1920x1080, up to 256 bounces, Russian roulette after bounce 8.  One step = one sample
-per pixel over the full image (one launch); the default K = 16 steps is the workload's 16 spp.
+with alias tables, 1920x1080, up to 256 bounces, Russian roulette after bounce 8.  One step = one frame
+of the workload: 16 samples per pixel over the full image in one launch.
 
 Prints one JSON line: value = path vertices per second of the plain tracer (rays traced through the
 medium per second, all inputs resident in HBM), with the trace kernel's roofline (HIP-event kernel time,
@@ -35,7 +35,7 @@ FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="sparse512")
     ap.add_argument("--tracking", choices=["local", "global"], default="global")
@@ -276,8 +276,9 @@ def main() -> int:
     pt = PathTracer(scene, stream=stream.handle)
     params = pt.params()
 
+    spp = cfg.spp
     for i in range(args.warmup):
-        pt.trace(1, params)
+        pt.trace(spp, params)
         stream.synchronize()
         print(f"[bench] warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     pt.stats.fill(0)
@@ -287,7 +288,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for i in range(args.steps):
         timers[i].start(stream.handle)
-        pt.trace(1, params)
+        pt.trace(spp, params)
         timers[i].stop(stream.handle)
     stream.synchronize()
     wall = time.perf_counter() - t0
@@ -321,11 +322,11 @@ def main() -> int:
         "config": {
             "workload": f"{cfg.name}: sparse {cfg.fbm.resolution}^3 fBm cloud, 25% of 8^3 leaves, sigma_t max "
                         f"{cfg.density_coeff:g}, albedo {cfg.albedo}, HG g={cfg.forwardness}, Hosek-Wilkie T=3 sun 30deg, "
-                        f"alias-table env sampling, {cfg.width}x{cfg.height}, {steps} spp, <= {cfg.max_path_length} bounces, "
+                        f"alias-table env sampling, {cfg.width}x{cfg.height}, {spp} spp per step, <= {cfg.max_path_length} bounces, "
                         f"RR after {cfg.rr_after_path_length}",
             "tracking": args.tracking,
             "image": [cfg.width, cfg.height],
-            "spp": steps,
+            "spp_per_step": spp,
         },
         "paths_per_s": c["paths"] / wall,
         "vertices_per_path": c["vertices"] / max(c["paths"], 1),

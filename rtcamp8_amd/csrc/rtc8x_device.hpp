@@ -148,9 +148,28 @@ __device__ __forceinline__ V3 world_to_index(const rtc8x_grid& g, float px, floa
               fmaf(x, m[6], fmaf(y, m[7], z * m[8]))};
 }
 
+struct F2 {
+    float x, y;
+};
+
+// Voxels (i,j,k) and (i,j,k+1) of one block (k & 7 != 7): one 8-byte load from a leaf (voxel order
+// puts k innermost), or the tile value twice.
+__device__ __forceinline__ F2 tap_pair(const rtc8x_grid& g, uint32_t e, uint32_t off)
+{
+    if (e & RTC8X_TILE_BIT) {
+        const float v = g.tile_values[e & ~RTC8X_TILE_BIT];
+        return F2{v, v};
+    }
+    F2 r;
+    __builtin_memcpy(&r, g.leaf_values + (size_t)e * RTC8X_LEAF_SIZE + off, sizeof(r));
+    return r;
+}
+
 // SampleFromVoxels<Acc,1,false>::operator() (util/SampleFromVoxels.h:419-460) with Floor (:58-65).
-// The 8 taps are fetched in two dependent rounds -- the 8 leaf-map entries, then the 8 voxel values,
-// each round issued together -- and combined with the reference's lerps in its order.
+// Fetches in two dependent rounds: the leaf-map entry of the stencil's block (plus the other blocks
+// only for the lanes whose stencil crosses a block face), then the voxels -- four 8-byte pair loads
+// when the stencil does not cross a z face of its block, else eight single loads.  The lerps are the
+// reference's, in its order.
 __device__ __forceinline__ float trilinear(const rtc8x_grid& g, V3 idx)
 {
     float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
@@ -160,22 +179,26 @@ __device__ __forceinline__ float trilinear(const rtc8x_grid& g, V3 idx)
     const int32_t bx0 = (i >> 3) - ob0, bx1 = ((i + 1) >> 3) - ob0;
     const int32_t by0 = (j >> 3) - ob1, by1 = ((j + 1) >> 3) - ob1;
     const int32_t bz0 = (k >> 3) - ob2, bz1 = ((k + 1) >> 3) - ob2;
+    const bool cross_x = (i & 7) == 7, cross_y = (j & 7) == 7, cross_z = (k & 7) == 7;
     uint32_t e000, e001, e010, e011, e100, e101, e110, e111;
     if (g.leaf_map && bx0 >= 0 && by0 >= 0 && bz0 >= 0 && bx1 < g.leaf_map_dims[0] && by1 < g.leaf_map_dims[1] &&
         bz1 < g.leaf_map_dims[2]) {
         const size_t d1 = (uint32_t)g.leaf_map_dims[1], d2 = (uint32_t)g.leaf_map_dims[2];
-        const uint32_t* m00 = g.leaf_map + ((size_t)bx0 * d1 + by0) * d2;
-        const uint32_t* m01 = g.leaf_map + ((size_t)bx0 * d1 + by1) * d2;
-        const uint32_t* m10 = g.leaf_map + ((size_t)bx1 * d1 + by0) * d2;
-        const uint32_t* m11 = g.leaf_map + ((size_t)bx1 * d1 + by1) * d2;
-        e000 = m00[bz0];
-        e001 = m00[bz1];
-        e010 = m01[bz0];
-        e011 = m01[bz1];
-        e100 = m10[bz0];
-        e101 = m10[bz1];
-        e110 = m11[bz0];
-        e111 = m11[bz1];
+        e000 = g.leaf_map[((size_t)bx0 * d1 + by0) * d2 + bz0];
+        e001 = e010 = e011 = e100 = e101 = e110 = e111 = e000;
+        if (cross_x || cross_y || cross_z) {
+            const uint32_t* m00 = g.leaf_map + ((size_t)bx0 * d1 + by0) * d2;
+            const uint32_t* m01 = g.leaf_map + ((size_t)bx0 * d1 + by1) * d2;
+            const uint32_t* m10 = g.leaf_map + ((size_t)bx1 * d1 + by0) * d2;
+            const uint32_t* m11 = g.leaf_map + ((size_t)bx1 * d1 + by1) * d2;
+            e001 = m00[bz1];
+            e010 = m01[bz0];
+            e011 = m01[bz1];
+            e100 = m10[bz0];
+            e101 = m10[bz1];
+            e110 = m11[bz0];
+            e111 = m11[bz1];
+        }
     } else {
         e000 = find_leaf(g, i, j, k);
         e001 = find_leaf(g, i, j, k + 1);
@@ -186,14 +209,30 @@ __device__ __forceinline__ float trilinear(const rtc8x_grid& g, V3 idx)
         e110 = find_leaf(g, i + 1, j + 1, k);
         e111 = find_leaf(g, i + 1, j + 1, k + 1);
     }
-    const float v000 = *voxel_ptr(g, e000, i, j, k);
-    const float v001 = *voxel_ptr(g, e001, i, j, k + 1);
-    const float v011 = *voxel_ptr(g, e011, i, j + 1, k + 1);
-    const float v010 = *voxel_ptr(g, e010, i, j + 1, k);
-    const float v110 = *voxel_ptr(g, e110, i + 1, j + 1, k);
-    const float v111 = *voxel_ptr(g, e111, i + 1, j + 1, k + 1);
-    const float v101 = *voxel_ptr(g, e101, i + 1, j, k + 1);
-    const float v100 = *voxel_ptr(g, e100, i + 1, j, k);
+    float v000, v001, v010, v011, v100, v101, v110, v111;
+    if (!cross_z) {
+        const F2 p00 = tap_pair(g, e000, leaf_offset(i, j, k));
+        const F2 p01 = tap_pair(g, e010, leaf_offset(i, j + 1, k));
+        const F2 p10 = tap_pair(g, e100, leaf_offset(i + 1, j, k));
+        const F2 p11 = tap_pair(g, e110, leaf_offset(i + 1, j + 1, k));
+        v000 = p00.x;
+        v001 = p00.y;
+        v010 = p01.x;
+        v011 = p01.y;
+        v100 = p10.x;
+        v101 = p10.y;
+        v110 = p11.x;
+        v111 = p11.y;
+    } else {
+        v000 = *voxel_ptr(g, e000, i, j, k);
+        v001 = *voxel_ptr(g, e001, i, j, k + 1);
+        v011 = *voxel_ptr(g, e011, i, j + 1, k + 1);
+        v010 = *voxel_ptr(g, e010, i, j + 1, k);
+        v110 = *voxel_ptr(g, e110, i + 1, j + 1, k);
+        v111 = *voxel_ptr(g, e111, i + 1, j + 1, k + 1);
+        v101 = *voxel_ptr(g, e101, i + 1, j, k + 1);
+        v100 = *voxel_ptr(g, e100, i + 1, j, k);
+    }
     float vy = v000 + z * (v001 - v000);
     float vy1 = v010 + z * (v011 - v010);
     float vx = vy + y * (vy1 - vy);
@@ -275,48 +314,84 @@ __device__ __forceinline__ float ratio_global(const rtc8x_grid& g, Medium& m, Pc
 }
 
 // 3D DDA over the 8^3 majorant cells in index space; boundaries recomputed from the cell index.
+// Per-axis state is held in named scalars and selected by axis (no indexed arrays, so it stays in
+// registers).
 struct Dda {
-    int32_t c[3], step[3];
-    float io[3], inv[3], tnext[3];
-    __device__ __forceinline__ float boundary(const rtc8x_grid& g, int a) const
+    int32_t c0, c1, c2;      // cell
+    int32_t s0, s1, s2;      // step per axis (-1, 0, +1)
+    float io0, io1, io2;     // ray origin in index space
+    float iv0, iv1, iv2;     // 1 / index-space direction
+    float tn0, tn1, tn2;     // ray parameter of the next cell boundary per axis
+    static __device__ __forceinline__ float self(int a, float x, float y, float z) { return a == 0 ? x : (a == 1 ? y : z); }
+    static __device__ __forceinline__ int32_t seli(int a, int32_t x, int32_t y, int32_t z)
     {
-        int32_t b = g.cell_origin[a] + 8 * (c[a] + (step[a] > 0 ? 1 : 0));
-        return ((float)b - io[a]) * inv[a];
+        return a == 0 ? x : (a == 1 ? y : z);
+    }
+    __device__ __forceinline__ int32_t cell(int a) const { return seli(a, c0, c1, c2); }
+    __device__ __forceinline__ float tnext(int a) const { return self(a, tn0, tn1, tn2); }
+    static __device__ __forceinline__ float boundary_of(int32_t origin, int32_t c, int32_t st, float io, float iv)
+    {
+        int32_t b = origin + 8 * (c + (st > 0 ? 1 : 0));
+        return ((float)b - io) * iv;
+    }
+    static __device__ __forceinline__ void axis_init(float t, float io, float id, int32_t origin, int32_t dims,
+                                                     int32_t& c, int32_t& st, float& iv, float& tn)
+    {
+        float pa = fmaf(t, id, io);
+        float cf = floorf((pa - (float)origin) * 0.125f);
+        c = cf < 0.0f ? 0 : (cf > (float)(dims - 1) ? dims - 1 : (int32_t)cf);
+        st = id > 0.0f ? 1 : (id < 0.0f ? -1 : 0);
+        iv = 1.0f / id;
+        tn = st != 0 ? boundary_of(origin, c, st, io, iv) : __builtin_huge_valf();
     }
     __device__ __forceinline__ void init(const rtc8x_grid& g, V3 o, V3 d, float t)
     {
         const float* mm = g.inv_mat;
         float x = o.x - g.vec[0], y = o.y - g.vec[1], z = o.z - g.vec[2];
-        io[0] = fmaf(x, mm[0], fmaf(y, mm[1], z * mm[2]));
-        io[1] = fmaf(x, mm[3], fmaf(y, mm[4], z * mm[5]));
-        io[2] = fmaf(x, mm[6], fmaf(y, mm[7], z * mm[8]));
-        float id[3];
-        id[0] = fmaf(d.x, mm[0], fmaf(d.y, mm[1], d.z * mm[2]));
-        id[1] = fmaf(d.x, mm[3], fmaf(d.y, mm[4], d.z * mm[5]));
-        id[2] = fmaf(d.x, mm[6], fmaf(d.y, mm[7], d.z * mm[8]));
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float pa = fmaf(t, id[a], io[a]);
-            float cf = floorf((pa - (float)g.cell_origin[a]) * 0.125f);
-            int32_t cc = cf < 0.0f ? 0 : (cf > (float)(g.cell_dims[a] - 1) ? g.cell_dims[a] - 1 : (int32_t)cf);
-            c[a] = cc;
-            step[a] = id[a] > 0.0f ? 1 : (id[a] < 0.0f ? -1 : 0);
-            inv[a] = 1.0f / id[a];
-            tnext[a] = step[a] != 0 ? boundary(g, a) : __builtin_huge_valf();
-        }
+        io0 = fmaf(x, mm[0], fmaf(y, mm[1], z * mm[2]));
+        io1 = fmaf(x, mm[3], fmaf(y, mm[4], z * mm[5]));
+        io2 = fmaf(x, mm[6], fmaf(y, mm[7], z * mm[8]));
+        const float id0 = fmaf(d.x, mm[0], fmaf(d.y, mm[1], d.z * mm[2]));
+        const float id1 = fmaf(d.x, mm[3], fmaf(d.y, mm[4], d.z * mm[5]));
+        const float id2 = fmaf(d.x, mm[6], fmaf(d.y, mm[7], d.z * mm[8]));
+        axis_init(t, io0, id0, g.cell_origin[0], g.cell_dims[0], c0, s0, iv0, tn0);
+        axis_init(t, io1, id1, g.cell_origin[1], g.cell_dims[1], c1, s1, iv1, tn1);
+        axis_init(t, io2, id2, g.cell_origin[2], g.cell_dims[2], c2, s2, iv2, tn2);
     }
     __device__ __forceinline__ int axis() const
     {
         int a = 0;
-        if (tnext[1] < tnext[a]) a = 1;
-        if (tnext[2] < tnext[a]) a = 2;
+        float best = tn0;
+        if (tn1 < best) {
+            a = 1;
+            best = tn1;
+        }
+        if (tn2 < best) a = 2;
         return a;
+    }
+    // Steps the cell along axis a; false when it leaves the cell grid.  Updates that axis' boundary.
+    // Written with value selects and unconditional stores: per-axis branches that each store a field
+    // get merged into one store through a selected field address, which keeps the state in memory.
+    __device__ __forceinline__ bool step(const rtc8x_grid& g, int a)
+    {
+        const int32_t st = seli(a, s0, s1, s2);
+        const int32_t c = seli(a, c0, c1, c2) + st;
+        if (c < 0 || c >= seli(a, g.cell_dims[0], g.cell_dims[1], g.cell_dims[2])) return false;
+        const float tn = boundary_of(seli(a, g.cell_origin[0], g.cell_origin[1], g.cell_origin[2]), c, st,
+                                     self(a, io0, io1, io2), self(a, iv0, iv1, iv2));
+        c0 = a == 0 ? c : c0;
+        c1 = a == 1 ? c : c1;
+        c2 = a == 2 ? c : c2;
+        tn0 = a == 0 ? tn : tn0;
+        tn1 = a == 1 ? tn : tn1;
+        tn2 = a == 2 ? tn : tn2;
+        return true;
     }
 };
 
 __device__ __forceinline__ float cell_majorant(const rtc8x_grid& g, const Medium& m, const Dda& s)
 {
-    size_t idx = ((size_t)s.c[0] * (size_t)g.cell_dims[1] + (size_t)s.c[1]) * (size_t)g.cell_dims[2] + (size_t)s.c[2];
+    size_t idx = ((size_t)s.c0 * (size_t)g.cell_dims[1] + (size_t)s.c1) * (size_t)g.cell_dims[2] + (size_t)s.c2;
     return m.coeff * g.cell_majorants[idx];
 }
 
@@ -329,7 +404,8 @@ __device__ __forceinline__ bool track_local(const rtc8x_grid& g, Medium& m, Pcg&
     float tau = -rtc8x_logf(1.0f - rng.uni());
     for (;;) {
         int a = s.axis();
-        float tend = s.tnext[a] < t1 ? s.tnext[a] : t1;
+        const float tna = s.tnext(a);
+        float tend = tna < t1 ? tna : t1;
         float mu = cell_majorant(g, m, s);
         if (mu > 0.0f) {
             for (;;) {
@@ -348,11 +424,9 @@ __device__ __forceinline__ bool track_local(const rtc8x_grid& g, Medium& m, Pcg&
                 tau = -rtc8x_logf(1.0f - rng.uni());
             }
         }
-        if (!(s.tnext[a] < t1)) return false;
+        if (!(tna < t1)) return false;
         t = tend;
-        s.c[a] += s.step[a];
-        if (s.c[a] < 0 || s.c[a] >= g.cell_dims[a]) return false;
-        s.tnext[a] = s.boundary(g, a);
+        if (!s.step(g, a)) return false;
     }
 }
 
@@ -365,7 +439,8 @@ __device__ __forceinline__ float ratio_local(const rtc8x_grid& g, Medium& m, Pcg
     float tau = -rtc8x_logf(1.0f - rng.uni());
     for (;;) {
         int a = s.axis();
-        float tend = s.tnext[a] < t1 ? s.tnext[a] : t1;
+        const float tna = s.tnext(a);
+        float tend = tna < t1 ? tna : t1;
         float mu = cell_majorant(g, m, s);
         if (mu > 0.0f) {
             for (;;) {
@@ -381,11 +456,9 @@ __device__ __forceinline__ float ratio_local(const rtc8x_grid& g, Medium& m, Pcg
                 tau = -rtc8x_logf(1.0f - rng.uni());
             }
         }
-        if (!(s.tnext[a] < t1)) break;
+        if (!(tna < t1)) break;
         t = tend;
-        s.c[a] += s.step[a];
-        if (s.c[a] < 0 || s.c[a] >= g.cell_dims[a]) break;
-        s.tnext[a] = s.boundary(g, a);
+        if (!s.step(g, a)) break;
     }
     return vis;
 }

@@ -27,8 +27,9 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h)
 
 // Each lane performs `steps` lookups.  A lookup reads the 8 corners of a 2x2x2 stencil at a random
 // position inside a random leaf (leaf voxel offset x<<6 | y<<3 | z, the NanoVDB leaf order), exactly
-// the 8 taps of a trilinear density lookup, and the next lookup's position depends on the values read,
-// so lookups are serially dependent while the 8 taps of one lookup are issued together.
+// the 8 taps of a trilinear density lookup, as four 8-byte loads of z-adjacent pairs (the fewest
+// memory instructions that fetch a stencil); the next lookup's position depends on the values read,
+// so lookups are serially dependent while the taps of one lookup are issued together.
 __global__ void gather_probe_kernel(const float* __restrict__ values, uint32_t num_leaves, uint32_t steps,
                                     uint32_t seed, float* __restrict__ out)
 {
@@ -41,7 +42,12 @@ __global__ void gather_probe_kernel(const float* __restrict__ values, uint32_t n
         const uint32_t r = mix32(h + 0x632be5abu);
         const uint32_t x = (r & 0xffu) % 7u, y = ((r >> 8) & 0xffu) % 7u, z = ((r >> 16) & 0xffu) % 7u;
         const float* b = values + (size_t)leaf * 512 + (x << 6 | y << 3 | z);
-        const float v = ((b[0] + b[1]) + (b[8] + b[9])) + ((b[64] + b[65]) + (b[72] + b[73]));
+        float2 q[4];
+        __builtin_memcpy(&q[0], b, 8);
+        __builtin_memcpy(&q[1], b + 8, 8);
+        __builtin_memcpy(&q[2], b + 64, 8);
+        __builtin_memcpy(&q[3], b + 72, 8);
+        const float v = ((q[0].x + q[0].y) + (q[1].x + q[1].y)) + ((q[2].x + q[2].y) + (q[3].x + q[3].y));
         acc += v;
         h = mix32(h ^ __float_as_uint(v) ^ s);
     }

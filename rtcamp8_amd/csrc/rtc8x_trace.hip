@@ -456,6 +456,9 @@ hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t 
     return hipGetLastError();
 }
 
+template <int TRACKING>
+hipError_t launch_wavefront(const rtc8x_frame_params& p, uint32_t num_samples, hipStream_t stream);
+
 template <int ENTRY, bool COUNT_ONLY>
 hipError_t launch_tracking(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t ns, const uint32_t* offsets,
                            uint32_t* counts, hipStream_t stream)
@@ -529,8 +532,10 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
     }
     hipStream_t s = (hipStream_t)stream;
     if (entry == RTC8X_ENTRY_PATH_TRACE) {
-        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE, false>(p, p.image_width, p.image_height, num_samples,
-                                                                        nullptr, nullptr, s)));
+        if (p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT)
+            RTC8X_HIP(dev::launch_wavefront<RTC8X_TRACKING_LOCAL_MAJORANT>(p, num_samples, s));
+        else
+            RTC8X_HIP(dev::launch_wavefront<RTC8X_TRACKING_GLOBAL_MAJORANT>(p, num_samples, s));
         return RTC8X_OK;
     }
     const rtc8x_nrc_buffers& nb = p.nrc;

@@ -96,6 +96,17 @@ def test_plain_tracer_sparse_constant_sky_bit_exact(tracking):
     assert res["vertices"] == int(res["cpu_stats"][_abi.STAT_VERTICES])
 
 
+@pytest.mark.parametrize("tracking", [0, 1])
+def test_plain_tracer_ragged_image_bit_exact(tracking):
+    """An image that is not a multiple of the 8x8 work tiles, several samples per launch."""
+    res = trace_parity_case("dense128", width=37, height=29, spp=4, tracking=tracking, fbm_resolution=64,
+                            samples_per_launch=2)
+    assert res["rng_equal"] and res["vertices_equal"]
+    assert np.array_equal(res["gpu_accum"].view(np.uint64), res["cpu_accum"].view(np.uint64))
+    assert res["paths"] == 37 * 29 * 4
+    assert res["lookups"] == int(res["cpu_stats"][2])
+
+
 def test_schlick_reference_mode_bit_exact():
     """Reference parity mode: Schlick phase, RR from the first vertex, unbounded paths, env only."""
     from rtcamp8_amd import _abi

@@ -53,6 +53,9 @@ int oracle_trace_f32(const rtc8x_frame_params* p, int32_t entry, uint32_t num_sa
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
 int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
+/* EnvironmentalLight::sample / evaluate (common_renderer_types.h:988-1015, 2419-2458) in FP64. */
+void oracle_env_sample_f64(const rtc8x_env_light* e, double u0, double u1, double dir[3], double* pdf, double rgb[3]);
+void oracle_env_evaluate_f64(const rtc8x_env_light* e, const double dir[3], double* pdf, double rgb[3]);
 
 
 /* NRC data path (renderer/gpu_kernels/nrc_set_up_kernels.cu) on host copies of params->nrc. */

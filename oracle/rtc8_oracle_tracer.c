@@ -69,3 +69,28 @@ int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_sa
 {
     return oracle_trace_impl_f64(p, entry, num_samples, y0, y1, nthreads);
 }
+
+/* Environment sampling hooks for the tests (FP64 instance). */
+void oracle_env_sample_f64(const rtc8x_env_light* e, double u0, double u1, double dir[3], double* pdf, double rgb[3])
+{
+    vec3_f64 d;
+    rgb_f64 c = env_sample_f64(e, u0, u1, &d, pdf);
+    dir[0] = d.x;
+    dir[1] = d.y;
+    dir[2] = d.z;
+    rgb[0] = c.r;
+    rgb[1] = c.g;
+    rgb[2] = c.b;
+}
+
+void oracle_env_evaluate_f64(const rtc8x_env_light* e, const double dir[3], double* pdf, double rgb[3])
+{
+    vec3_f64 d;
+    d.x = dir[0];
+    d.y = dir[1];
+    d.z = dir[2];
+    rgb_f64 c = env_evaluate_f64(e, d, pdf);
+    rgb[0] = c.r;
+    rgb[1] = c.g;
+    rgb[2] = c.b;
+}

@@ -6,7 +6,7 @@
 #include <cstdio>
 #include <string>
 
-#include "../../include/rtc8x.h"
+#include "rtc8x.h"
 
 namespace rtc8x {
 

@@ -9,8 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../../include/rtc8x.h"
-#include "../../include/rtc8x_detmath.h"
+#include "rtc8x.h"
+#include "rtc8x_detmath.h"
 
 namespace rtc8x {
 namespace dev {

@@ -51,6 +51,8 @@ def oracle() -> C.CDLL:
                                        C.c_uint32]),
         "oracle_trace_f64": (C.c_int, [C.POINTER(_abi.FrameParams), C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_uint32]),
+        "oracle_env_sample_f64": (None, [C.POINTER(_abi.EnvLight), C.c_double, C.c_double, _P, _P, _P]),
+        "oracle_env_evaluate_f64": (None, [C.POINTER(_abi.EnvLight), _P, _P, _P]),
         "oracle_nrc_prepare": (None, [C.POINTER(_abi.FrameParams), C.c_int32]),
         "oracle_nrc_propagate": (None, [C.POINTER(_abi.FrameParams)]),
         "oracle_nrc_shuffle": (None, [C.POINTER(_abi.FrameParams)]),

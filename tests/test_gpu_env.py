@@ -1,0 +1,129 @@
+"""Environment-light importance sampling (Hosek-Wilkie lat-long sky): the library's alias tables and
+hierarchical importance map against their definitions, sampled pdf against evaluated pdf, the sample
+histogram against the pdf, and the tracer with a textured sky bit for bit against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle_lib import download_env, oracle, ptr, trace_parity_case
+
+pytestmark = pytest.mark.gpu
+
+SKY_W, SKY_H = 256, 128
+
+
+def _scene(sampler):
+    from rtcamp8_amd import scene as sc
+
+    cfg = sc.CONFIGS["sparse512"].with_(width=32, height=16, sky_width=SKY_W, sky_height=SKY_H, env_sampler=sampler,
+                                        fbm=sc.FbmConfig(resolution=64, seed=0x5EED0002, sparse=True,
+                                                         active_leaf_fraction=0.25, contrast=4.0))
+    return sc.DeviceScene(cfg)
+
+
+def _alias_pmf(entries: np.ndarray, n: int) -> np.ndarray:
+    """The distribution an alias table draws from: P(i) = (pf_i + sum over j aliasing to i of (1 - pf_j)) / n."""
+    second = entries[0::2]
+    pf = entries[1::2].view(np.float32).astype(np.float64)
+    p = pf.copy()
+    np.add.at(p, second, 1.0 - pf)
+    return p / n
+
+
+def test_alias_tables_reproduce_the_texel_pmf():
+    from rtcamp8_amd import _abi
+
+    ds = _scene(_abi.ENV_SAMPLER_ALIAS)
+    try:
+        he, keep = download_env(ds.env)
+        w = keep["weights"].astype(np.float64).reshape(SKY_H, SKY_W)
+        assert np.all(w >= 0) and w.sum() > 0
+        assert abs(w.sum() - he.alias_integral) <= 1e-5 * w.sum()
+        rows = _alias_pmf(keep["rows"], SKY_H)
+        assert np.allclose(rows, w.sum(axis=1) / w.sum(), rtol=0, atol=1e-6 * rows.max())
+        cols = keep["cols"].reshape(SKY_H, 2 * SKY_W)
+        for y in range(0, SKY_H, 7):
+            if w[y].sum() == 0:
+                continue
+            p = _alias_pmf(cols[y], SKY_W)
+            assert np.allclose(p, w[y] / w[y].sum(), rtol=0, atol=1e-6 * p.max()), y
+        # texel weights: luminance * pi/2 * sin(theta) of the texel row centre (compute_light_probs.cu:165-178)
+        tex = keep["texels"].reshape(SKY_H, SKY_W, 4).astype(np.float64)
+        lum = 0.2126729 * tex[..., 0] + 0.7151522 * tex[..., 1] + 0.0721750 * tex[..., 2]
+        sin_t = np.sin(np.pi * (np.arange(SKY_H) + 0.5) / SKY_H)[:, None]
+        assert np.allclose(w, lum * (np.pi / 2) * sin_t, rtol=1e-5, atol=1e-30)
+    finally:
+        ds.close()
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_sampled_pdf_matches_evaluated_pdf(sampler):
+    ds = _scene(sampler)
+    try:
+        he, keep = download_env(ds.env)
+        rng = np.random.default_rng(30 + sampler)
+        n = 20000
+        u = rng.random((n, 2))
+        d, pdf, rgb = np.zeros(3), np.zeros(1), np.zeros(3)
+        pdf2, rgb2 = np.zeros(1), np.zeros(3)
+        # the alias sampler divides the same texel weight on both sides (exact); the hierarchical one
+        # multiplies ratios of FP32 box-averaged mips, equal to the texel's share only to FP32 rounding
+        rtol = 1e-12 if sampler == 1 else 2e-5
+        agree, total = 0, 0
+        for i in range(n):
+            oracle().oracle_env_sample_f64(C.byref(he), float(u[i, 0]), float(u[i, 1]), ptr(d), ptr(pdf), ptr(rgb))
+            if pdf[0] == 0:
+                continue
+            assert abs(np.linalg.norm(d) - 1.0) < 1e-12
+            oracle().oracle_env_evaluate_f64(C.byref(he), ptr(d), ptr(pdf2), ptr(rgb2))
+            total += 1
+            if abs(pdf2[0] - pdf[0]) <= rtol * pdf[0] and np.allclose(rgb, rgb2, rtol=1e-12, atol=0):
+                agree += 1
+        # a direction that lands exactly on a texel edge may map to the neighbouring texel
+        assert total > 0.99 * n and agree >= 0.999 * total, (agree, total)
+    finally:
+        ds.close()
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_sample_histogram_follows_the_pdf(sampler):
+    """Counts of sampled texel rows against the row marginal of the importance weights (chi-square)."""
+    from scipy.stats import chisquare
+
+    ds = _scene(sampler)
+    try:
+        he, keep = download_env(ds.env)
+        if sampler == 1:
+            w = keep["weights"].astype(np.float64).reshape(SKY_H, SKY_W)
+        else:
+            w = keep["mips"][: he.map_dim_x * he.map_dim_y].astype(np.float64).reshape(he.map_dim_y, he.map_dim_x)
+        marg = w.sum(axis=1) / w.sum()
+        rng = np.random.default_rng(40 + sampler)
+        n = 60000
+        u = rng.random((n, 2))
+        d, pdf, rgb = np.zeros(3), np.zeros(1), np.zeros(3)
+        rows = np.zeros(marg.size, np.int64)
+        for i in range(n):
+            oracle().oracle_env_sample_f64(C.byref(he), float(u[i, 0]), float(u[i, 1]), ptr(d), ptr(pdf), ptr(rgb))
+            theta = np.arccos(np.clip(d[1], -1, 1))
+            rows[min(int(theta / np.pi * marg.size), marg.size - 1)] += 1
+        exp = marg * n
+        keep_bins = exp >= 5
+        obs = np.append(rows[keep_bins], rows[~keep_bins].sum())
+        ex = np.append(exp[keep_bins], exp[~keep_bins].sum())
+        obs, ex = obs[ex > 0], ex[ex > 0]
+        ex *= obs.sum() / ex.sum()
+        assert chisquare(obs, ex).pvalue > 1e-4
+    finally:
+        ds.close()
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_tracer_with_textured_sky_bit_exact(sampler):
+    res = trace_parity_case("sparse512", width=40, height=24, spp=2, tracking=0, fbm_resolution=128,
+                            sky_width=SKY_W, sky_height=SKY_H, env_sampler=sampler, max_path_length=64,
+                            samples_per_launch=1)
+    assert res["rng_equal"] and res["vertices_equal"]
+    assert np.array_equal(res["gpu_accum"].view(np.uint64), res["cpu_accum"].view(np.uint64))
+    assert res["lookups"] == int(res["cpu_stats"][2])

@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nrc-steps", type=int, default=8, help="steps of each NRC leg (0 skips them)")
+    ap.add_argument("--rmse-ref-spp", type=int, default=256,
+                    help="spp of each half of the equal-RMSE reference (0 skips the equal-RMSE leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -233,6 +235,34 @@ def latency_roofline(scene, lookups_per_path: float, trace_lookups_per_s: float,
     return res
 
 
+def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
+    """Wall time to equal RMSE: NRC-accelerated rendering vs the plain tracer on the same frame.
+
+    The converged reference is the plain tracer at 2 x ref_spp (two independent halves A, B); the MSE
+    of an estimate X is mean((X - A)(X - B)), free of reference noise.  The plain tracer's MSE falls
+    as 1/spp, so it needs spp * mse_plain / mse_nrc samples to match the NRC image; its time for that
+    is scaled from the measured 16-spp launch."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd.convergence import cross_mse, nrc_image, plain_image
+
+    a, ta = plain_image(scene, ref_spp, seed=0xA11CE, stream=stream.handle)
+    b, _ = plain_image(scene, ref_spp, seed=0xB0B, stream=stream.handle)
+    x, tx = plain_image(scene, spp, seed=0xC0FFEE, stream=stream.handle)
+    mean = float((a + b).mean() / 2)
+    mse_plain = cross_mse(x, a, b)
+    res = {"reference": f"plain tracer, 2 x {ref_spp} spp with independent seeds (cross-product MSE)",
+           "spp": spp, "plain_seconds": tx, "plain_rel_rmse": float(np.sqrt(max(mse_plain, 0.0)) / mean)}
+    for name, prec in (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16)):
+        run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec)
+        mse = cross_mse(run.image, a, b)
+        n_star = spp * mse_plain / mse if mse > 0 else float("inf")
+        t_equal = tx * n_star / spp
+        res[name] = {"nrc_seconds": run.seconds, "nrc_rel_rmse": float(np.sqrt(max(mse, 0.0)) / mean),
+                     "plain_spp_for_equal_rmse": n_star, "plain_seconds_for_equal_rmse": t_equal,
+                     "speedup": t_equal / run.seconds}
+    return res
+
+
 def calibrate_copy(mib: int, steps: int) -> int:
     """HBM counter calibration: `steps` launches of the plain copy kernel over `mib` MiB."""
     from rtcamp8_amd._lib import call
@@ -360,6 +390,12 @@ def main() -> int:
         print(f"[bench] NRC render: {rend['ms_per_spp']:.2f} ms/spp", file=sys.stderr, flush=True)
         result["nrc_queries_per_s"] = mlp["queries_per_s"]
         result["nrc"] = {"network": mlp, "render": rend}
+    if args.rmse_ref_spp > 0:
+        print("[bench] equal-RMSE leg ...", file=sys.stderr, flush=True)
+        eq = equal_rmse_leg(scene, args.rmse_ref_spp, cfg.spp, stream)
+        result["equal_rmse"] = eq
+        print(f"[bench] equal RMSE: plain rel RMSE {eq['plain_rel_rmse']:.4f} at {cfg.spp} spp; NRC fp16 "
+              f"{eq['fp16']['nrc_rel_rmse']:.4f} -> speedup {eq['fp16']['speedup']:.2f}x", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)

@@ -190,6 +190,13 @@ class NrcRenderer:
         call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(cfg.rng_seed))
         self.accum.fill(0)
 
+    def reset(self, seed: int) -> None:
+        """Restarts the accumulation from new per-pixel seeds (the network keeps its training)."""
+        cfg = self.scene.cfg
+        call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(seed))
+        self.accum.fill(0)
+        self.samples = 0
+
     def params(self) -> _abi.FrameParams:
         p = self.scene.params()
         p.rng_states = self.rng.ptr

@@ -385,6 +385,13 @@ typedef struct rtc8x_frame_params {
     double* accum;                           /* per-pixel RGB sums, FP64 (accumBuffer) */
     uint32_t* path_vertices;                 /* optional per-pixel vertex count sums */
     uint64_t* stats;                         /* optional RTC8X_STAT_COUNT counters */
+    /* Optional scheduling (results do not depend on it): pixel_cost[i] receives the density lookups of
+     * launch pixel i (row-major over the launch) in this launch; work_order, a permutation of the
+     * launch's pixels, is the order in which lanes take them (see rtc8x_order_by_cost).  Without
+     * work_order lanes take 8x8 pixel tiles in order (GENERATE_TRAINING_DATA: the unbiased-suffix
+     * pixels first). */
+    uint32_t* pixel_cost;
+    const uint32_t* work_order;
     rtc8x_nrc_buffers nrc;
 } rtc8x_frame_params;
 
@@ -394,6 +401,13 @@ typedef struct rtc8x_frame_params {
  * must be 1. */
 rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t entry,
                          uint32_t num_samples);
+
+/* Work order for the next launch from the last launch's pixel_cost: pixels sorted by decreasing
+ * floor(log2(cost + 1)), row-major within a class (a stable counting sort on the device), so the
+ * longest paths start first.  No counterpart in the reference (its OptiX launch has no such control).
+ * scratch: rtc8x_order_scratch_bytes(n) bytes. */
+rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, uint32_t n, uint32_t* order, void* scratch);
+size_t rtc8x_order_scratch_bytes(uint32_t n);
 
 /* ------------------------------------------------------------------------------------------------
  * NRC data-path kernels (renderer/gpu_kernels/nrc_set_up_kernels.cu).  They read the same

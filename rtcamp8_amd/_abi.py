@@ -207,6 +207,8 @@ class FrameParams(C.Structure):
         ("accum", C.c_void_p),
         ("path_vertices", C.c_void_p),
         ("stats", C.c_void_p),
+        ("pixel_cost", C.c_void_p),
+        ("work_order", C.c_void_p),
         ("nrc", NrcBuffers),
     ]
 
@@ -311,6 +313,8 @@ PROTOTYPES = {
     "rtc8x_env_release": (_S, [C.POINTER(EnvLight)]),
     "rtc8x_camera_setup": (_S, [C.POINTER(Camera), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float]),
     "rtc8x_trace": (_S, [_P, C.POINTER(FrameParams), C.c_int32, C.c_uint32]),
+    "rtc8x_order_by_cost": (_S, [_P, _P, C.c_uint32, _P, _P]),
+    "rtc8x_order_scratch_bytes": (C.c_size_t, [C.c_uint32]),
     "rtc8x_seed_rng": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint64]),
     "rtc8x_sky_bake": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, _P]),
     "rtc8x_sky_eval": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, _P, _P, _P]),

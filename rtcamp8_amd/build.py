@@ -25,7 +25,6 @@ SOURCES = [
     "rtc8x_env.hip",
     "rtc8x_sky.hip",
     "rtc8x_trace.hip",
-    "rtc8x_wavefront.hip",
     "rtc8x_nrc_data.hip",
     "rtc8x_nrc_mlp.hip",
     "rtc8x_measure.hip",

@@ -484,6 +484,33 @@ __device__ __forceinline__ float transmittance(const rtc8x_grid& g, Medium& m, P
     return ratio_global(g, m, rng, o, d, t0, t1);
 }
 
+// RadianceQuery of a medium vertex (optix_pathtracing_kernels.cu:14-29): AABB-local position, the
+// reference's toPolarYUp(&phi, &theta) argument order (polar angle stored in *_phi), roughness 0,
+// diffuse reflectance = albedo.
+__device__ __forceinline__ void make_query(const rtc8x_frame_params& p, V3 pos, V3 vout, rtc8x_radiance_query* q)
+{
+    float pp[3] = {pos.x, pos.y, pos.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float num = pp[a] - p.world_aabb_min[a];
+        float den = p.world_aabb_max[a] - p.world_aabb_min[a];
+        q->position[a] = den != 0.0f ? num / den : 0.0f;
+    }
+    const float two_pi = 2.0f * RTC8X_PI_F;
+    float y = vout.y < -1.0f ? -1.0f : (vout.y > 1.0f ? 1.0f : vout.y);
+    float th = rtc8x_acosf(y);
+    float ph = rtc8x_fmodf(rtc8x_atan2f(-vout.x, vout.z) + two_pi, two_pi);
+    q->normal_phi = th;
+    q->normal_theta = ph;
+    q->vout_phi = th;
+    q->vout_theta = ph;
+    q->roughness = 0.0f;
+    q->diffuse_reflectance[0] = p.scattering_albedo;
+    q->diffuse_reflectance[1] = p.scattering_albedo;
+    q->diffuse_reflectance[2] = p.scattering_albedo;
+    q->specular_reflectance[0] = q->specular_reflectance[1] = q->specular_reflectance[2] = 0.0f;
+}
+
 // -------------------------------------------------------------- phase functions
 // SchlickPhaseFunction (common_renderer_types.h:2063-2097) and Henyey-Greenstein.
 __device__ __forceinline__ float phase_value(int kind, float k, float cos_t)

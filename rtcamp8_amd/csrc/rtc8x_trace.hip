@@ -1,13 +1,22 @@
-// rtc8x_trace.hip -- the path-tracing launch (replaces the three OptiX ray-generation programs,
-// renderer/gpu_kernels/optix_pathtracing_kernels.cu:31-448, for volume-only scenes).
+// rtc8x_trace.hip -- the path-tracing launch: the three OptiX ray-generation programs
+// (pathTrace, generateTrainingData, pathTraceWithNRC; renderer/gpu_kernels/optix_pathtracing_kernels.cu
+// :31-448) for volume-only scenes, as one persistent per-lookup state machine.
 //
-// One lane traces one pixel's paths.  A 256-lane workgroup covers a 16x16 pixel tile (a 64-lane
-// wavefront is a 16x4 strip), and workgroup ids are remapped so that the tiles one XCD receives are
-// contiguous in the image (the dispatcher deals workgroups round-robin over the 8 XCDs, each with
-// its own L2): neighbouring tiles then share that XCD's L2 for the grid's upper/lower tables and
-// the leaves near the camera.
+// A lane owns one pixel at a time (all of its samples, so the pixel's PCG stream is consumed in the
+// reference's order) and steps its path through free-flight tracking towards the next scattering
+// vertex, ratio tracking of the NEE shadow ray, and vertex events (camera ray, escape, scattering with
+// Russian roulette and NRC termination, NEE set-up, phase sampling, training records, path end).  Each
+// iteration of the main loop takes one tentative collision, with its density lookup, for every lane
+// that is stepping a segment; a lane that reaches a vertex event parks, and the parked lanes run their
+// events together once they are half of the wavefront's active lanes, so that each event's code is
+// issued once for many lanes.  A lane whose pixel is done takes the next one from a work counter
+// (8x8 pixel tiles or a caller-given order, one atomic per wavefront).  Per pixel, the random numbers
+// and floating-point operations are the reference's, so results are bit-identical to tracing each
+// pixel's paths one after another.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <cmath>
 #include <cstring>
 
 #include "rtc8x_common.hpp"
@@ -17,406 +26,570 @@ namespace rtc8x {
 namespace dev {
 
 constexpr uint32_t kHardMaxPathLength = 4096u;
-constexpr uint32_t kTileW = 16, kTileH = 16;
+constexpr uint32_t kCounterSlots = 1024;
+__device__ uint32_t g_work_counter[kCounterSlots];
 
-struct PathCtx {
-    Medium med;
-    float prob_env;
-    uint32_t scatters;
+__global__ void reset_counter_kernel(uint32_t slot) { g_work_counter[slot] = 0u; }
+
+template <int TRACKING>
+struct Seg;
+
+// Global majorant, the reference's loops (optix_pathtracing_kernels.cu:110-125,
+// renderer_kernel_common.h:371-382).
+template <>
+struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
+    V3 o, d;
+    float t, t1;
+    __device__ __forceinline__ bool init(const rtc8x_grid& g, const Medium& m, V3 ro, V3 rd, float tmax)
+    {
+        float t0;
+        if (!(m.majorant > 0.0f)) return false;
+        if (!clip(g, ro, rd, tmax, t0, t1)) return false;
+        t = fmaxf(0.0f, t0);
+        o = ro;
+        d = rd;
+        return true;
+    }
+    __device__ __forceinline__ bool advance(const rtc8x_grid&, const Medium& m, Pcg& rng, float& mu)
+    {
+        t += -rtc8x_logf(1.0f - rng.uni()) / m.majorant;
+        if (t > t1) return false;
+        mu = m.majorant;
+        return true;
+    }
+    // returns true when the tentative collision is real
+    __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu) { return rng.uni() < sigma / mu; }
+    __device__ __forceinline__ void after_shadow(Pcg&, float sigma, float mu, float& vis) { vis *= 1.0f - sigma / mu; }
 };
 
-// performNextEventEstimation (renderer_kernel_common.h:297-403), environment at infinity + delta sun.
-template <int TRACKING>
-__device__ __forceinline__ RGB nee(const rtc8x_frame_params& p, PathCtx& c, Pcg& rng, V3 pos, const Frame& fr)
-{
-    const RGB zero{0.0f, 0.0f, 0.0f};
-    float u_light = rng.uni();
-    const bool have_env = p.env.kind != RTC8X_ENV_NONE, have_sun = p.sun.enabled != 0;
-    bool use_sun = false;
-    float prob_cur = 1.0f;
-    if (have_env && have_sun) {
-        if (u_light < p.sun.select_prob) {
-            use_sun = true;
-            prob_cur = p.sun.select_prob;
-        } else {
-            prob_cur = 1.0f - p.sun.select_prob;
+// Local majorants over 8^3 cells, the loops of track_local / ratio_local made resumable.
+template <>
+struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
+    V3 o, d;
+    float t, t1, tau, mu_cell, tend;
+    int32_t axis, in_cell;
+    Dda s;
+    __device__ __forceinline__ bool init(const rtc8x_grid& g, const Medium& m, V3 ro, V3 rd, float tmax)
+    {
+        float t0;
+        if (!(m.majorant > 0.0f)) return false;
+        if (!clip(g, ro, rd, tmax, t0, t1)) return false;
+        t = fmaxf(0.0f, t0);
+        o = ro;
+        d = rd;
+        s.init(g, ro, rd, t);
+        in_cell = 0;
+        return true;
+    }
+    // first call after init draws tau (as track_local does right after the DDA set-up)
+    __device__ __forceinline__ void first_tau(Pcg& rng) { tau = -rtc8x_logf(1.0f - rng.uni()); }
+    __device__ __forceinline__ bool advance(const rtc8x_grid& g, const Medium& m, Pcg&, float& mu)
+    {
+        for (;;) {
+            if (!in_cell) {
+                axis = s.axis();
+                const float tna = s.tnext(axis);
+                tend = tna < t1 ? tna : t1;
+                mu_cell = cell_majorant(g, m, s);
+                in_cell = 1;
+            }
+            if (mu_cell > 0.0f) {
+                float tc = t + tau / mu_cell;
+                if (tc < tend) {
+                    t = tc;
+                    mu = mu_cell;
+                    return true;
+                }
+                tau = tau - mu_cell * (tend - t);
+                if (tau < 0.0f) tau = 0.0f;
+            }
+            if (!(s.tnext(axis) < t1)) return false;
+            t = tend;
+            if (!s.step(g, axis)) return false;
+            in_cell = 0;
         }
-    } else if (have_sun) {
-        use_sun = true;
     }
-    float u0 = rng.uni();
-    float u1 = rng.uni();
-    if (!have_env && !have_sun) return zero;
-
-    V3 lpos;
-    RGB emittance;
-    float area_pd = 0.0f;
-    if (use_sun) {
-        lpos = V3{p.sun.direction[0], p.sun.direction[1], p.sun.direction[2]};
-        emittance = RGB{p.sun.irradiance[0], p.sun.irradiance[1], p.sun.irradiance[2]};
-    } else {
-        V3 dir;
-        float dir_pd;
-        RGB radiance = env_sample(p.env, u0, u1, dir, dir_pd);
-        emittance = scale(radiance, RTC8X_PI_F);
-        lpos = dir;
-        area_pd = dir_pd;
-        if (area_pd == 0.0f) return zero;
-        area_pd *= prob_cur;
+    __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu)
+    {
+        if (rng.uni() < sigma / mu) return true;
+        tau = -rtc8x_logf(1.0f - rng.uni());
+        return false;
     }
-    V3 lnormal{-lpos.x, -lpos.y, -lpos.z};
-    V3 sdir{-lpos.x, -lpos.y, -lpos.z};
-    float lp_cos = dot(sdir, lnormal);
-    if (lp_cos <= 0.0f) return zero;
-    float trace_len = 2.0f * p.world_radius;
-    V3 lpoint{pos.x + trace_len * -sdir.x, pos.y + trace_len * -sdir.y, pos.z + trace_len * -sdir.z};
-    float vis = transmittance<TRACKING>(p.grid, c.med, rng, lpoint, sdir, trace_len);
-    if (!(vis > 0.0f)) return zero;
-    V3 vin = to_local(fr, V3{-sdir.x, -sdir.y, -sdir.z});
-    float pval = phase_value(p.phase, p.scattering_forwardness, -vin.z);
-    float alb = p.scattering_albedo;
-    RGB fs{alb * pval, alb * pval, alb * pval};
-    if (use_sun) return scale(mul(fs, emittance), vis / prob_cur);
-    float bsdf_pd = pval * lp_cos / 1.0f;
-    if (!isfinite(bsdf_pd)) bsdf_pd = 0.0f;
-    float light_pd = area_pd;
-    float mis = light_pd * light_pd / (bsdf_pd * bsdf_pd + light_pd * light_pd);
-    RGB le = scale(emittance, 1.0f / RTC8X_PI_F);
-    float g_term = lp_cos * 1.0f / 1.0f;
-    return scale(mul(fs, le), vis * mis * g_term / area_pd);
-}
+    __device__ __forceinline__ void after_shadow(Pcg& rng, float sigma, float mu, float& vis)
+    {
+        vis *= 1.0f - sigma / mu;
+        tau = -rtc8x_logf(1.0f - rng.uni());
+    }
+};
 
-// RadianceQuery of a medium vertex (optix_pathtracing_kernels.cu:14-29)
-__device__ __forceinline__ void make_query(const rtc8x_frame_params& p, V3 pos, V3 vout, rtc8x_radiance_query* q)
+enum : uint32_t {
+    kModeWork = 0,      // needs a pixel
+    kModeTrack = 1,     // free-flight tracking segment in progress
+    kModeShadow = 2,    // NEE ratio-tracking segment in progress
+    kModeIdle = 3,      // no work left
+    kModeStart = 16,    // start a path (camera ray)
+    kModeEscape = 33,   // tracking segment ended without a collision
+    kModeNeeDone = 34,  // shadow segment ended
+    kModeScatter = 49   // real collision at sg.t
+};
+
+struct Lane {
+    uint32_t lx, ly, lin, sample, cost0;
+    Pcg rng;
+    V3 org, dir, pos;
+    RGB tp, contribution, nee_f;
+    float dir_pd, init_importance, vis, nee_k1, nee_k2, nee_k3;
+    uint32_t path_length, mode;
+    bool nee_sun;
+    // NRC state (optix_pathtracing_kernels.cu:57-75, 200-214, 358-410)
+    bool unbiased;
+    float primary_spread, cur_sqrt_spread;
+    uint32_t prev_idx, records, train_base;
+    RGB prev_local_tp, nee_c;
+};
+
+template <int TRACKING>
+__device__ __forceinline__ bool start_segment(const rtc8x_grid& g, const Medium& med, Seg<TRACKING>& sg, Pcg& rng, V3 o,
+                                              V3 d, float tmax)
 {
-    float pp[3] = {pos.x, pos.y, pos.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float num = pp[a] - p.world_aabb_min[a];
-        float den = p.world_aabb_max[a] - p.world_aabb_min[a];
-        q->position[a] = den != 0.0f ? num / den : 0.0f;
-    }
-    const float two_pi = 2.0f * RTC8X_PI_F;
-    float y = vout.y < -1.0f ? -1.0f : (vout.y > 1.0f ? 1.0f : vout.y);
-    float th = rtc8x_acosf(y);
-    float ph = rtc8x_fmodf(rtc8x_atan2f(-vout.x, vout.z) + two_pi, two_pi);
-    q->normal_phi = th;
-    q->normal_theta = ph;
-    q->vout_phi = th;
-    q->vout_theta = ph;
-    q->roughness = 0.0f;
-    q->diffuse_reflectance[0] = p.scattering_albedo;
-    q->diffuse_reflectance[1] = p.scattering_albedo;
-    q->diffuse_reflectance[2] = p.scattering_albedo;
-    q->specular_reflectance[0] = q->specular_reflectance[1] = q->specular_reflectance[2] = 0.0f;
+    if (!sg.init(g, med, o, d, tmax)) return false;
+    if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) sg.first_tau(rng);
+    return true;
 }
 
-// One path of pathTrace_generic<withNRC, genData> (optix_pathtracing_kernels.cu:31-434).
-// GEN_DATA paths take training-record indices [train_base, train_base + n) (pixel order); with
-// COUNT_ONLY nothing is written and the number of record attempts is returned.
+// pathTrace_generic<withNRC, genData> (optix_pathtracing_kernels.cu:31-434) for volume-only scenes, as
+// a per-lookup state machine over a persistent grid.  ENTRY selects the ray-generation program;
+// COUNT_ONLY (GENERATE_TRAINING_DATA) traces the paths without writing anything and stores each
+// pixel's number of training-record attempts, from which the records' indices are assigned in pixel
+// order (record_offsets) before the writing pass.
 template <int ENTRY, int TRACKING, bool COUNT_ONLY>
-__device__ uint32_t trace_path(const rtc8x_frame_params& p, PathCtx& c, uint32_t lx, uint32_t ly, uint32_t launch_w,
-                               uint32_t launch_h, Pcg& rng, uint32_t train_base, RGB& contribution_out,
-                               uint32_t& path_length_out)
+__global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
+                                                        uint32_t launch_h, uint32_t num_samples, uint32_t slot,
+                                                        const uint32_t* __restrict__ record_offsets,
+                                                        uint32_t* __restrict__ record_counts)
 {
     constexpr bool with_nrc = ENTRY != RTC8X_ENTRY_PATH_TRACE;
     constexpr bool gen_data = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA;
     const rtc8x_nrc_buffers& nb = p.nrc;
-    const uint32_t linear = ly * launch_w + lx;
-
-    // camera ray (optix_pathtracing_kernels.cu:43-55)
-    float px = ((float)lx + rng.uni()) / (float)launch_w;
-    float py = ((float)ly + rng.uni()) / (float)launch_h;
-    const float vh = p.camera.vh, vw = p.camera.vw;
-    const float sensor_area = vw * vh;
-    V3 org{p.camera.position[0], p.camera.position[1], p.camera.position[2]};
-    V3 ldir = normalize(V3{vw * (-0.5f + px), vh * (0.5f - py), -1.0f});
-    const float* M = p.camera.orientation;
-    V3 dir{dot(V3{M[0], M[1], M[2]}, ldir), dot(V3{M[3], M[4], M[5]}, ldir), dot(V3{M[6], M[7], M[8]}, ldir)};
-    dir = normalize(dir);
-    float az = ldir.z < 0.0f ? -ldir.z : ldir.z;
-    float dir_pd = 1.0f / (az * (az * az) * sensor_area);
-
-    bool unbiased = false;
-    uint32_t prev_idx = RTC8X_INVALID_VERTEX_INDEX;
-    RGB prev_local_tp{0.0f, 0.0f, 0.0f};
-    uint32_t records = 0;
-    if (gen_data) {
-        uint32_t tl = (ly % 4u) * 4u + (lx % 4u);
-        unbiased = ((tl + nb.offset_to_select_unbiased_path) % 16u) == 0;
-    }
-
-    RGB contribution{0.0f, 0.0f, 0.0f};
-    float rinv = 1.0f / dir_pd;
-    RGB tp{rinv, rinv, rinv};
-    const float init_importance = luminance(tp);
-    uint32_t path_length = 0;
-    float primary_spread = 0.0f, cur_sqrt_spread = 0.0f;
-    uint32_t rr_after = p.rr_after_path_length;
-    if (gen_data && rr_after < 2) rr_after = 2;
+    const uint32_t W = launch_w, H = launch_h;
+    const uint32_t tiles_x = (W + 7) / 8;
+    // GENERATE_TRAINING_DATA hands out the unbiased-suffix pixels first (the one pixel of each 4x4 block
+    // with ((ly % 4) * 4 + lx % 4 + offset) % 16 == 0): their paths run to full length, so starting
+    // them first keeps the launch from waiting on them at the end.
+    const uint32_t first_c = (16u - nb.offset_to_select_unbiased_path % 16u) % 16u;
+    const uint32_t first_rx = first_c % 4u, first_ry = first_c / 4u;
+    const uint32_t first_nx = W > first_rx ? (W - first_rx + 3u) / 4u : 0u;
+    const uint32_t first_ny = H > first_ry ? (H - first_ry + 3u) / 4u : 0u;
+    const uint32_t n_first = (gen_data && !p.work_order) ? first_nx * first_ny : 0u;
+    const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + 7) / 8) * 64;
+    const uint32_t lane_id = threadIdx.x & 63;
     uint32_t max_len = (ENTRY == RTC8X_ENTRY_PATH_TRACE && p.max_path_length) ? p.max_path_length : kHardMaxPathLength;
     if (max_len > kHardMaxPathLength) max_len = kHardMaxPathLength;
+    uint32_t rr_after = p.rr_after_path_length;
+    if (gen_data && rr_after < 2) rr_after = 2;
+    const float prob_env = (p.sun.enabled && p.env.kind != RTC8X_ENV_NONE) ? 1.0f - p.sun.select_prob : 1.0f;
 
+    Medium med;
+    med.coeff = p.density_coeff;
+    med.majorant = p.density_coeff * p.grid.max_value;
+    med.lookups = 0;
+    uint32_t paths = 0, vertices = 0, scatters = 0;
+
+    Lane L;
+    L.mode = kModeWork;
+    L.lin = 0;
+    Seg<TRACKING> sg;
+    float mu = 0.0f;
     for (;;) {
-        ++path_length;
-        float t_hit = 1e10f;
-        bool vol = free_flight<TRACKING>(p.grid, c.med, rng, org, dir, 1e10f, t_hit);
-        if (!vol) {
-            // escaped to the environment (optix_pathtracing_kernels.cu:130-174)
-            if (p.env.kind != RTC8X_ENV_NONE) {
-                float hyp_pd;
-                RGB radiance = env_evaluate(p.env, dir, hyp_pd);
-                float mis = 1.0f;
-                if (path_length > 1) {
-                    float light_pd = c.prob_env * hyp_pd;
-                    float bsdf_pd = dir_pd;
-                    mis = bsdf_pd * bsdf_pd / (bsdf_pd * bsdf_pd + light_pd * light_pd);
+        // Vertex events are batched: a lane that reaches one parks until half of the wavefront's active
+        // lanes have, then all of them run their events together (each event block is issued once for
+        // many lanes instead of once per lane), while the others keep stepping through lookups.
+        const uint64_t parked = __ballot(L.mode != kModeTrack && L.mode != kModeShadow && L.mode != kModeIdle);
+        const uint64_t stepping = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
+        if (parked == 0 && stepping == 0) break;
+        const bool event_pass = stepping == 0 || 2 * __popcll(parked) >= __popcll(parked | stepping);
+        // ---- events until this lane has a tracking or shadow segment under way, or no work left
+        while (event_pass && L.mode != kModeIdle && L.mode != kModeTrack && L.mode != kModeShadow) {
+            if (L.mode == kModeWork) {
+                // the next pixel from the work counter, one atomic per wavefront
+                const uint64_t ask = __ballot(1);
+                const uint32_t leader = (uint32_t)__ffsll((long long)ask) - 1u;
+                uint32_t base = 0;
+                if (lane_id == leader) base = atomicAdd(&g_work_counter[slot], (uint32_t)__popcll(ask));
+                base = __shfl(base, (int)leader, 64);
+                const uint32_t w = base + (uint32_t)__popcll(ask & ((1ull << lane_id) - 1ull));
+                if (w >= total) {
+                    L.mode = kModeIdle;
+                    continue;
                 }
-                RGB implicit_cont = scale(radiance, mis);
-                if (!gen_data) contribution = add(contribution, mul(tp, implicit_cont));
-                if (gen_data && !COUNT_ONLY && prev_idx != RTC8X_INVALID_VERTEX_INDEX) {
-                    float* tgt = nb.train_targets[0] + 3 * (size_t)prev_idx;
-                    RGB a = mul(prev_local_tp, implicit_cont);
-                    tgt[0] = tgt[0] + a.r;
-                    tgt[1] = tgt[1] + a.g;
-                    tgt[2] = tgt[2] + a.b;
+                if (p.work_order) {
+                    const uint32_t lin = p.work_order[w];
+                    if (lin >= W * H) continue;
+                    L.lx = lin % W;
+                    L.ly = lin / W;
+                } else if (gen_data && w < n_first) {
+                    L.lx = 4 * (w % first_nx) + first_rx;
+                    L.ly = 4 * (w / first_nx) + first_ry;
+                } else {
+                    const uint32_t wt = gen_data ? w - n_first : w;
+                    const uint32_t tile = wt >> 6, within = wt & 63u;
+                    L.lx = (tile % tiles_x) * 8 + (within & 7u);
+                    L.ly = (tile / tiles_x) * 8 + (within >> 3);
+                    if (L.lx >= W || L.ly >= H) continue;
+                    if (gen_data && (L.lx & 3u) == first_rx && (L.ly & 3u) == first_ry) continue;  // taken first
                 }
+                L.lin = L.ly * W + L.lx;
+                L.cost0 = med.lookups;
+                L.rng.state = p.rng_states[(size_t)L.ly * p.image_width + L.lx];
+                L.sample = 0;
+                if (gen_data) L.train_base = record_offsets ? record_offsets[L.lin] : 0u;
+                L.mode = kModeStart;
             }
-            if (!COUNT_ONLY) {
+            if (L.mode == kModeStart) {
+                // camera ray (optix_pathtracing_kernels.cu:43-55)
+                float px = ((float)L.lx + L.rng.uni()) / (float)W;
+                float py = ((float)L.ly + L.rng.uni()) / (float)H;
+                const float vh = p.camera.vh, vw = p.camera.vw;
+                const float sensor_area = vw * vh;
+                L.org = V3{p.camera.position[0], p.camera.position[1], p.camera.position[2]};
+                V3 ldir = normalize(V3{vw * (-0.5f + px), vh * (0.5f - py), -1.0f});
+                const float* M = p.camera.orientation;
+                V3 dir{dot(V3{M[0], M[1], M[2]}, ldir), dot(V3{M[3], M[4], M[5]}, ldir), dot(V3{M[6], M[7], M[8]}, ldir)};
+                L.dir = normalize(dir);
+                float az = ldir.z < 0.0f ? -ldir.z : ldir.z;
+                L.dir_pd = 1.0f / (az * (az * az) * sensor_area);
+                L.contribution = RGB{0.0f, 0.0f, 0.0f};
+                float rinv = 1.0f / L.dir_pd;
+                L.tp = RGB{rinv, rinv, rinv};
+                L.init_importance = luminance(L.tp);
+                L.path_length = 1;
+                if (with_nrc) {
+                    L.primary_spread = 0.0f;
+                    L.cur_sqrt_spread = 0.0f;
+                }
                 if (gen_data) {
-                    rtc8x_training_suffix_terminal ti;
-                    ti.prev_vertex_index = prev_idx;
-                    ti.has_query = 0;
-                    ti.path_length = path_length & 0xFFu;
-                    ti.is_unbiased = unbiased ? 1u : 0u;
-                    nb.train_suffix_terminals[linear] = ti;
-                } else if (with_nrc) {
-                    rtc8x_terminal_info ti;
-                    ti.throughput[0] = tp.r;
-                    ti.throughput[1] = tp.g;
-                    ti.throughput[2] = tp.b;
-                    ti.has_query = (path_length & 0xFFu) << 8;
-                    nb.inference_terminals[linear] = ti;
+                    const uint32_t tl = (L.ly % 4u) * 4u + (L.lx % 4u);
+                    L.unbiased = ((tl + nb.offset_to_select_unbiased_path) % 16u) == 0;
+                    L.prev_idx = RTC8X_INVALID_VERTEX_INDEX;
+                    L.prev_local_tp = RGB{0.0f, 0.0f, 0.0f};
+                    L.records = 0;
                 }
+                L.mode = start_segment<TRACKING>(p.grid, med, sg, L.rng, L.org, L.dir, 1e10f) ? kModeTrack : kModeEscape;
             }
-            break;
-        }
-
-        // medium interaction (optix_pathtracing_kernels.cu:176-182)
-        c.scatters++;
-        V3 pos{org.x + t_hit * dir.x, org.y + t_hit * dir.y, org.z + t_hit * dir.z};
-        V3 vout{-dir.x, -dir.y, -dir.z};
-        Frame fr = make_frame(vout);
-
-        if (with_nrc) {
-            // path spread (optix_pathtracing_kernels.cu:200-210)
-            float ddx = pos.x - org.x, ddy = pos.y - org.y, ddz = pos.z - org.z;
-            float dist2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (path_length == 1)
-                primary_spread = dist2 / (4.0f * RTC8X_PI_F * 1.0f);
-            else
-                cur_sqrt_spread += sqrtf(dist2 / (dir_pd * 1.0f));
-        }
-
-        // Russian roulette (optix_pathtracing_kernels.cu:254-268)
-        float continue_prob = 1.0f;
-        bool rr_terminated = false;
-        if (path_length > rr_after) {
-            float q = luminance(tp) / init_importance;
-            continue_prob = fminf(q, 1.0f);
-            if (rng.uni() >= continue_prob) rr_terminated = true;
-        }
-        if (!with_nrc && rr_terminated) break;
-
-        if (with_nrc) {
-            bool ends_with_cache = rr_terminated;
-            const float pt = nb.path_termination_factor;
-            if (gen_data) {
-                if (!unbiased) ends_with_cache |= cur_sqrt_spread * cur_sqrt_spread > 2.0f * pt * primary_spread;
-            } else {
-                ends_with_cache |= cur_sqrt_spread * cur_sqrt_spread > pt * primary_spread;
-            }
-            if (ends_with_cache) {
-                if (!COUNT_ONLY) {
-                    make_query(p, pos, vout, &nb.inference_queries[linear]);
-                    if (gen_data) {
-                        rtc8x_training_suffix_terminal ti;
-                        ti.prev_vertex_index = prev_idx;
-                        ti.has_query = 1;
-                        ti.path_length = path_length & 0xFFu;
-                        ti.is_unbiased = unbiased ? 1u : 0u;
-                        nb.train_suffix_terminals[linear] = ti;
-                    } else {
-                        rtc8x_terminal_info ti;
-                        ti.throughput[0] = tp.r;
-                        ti.throughput[1] = tp.g;
-                        ti.throughput[2] = tp.b;
-                        ti.has_query = 1u | ((path_length & 0xFFu) << 8);
-                        nb.inference_terminals[linear] = ti;
+            bool after_nee = false, path_end = false;
+            if (L.mode == kModeEscape) {
+                // escaped to the environment (optix_pathtracing_kernels.cu:130-174)
+                if (p.env.kind != RTC8X_ENV_NONE) {
+                    float hyp_pd;
+                    RGB radiance = env_evaluate(p.env, L.dir, hyp_pd);
+                    float mis = 1.0f;
+                    if (L.path_length > 1) {
+                        float light_pd = prob_env * hyp_pd;
+                        float bsdf_pd = L.dir_pd;
+                        mis = bsdf_pd * bsdf_pd / (bsdf_pd * bsdf_pd + light_pd * light_pd);
+                    }
+                    const RGB implicit_cont = scale(radiance, mis);
+                    if (!gen_data) L.contribution = add(L.contribution, mul(L.tp, implicit_cont));
+                    if (gen_data && !COUNT_ONLY && L.prev_idx != RTC8X_INVALID_VERTEX_INDEX) {
+                        float* tgt = nb.train_targets[0] + 3 * (size_t)L.prev_idx;
+                        const RGB a = mul(L.prev_local_tp, implicit_cont);
+                        tgt[0] = tgt[0] + a.r;
+                        tgt[1] = tgt[1] + a.g;
+                        tgt[2] = tgt[2] + a.b;
                     }
                 }
-                break;
+                if (!COUNT_ONLY) {
+                    if (gen_data) {
+                        rtc8x_training_suffix_terminal ti;
+                        ti.prev_vertex_index = L.prev_idx;
+                        ti.has_query = 0;
+                        ti.path_length = L.path_length & 0xFFu;
+                        ti.is_unbiased = L.unbiased ? 1u : 0u;
+                        nb.train_suffix_terminals[L.lin] = ti;
+                    } else if (with_nrc) {
+                        rtc8x_terminal_info ti;
+                        ti.throughput[0] = L.tp.r;
+                        ti.throughput[1] = L.tp.g;
+                        ti.throughput[2] = L.tp.b;
+                        ti.has_query = (L.path_length & 0xFFu) << 8;
+                        nb.inference_terminals[L.lin] = ti;
+                    }
+                }
+                path_end = true;
+            } else if (L.mode == kModeNeeDone) {
+                // end of the NEE shadow ray (performNextEventEstimation, renderer_kernel_common.h:297-403)
+                RGB nee_c{0.0f, 0.0f, 0.0f};
+                if (L.vis > 0.0f)
+                    nee_c = L.nee_sun ? scale(L.nee_f, L.vis / L.nee_k1)
+                                      : scale(L.nee_f, L.vis * L.nee_k1 * L.nee_k2 / L.nee_k3);
+                if (!gen_data) L.contribution = add(L.contribution, mul(L.tp, nee_c));
+                if (gen_data) L.nee_c = nee_c;
+                after_nee = true;
+            } else if (L.mode == kModeScatter) {
+                // medium interaction at sg.t (optix_pathtracing_kernels.cu:176-268)
+                scatters++;
+                const float th = sg.t;
+                L.pos = V3{L.org.x + th * L.dir.x, L.org.y + th * L.dir.y, L.org.z + th * L.dir.z};
+                if (with_nrc) {
+                    // path spread (optix_pathtracing_kernels.cu:200-210)
+                    float ddx = L.pos.x - L.org.x, ddy = L.pos.y - L.org.y, ddz = L.pos.z - L.org.z;
+                    float dist2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (L.path_length == 1)
+                        L.primary_spread = dist2 / (4.0f * RTC8X_PI_F * 1.0f);
+                    else
+                        L.cur_sqrt_spread += sqrtf(dist2 / (L.dir_pd * 1.0f));
+                }
+                // Russian roulette (optix_pathtracing_kernels.cu:254-268)
+                float continue_prob = 1.0f;
+                bool rr_terminated = false;
+                if (L.path_length > rr_after) {
+                    float q = luminance(L.tp) / L.init_importance;
+                    continue_prob = fminf(q, 1.0f);
+                    if (L.rng.uni() >= continue_prob) rr_terminated = true;
+                }
+                if (!with_nrc && rr_terminated) path_end = true;
+                if (with_nrc) {
+                    // termination into the cache (optix_pathtracing_kernels.cu:282-330)
+                    bool ends_with_cache = rr_terminated;
+                    const float pt = nb.path_termination_factor;
+                    if (gen_data) {
+                        if (!L.unbiased)
+                            ends_with_cache |= L.cur_sqrt_spread * L.cur_sqrt_spread > 2.0f * pt * L.primary_spread;
+                    } else {
+                        ends_with_cache |= L.cur_sqrt_spread * L.cur_sqrt_spread > pt * L.primary_spread;
+                    }
+                    if (ends_with_cache) {
+                        if (!COUNT_ONLY) {
+                            const V3 vout{-L.dir.x, -L.dir.y, -L.dir.z};
+                            make_query(p, L.pos, vout, &nb.inference_queries[L.lin]);
+                            if (gen_data) {
+                                rtc8x_training_suffix_terminal ti;
+                                ti.prev_vertex_index = L.prev_idx;
+                                ti.has_query = 1;
+                                ti.path_length = L.path_length & 0xFFu;
+                                ti.is_unbiased = L.unbiased ? 1u : 0u;
+                                nb.train_suffix_terminals[L.lin] = ti;
+                            } else {
+                                rtc8x_terminal_info ti;
+                                ti.throughput[0] = L.tp.r;
+                                ti.throughput[1] = L.tp.g;
+                                ti.throughput[2] = L.tp.b;
+                                ti.has_query = 1u | ((L.path_length & 0xFFu) << 8);
+                                nb.inference_terminals[L.lin] = ti;
+                            }
+                        }
+                        path_end = true;
+                    }
+                }
+                if (!path_end) {
+                    L.tp = scale(L.tp, 1.0f / continue_prob);
+                    // NEE set-up: light choice and sample, everything of the estimate except visibility
+                    const bool have_env = p.env.kind != RTC8X_ENV_NONE, have_sun = p.sun.enabled != 0;
+                    float u_light = L.rng.uni();
+                    bool use_sun = false;
+                    float prob_cur = 1.0f;
+                    if (have_env && have_sun) {
+                        if (u_light < p.sun.select_prob) {
+                            use_sun = true;
+                            prob_cur = p.sun.select_prob;
+                        } else {
+                            prob_cur = 1.0f - p.sun.select_prob;
+                        }
+                    } else if (have_sun) {
+                        use_sun = true;
+                    }
+                    float u0 = L.rng.uni();
+                    float u1 = L.rng.uni();
+                    bool zero = !have_env && !have_sun;
+                    V3 lpos{0.0f, 0.0f, 0.0f};
+                    RGB emittance{0.0f, 0.0f, 0.0f};
+                    float area_pd = 0.0f;
+                    if (!zero) {
+                        if (use_sun) {
+                            lpos = V3{p.sun.direction[0], p.sun.direction[1], p.sun.direction[2]};
+                            emittance = RGB{p.sun.irradiance[0], p.sun.irradiance[1], p.sun.irradiance[2]};
+                        } else {
+                            V3 ldir;
+                            float ldir_pd;
+                            RGB radiance = env_sample(p.env, u0, u1, ldir, ldir_pd);
+                            emittance = scale(radiance, RTC8X_PI_F);
+                            lpos = ldir;
+                            area_pd = ldir_pd;
+                            if (area_pd == 0.0f) zero = true;
+                            area_pd *= prob_cur;
+                        }
+                    }
+                    V3 sdir{-lpos.x, -lpos.y, -lpos.z};
+                    float lp_cos = 0.0f;
+                    if (!zero) {
+                        V3 lnormal{-lpos.x, -lpos.y, -lpos.z};
+                        lp_cos = dot(sdir, lnormal);
+                        if (lp_cos <= 0.0f) zero = true;
+                    }
+                    if (zero) {
+                        const RGB nee_c{0.0f, 0.0f, 0.0f};
+                        if (!gen_data) L.contribution = add(L.contribution, mul(L.tp, nee_c));
+                        if (gen_data) L.nee_c = nee_c;
+                        after_nee = true;
+                    } else {
+                        const Frame fr = make_frame(V3{-L.dir.x, -L.dir.y, -L.dir.z});
+                        V3 vin = to_local(fr, V3{-sdir.x, -sdir.y, -sdir.z});
+                        float pval = phase_value(p.phase, p.scattering_forwardness, -vin.z);
+                        float alb = p.scattering_albedo;
+                        RGB fs{alb * pval, alb * pval, alb * pval};
+                        L.nee_sun = use_sun;
+                        if (use_sun) {
+                            L.nee_f = mul(fs, emittance);
+                            L.nee_k1 = prob_cur;
+                        } else {
+                            float bsdf_pd = pval * lp_cos / 1.0f;
+                            if (!isfinite(bsdf_pd)) bsdf_pd = 0.0f;
+                            float light_pd = area_pd;
+                            L.nee_k1 = light_pd * light_pd / (bsdf_pd * bsdf_pd + light_pd * light_pd);
+                            L.nee_f = mul(fs, scale(emittance, 1.0f / RTC8X_PI_F));
+                            L.nee_k2 = lp_cos * 1.0f / 1.0f;
+                            L.nee_k3 = area_pd;
+                        }
+                        const float trace_len = 2.0f * p.world_radius;
+                        V3 lpoint{L.pos.x + trace_len * -sdir.x, L.pos.y + trace_len * -sdir.y,
+                                  L.pos.z + trace_len * -sdir.z};
+                        L.vis = 1.0f;
+                        L.mode = start_segment<TRACKING>(p.grid, med, sg, L.rng, lpoint, sdir, trace_len) ? kModeShadow
+                                                                                                        : kModeNeeDone;
+                        continue;
+                    }
+                }
+            }
+            if (after_nee) {
+                if (L.path_length >= max_len) {
+                    path_end = true;
+                } else {
+                    // phase sampling (optix_pathtracing_kernels.cu:341-356)
+                    float us0 = L.rng.uni();
+                    float us1 = L.rng.uni();
+                    V3 dl;
+                    float pdf = phase_sample(p.phase, p.scattering_forwardness, us0, us1, dl);
+                    if (pdf == 0.0f) {
+                        path_end = true;
+                    } else {
+                        const V3 vout{-L.dir.x, -L.dir.y, -L.dir.z};
+                        const Frame fr = make_frame(vout);
+                        const float alb = p.scattering_albedo;
+                        RGB fsv{alb * pdf, alb * pdf, alb * pdf};
+                        L.dir = from_local(fr, dl);
+                        const RGB local_tp = scale(fsv, 1.0f / pdf);
+                        L.tp = mul(L.tp, local_tp);
+                        L.org = L.pos;
+                        L.dir_pd = pdf;
+                        bool record_end = false;
+                        if (gen_data) {
+                            // training vertex record (optix_pathtracing_kernels.cu:358-410)
+                            const uint32_t idx = L.train_base + L.records;
+                            L.records++;
+                            if (!COUNT_ONLY) {
+                                rtc8x_radiance_query q;
+                                make_query(p, L.pos, vout, &q);
+                                if (idx < RTC8X_TRAIN_BUFFER_SIZE) {
+                                    nb.train_queries[0][idx] = q;
+                                    rtc8x_training_vertex_info vi;
+                                    vi.local_throughput[0] = local_tp.r;
+                                    vi.local_throughput[1] = local_tp.g;
+                                    vi.local_throughput[2] = local_tp.b;
+                                    vi.prev_vertex_index = L.prev_idx;
+                                    vi.path_length = L.path_length & 0xFFu;
+                                    vi.is_unbiased = L.unbiased ? 1u : 0u;
+                                    nb.train_vertex_info[idx] = vi;
+                                    float* tgt = nb.train_targets[0] + 3 * (size_t)idx;
+                                    tgt[0] = L.nee_c.r;
+                                    tgt[1] = L.nee_c.g;
+                                    tgt[2] = L.nee_c.b;
+                                    L.prev_local_tp = local_tp;
+                                    L.prev_idx = idx;
+                                } else {
+                                    nb.inference_queries[L.lin] = q;
+                                    rtc8x_training_suffix_terminal ti;
+                                    ti.prev_vertex_index = L.prev_idx;
+                                    ti.has_query = 1;
+                                    ti.path_length = L.path_length & 0xFFu;
+                                    ti.is_unbiased = L.unbiased ? 1u : 0u;
+                                    nb.train_suffix_terminals[L.lin] = ti;
+                                    record_end = true;
+                                }
+                            }
+                        }
+                        if (record_end) {
+                            path_end = true;
+                        } else {
+                            ++L.path_length;
+                            L.mode = start_segment<TRACKING>(p.grid, med, sg, L.rng, L.org, L.dir, 1e10f) ? kModeTrack
+                                                                                                         : kModeEscape;
+                            continue;
+                        }
+                    }
+                }
+            }
+            if (path_end) {
+                paths++;
+                vertices += L.path_length;
+                if (COUNT_ONLY) {
+                    record_counts[L.lin] = L.records;
+                } else {
+                    const bool ok = all_nonneg_finite(L.contribution);
+                    if (ENTRY == RTC8X_ENTRY_PATH_TRACE) {
+                        if (ok) {
+                            double* a = p.accum + 3 * (size_t)L.lin;
+                            a[0] += (double)L.contribution.r;
+                            a[1] += (double)L.contribution.g;
+                            a[2] += (double)L.contribution.b;
+                        }
+                    } else if (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
+                        float* pf = nb.per_frame_contribution + 3 * (size_t)L.lin;
+                        pf[0] = ok ? L.contribution.r : 0.0f;
+                        pf[1] = ok ? L.contribution.g : 0.0f;
+                        pf[2] = ok ? L.contribution.b : 0.0f;
+                    }
+                    if (p.path_vertices) p.path_vertices[L.lin] += L.path_length;
+                }
+                if (++L.sample < num_samples) {
+                    L.mode = kModeStart;
+                } else {
+                    if (!COUNT_ONLY) {
+                        p.rng_states[(size_t)L.ly * p.image_width + L.lx] = L.rng.state;
+                        if (p.pixel_cost) p.pixel_cost[L.lin] = med.lookups - L.cost0;
+                    }
+                    L.mode = kModeWork;
+                }
             }
         }
-
-        tp = scale(tp, 1.0f / continue_prob);
-        RGB nee_c = nee<TRACKING>(p, c, rng, pos, fr);
-        if (!gen_data) contribution = add(contribution, mul(tp, nee_c));
-        if (path_length >= max_len) break;
-
-        // phase sampling (optix_pathtracing_kernels.cu:341-356)
-        float us0 = rng.uni();
-        float us1 = rng.uni();
-        V3 dl;
-        float pdf = phase_sample(p.phase, p.scattering_forwardness, us0, us1, dl);
-        if (pdf == 0.0f) break;
-        const float alb = p.scattering_albedo;
-        RGB fsv{alb * pdf, alb * pdf, alb * pdf};
-        dir = from_local(fr, dl);
-        RGB local_tp = scale(fsv, 1.0f / pdf);
-        tp = mul(tp, local_tp);
-        org = pos;
-        dir_pd = pdf;
-
-        if (gen_data) {
-            // training vertex record (optix_pathtracing_kernels.cu:358-410)
-            uint32_t idx = train_base + records;
-            records++;
-            if (COUNT_ONLY) continue;
-            rtc8x_radiance_query q;
-            make_query(p, pos, vout, &q);
-            if (idx < RTC8X_TRAIN_BUFFER_SIZE) {
-                nb.train_queries[0][idx] = q;
-                rtc8x_training_vertex_info vi;
-                vi.local_throughput[0] = local_tp.r;
-                vi.local_throughput[1] = local_tp.g;
-                vi.local_throughput[2] = local_tp.b;
-                vi.prev_vertex_index = prev_idx;
-                vi.path_length = path_length & 0xFFu;
-                vi.is_unbiased = unbiased ? 1u : 0u;
-                nb.train_vertex_info[idx] = vi;
-                float* tgt = nb.train_targets[0] + 3 * (size_t)idx;
-                tgt[0] = nee_c.r;
-                tgt[1] = nee_c.g;
-                tgt[2] = nee_c.b;
-                prev_local_tp = local_tp;
-                prev_idx = idx;
-            } else {
-                nb.inference_queries[linear] = q;
-                rtc8x_training_suffix_terminal ti;
-                ti.prev_vertex_index = prev_idx;
-                ti.has_query = 1;
-                ti.path_length = path_length & 0xFFu;
-                ti.is_unbiased = unbiased ? 1u : 0u;
-                nb.train_suffix_terminals[linear] = ti;
-                break;
-            }
-        }
-    }
-    contribution_out = contribution;
-    path_length_out = path_length;
-    return records;
-}
-
-// Maps a workgroup id to a tile so that each XCD's share of the grid is one contiguous run of tiles.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t nblocks)
-{
-    const uint32_t nx = 8;
-    uint32_t q = nblocks / nx, r = nblocks % nx;
-    uint32_t x = bid % nx, k = bid / nx;
-    return x * q + (x < r ? x : r) + k;
-}
-
-__device__ __forceinline__ void add_stats(uint64_t* stats, uint32_t paths, uint32_t vertices, uint32_t lookups,
-                                          uint32_t scatters)
-{
-    // wavefront reduction, then one atomic per wavefront per counter
-    uint64_t v[4] = {paths, vertices, lookups, scatters};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        uint64_t x = v[i];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        v[i] = x;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd((unsigned long long*)&stats[RTC8X_STAT_PATHS], (unsigned long long)v[0]);
-        atomicAdd((unsigned long long*)&stats[RTC8X_STAT_VERTICES], (unsigned long long)v[1]);
-        atomicAdd((unsigned long long*)&stats[RTC8X_STAT_LOOKUPS], (unsigned long long)v[2]);
-        atomicAdd((unsigned long long*)&stats[RTC8X_STAT_SCATTER], (unsigned long long)v[3]);
-    }
-}
-
-template <int ENTRY, int TRACKING, bool COUNT_ONLY>
-__global__ __launch_bounds__(256) void trace_kernel(const rtc8x_frame_params p, uint32_t launch_w, uint32_t launch_h,
-                                                    uint32_t num_samples, const uint32_t* __restrict__ record_offsets,
-                                                    uint32_t* __restrict__ record_counts)
-{
-    const uint32_t tiles_x = (launch_w + kTileW - 1) / kTileW;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-    const uint32_t lx = tx * kTileW + (threadIdx.x % kTileW);
-    const uint32_t ly = ty * kTileH + (threadIdx.x / kTileW);
-    const bool active = lx < launch_w && ly < launch_h;
-
-    PathCtx c;
-    c.med.coeff = p.density_coeff;
-    c.med.majorant = p.density_coeff * p.grid.max_value;
-    c.med.lookups = 0;
-    c.scatters = 0;
-    c.prob_env = (p.sun.enabled && p.env.kind != RTC8X_ENV_NONE) ? 1.0f - p.sun.select_prob : 1.0f;
-    uint32_t paths = 0, vertices = 0;
-
-    if (active) {
-        uint64_t* rs = &p.rng_states[(size_t)ly * p.image_width + lx];
-        Pcg rng{*rs};
-        const uint32_t lin = ly * launch_w + lx;
-        const uint32_t base = record_offsets ? record_offsets[lin] : 0u;
-        double acc[3] = {0.0, 0.0, 0.0};
-        if (ENTRY == RTC8X_ENTRY_PATH_TRACE && !COUNT_ONLY) {
-            const double* a = p.accum + 3 * (size_t)lin;
-            acc[0] = a[0];
-            acc[1] = a[1];
-            acc[2] = a[2];
-        }
-        uint32_t vsum = 0;
-        for (uint32_t s = 0; s < num_samples; ++s) {
-            RGB contrib;
-            uint32_t plen;
-            uint32_t rec = trace_path<ENTRY, TRACKING, COUNT_ONLY>(p, c, lx, ly, launch_w, launch_h, rng, base, contrib, plen);
-            paths++;
-            vertices += plen;
-            vsum += plen;
-            if (COUNT_ONLY) {
-                record_counts[lin] = rec;
-                continue;
-            }
-            if (ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA) continue;
-            bool ok = all_nonneg_finite(contrib);
-            if (ENTRY == RTC8X_ENTRY_PATH_TRACE) {
-                if (ok) {
-                    acc[0] += (double)contrib.r;
-                    acc[1] += (double)contrib.g;
-                    acc[2] += (double)contrib.b;
+        // ---- one tentative collision, and its density lookup, per stepping lane
+        if (L.mode == kModeTrack || L.mode == kModeShadow) {
+            if (sg.advance(p.grid, med, L.rng, mu)) {
+                const float sigma = sample_density(p.grid, med, sg.o, sg.d, sg.t);
+                if (L.mode == kModeTrack) {
+                    if (sg.after_track(L.rng, sigma, mu)) L.mode = kModeScatter;
+                } else {
+                    sg.after_shadow(L.rng, sigma, mu, L.vis);
                 }
             } else {
-                float* pf = p.nrc.per_frame_contribution + 3 * (size_t)lin;
-                pf[0] = ok ? contrib.r : 0.0f;
-                pf[1] = ok ? contrib.g : 0.0f;
-                pf[2] = ok ? contrib.b : 0.0f;
+                L.mode = L.mode == kModeTrack ? kModeEscape : kModeNeeDone;
             }
-        }
-        if (!COUNT_ONLY) {
-            *rs = rng.state;
-            if (ENTRY == RTC8X_ENTRY_PATH_TRACE) {
-                double* a = p.accum + 3 * (size_t)lin;
-                a[0] = acc[0];
-                a[1] = acc[1];
-                a[2] = acc[2];
-            }
-            if (p.path_vertices) p.path_vertices[lin] += vsum;
         }
     }
-    if (p.stats && !COUNT_ONLY) add_stats(p.stats, paths, vertices, c.med.lookups, c.scatters);
+    if (p.stats && !COUNT_ONLY) {
+        uint64_t v[4] = {paths, vertices, med.lookups, scatters};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint64_t x = v[i];
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            v[i] = x;
+        }
+        if (lane_id == 0) {
+            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_PATHS], (unsigned long long)v[0]);
+            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_VERTICES], (unsigned long long)v[1]);
+            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_LOOKUPS], (unsigned long long)v[2]);
+            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_SCATTER], (unsigned long long)v[3]);
+        }
+    }
 }
 
 // Exclusive prefix sum of the per-path record counts (one workgroup, a few hundred thousand entries).
@@ -446,18 +619,117 @@ __global__ __launch_bounds__(1024) void record_prefix_kernel(const uint32_t* __r
     if (tid == 1023) *total_out = part[1023] > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)part[1023];
 }
 
+
+
+// ---------------------------------------------------------------- cost-ordered work lists
+// A stable counting sort of the pixels by class = 31 - floor(log2(cost + 1)) (costliest first):
+// per-block class histograms, one exclusive scan over (class, block), and a stable scatter.
+constexpr uint32_t kOrderBlock = 1024;  // pixels per workgroup (256 lanes x 4)
+constexpr uint32_t kClasses = 32;
+
+__device__ __forceinline__ uint32_t cost_class(uint32_t c)
+{
+    return (uint32_t)__clz((c < 0xFFFFFFFEu ? c : 0xFFFFFFFEu) + 1u);  // 0 (costliest) .. 31 (no lookups)
+}
+
+__global__ __launch_bounds__(256) void order_hist_kernel(const uint32_t* __restrict__ cost, uint32_t n,
+                                                         uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[kClasses];
+    if (threadIdx.x < kClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kOrderBlock;
+    for (uint32_t k = 0; k < kOrderBlock / 256; ++k) {
+        const uint32_t i = b0 + k * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[cost_class(cost[i])], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of hist[class][block] in class-major order, in place (one workgroup).
+__global__ __launch_bounds__(1024) void order_scan_kernel(uint32_t* __restrict__ hist, uint32_t m)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t chunk = (m + 1023) / 1024;
+    const uint32_t b = tid * chunk, e = min(m, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += hist[i];
+    part[tid] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - s;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+// Stable scatter: within a block, pixels keep their index order inside each class.
+__global__ __launch_bounds__(256) void order_scatter_kernel(const uint32_t* __restrict__ cost, uint32_t n,
+                                                            const uint32_t* __restrict__ offsets,
+                                                            uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t base[kClasses];
+    __shared__ uint32_t cnt[kClasses][256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t b0 = blockIdx.x * kOrderBlock + t * (kOrderBlock / 256);  // 4 consecutive pixels per lane
+    uint32_t cls[kOrderBlock / 256];
+    for (uint32_t c = 0; c < kClasses; ++c) cnt[c][t] = 0;
+    for (uint32_t k = 0; k < kOrderBlock / 256; ++k) {
+        const uint32_t i = b0 + k;
+        cls[k] = i < n ? cost_class(cost[i]) : kClasses;
+        if (cls[k] < kClasses) cnt[cls[k]][t] += 1;
+    }
+    if (t < kClasses) base[t] = offsets[t * gridDim.x + blockIdx.x];
+    __syncthreads();
+    // exclusive scan of cnt[c][0..255] for every class (one lane per class)
+    if (t < kClasses) {
+        uint32_t run = 0;
+        for (uint32_t j = 0; j < 256; ++j) {
+            const uint32_t v = cnt[t][j];
+            cnt[t][j] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < kOrderBlock / 256; ++k) {
+        if (cls[k] >= kClasses) continue;
+        uint32_t before = 0;
+        for (uint32_t q = 0; q < k; ++q) before += cls[q] == cls[k] ? 1u : 0u;
+        order[base[cls[k]] + cnt[cls[k]][t] + before] = b0 + k;
+    }
+}
+
+// One persistent launch: as many 256-lane workgroups as fit on the device at once, a fresh work
+// counter slot (cleared by a one-lane kernel on the same stream; launches on different streams take
+// different slots).
 template <int ENTRY, int TRACKING, bool COUNT_ONLY>
 hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t num_samples, const uint32_t* offsets,
                   uint32_t* counts, hipStream_t stream)
 {
-    const uint32_t tiles = ((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH);
-    hipLaunchKernelGGL((trace_kernel<ENTRY, TRACKING, COUNT_ONLY>), dim3(tiles), dim3(kTileW * kTileH), 0, stream, p,
-                       w, h, num_samples, offsets, counts);
+    static std::atomic<uint32_t> next_slot{0};
+    const uint32_t slot = next_slot.fetch_add(1) % kCounterSlots;
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wavefront_kernel<ENTRY, TRACKING, COUNT_ONLY>, 256, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, slot);
+    hipLaunchKernelGGL((wavefront_kernel<ENTRY, TRACKING, COUNT_ONLY>), dim3((uint32_t)(cus * per_cu)), dim3(256), 0,
+                       stream, p, w, h, num_samples, slot, offsets, counts);
     return hipGetLastError();
 }
-
-template <int TRACKING>
-hipError_t launch_wavefront(const rtc8x_frame_params& p, uint32_t num_samples, hipStream_t stream);
 
 template <int ENTRY, bool COUNT_ONLY>
 hipError_t launch_tracking(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t ns, const uint32_t* offsets,
@@ -472,6 +744,29 @@ hipError_t launch_tracking(const rtc8x_frame_params& p, uint32_t w, uint32_t h, 
 }  // namespace rtc8x
 
 using namespace rtc8x;
+
+extern "C" size_t rtc8x_order_scratch_bytes(uint32_t n)
+{
+    const size_t blocks = ((size_t)n + dev::kOrderBlock - 1) / dev::kOrderBlock;
+    return blocks * dev::kClasses * sizeof(uint32_t);
+}
+
+extern "C" rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, uint32_t n, uint32_t* order,
+                                            void* scratch)
+{
+    RTC8X_REQUIRE(cost && order && scratch, "rtc8x_order_by_cost: NULL argument");
+    if (n == 0) return RTC8X_OK;
+    const uint32_t blocks = (n + dev::kOrderBlock - 1) / dev::kOrderBlock;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* hist = (uint32_t*)scratch;
+    hipLaunchKernelGGL(dev::order_hist_kernel, dim3(blocks), dim3(256), 0, s, cost, n, hist);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::order_scan_kernel, dim3(1), dim3(1024), 0, s, hist, blocks * dev::kClasses);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::order_scatter_kernel, dim3(blocks), dim3(256), 0, s, cost, n, hist, order);
+    RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}
 
 static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
@@ -530,12 +825,12 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
             RTC8X_REQUIRE(p.sun.select_prob > 0.0f && p.sun.select_prob < 1.0f,
                           "rtc8x_trace: sun.select_prob must be in (0,1) when an environment is present");
     }
+    RTC8X_REQUIRE(entry != RTC8X_ENTRY_GENERATE_TRAINING_DATA || (!p.work_order && !p.pixel_cost),
+                  "rtc8x_trace: work_order / pixel_cost are for the full-image entries (the training size varies)");
     hipStream_t s = (hipStream_t)stream;
     if (entry == RTC8X_ENTRY_PATH_TRACE) {
-        if (p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT)
-            RTC8X_HIP(dev::launch_wavefront<RTC8X_TRACKING_LOCAL_MAJORANT>(p, num_samples, s));
-        else
-            RTC8X_HIP(dev::launch_wavefront<RTC8X_TRACKING_GLOBAL_MAJORANT>(p, num_samples, s));
+        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE, false>(p, p.image_width, p.image_height, num_samples,
+                                                                        nullptr, nullptr, s)));
         return RTC8X_OK;
     }
     const rtc8x_nrc_buffers& nb = p.nrc;

@@ -179,6 +179,11 @@ class NrcRenderer:
         self.accum = DeviceBuffer(24 * n)
         self.stats_buf = DeviceBuffer.zeros(8 * _abi.STAT_COUNT)
         self.buffers = NrcBuffers(cfg.width, cfg.height)
+        # cost-ordered scheduling of the render launches (the full-image entry)
+        self.cost = DeviceBuffer.zeros(4 * n)
+        self.order = DeviceBuffer(4 * n)
+        self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n))
+        self.have_cost = False
         self.cache = NeuralRadianceCache()
         self.cache.initialize(num_hidden_layers, learning_rate, precision)
         self.frame_rng = Mt19937(frame_rng_seed)
@@ -235,6 +240,12 @@ class NrcRenderer:
     def render_sample(self) -> None:
         s = C.c_void_p(self.stream)
         p = self.params()
+        if self.have_cost:
+            call("rtc8x_order_by_cost", s, C.c_void_p(self.cost.ptr), self.width * self.height, C.c_void_p(self.order.ptr),
+                 C.c_void_p(self.order_scratch.ptr))
+            p.work_order = self.order.ptr
+        p.pixel_cost = self.cost.ptr
+        self.have_cost = True
         call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
         nq = round_up(self.width * self.height, 128)
         self.cache.infer(self.stream, self.buffers.inference_queries.ptr, nq, self.buffers.inferred_radiance.ptr)
@@ -262,5 +273,5 @@ class NrcRenderer:
     def close(self) -> None:
         self.cache.finalize()
         self.buffers.free()
-        for b in (self.rng, self.accum, self.stats_buf):
+        for b in (self.rng, self.accum, self.stats_buf, self.cost, self.order, self.order_scratch):
             b.free()

@@ -8,13 +8,13 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._lib import call
+from ._lib import call, lib
 from .device import DeviceBuffer
 from .scene import DeviceScene
 
 
 class PathTracer:
-    def __init__(self, scene: DeviceScene, stream=None, track_vertices: bool = False):
+    def __init__(self, scene: DeviceScene, stream=None, track_vertices: bool = False, schedule: bool = True):
         self.scene = scene
         self.stream = stream
         cfg = scene.cfg
@@ -23,6 +23,12 @@ class PathTracer:
         self.accum = DeviceBuffer(24 * n)
         self.vertices = DeviceBuffer(4 * n) if track_vertices else None
         self.stats = DeviceBuffer(8 * _abi.STAT_COUNT)
+        # cost-ordered scheduling: each launch records per-pixel lookups, the next starts the costliest first
+        self.schedule = schedule
+        self.cost = DeviceBuffer.zeros(4 * n) if schedule else None
+        self.order = DeviceBuffer(4 * n) if schedule else None
+        self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n)) if schedule else None
+        self.have_cost = False
         self.samples = 0
         self.reset()
 
@@ -44,8 +50,22 @@ class PathTracer:
         p.stats = self.stats.ptr
         return p
 
+    def _schedule(self, p: _abi.FrameParams) -> None:
+        if not self.schedule:
+            return
+        cfg = self.scene.cfg
+        if self.have_cost:
+            call("rtc8x_order_by_cost", C.c_void_p(self.stream), C.c_void_p(self.cost.ptr), cfg.width * cfg.height,
+                 C.c_void_p(self.order.ptr), C.c_void_p(self.order_scratch.ptr))
+            p.work_order = self.order.ptr
+        else:
+            p.work_order = None
+        p.pixel_cost = self.cost.ptr
+        self.have_cost = True
+
     def trace(self, samples_per_launch: int = 1, params: _abi.FrameParams | None = None) -> None:
         p = params if params is not None else self.params()
+        self._schedule(p)
         call("rtc8x_trace", C.c_void_p(self.stream), C.byref(p), _abi.ENTRY_PATH_TRACE, int(samples_per_launch))
         self.samples += int(samples_per_launch)
 
@@ -60,6 +80,6 @@ class PathTracer:
                 "lookups": int(s[_abi.STAT_LOOKUPS]), "scatters": int(s[_abi.STAT_SCATTER])}
 
     def close(self) -> None:
-        for b in (self.rng, self.accum, self.vertices, self.stats):
+        for b in (self.rng, self.accum, self.vertices, self.stats, self.cost, self.order, self.order_scratch):
             if b is not None:
                 b.free()

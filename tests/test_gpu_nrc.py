@@ -225,14 +225,22 @@ def test_training_data_path_bit_exact(tracking):
     ds.close()
 
 
-def test_render_with_nrc_bit_exact():
-    """PATH_TRACE_WITH_NRC trace -> (synthetic) inference -> accumulate, GPU vs oracle."""
+@pytest.mark.parametrize("ordered", [False, True])
+def test_render_with_nrc_bit_exact(ordered):
+    """PATH_TRACE_WITH_NRC trace -> (synthetic) inference -> accumulate, GPU vs oracle; optionally with
+    lanes taking the pixels in a random work order (results must not depend on it)."""
     from rtcamp8_amd import _abi
     from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
 
     cfg, ds, r = _nrc_case()
     b = r.buffers
     p = r.params()
+    if ordered:
+        perm = np.random.default_rng(21).permutation(cfg.width * cfg.height).astype(np.uint32)
+        dperm = DeviceBuffer.from_array(perm)
+        p.work_order = dperm.ptr
+        p.pixel_cost = r.cost.ptr
     rng0 = r.rng.download(np.uint64, cfg.width * cfg.height)
     before = HostNrc(b)
     call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)

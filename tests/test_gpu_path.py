@@ -141,3 +141,25 @@ def test_uploaded_grid_matches_nanovdb_fixture():
         assert np.array_equal(dval.download(np.float32, n).view(np.uint32), fx["p_value"].view(np.uint32))
     finally:
         call("rtc8x_grid_destroy", C.byref(dg))
+
+
+def test_order_by_cost_is_a_stable_costliest_first_permutation():
+    from rtcamp8_amd._lib import call, lib
+    from rtcamp8_amd.device import DeviceBuffer
+
+    rng = np.random.default_rng(5)
+    for n in (1, 1000, 1024, 300001):
+        cost = (rng.integers(0, 1 << rng.integers(1, 30, n), dtype=np.int64) % (1 << 31)).astype(np.uint32)
+        cost[: min(n, 7)] = [0, 1, 2, 3, 0xFFFFFFFF, 0xFFFFFFFE, 5][: min(n, 7)]
+        dc, do = DeviceBuffer.from_array(cost), DeviceBuffer(4 * n)
+        ds = DeviceBuffer(max(int(lib().rtc8x_order_scratch_bytes(n)), 4))
+        call("rtc8x_order_by_cost", None, C.c_void_p(dc.ptr), n, C.c_void_p(do.ptr), C.c_void_p(ds.ptr))
+        order = do.download(np.uint32, n).astype(np.int64)
+        assert np.array_equal(np.sort(order), np.arange(n))
+        cls = 31 - np.floor(np.log2(np.minimum(cost.astype(np.float64), 0xFFFFFFFE) + 1)).astype(np.int64)
+        key = cls[order]
+        assert np.all(np.diff(key) >= 0)  # costliest class first
+        same = np.diff(key) == 0
+        assert np.all(np.diff(order)[same] > 0)  # row-major inside a class
+        for b_ in (dc, do, ds):
+            b_.free()

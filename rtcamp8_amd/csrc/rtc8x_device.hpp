@@ -248,6 +248,13 @@ struct Medium {
     uint32_t lookups;
 };
 
+// Density coefficient times the trilinear grid value at o + t d (no lookup counting).
+__device__ __forceinline__ float density_at(const rtc8x_grid& g, float coeff, V3 o, V3 d, float t)
+{
+    V3 idx = world_to_index(g, fmaf(t, d.x, o.x), fmaf(t, d.y, o.y), fmaf(t, d.z, o.z));
+    return coeff * trilinear(g, idx);
+}
+
 __device__ __forceinline__ float sample_density(const rtc8x_grid& g, Medium& m, V3 o, V3 d, float t)
 {
     V3 idx = world_to_index(g, fmaf(t, d.x, o.x), fmaf(t, d.y, o.y), fmaf(t, d.z, o.z));

@@ -8,7 +8,7 @@
 // Russian roulette and NRC termination, NEE set-up, phase sampling, training records, path end).  Each
 // iteration of the main loop takes one tentative collision, with its density lookup, for every lane
 // that is stepping a segment; a lane that reaches a vertex event parks, and the parked lanes run their
-// events together once they are half of the wavefront's active lanes, so that each event's code is
+// events together once they are a quarter of the wavefront's active lanes, so that each event's code is
 // issued once for many lanes.  A lane whose pixel is done takes the next one from a work counter
 // (8x8 pixel tiles or a caller-given order, one atomic per wavefront).  Per pixel, the random numbers
 // and floating-point operations are the reference's, so results are bit-identical to tracing each
@@ -26,6 +26,7 @@ namespace rtc8x {
 namespace dev {
 
 constexpr uint32_t kHardMaxPathLength = 4096u;
+constexpr uint32_t kSpeculate = 2;  // tentative collisions looked up together per lane (global majorant)
 constexpr uint32_t kCounterSlots = 1024;
 __device__ uint32_t g_work_counter[kCounterSlots];
 
@@ -201,13 +202,13 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
     Seg<TRACKING> sg;
     float mu = 0.0f;
     for (;;) {
-        // Vertex events are batched: a lane that reaches one parks until half of the wavefront's active
+        // Vertex events are batched: a lane that reaches one parks until a quarter of the wavefront's active
         // lanes have, then all of them run their events together (each event block is issued once for
         // many lanes instead of once per lane), while the others keep stepping through lookups.
         const uint64_t parked = __ballot(L.mode != kModeTrack && L.mode != kModeShadow && L.mode != kModeIdle);
         const uint64_t stepping = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
         if (parked == 0 && stepping == 0) break;
-        const bool event_pass = stepping == 0 || 2 * __popcll(parked) >= __popcll(parked | stepping);
+        const bool event_pass = stepping == 0 || 4 * __popcll(parked) >= __popcll(parked | stepping);
         // ---- events until this lane has a tracking or shadow segment under way, or no work left
         while (event_pass && L.mode != kModeIdle && L.mode != kModeTrack && L.mode != kModeShadow) {
             if (L.mode == kModeWork) {
@@ -561,8 +562,74 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                 }
             }
         }
-        // ---- one tentative collision, and its density lookup, per stepping lane
-        if (L.mode == kModeTrack || L.mode == kModeShadow) {
+        // ---- tentative collisions, with their density lookups, for every stepping lane
+        if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
+            // kSpeculate collisions at once: the positions depend only on the random numbers, so their
+            // lookups are issued together.  The numbers are drawn in the sequential algorithm's order
+            // (tracking: step, accept, step, accept, ...; ratio tracking: step, step, ...) and the
+            // results are consumed in order; after an acceptance or the segment's end the PCG state is
+            // rewound to where the sequential loop would stand, so later draws are unaffected.
+            if (L.mode == kModeTrack || L.mode == kModeShadow) {
+                const bool track = L.mode == kModeTrack;
+                Pcg r = L.rng;
+                float tk[kSpeculate], ua[kSpeculate];
+                uint64_t st_after[kSpeculate];
+                uint32_t n = 0;
+                bool ended = false;
+                uint64_t st_end = 0;
+                float t = sg.t;
+#pragma unroll
+                for (uint32_t k = 0; k < kSpeculate; ++k) {
+                    tk[k] = sg.t;
+                    ua[k] = 0.0f;
+                    st_after[k] = 0;
+                    if (!ended) {
+                        t += -rtc8x_logf(1.0f - r.uni()) / med.majorant;
+                        if (t > sg.t1) {
+                            ended = true;
+                            st_end = r.state;
+                        } else {
+                            tk[k] = t;
+                            if (track) ua[k] = r.uni();
+                            st_after[k] = r.state;
+                            n = k + 1;
+                        }
+                    }
+                }
+                float sig[kSpeculate];
+#pragma unroll
+                for (uint32_t k = 0; k < kSpeculate; ++k) sig[k] = density_at(p.grid, med.coeff, sg.o, sg.d, tk[k]);
+                bool accepted = false;
+                float t_acc = 0.0f;
+                uint64_t st_acc = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < kSpeculate; ++k) {
+                    if (!accepted && k < n) {
+                        med.lookups++;
+                        if (track) {
+                            if (ua[k] < sig[k] / med.majorant) {
+                                accepted = true;
+                                t_acc = tk[k];
+                                st_acc = st_after[k];
+                            }
+                        } else {
+                            L.vis *= 1.0f - sig[k] / med.majorant;
+                        }
+                    }
+                }
+                if (accepted) {
+                    sg.t = t_acc;
+                    L.rng.state = st_acc;
+                    L.mode = kModeScatter;
+                } else if (ended) {
+                    L.rng.state = st_end;
+                    L.mode = track ? kModeEscape : kModeNeeDone;
+                } else {
+                    sg.t = tk[kSpeculate - 1];
+                    L.rng.state = st_after[kSpeculate - 1];
+                }
+            }
+        } else if (L.mode == kModeTrack || L.mode == kModeShadow) {
             if (sg.advance(p.grid, med, L.rng, mu)) {
                 const float sigma = sample_density(p.grid, med, sg.o, sg.d, sg.t);
                 if (L.mode == kModeTrack) {

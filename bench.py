@@ -238,7 +238,8 @@ def latency_roofline(scene, lookups_per_path: float, trace_lookups_per_s: float,
 def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
     """Wall time to equal RMSE: NRC-accelerated rendering vs the plain tracer on the same frame.
 
-    The converged reference is the plain tracer at 2 x ref_spp (two independent halves A, B); the MSE
+    NRC runs the reference's batch schedule (training iterations per frame, then the frame's samples) with
+    4 spp per frame.  The converged reference is the plain tracer at 2 x ref_spp (two independent halves A, B); the MSE
     of an estimate X is mean((X - A)(X - B)), free of reference noise.  The plain tracer's MSE falls
     as 1/spp, so it needs spp * mse_plain / mse_nrc samples to match the NRC image; its time for that
     is scaled from the measured 16-spp launch."""
@@ -251,9 +252,10 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
     mean = float((a + b).mean() / 2)
     mse_plain = cross_mse(x, a, b)
     res = {"reference": f"plain tracer, 2 x {ref_spp} spp with independent seeds (cross-product MSE)",
+           "nrc_schedule": "frame 0: 8 training iterations, later frames 1; 4 spp traced into the cache per frame",
            "spp": spp, "plain_seconds": tx, "plain_rel_rmse": float(np.sqrt(max(mse_plain, 0.0)) / mean)}
     for name, prec in (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16)):
-        run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec)
+        run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec, spp_per_frame=4)
         mse = cross_mse(run.image, a, b)
         n_star = spp * mse_plain / mse if mse > 0 else float("inf")
         t_equal = tx * n_star / spp

@@ -434,7 +434,7 @@ size_t rtc8x_nrc_scratch_bytes(uint32_t width, uint32_t height);
  * network_interface.cu:50-159) and the tiny-cuda-nn objects behind it.
  *
  * Input: n RadianceQuery records (14 floats each, AoS).  Encoding (64 dims): TriangleWave with 12
- * frequencies on the 3 position dims (36), OneBlob with 4 bins on the 5 angle/roughness dims (20;
+ * frequencies on the 3 position dims (36; |2 frac(x 2^(f-1)) - 1|, period 2 at the base frequency), OneBlob with 4 bins on the 5 angle/roughness dims (20;
  * polar angles scaled by 1/pi, azimuths by 1/(2 pi)), identity on the 6 reflectance dims, then a
  * constant 1 (the first layer's bias) and one zero pad.  Network: num_hidden_layers ReLU layers of
  * width 64 and a linear layer to RGB, no biases (FullyFusedMLP).  Loss: relative L2 over luminance,

@@ -65,7 +65,7 @@ void oracle_nrc_encode(const rtc8x_radiance_query* q, uint32_t n, double* out)
         double* e = out + (size_t)s * 64;
         for (int d = 0; d < 3; ++d)
             for (int f = 0; f < 12; ++f) {
-                double t = (double)q[s].position[d] * ldexp(1.0, f);
+                double t = (double)q[s].position[d] * ldexp(1.0, f - 1);  /* period 2 / 2^f */
                 t = t - floor(t);
                 e[12 * d + f] = fabs(2.0 * t - 1.0);
             }

@@ -60,9 +60,11 @@ __host__ __device__ constexpr uint32_t num_params_t(uint32_t L) { return L * kLa
 // Input encoding of one RadianceQuery (renderer/renderer_shared.h:46-55), 64 features.
 __device__ __forceinline__ void encode(const rtc8x_radiance_query& q, float* e)
 {
+    // Triangle waves of period 2 / 2^f: tri(x 2^f) = |2 frac(x 2^(f-1)) - 1|.  The base frequency is
+    // monotonic on [0, 1]; with period 1 at every frequency the encoding would not tell x from 1 - x.
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        float sc = 1.0f;
+        float sc = 0.5f;
 #pragma unroll
         for (int f = 0; f < 12; ++f) {
             float t = q.position[d] * sc;

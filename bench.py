@@ -370,8 +370,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("trace_kernel", cfg.name),
-            "kernel": "trace_kernel<PATH_TRACE>",
+            "traffic": traffic_from_profiles("wavefront_kernel", cfg.name),
+            "kernel": "wavefront_kernel<PATH_TRACE, GLOBAL_MAJORANT>",
             "kernel_ms_avg": float(np.mean(kernel_ms)),
             "algorithmic_bytes_per_launch": lookups_per_launch * BYTES_PER_LOOKUP,
         },

@@ -121,9 +121,11 @@ def random_leaf_queries(scene, n: int, seed: int) -> np.ndarray:
     return q
 
 
-def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 21, n_train: int = 1 << 16):
-    """Standalone network step: train on 2^16 samples, then infer 2^21 queries (5 hidden layers of 64,
-    FP16 storage, FP32 accumulation)."""
+def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 21, n_train: int = 1 << 16,
+                encoding: int = 0):
+    """Standalone network step: train on 2^16 samples, then infer 2^21 queries (FP16 storage, FP32
+    accumulation); the benchmark configuration is 5 hidden layers of 64 with the frequency (triangle
+    wave) encoding, the renderer's default 2 hidden layers with the hash grid."""
     from rtcamp8_amd import _abi
     from rtcamp8_amd.device import DeviceBuffer, EventTimer
     from rtcamp8_amd.nrc import NeuralRadianceCache
@@ -132,7 +134,7 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     t = (0.2 + np.random.default_rng(99).random((n_train, 3))).astype(np.float32)
     dq, dt, dout = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t), DeviceBuffer(12 * n_infer)
     cache = NeuralRadianceCache()
-    cache.initialize(num_hidden_layers=hidden_layers, learning_rate=1e-2, precision=_abi.NRC_FP16)
+    cache.initialize(num_hidden_layers=hidden_layers, learning_rate=1e-2, precision=_abi.NRC_FP16, encoding=encoding)
     for _ in range(2):
         cache.train(None, dq.ptr, dt.ptr, n_train)
         cache.infer(None, dq.ptr, n_infer, dout.ptr)
@@ -157,6 +159,7 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     train_flop = 3.0 * fwd_flop  # forward + backward (input and weight gradients)
     return {
         "hidden_layers": hidden_layers, "precision": "fp16 storage, fp32 accumulate",
+        "encoding": "hash grid (16 levels x 2, 2^15)" if encoding == 1 else "triangle waves (12 frequencies)",
         "infer_queries": n_infer, "train_samples": n_train,
         "infer_ms": infer_ms, "train_ms": train_ms,
         "queries_per_s": n_infer / (infer_ms * 1e-3),
@@ -385,13 +388,14 @@ def main() -> int:
     if args.nrc_steps > 0:
         print("[bench] NRC network leg ...", file=sys.stderr, flush=True)
         mlp = nrc_mlp_leg(scene, args.nrc_steps)
+        mlp_hash = nrc_mlp_leg(scene, args.nrc_steps, hidden_layers=2, encoding=1)
         print(f"[bench] NRC network: infer {mlp['infer_ms']:.2f} ms / 2^21, train {mlp['train_ms']:.2f} ms / 2^16",
               file=sys.stderr, flush=True)
         print("[bench] NRC render leg ...", file=sys.stderr, flush=True)
         rend = nrc_render_leg(scene, args.nrc_steps, stream)
         print(f"[bench] NRC render: {rend['ms_per_spp']:.2f} ms/spp", file=sys.stderr, flush=True)
         result["nrc_queries_per_s"] = mlp["queries_per_s"]
-        result["nrc"] = {"network": mlp, "render": rend}
+        result["nrc"] = {"network": mlp, "network_hash_grid": mlp_hash, "render": rend}
     if args.rmse_ref_spp > 0:
         print("[bench] equal-RMSE leg ...", file=sys.stderr, flush=True)
         eq = equal_rmse_leg(scene, args.rmse_ref_spp, cfg.spp, stream)

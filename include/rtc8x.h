@@ -451,16 +451,39 @@ typedef enum rtc8x_nrc_precision {
     RTC8X_NRC_BF16 = 1
 } rtc8x_nrc_precision;
 
+typedef enum rtc8x_nrc_encoding {
+    RTC8X_NRC_TRIANGLE_WAVE = 0,  /* PositionEncoding::TriangleWave (network_interface.cu:76-98) */
+    RTC8X_NRC_HASH_GRID = 1       /* PositionEncoding::HashGrid, the reference's default (network_interface.cu:99-123,
+                                     renderer_scene.cpp:50) */
+} rtc8x_nrc_encoding;
+
 typedef struct rtc8x_nrc_desc {
     uint32_t num_hidden_layers;  /* renderer_scene.cpp:50 uses 2; the standalone benchmark 5 (1..8) */
     float learning_rate;         /* 1e-2 */
     float beta1, beta2;          /* 0.9, 0.99 */
-    float epsilon;               /* 1e-8 (TriangleWave configuration, network_interface.cu:96) */
+    float epsilon;               /* 1e-8 with TriangleWave, 1e-15 with HashGrid (network_interface.cu:96,122) */
     float l2_reg;                /* 1e-6 */
     float ema_decay;             /* 0.99 */
     int32_t precision;           /* rtc8x_nrc_precision */
     uint64_t seed;               /* weight initialisation (Xavier-uniform from PCG32) */
+    int32_t encoding;            /* rtc8x_nrc_encoding */
+    uint32_t reserved;
 } rtc8x_nrc_desc;
+
+/* HashGrid position encoding (multiresolution hash encoding, Mueller et al. 2022, with the reference's
+ * settings, network_interface.cu:103-111): 16 levels x 2 features, 2^15 entries per level, base
+ * resolution 16, resolution x2 per level.  Level l has scale s = 16 * 2^l - 1 and resolution
+ * r = ceil(s) + 1; a position p in [0,1]^3 is trilinearly interpolated between the corners
+ * floor(p s + 1/2) + {0,1} (coordinates 0..r).  A level whose (r + 1)^3 corners fit the table is
+ * indexed densely, x + y (r + 1) + z (r + 1)^2; the others by the spatial hash
+ * (x * 1) ^ (y * 2654435761) ^ (z * 805459861) mod 2^15.  The 32 grid features take the place of the
+ * 36 triangle waves; the 64-feature input is then [grid 32 | OneBlob 20 | identity 6 | 1 | 0 x 5].
+ * The tables (FP32, initialised uniformly in [-1e-4, 1e-4]) follow the MLP weights in the parameter
+ * vector. */
+#define RTC8X_NRC_HASH_LEVELS 16
+#define RTC8X_NRC_HASH_FEATURES 2
+#define RTC8X_NRC_HASH_LOG2_TABLE 15
+#define RTC8X_NRC_HASH_BASE_RESOLUTION 16
 
 #define RTC8X_NRC_INPUT_DIMS 14
 #define RTC8X_NRC_ENCODED_DIMS 64
@@ -487,6 +510,10 @@ rtc8x_status rtc8x_nrc_set_weights(rtc8x_nrc* nrc, const float* host, size_t cou
 /* Parity probes: the 64-dim encoding of n queries (out[64n]), and the summed per-sample loss gradient
  * with respect to the weights for one batch without an update (grad[num_params], device). */
 rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t n, float* out);
+/* The network's own input encoding (either kind), with its training (which = 0) or inference (1)
+ * hash tables; out[64n]. */
+rtc8x_status rtc8x_nrc_encode_net(const rtc8x_nrc* nrc, void* stream, const float* in, uint32_t n, float* out,
+                                  int32_t which);
 rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* nrc, void* stream, const float* in, const float* target, uint32_t n,
                                 float* grad);
 

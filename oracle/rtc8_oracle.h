@@ -73,6 +73,15 @@ double oracle_nrc_loss(const float* weights, uint32_t L, const rtc8x_radiance_qu
                        int round_mode);
 double oracle_nrc_gradient(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target,
                            uint32_t n, double* grad, int round_mode);
+/* Either encoding (rtc8x_nrc_encoding); with the hash grid the tables follow the MLP weights in
+ * `weights` / `grad` (oracle_nrc_num_params_ex floats in all). */
+uint32_t oracle_nrc_num_params_ex(uint32_t L, int32_t encoding);
+void oracle_nrc_encode_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q, uint32_t n,
+                          double* out);
+void oracle_nrc_forward_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q, uint32_t n,
+                           double* out3, int round_mode);
+double oracle_nrc_gradient_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q,
+                              const float* target, uint32_t n, double* grad, int round_mode);
 void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_sum, uint32_t P, uint32_t n,
                      float lr, float beta1, float beta2, float eps, float l2, float decay, uint64_t step);
 float oracle_round_fp16(float x);

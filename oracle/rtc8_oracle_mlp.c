@@ -85,15 +85,129 @@ void oracle_nrc_encode(const rtc8x_radiance_query* q, uint32_t n, double* out)
     }
 }
 
+/* ---- HashGrid position encoding (multiresolution hash encoding, Mueller et al. 2022) with the
+ * reference's settings (renderer/network_interface.cu:103-111): 16 levels x 2 features, 2^15 entries,
+ * base resolution 16, x2 per level.  Level geometry and indexing as include/rtc8x.h states them. */
+#define HL RTC8X_NRC_HASH_LEVELS
+#define HT (1u << RTC8X_NRC_HASH_LOG2_TABLE)
+
+typedef struct {
+    double scale[HL];
+    uint32_t stride[HL], offset[HL];
+    uint32_t total;
+} hash_geom;
+
+static void hash_geometry(hash_geom* g)
+{
+    uint32_t off = 0;
+    for (uint32_t l = 0; l < HL; ++l) {
+        double sc = ldexp(1.0, (int)l) * RTC8X_NRC_HASH_BASE_RESOLUTION - 1.0;
+        uint64_t r1 = (uint64_t)ceil(sc) + 2;
+        uint64_t dense = r1 * r1 * r1;
+        uint32_t entries = dense <= HT ? (uint32_t)((dense + 7) / 8 * 8) : HT;
+        g->scale[l] = sc;
+        g->stride[l] = dense <= HT ? (uint32_t)r1 : 0u;
+        g->offset[l] = off;
+        off += entries * RTC8X_NRC_HASH_FEATURES;
+    }
+    g->total = off;
+}
+
+/* the 8 corners of level l around position p in [0,1]^3: table float index and trilinear weight */
+static void hash_corners(const hash_geom* g, uint32_t l, const double p[3], uint32_t idx[8], double w[8])
+{
+    /* the cell position in FP32, as the method computes it (p s + 1/2, two roundings): at the fine
+     * levels (s up to 524287) FP32 quantises the fraction, and that is part of the encoding */
+    uint32_t gi[3];
+    double f[3];
+    for (int d = 0; d < 3; ++d) {
+        float pos = (float)p[d] * (float)g->scale[l];
+        pos = pos + 0.5f;
+        float fl = floorf(pos);
+        f[d] = (double)(pos - fl);
+        gi[d] = (uint32_t)fl;
+    }
+    for (uint32_t c = 0; c < 8; ++c) {
+        uint32_t x = gi[0] + (c & 1u), y = gi[1] + ((c >> 1) & 1u), z = gi[2] + (c >> 2);
+        w[c] = ((c & 1u) ? f[0] : 1.0 - f[0]) * ((c & 2u) ? f[1] : 1.0 - f[1]) * ((c & 4u) ? f[2] : 1.0 - f[2]);
+        uint32_t st = g->stride[l];
+        uint32_t e = st ? x + y * st + z * st * st : ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & (HT - 1u);
+        idx[c] = g->offset[l] + 2u * e;
+    }
+}
+
+static void query_p01(const rtc8x_radiance_query* q, double p[3])
+{
+    for (int d = 0; d < 3; ++d) {
+        float v = q->position[d];
+        v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        p[d] = (double)v;
+    }
+}
+
+/* [grid 32 | OneBlob 20 | identity 6 | 1 | 0 x 5] */
+static void encode_hash(const rtc8x_radiance_query* q, const float* tables, const hash_geom* g, double* e)
+{
+    const double pi = 3.14159265358979323846;
+    double p[3];
+    query_p01(q, p);
+    for (uint32_t l = 0; l < HL; ++l) {
+        uint32_t idx[8];
+        double w[8];
+        hash_corners(g, l, p, idx, w);
+        double a0 = 0.0, a1 = 0.0;
+        for (int c = 0; c < 8; ++c) {
+            a0 += w[c] * (double)tables[idx[c]];
+            a1 += w[c] * (double)tables[idx[c] + 1];
+        }
+        e[2 * l] = a0;
+        e[2 * l + 1] = a1;
+    }
+    double a[5] = {q->normal_phi / pi, q->normal_theta / (2 * pi), q->vout_phi / pi, q->vout_theta / (2 * pi), q->roughness};
+    for (int j = 0; j < 5; ++j)
+        for (int b = 0; b < 4; ++b) {
+            double dd = a[j] - (b + 0.5) * 0.25;
+            e[32 + 4 * j + b] = exp(-8.0 * dd * dd);
+        }
+    for (int c = 0; c < 3; ++c) {
+        e[52 + c] = q->diffuse_reflectance[c];
+        e[55 + c] = q->specular_reflectance[c];
+    }
+    e[58] = 1.0;
+    for (int j = 59; j < 64; ++j) e[j] = 0.0;
+}
+
+uint32_t oracle_nrc_num_params_ex(uint32_t L, int32_t encoding)
+{
+    hash_geom g;
+    hash_geometry(&g);
+    return oracle_nrc_num_params(L) + (encoding == RTC8X_NRC_HASH_GRID ? g.total : 0u);
+}
+
+void oracle_nrc_encode_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q, uint32_t n,
+                          double* out)
+{
+    if (encoding != RTC8X_NRC_HASH_GRID) {
+        oracle_nrc_encode(q, n, out);
+        return;
+    }
+    hash_geom g;
+    hash_geometry(&g);
+    for (uint32_t s = 0; s < n; ++s) encode_hash(&q[s], weights + oracle_nrc_num_params(L), &g, out + (size_t)s * 64);
+}
+
 static double luminance(const double* p) { return 0.2126729 * p[0] + 0.7151522 * p[1] + 0.0721750 * p[2]; }
 
 /* Forward + (optionally) backward for n samples.  Returns the summed per-sample loss
  * sum_c (p_c - t_c)^2 / (lum(p)^2 + 0.01); grad (if not NULL) receives the summed per-sample gradient
  * of that loss with respect to the weights (the library's unscaled backward). */
-static double run(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target, uint32_t n,
-                  double* out3, double* grad, int mode)
+static double run(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q, const float* target,
+                  uint32_t n, double* out3, double* grad, int mode)
 {
-    uint32_t P = oracle_nrc_num_params(L);
+    uint32_t P = oracle_nrc_num_params(L);  /* MLP part; hash tables follow it in `weights` and `grad` */
+    uint32_t P_all = oracle_nrc_num_params_ex(L, encoding);
+    hash_geom hg;
+    hash_geometry(&hg);
     double* w = (double*)malloc(sizeof(double) * P);
     for (uint32_t i = 0; i < P; ++i) w[i] = rnd((double)weights[i], mode);
     double* h = (double*)malloc(sizeof(double) * 64 * (L + 1));
@@ -101,9 +215,12 @@ static double run(const float* weights, uint32_t L, const rtc8x_radiance_query* 
     double* dh = (double*)malloc(sizeof(double) * 64);
     double enc[64];
     double total = 0.0;
-    if (grad) memset(grad, 0, sizeof(double) * P);
+    if (grad) memset(grad, 0, sizeof(double) * P_all);
     for (uint32_t s = 0; s < n; ++s) {
-        oracle_nrc_encode(&q[s], 1, enc);
+        if (encoding == RTC8X_NRC_HASH_GRID)
+            encode_hash(&q[s], weights + P, &hg, enc);
+        else
+            oracle_nrc_encode(&q[s], 1, enc);
         for (int j = 0; j < 64; ++j) h[j] = rnd(enc[j], mode);
         for (uint32_t l = 0; l < L; ++l) {
             const double* wl = w + (size_t)l * LAYER;
@@ -147,6 +264,27 @@ static double run(const float* weights, uint32_t L, const rtc8x_radiance_query* 
             const double* hin = h + 64 * l;
             for (int o = 0; o < W; ++o)
                 for (int i = 0; i < W; ++i) grad[(size_t)l * LAYER + o * W + i] += dz[o] * hin[i];
+            if (l == 0 && encoding == RTC8X_NRC_HASH_GRID) {
+                /* gradient of the 32 grid features, scattered with the trilinear weights */
+                const double* w0 = w;
+                double p[3];
+                query_p01(&q[s], p);
+                for (uint32_t lv = 0; lv < HL; ++lv) {
+                    double de[2];
+                    for (int f = 0; f < 2; ++f) {
+                        double acc = 0.0;
+                        for (int o = 0; o < W; ++o) acc += w0[o * W + 2 * lv + f] * dz[o];
+                        de[f] = acc;
+                    }
+                    uint32_t idx[8];
+                    double cw[8];
+                    hash_corners(&hg, lv, p, idx, cw);
+                    for (int c = 0; c < 8; ++c) {
+                        grad[P + idx[c]] += cw[c] * de[0];
+                        grad[P + idx[c] + 1] += cw[c] * de[1];
+                    }
+                }
+            }
             if (l == 0) break;
             const double* wl = w + (size_t)l * LAYER;
             for (int i = 0; i < W; ++i) {
@@ -166,19 +304,31 @@ static double run(const float* weights, uint32_t L, const rtc8x_radiance_query* 
 void oracle_nrc_forward(const float* weights, uint32_t L, const rtc8x_radiance_query* q, uint32_t n, double* out3,
                         int round_mode)
 {
-    run(weights, L, q, NULL, n, out3, NULL, round_mode);
+    run(weights, L, RTC8X_NRC_TRIANGLE_WAVE, q, NULL, n, out3, NULL, round_mode);
 }
 
 double oracle_nrc_loss(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target, uint32_t n,
                        int round_mode)
 {
-    return run(weights, L, q, target, n, NULL, NULL, round_mode);
+    return run(weights, L, RTC8X_NRC_TRIANGLE_WAVE, q, target, n, NULL, NULL, round_mode);
 }
 
 double oracle_nrc_gradient(const float* weights, uint32_t L, const rtc8x_radiance_query* q, const float* target,
                            uint32_t n, double* grad, int round_mode)
 {
-    return run(weights, L, q, target, n, NULL, grad, round_mode);
+    return run(weights, L, RTC8X_NRC_TRIANGLE_WAVE, q, target, n, NULL, grad, round_mode);
+}
+
+void oracle_nrc_forward_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q, uint32_t n,
+                           double* out3, int round_mode)
+{
+    run(weights, L, encoding, q, NULL, n, out3, NULL, round_mode);
+}
+
+double oracle_nrc_gradient_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q,
+                              const float* target, uint32_t n, double* grad, int round_mode)
+{
+    return run(weights, L, encoding, q, target, n, NULL, grad, round_mode);
 }
 
 /* One Adam + EMA step in FP32, in the library's expression order, from a summed gradient. */

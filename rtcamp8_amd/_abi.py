@@ -229,6 +229,7 @@ class HosekBand(C.Structure):
 SKY_BANDS = 16
 
 NRC_FP16, NRC_BF16 = 0, 1
+NRC_TRIANGLE_WAVE, NRC_HASH_GRID = 0, 1
 
 
 class NrcDesc(C.Structure):
@@ -242,6 +243,8 @@ class NrcDesc(C.Structure):
         ("ema_decay", C.c_float),
         ("precision", C.c_int32),
         ("seed", C.c_uint64),
+        ("encoding", C.c_int32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -331,5 +334,6 @@ PROTOTYPES = {
     "rtc8x_nrc_get_weights": (_S, [_P, C.c_int32, _P, C.c_size_t]),
     "rtc8x_nrc_set_weights": (_S, [_P, _P, C.c_size_t]),
     "rtc8x_nrc_encode": (_S, [_P, _P, C.c_uint32, _P]),
+    "rtc8x_nrc_encode_net": (_S, [_P, _P, _P, C.c_uint32, _P, C.c_int32]),
     "rtc8x_nrc_gradient": (_S, [_P, _P, _P, _P, C.c_uint32, _P]),
 }

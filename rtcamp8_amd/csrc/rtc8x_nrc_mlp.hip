@@ -54,6 +54,64 @@ constexpr uint32_t kColStride = 264;                     // LDS row stride of a 
 constexpr uint32_t kMaxTrain = 65536;
 constexpr uint32_t kMaxLayers = 8;
 
+constexpr uint32_t kHashLevels = RTC8X_NRC_HASH_LEVELS;
+constexpr uint32_t kHashT = 1u << RTC8X_NRC_HASH_LOG2_TABLE;
+constexpr int kEncTriangle = RTC8X_NRC_TRIANGLE_WAVE;
+constexpr int kEncHash = RTC8X_NRC_HASH_GRID;
+
+// Per-level geometry of the hash-grid encoding (the scale, the dense stride r + 1 or 0 when the level
+// is hashed, and the level's float offset in the tables).
+struct HashGeom {
+    float scale[kHashLevels];
+    uint32_t stride[kHashLevels];
+    uint32_t offset[kHashLevels];
+};
+
+// Fills the geometry; returns the number of table floats.
+inline uint32_t hash_geometry(HashGeom* g)
+{
+    uint32_t off = 0;
+    for (uint32_t l = 0; l < kHashLevels; ++l) {
+        const float sc = std::exp2((float)l) * (float)RTC8X_NRC_HASH_BASE_RESOLUTION - 1.0f;
+        const uint64_t r1 = (uint64_t)std::ceil(sc) + 2;  // corner coordinates 0..r, r = ceil(s) + 1
+        const uint64_t dense = r1 * r1 * r1;
+        const uint32_t entries = dense <= kHashT ? (uint32_t)((dense + 7) / 8 * 8) : kHashT;
+        g->scale[l] = sc;
+        g->stride[l] = dense <= kHashT ? (uint32_t)r1 : 0u;
+        g->offset[l] = off;
+        off += entries * RTC8X_NRC_HASH_FEATURES;
+    }
+    return off;
+}
+
+// Corner c (bit 0: x, 1: y, 2: z) of level l for the clamped position p: table float index and weight.
+__device__ __forceinline__ void hash_corner(const HashGeom& gm, uint32_t l, const uint32_t gi[3], const float f[3],
+                                            uint32_t c, uint32_t& idx, float& w)
+{
+    const uint32_t x = gi[0] + (c & 1u), y = gi[1] + ((c >> 1) & 1u), z = gi[2] + (c >> 2);
+    w = ((c & 1u) ? f[0] : 1.0f - f[0]) * ((c & 2u) ? f[1] : 1.0f - f[1]) * ((c & 4u) ? f[2] : 1.0f - f[2]);
+    const uint32_t st = gm.stride[l];
+    const uint32_t e = st ? x + y * st + z * st * st : ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & (kHashT - 1u);
+    idx = gm.offset[l] + 2u * e;
+}
+
+__device__ __forceinline__ void hash_cell(const HashGeom& gm, uint32_t l, const float p[3], uint32_t gi[3], float f[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float pos = p[d] * gm.scale[l] + 0.5f;
+        const float fl = floorf(pos);
+        f[d] = pos - fl;
+        gi[d] = (uint32_t)fl;
+    }
+}
+
+__device__ __forceinline__ void query_position01(const rtc8x_radiance_query& q, float p[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[d] = fminf(fmaxf(q.position[d], 0.0f), 1.0f);
+}
+
 __host__ __device__ constexpr uint32_t num_params(uint32_t L) { return L * kLayer + kOutPad * kWidth; }
 __host__ __device__ constexpr uint32_t num_params_t(uint32_t L) { return L * kLayer + kWidth * kOutT; }
 
@@ -94,12 +152,67 @@ __device__ __forceinline__ void encode(const rtc8x_radiance_query& q, float* e)
     e[63] = 0.0f;
 }
 
-template <typename T, bool TRAIN>
+// Input encoding with the hash grid: [grid 32 | OneBlob 20 | identity 6 | 1 | 0 x 5].
+__device__ __forceinline__ void encode_hash(const rtc8x_radiance_query& q, const float* __restrict__ tables,
+                                            const HashGeom& gm, float* e)
+{
+    float p[3];
+    query_position01(q, p);
+    for (uint32_t l = 0; l < kHashLevels; ++l) {
+        uint32_t gi[3];
+        float f[3];
+        hash_cell(gm, l, p, gi, f);
+        float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            uint32_t idx;
+            float w;
+            hash_corner(gm, l, gi, f, c, idx, w);
+            float2 v;
+            __builtin_memcpy(&v, tables + idx, sizeof(v));
+            a0 += w * v.x;
+            a1 += w * v.y;
+        }
+        e[2 * l] = a0;
+        e[2 * l + 1] = a1;
+    }
+    const float inv_pi = 1.0f / RTC8X_PI_F, inv_2pi = 1.0f / (2.0f * RTC8X_PI_F);
+    const float a[5] = {q.normal_phi * inv_pi, q.normal_theta * inv_2pi, q.vout_phi * inv_pi, q.vout_theta * inv_2pi,
+                        q.roughness};
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float d = a[j] - ((float)b + 0.5f) * 0.25f;
+            e[32 + 4 * j + b] = expf(-8.0f * d * d);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        e[52 + c] = q.diffuse_reflectance[c];
+        e[55 + c] = q.specular_reflectance[c];
+    }
+    e[58] = 1.0f;
+#pragma unroll
+    for (int j = 59; j < 64; ++j) e[j] = 0.0f;
+}
+
+template <int ENC>
+__device__ __forceinline__ void encode_any(const rtc8x_radiance_query& q, const float* tables, const HashGeom& gm, float* e)
+{
+    if constexpr (ENC == kEncHash)
+        encode_hash(q, tables, gm, e);
+    else
+        encode(q, e);
+}
+
+template <typename T, bool TRAIN, int ENC>
 __global__ __launch_bounds__(256) void forward_kernel(const T* __restrict__ w, uint32_t L,
                                                       const rtc8x_radiance_query* __restrict__ in, uint32_t n,
                                                       float* __restrict__ out, const float* __restrict__ target,
                                                       T* __restrict__ hbuf, T* __restrict__ dout,
-                                                      float* __restrict__ loss_part)
+                                                      float* __restrict__ loss_part, const float* __restrict__ tables,
+                                                      const HashGeom geom)
 {
     using V = typename Vec8<T>::type;
     __shared__ __align__(16) T act[4][64 * kActStride];
@@ -111,7 +224,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const T* __restrict__ w, u
     {
         float e[64];
         if (s < n) {
-            encode(in[s], e);
+            encode_any<ENC>(in[s], tables, geom, e);
         } else {
 #pragma unroll
             for (int j = 0; j < 64; ++j) e[j] = 0.0f;
@@ -199,10 +312,11 @@ __global__ __launch_bounds__(256) void forward_kernel(const T* __restrict__ w, u
     if (lane == 0) loss_part[blockIdx.x * 4 + wave] = loss;
 }
 
-template <typename T>
+template <typename T, int ENC>
 __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt, uint32_t L, uint32_t n,
                                                        const T* __restrict__ hbuf, const T* __restrict__ dout,
-                                                       float* __restrict__ gpart)
+                                                       float* __restrict__ gpart, const rtc8x_radiance_query* __restrict__ in,
+                                                       const HashGeom geom, float* __restrict__ ggrid)
 {
     using V = typename Vec8<T>::type;
     __shared__ __align__(16) T dzs[256 * kActStride];   // dZ, sample-major [sample][out]
@@ -242,8 +356,9 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
 #pragma unroll
             for (int i = 0; i < 4; ++i) gblk[goff + (mt * 16 + 4 * (lane >> 4) + i) * kWidth + nt * 16 + r16] = acc[i];
         }
-        if (l == 0) break;
-        // dH_l = dZ_l W_l for this wavefront's 64 samples (K = 16, padded to one 32-step, for the top layer)
+        if (l == 0 && ENC != kEncHash) break;
+        // dH_l = dZ_l W_l for this wavefront's 64 samples (K = 16, padded to one 32-step, for the top layer);
+        // for l = 0 with the hash grid, the gradient of the encoded input
         const T* WT = wt + (size_t)l * kLayer;
         const uint32_t ldk = top ? kOutT : kWidth;
         const int ksteps = top ? 1 : 2;
@@ -265,6 +380,38 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
             }
         }
         __syncthreads();
+        if (ENC == kEncHash && l == 0) {
+            // dE for the 32 grid features, FP32, sample-major in the (now free) H tile
+            float* dE = reinterpret_cast<float*>(hf);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        dE[(wave * 64 + mt * 16 + 4 * (lane >> 4) + i) * 32 + nt * 16 + r16] = acc[mt][nt][i];
+            __syncthreads();
+            // scatter into the tables' gradient with the trilinear weights
+            if (b0 + tid < n) {
+                float p[3];
+                query_position01(in[b0 + tid], p);
+                for (uint32_t lv = 0; lv < kHashLevels; ++lv) {
+                    uint32_t gi[3];
+                    float f[3];
+                    hash_cell(geom, lv, p, gi, f);
+                    const float g0 = dE[tid * 32 + 2 * lv], g1 = dE[tid * 32 + 2 * lv + 1];
+#pragma unroll
+                    for (uint32_t c = 0; c < 8; ++c) {
+                        uint32_t idx;
+                        float w;
+                        hash_corner(geom, lv, gi, f, c, idx, w);
+                        atomicAdd(ggrid + idx, w * g0);
+                        atomicAdd(ggrid + idx + 1, w * g1);
+                    }
+                }
+            }
+            break;
+        }
         // ReLU mask: H_l is the post-ReLU output of layer l-1, so the derivative is [H_l > 0]
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
@@ -283,14 +430,20 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
 }
 
 // Sum of the per-workgroup weight gradients in workgroup order; flags non-finite sums.
-__global__ void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P,
+// (Hash-grid table gradients, entries P .. P_total, are accumulated directly and only checked here.)
+__global__ void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P, uint32_t P_total,
                                    float* __restrict__ grad, uint32_t* __restrict__ bad)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    float g = 0.0f;
-    for (uint32_t b = 0; b < nblocks; ++b) g += gpart[(size_t)b * P + i];
-    grad[i] = g;
+    if (i >= P_total) return;
+    float g;
+    if (i < P) {
+        g = 0.0f;
+        for (uint32_t b = 0; b < nblocks; ++b) g += gpart[(size_t)b * P + i];
+        grad[i] = g;
+    } else {
+        g = grad[i];
+    }
     if (!isfinite(g)) atomicOr(bad, 1u);
 }
 
@@ -299,13 +452,14 @@ struct AdamArgs {
     float inv_scale;      // 1 / (3 n): the loss scale of the backward pass
     float bc1, bc2;       // 1 - beta^t
     float ema_bc;         // 1 - decay^t
-    uint32_t P, L;
+    uint32_t P, L;        // P = all parameters (MLP, then hash tables)
+    uint32_t P_mlp;
 };
 
 template <typename T>
 __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
                             float* __restrict__ ema, const float* __restrict__ grad, uint32_t* __restrict__ bad,
-                            T* __restrict__ w, T* __restrict__ wt, T* __restrict__ winf)
+                            T* __restrict__ w, T* __restrict__ wt, T* __restrict__ winf, float* __restrict__ grid_inf)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.P) return;
@@ -320,6 +474,10 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
     master[i] = w1;
     const float e = a.decay * ema[i] + (1.0f - a.decay) * w1;
     ema[i] = e;
+    if (i >= a.P_mlp) {  // hash table entry: the FP32 master is the training table
+        grid_inf[i - a.P_mlp] = e / a.ema_bc;
+        return;
+    }
     w[i] = (T)w1;
     winf[i] = (T)(e / a.ema_bc);
     const uint32_t l = i / kLayer, r = i % kLayer, out = r / kWidth, in = r % kWidth;
@@ -331,10 +489,14 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
 
 template <typename T>
 __global__ void convert_kernel(const float* __restrict__ src, uint32_t P, uint32_t L, T* __restrict__ w, T* __restrict__ wt,
-                               T* __restrict__ winf)
+                               T* __restrict__ winf, uint32_t P_total, float* __restrict__ grid_inf)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
+    if (i >= P_total) return;
+    if (i >= P) {
+        grid_inf[i - P] = src[i];
+        return;
+    }
     const T x = (T)src[i];
     w[i] = x;
     winf[i] = x;
@@ -345,12 +507,14 @@ __global__ void convert_kernel(const float* __restrict__ src, uint32_t P, uint32
         wt[(size_t)L * kLayer + in * kOutT + out] = x;
 }
 
-__global__ void encode_kernel(const rtc8x_radiance_query* __restrict__ in, uint32_t n, float* __restrict__ out)
+template <int ENC>
+__global__ void encode_kernel(const rtc8x_radiance_query* __restrict__ in, uint32_t n, float* __restrict__ out,
+                              const float* __restrict__ tables, const HashGeom geom)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     float e[64];
-    encode(in[s], e);
+    encode_any<ENC>(in[s], tables, geom, e);
     for (int j = 0; j < 64; ++j) out[(size_t)s * 64 + j] = e[j];
 }
 
@@ -359,39 +523,64 @@ __global__ void encode_kernel(const rtc8x_radiance_query* __restrict__ in, uint3
 
 using namespace rtc8x;
 
+// Parameter vector: the MLP's weights (P_mlp, [out][in] per layer), then the hash tables (P_grid floats,
+// zero without the hash grid).  master / m / v / ema / grad hold all P = P_mlp + P_grid.
 struct rtc8x_nrc {
     rtc8x_nrc_desc desc;
-    uint32_t L = 0, P = 0, PT = 0;
+    uint32_t L = 0, P = 0, P_mlp = 0, P_grid = 0, PT = 0;
+    int enc = RTC8X_NRC_TRIANGLE_WAVE;
+    nrc::HashGeom geom{};
     size_t tsize = 2;
     uint64_t step = 0;
     float *master = nullptr, *m = nullptr, *v = nullptr, *ema = nullptr, *grad = nullptr, *gpart = nullptr;
     float* loss_part = nullptr;
+    float* grid_inf = nullptr;  // EMA (debiased) hash tables for inference
     uint32_t* bad = nullptr;
     void *w = nullptr, *wt = nullptr, *winf = nullptr, *hbuf = nullptr, *dout = nullptr;
 };
 
 namespace {
 
-template <typename T>
-hipError_t launch_forward(rtc8x_nrc* h, bool train, const void* w, hipStream_t s, const float* in, uint32_t n, float* out,
-                          const float* target)
+template <typename T, int ENC>
+hipError_t launch_forward_enc(rtc8x_nrc* h, bool train, const void* w, const float* tables, hipStream_t s, const float* in,
+                              uint32_t n, float* out, const float* target)
 {
     const uint32_t blocks = (n + 255) / 256;
     if (train)
-        hipLaunchKernelGGL((nrc::forward_kernel<T, true>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
-                           (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part);
+        hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
+                           (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part, tables,
+                           h->geom);
     else
-        hipLaunchKernelGGL((nrc::forward_kernel<T, false>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
-                           (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr, (float*)nullptr);
+        hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
+                           (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr, (float*)nullptr,
+                           tables, h->geom);
     return hipGetLastError();
 }
 
+// train: the working weights and the FP32 master tables; inference: the EMA weights and tables
 template <typename T>
-hipError_t launch_backward(rtc8x_nrc* h, hipStream_t s, uint32_t n)
+hipError_t launch_forward(rtc8x_nrc* h, bool train, hipStream_t s, const float* in, uint32_t n, float* out,
+                          const float* target)
+{
+    const void* w = train ? h->w : h->winf;
+    const float* tables = train ? h->master + h->P_mlp : h->grid_inf;
+    if (h->enc == RTC8X_NRC_HASH_GRID)
+        return launch_forward_enc<T, RTC8X_NRC_HASH_GRID>(h, train, w, tables, s, in, n, out, target);
+    return launch_forward_enc<T, RTC8X_NRC_TRIANGLE_WAVE>(h, train, w, tables, s, in, n, out, target);
+}
+
+template <typename T>
+hipError_t launch_backward(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n)
 {
     const uint32_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL((nrc::backward_kernel<T>), dim3(blocks), dim3(256), 0, s, (const T*)h->wt, h->L, n,
-                       (const T*)h->hbuf, (const T*)h->dout, h->gpart);
+    if (h->enc == RTC8X_NRC_HASH_GRID)
+        hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, (const T*)h->wt,
+                           h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, (const rtc8x_radiance_query*)in, h->geom,
+                           h->grad + h->P_mlp);
+    else
+        hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_TRIANGLE_WAVE>), dim3(blocks), dim3(256), 0, s,
+                           (const T*)h->wt, h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart,
+                           (const rtc8x_radiance_query*)in, h->geom, (float*)nullptr);
     return hipGetLastError();
 }
 
@@ -399,15 +588,15 @@ template <typename T>
 hipError_t launch_adam(rtc8x_nrc* h, hipStream_t s, const nrc::AdamArgs& a)
 {
     hipLaunchKernelGGL((nrc::adam_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, a, h->master, h->m, h->v, h->ema,
-                       h->grad, h->bad, (T*)h->w, (T*)h->wt, (T*)h->winf);
+                       h->grad, h->bad, (T*)h->w, (T*)h->wt, (T*)h->winf, h->grid_inf);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_convert(rtc8x_nrc* h, hipStream_t s)
 {
-    hipLaunchKernelGGL((nrc::convert_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, h->master, h->P, h->L, (T*)h->w,
-                       (T*)h->wt, (T*)h->winf);
+    hipLaunchKernelGGL((nrc::convert_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, h->master, h->P_mlp, h->L,
+                       (T*)h->w, (T*)h->wt, (T*)h->winf, h->P, h->grid_inf);
     return hipGetLastError();
 }
 
@@ -429,12 +618,13 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
 rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const float* target, uint32_t n)
 {
     const bool bf = is_bf16(h);
-    RTC8X_HIP(bf ? launch_forward<__bf16>(h, true, h->w, s, in, n, nullptr, target)
-                 : launch_forward<_Float16>(h, true, h->w, s, in, n, nullptr, target));
-    RTC8X_HIP(bf ? launch_backward<__bf16>(h, s, n) : launch_backward<_Float16>(h, s, n));
+    RTC8X_HIP(bf ? launch_forward<__bf16>(h, true, s, in, n, nullptr, target)
+                 : launch_forward<_Float16>(h, true, s, in, n, nullptr, target));
+    if (h->P_grid) RTC8X_HIP(hipMemsetAsync(h->grad + h->P_mlp, 0, sizeof(float) * h->P_grid, s));
+    RTC8X_HIP(bf ? launch_backward<__bf16>(h, s, in, n) : launch_backward<_Float16>(h, s, in, n));
     RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
-    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->gpart, (n + 255) / 256, h->P,
-                       h->grad, h->bad);
+    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->gpart, (n + 255) / 256,
+                       h->P_mlp, h->P, h->grad, h->bad);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
@@ -444,7 +634,8 @@ rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const f
 extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
 {
     if (!h) return RTC8X_OK;
-    void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout};
+    void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout,
+                    h->grid_inf};
     for (void* p : ptrs)
         if (p) RTC8X_HIP(hipFree(p));
     delete h;
@@ -460,19 +651,25 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
     RTC8X_REQUIRE(d->learning_rate > 0.0f && d->beta1 >= 0.0f && d->beta1 < 1.0f && d->beta2 >= 0.0f && d->beta2 < 1.0f &&
                       d->epsilon > 0.0f && d->ema_decay >= 0.0f && d->ema_decay < 1.0f && d->l2_reg >= 0.0f,
                   "rtc8x_nrc_create: bad optimiser parameters");
+    RTC8X_REQUIRE(d->encoding == RTC8X_NRC_TRIANGLE_WAVE || d->encoding == RTC8X_NRC_HASH_GRID,
+                  "rtc8x_nrc_create: unknown encoding %d", d->encoding);
     *out = nullptr;
     rtc8x_nrc* h = new rtc8x_nrc();
     h->desc = *d;
+    h->enc = d->encoding;
     h->L = d->num_hidden_layers;
-    h->P = nrc::num_params(h->L);
+    h->P_mlp = nrc::num_params(h->L);
+    h->P_grid = h->enc == RTC8X_NRC_HASH_GRID ? nrc::hash_geometry(&h->geom) : 0u;
+    h->P = h->P_mlp + h->P_grid;
     h->PT = nrc::num_params_t(h->L);
-    const size_t P = h->P;
+    const size_t P = h->P, P_mlp = h->P_mlp;
     const size_t max_blocks = nrc::kMaxTrain / 256;
     auto alloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
     bool ok = alloc((void**)&h->master, 4 * P) && alloc((void**)&h->m, 4 * P) && alloc((void**)&h->v, 4 * P) &&
-              alloc((void**)&h->ema, 4 * P) && alloc((void**)&h->grad, 4 * P) && alloc((void**)&h->gpart, 4 * P * max_blocks) &&
-              alloc((void**)&h->loss_part, 4 * 4 * max_blocks) && alloc((void**)&h->bad, 4) && alloc(&h->w, h->tsize * P) &&
-              alloc(&h->wt, h->tsize * h->PT) && alloc(&h->winf, h->tsize * P) &&
+              alloc((void**)&h->ema, 4 * P) && alloc((void**)&h->grad, 4 * P) &&
+              alloc((void**)&h->gpart, 4 * P_mlp * max_blocks) && alloc((void**)&h->loss_part, 4 * 4 * max_blocks) &&
+              alloc((void**)&h->bad, 4) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
+              alloc(&h->winf, h->tsize * P_mlp) && (h->P_grid == 0 || alloc((void**)&h->grid_inf, 4 * (size_t)h->P_grid)) &&
               alloc(&h->hbuf, h->tsize * 64 * (size_t)(h->L + 1) * nrc::kMaxTrain) &&
               alloc(&h->dout, h->tsize * nrc::kOutPad * (size_t)nrc::kMaxTrain);
     if (!ok) {
@@ -496,6 +693,8 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
         for (uint32_t r = 0; r < rows; ++r)
             for (uint32_t c = 0; c < nrc::kWidth; ++c) init[(size_t)l * nrc::kLayer + r * nrc::kWidth + c] = (float)((2.0 * uni() - 1.0) * lim);
     }
+    // hash tables: uniform in [-1e-4, 1e-4] (the initialisation the hash-encoding paper gives)
+    for (size_t i = P_mlp; i < P; ++i) init[i] = (float)((2.0 * uni() - 1.0) * 1e-4);
     hipError_t e = hipMemcpy(h->master, init.data(), 4 * P, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         rtc8x_nrc_destroy(h);
@@ -538,8 +737,8 @@ extern "C" rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* h, void* stream, const float*
     RTC8X_REQUIRE(h && in && out, "rtc8x_nrc_infer: NULL argument");
     if (n == 0) return RTC8X_OK;
     hipStream_t s = (hipStream_t)stream;
-    RTC8X_HIP(is_bf16(h) ? launch_forward<__bf16>(h, false, h->winf, s, in, n, out, nullptr)
-                         : launch_forward<_Float16>(h, false, h->winf, s, in, n, out, nullptr));
+    RTC8X_HIP(is_bf16(h) ? launch_forward<__bf16>(h, false, s, in, n, out, nullptr)
+                         : launch_forward<_Float16>(h, false, s, in, n, out, nullptr));
     return RTC8X_OK;
 }
 
@@ -577,6 +776,7 @@ extern "C" rtc8x_status rtc8x_nrc_train(rtc8x_nrc* h, void* stream, const float*
     a.bc2 = (float)(1.0 - std::pow((double)d.beta2, (double)h->step));
     a.ema_bc = (float)(1.0 - std::pow((double)d.ema_decay, (double)h->step));
     a.P = h->P;
+    a.P_mlp = h->P_mlp;
     a.L = h->L;
     RTC8X_HIP(is_bf16(h) ? launch_adam<__bf16>(h, s, a) : launch_adam<_Float16>(h, s, a));
     if (loss_on_cpu) {
@@ -595,8 +795,25 @@ extern "C" rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t
 {
     RTC8X_REQUIRE(in && out, "rtc8x_nrc_encode: NULL argument");
     if (n == 0) return RTC8X_OK;
-    hipLaunchKernelGGL(nrc::encode_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       (const rtc8x_radiance_query*)in, n, out);
+    nrc::HashGeom g{};
+    hipLaunchKernelGGL(nrc::encode_kernel<RTC8X_NRC_TRIANGLE_WAVE>, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, (const rtc8x_radiance_query*)in, n, out, (const float*)nullptr, g);
+    RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_nrc_encode_net(const rtc8x_nrc* h, void* stream, const float* in, uint32_t n, float* out,
+                                             int32_t which)
+{
+    RTC8X_REQUIRE(h && in && out && (which == 0 || which == 1), "rtc8x_nrc_encode_net: bad arguments");
+    if (n == 0) return RTC8X_OK;
+    const float* tables = which == 0 ? h->master + h->P_mlp : h->grid_inf;
+    if (h->enc == RTC8X_NRC_HASH_GRID)
+        hipLaunchKernelGGL(nrc::encode_kernel<RTC8X_NRC_HASH_GRID>, dim3((n + 255) / 256), dim3(256), 0,
+                           (hipStream_t)stream, (const rtc8x_radiance_query*)in, n, out, tables, h->geom);
+    else
+        hipLaunchKernelGGL(nrc::encode_kernel<RTC8X_NRC_TRIANGLE_WAVE>, dim3((n + 255) / 256), dim3(256), 0,
+                           (hipStream_t)stream, (const rtc8x_radiance_query*)in, n, out, tables, h->geom);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }

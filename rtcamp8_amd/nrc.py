@@ -52,12 +52,17 @@ class NeuralRadianceCache:
         self.handle = None
 
     def initialize(self, num_hidden_layers: int = 2, learning_rate: float = 1e-2, precision: int = _abi.NRC_FP16,
-                   seed: int = 0x9E3779B97F4A7C15, ema_decay: float = 0.99) -> None:
-        d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, 1e-8, 1e-6, ema_decay, precision, seed)
+                   seed: int = 0x9E3779B97F4A7C15, ema_decay: float = 0.99,
+                   encoding: int = _abi.NRC_HASH_GRID) -> None:
+        """initialize(posEnc, numHiddenLayers, learningRate) (network_interface.cu:48-134); Adam epsilon
+        1e-15 with the hash grid and 1e-8 with triangle waves, as the reference configures them."""
+        eps = 1e-15 if encoding == _abi.NRC_HASH_GRID else 1e-8
+        d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, eps, 1e-6, ema_decay, precision, seed, encoding, 0)
         h = C.c_void_p()
         call("rtc8x_nrc_create", C.byref(d), C.byref(h))
         self.handle = h.value
         self.num_hidden_layers = num_hidden_layers
+        self.encoding = encoding
 
     def finalize(self) -> None:
         if self.handle:
@@ -169,7 +174,8 @@ class NrcRenderer:
     accumulate.  The plain tracer (rtcamp8_amd.tracer) stays the unbiased reference."""
 
     def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
-                 precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121):
+                 precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121,
+                 encoding: int = _abi.NRC_HASH_GRID):
         cfg = scene.cfg
         self.scene = scene
         self.stream = stream
@@ -185,7 +191,7 @@ class NrcRenderer:
         self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n))
         self.have_cost = False
         self.cache = NeuralRadianceCache()
-        self.cache.initialize(num_hidden_layers, learning_rate, precision)
+        self.cache.initialize(num_hidden_layers, learning_rate, precision, encoding=encoding)
         self.frame_rng = Mt19937(frame_rng_seed)
         self.buf_idx = 1  # the first iteration toggles to 0 (renderer_main.cpp:1558-1559)
         self.radiance_scale = radiance_scale

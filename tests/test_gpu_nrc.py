@@ -12,15 +12,18 @@ from test_oracle_nrc import random_queries, random_weights
 pytestmark = pytest.mark.gpu
 
 
-def _cache(L, precision):
+def _cache(L, precision, encoding):
     from rtcamp8_amd.nrc import NeuralRadianceCache
 
     c = NeuralRadianceCache()
-    c.initialize(num_hidden_layers=L, learning_rate=1e-2, precision=precision)
+    c.initialize(num_hidden_layers=L, learning_rate=1e-2, precision=precision, encoding=encoding)
     return c
 
 
-def test_encoding_matches_oracle():
+ENCODINGS = [0, 1]  # triangle waves, hash grid
+
+
+def test_triangle_encoding_matches_oracle():
     from rtcamp8_amd._lib import call
     from rtcamp8_amd.device import DeviceBuffer
 
@@ -34,89 +37,124 @@ def test_encoding_matches_oracle():
     assert np.max(np.abs(got - ref)) < 2e-6
 
 
+def test_hash_encoding_matches_oracle():
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
+
+    L, n = 2, 3000
+    q = random_queries(n, seed=9)
+    q[:5, 0:3] = [[0, 0, 0], [1, 1, 1], [1, 0, 1], [0.5, 0.5, 0.5], [1e-7, 1 - 1e-7, 0.25]]  # corners, faces
+    w = random_weights(L, seed=19, encoding=1)
+    c = _cache(L, 0, 1)
+    c.set_weights(w)
+    dq, de = DeviceBuffer.from_array(q), DeviceBuffer(4 * 64 * n)
+    call("rtc8x_nrc_encode_net", C.c_void_p(c.handle), None, C.c_void_p(dq.ptr), n, C.c_void_p(de.ptr), 0)
+    got = de.download(np.float32, 64 * n).astype(np.float64).reshape(n, 64)
+    ref = np.zeros(64 * n)
+    oracle().oracle_nrc_encode_ex(ptr(w), L, 1, ptr(q), n, ptr(ref))
+    ref = ref.reshape(n, 64)
+    # FP32 interpolation on the GPU: a cell's weights carry ~1e-7 relative rounding per level
+    assert np.max(np.abs(got - ref)) < 2e-6
+    c.finalize()
+
+
+@pytest.mark.parametrize("encoding", ENCODINGS)
 @pytest.mark.parametrize("L,precision,mode,tol", [(2, 0, 1, 4e-3), (5, 0, 1, 4e-3), (2, 1, 2, 2e-2), (5, 1, 2, 2e-2)])
-def test_inference_matches_oracle(L, precision, mode, tol):
+def test_inference_matches_oracle(L, precision, mode, tol, encoding):
     from rtcamp8_amd.device import DeviceBuffer
 
     n = 1000  # not a multiple of the 256-query workgroup: exercises the tail
     q = random_queries(n, seed=10)
-    w = random_weights(L, seed=11, scale=1.2)
-    c = _cache(L, precision)
+    w = random_weights(L, seed=11, scale=1.2, encoding=encoding, table_scale=1.0)
+    c = _cache(L, precision, encoding)
     c.set_weights(w)
     dq, do = DeviceBuffer.from_array(q), DeviceBuffer(12 * n)
     c.infer(None, dq.ptr, n, do.ptr)
     got = do.download(np.float32, 3 * n).astype(np.float64)
     ref = np.zeros(3 * n)
-    oracle().oracle_nrc_forward(ptr(w), L, ptr(q), n, ptr(ref), mode)
+    oracle().oracle_nrc_forward_ex(ptr(w), L, encoding, ptr(q), n, ptr(ref), mode)
     rms = np.sqrt(np.mean(ref ** 2))
     assert rms > 1e-3
     assert np.max(np.abs(got - ref)) < tol * rms, np.max(np.abs(got - ref)) / rms
     c.finalize()
 
 
+@pytest.mark.parametrize("encoding", ENCODINGS)
 @pytest.mark.parametrize("L,precision,mode,tol", [(2, 0, 1, 5e-3), (5, 0, 1, 5e-3), (3, 1, 2, 3e-2)])
-def test_gradient_matches_oracle(L, precision, mode, tol):
+def test_gradient_matches_oracle(L, precision, mode, tol, encoding):
     from rtcamp8_amd.device import DeviceBuffer
 
     n = 700
     q = random_queries(n, seed=12)
     t = np.random.default_rng(13).random((n, 3)).astype(np.float32)
-    w = random_weights(L, seed=14, scale=1.2)
-    c = _cache(L, precision)
+    w = random_weights(L, seed=14, scale=1.2, encoding=encoding, table_scale=1.0)
+    P = L * 4096 + 16 * 64
+    c = _cache(L, precision, encoding)
     c.set_weights(w)
     dq, dt = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t)
     got = c.gradient(None, dq.ptr, dt.ptr, n).astype(np.float64)
     ref = np.zeros(w.size)
-    oracle().oracle_nrc_gradient(ptr(w), L, ptr(q), ptr(t), n, ptr(ref), mode)
-    for layer in range(L + 1):  # every layer on its own, so a small layer cannot hide in the total
-        sl = slice(layer * 4096, (layer + 1) * 4096 if layer < L else layer * 4096 + 3 * 64)
+    oracle().oracle_nrc_gradient_ex(ptr(w), L, encoding, ptr(q), ptr(t), n, ptr(ref), mode)
+    sections = [slice(layer * 4096, (layer + 1) * 4096 if layer < L else layer * 4096 + 3 * 64) for layer in range(L + 1)]
+    if encoding == 1:
+        sections.append(slice(P, w.size))  # the hash tables
+    for k, sl in enumerate(sections):  # every layer on its own, so a small layer cannot hide in the total
         err = np.linalg.norm(got[sl] - ref[sl]) / np.linalg.norm(ref[sl])
-        assert err < tol, (layer, err)
+        assert err < tol, (k, err)
     exact = np.zeros(w.size)
-    oracle().oracle_nrc_gradient(ptr(w), L, ptr(q), ptr(t), n, ptr(exact), 0)
+    oracle().oracle_nrc_gradient_ex(ptr(w), L, encoding, ptr(q), ptr(t), n, ptr(exact), 0)
     assert np.linalg.norm(got - exact) / np.linalg.norm(exact) < 10 * tol
-    assert np.all(got[L * 4096 + 3 * 64:] == 0)
+    assert np.all(got[L * 4096 + 3 * 64:P] == 0)
     c.finalize()
 
 
-def test_adam_step_bit_exact():
+@pytest.mark.parametrize("encoding", ENCODINGS)
+def test_adam_step_bit_exact(encoding):
     from rtcamp8_amd.device import DeviceBuffer
 
     L, n = 2, 2048
     q = random_queries(n, seed=15)
     t = np.random.default_rng(16).random((n, 3)).astype(np.float32)
-    w = random_weights(L, seed=17, scale=1.0)
-    c = _cache(L, 0)
+    w = random_weights(L, seed=17, scale=1.0, encoding=encoding)
+    c = _cache(L, 0, encoding)
     c.set_weights(w)
+    eps = 1e-15 if encoding == 1 else 1e-8
     dq, dt = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t)
     g = c.gradient(None, dq.ptr, dt.ptr, n)
     c.train(None, dq.ptr, dt.ptr, n)
     master = c.get_weights(0)
     ema = c.get_weights(1)
     om, m, v, oe = w.copy(), np.zeros_like(w), np.zeros_like(w), np.zeros_like(w)
-    oracle().oracle_nrc_adam(ptr(om), ptr(m), ptr(v), ptr(oe), ptr(g), w.size, n, 1e-2, 0.9, 0.99, 1e-8, 1e-6, 0.99, 1)
-    assert np.array_equal(master.view(np.uint32), om.view(np.uint32))
+    oracle().oracle_nrc_adam(ptr(om), ptr(m), ptr(v), ptr(oe), ptr(g), w.size, n, 1e-2, 0.9, 0.99, eps, 1e-6, 0.99, 1)
     # the debiasing divisor is 1 - decay^t in double from the FP32 decay, rounded to FP32
     debias = np.float32(1.0 - float(np.float32(0.99)))
-    assert np.array_equal(ema.view(np.uint32), (oe / debias).view(np.uint32))
-    # the FP16 training copies written by the optimiser step: the gradient at the new weights matches
-    # the oracle's at the new master weights
+    P = L * 4096 + 16 * 64
+    assert np.array_equal(master[:P].view(np.uint32), om[:P].view(np.uint32))
+    assert np.array_equal(ema[:P].view(np.uint32), (oe[:P] / debias).view(np.uint32))
+    if encoding == 1:
+        # the tables' gradient is scattered with atomics, so the step's own gradient can differ from `g`
+        # in the last bits; Adam's first step moves each entry by about lr * sign(g)
+        diff = np.abs(master[P:] - om[P:]) > 1e-6 * np.maximum(np.abs(om[P:]), 1e-3)
+        assert diff.mean() < 1e-3, diff.mean()
+    # the working copies written by the optimiser step: the gradient at the new weights matches the
+    # oracle's at the new master weights
     g2 = c.gradient(None, dq.ptr, dt.ptr, n).astype(np.float64)
     ref = np.zeros(w.size)
-    oracle().oracle_nrc_gradient(ptr(master), L, ptr(q), ptr(t), n, ptr(ref), 1)
+    oracle().oracle_nrc_gradient_ex(ptr(master), L, encoding, ptr(q), ptr(t), n, ptr(ref), 1)
     assert np.linalg.norm(g2 - ref) / np.linalg.norm(ref) < 5e-3
     c.finalize()
 
 
+@pytest.mark.parametrize("encoding", ENCODINGS)
 @pytest.mark.parametrize("precision", [0, 1])
-def test_training_reduces_loss(precision):
-    """Online training on a smooth synthetic radiance field drops the relative L2 loss by > 10x.  Every
-    triangle wave tri(2^k x) is symmetric under x -> 1 - x on [0, 1], so the field is chosen with that
-    symmetry in each position coordinate (a field without it is not representable by the encoding)."""
+def test_training_reduces_loss(precision, encoding):
+    """Online training on a smooth synthetic radiance field drops the relative L2 loss by > 10x.  The
+    field is mirror-symmetric in each position coordinate so that it is representable by both
+    encodings."""
     from rtcamp8_amd.device import DeviceBuffer
 
     L, n = 3, 16384
-    c = _cache(L, precision)
+    c = _cache(L, precision, encoding)
     losses = []
     for step in range(300):
         q = random_queries(n, seed=100 + step)

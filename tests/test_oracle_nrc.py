@@ -23,11 +23,15 @@ def random_queries(n, seed=1):
     return q
 
 
-def random_weights(L, seed=2, scale=0.3):
+def random_weights(L, seed=2, scale=0.3, encoding=0, table_scale=0.1):
+    """MLP weights (output rows beyond RGB zero) followed, with the hash grid, by random tables."""
     P = oracle().oracle_nrc_num_params(L)
+    P_all = oracle().oracle_nrc_num_params_ex(L, encoding)
     rng = np.random.default_rng(seed)
-    w = (rng.standard_normal(P) * scale / 8).astype(np.float32)
-    w[L * 4096 + 3 * 64:] = 0.0  # padded output rows
+    w = np.zeros(P_all, dtype=np.float32)
+    w[:P] = (rng.standard_normal(P) * scale / 8).astype(np.float32)
+    w[L * 4096 + 3 * 64:P] = 0.0  # padded output rows
+    w[P:] = (rng.uniform(-1, 1, P_all - P) * table_scale).astype(np.float32)
     return w
 
 
@@ -122,3 +126,64 @@ def test_encoding_ranges():
     assert np.all(e[:, :56] >= 0) and np.all(e[:, :56] <= 1)
     assert np.all(e[:, 62] == 1) and np.all(e[:, 63] == 0)
     assert np.allclose(e[:, 56:59], 0.99, atol=1e-6)
+
+
+def test_hash_grid_geometry():
+    """Level 0 (scale 15) is dense with (15 + 2)^3 corners rounded up to 8; the 15 finer levels are hashed."""
+    L = 2
+    total = oracle().oracle_nrc_num_params_ex(L, 1) - oracle().oracle_nrc_num_params(L)
+    assert total == 2 * (4920 + 15 * 32768)
+
+
+def test_hash_encoding_is_continuous_and_interpolates():
+    """Trilinear interpolation is continuous across cell faces and exact at the dense level's corners."""
+    L = 2
+    w = random_weights(L, seed=7, encoding=1)
+    P = oracle().oracle_nrc_num_params(L)
+    q = random_queries(2, seed=8)
+    q[1] = q[0]
+    q[0, 0] = 5.0 / 15.0 - 0.5 / 15.0 - 1e-7  # just below a level-0 cell face (p * 15 + 0.5 = 5)
+    q[1, 0] = 5.0 / 15.0 - 0.5 / 15.0 + 1e-7
+    e = np.zeros(2 * 64)
+    oracle().oracle_nrc_encode_ex(ptr(w), L, 1, ptr(q), 2, ptr(e))
+    e = e.reshape(2, 64)
+    # across the level-0 face the coarse features match (finer levels move by their scale x 2e-7)
+    assert np.max(np.abs(e[0, :8] - e[1, :8])) < 1e-5
+    # at a level-0 corner the features are the table entry there
+    q2 = random_queries(1, seed=9)
+    q2[0, 0:3] = [(3 - 0.5) / 15.0, (4 - 0.5) / 15.0, (6 - 0.5) / 15.0]  # corner (3, 4, 6)
+    e2 = np.zeros(64)
+    oracle().oracle_nrc_encode_ex(ptr(w), L, 1, ptr(q2), 1, ptr(e2))
+    idx = 3 + 4 * 17 + 6 * 17 * 17
+    assert np.allclose(e2[0:2], w[P + 2 * idx:P + 2 * idx + 2], rtol=0, atol=1e-6)
+    assert np.all(e2[32:52] >= 0) and e2[58] == 1 and np.all(e2[59:] == 0)
+
+
+def test_hash_gradient_matches_central_differences():
+    L, n = 2, 16
+    q = random_queries(n, seed=10)
+    t = np.random.default_rng(11).random((n, 3)).astype(np.float32)
+    w = random_weights(L, seed=12, encoding=1)
+    P = oracle().oracle_nrc_num_params(L)
+    grad = np.zeros(w.size)
+    oracle().oracle_nrc_gradient_ex(ptr(w), L, 1, ptr(q), ptr(t), n, ptr(grad), 0)
+    base = np.zeros(3 * n)
+    oracle().oracle_nrc_forward_ex(ptr(w), L, 1, ptr(q), n, ptr(base), 0)
+    denom = (base.reshape(n, 3) @ np.array([0.2126729, 0.7151522, 0.0721750])) ** 2 + 0.01
+
+    def frozen_loss(wv):
+        out = np.zeros(3 * n)
+        oracle().oracle_nrc_forward_ex(ptr(wv), L, 1, ptr(q), n, ptr(out), 0)
+        return float((((out.reshape(n, 3) - t) ** 2).sum(axis=1) / denom).sum())
+
+    touched = np.flatnonzero(grad[P:]) + P
+    assert touched.size > 100
+    rng = np.random.default_rng(13)
+    idx = np.concatenate([rng.choice(touched, 25, replace=False), rng.choice(4096, 5, replace=False)])
+    for i in idx:
+        h = 1e-4
+        wp, wm = w.copy(), w.copy()
+        wp[i] += np.float32(h)
+        wm[i] -= np.float32(h)
+        fd = (frozen_loss(wp) - frozen_loss(wm)) / (float(wp[i]) - float(wm[i]))
+        assert abs(fd - grad[i]) <= 1e-5 * max(1.0, abs(grad[i])), (i, fd, grad[i])

@@ -30,6 +30,12 @@ METRIC = "path vertices/sec + NRC queries/sec; wall time to equal RMSE vs FP64 o
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LOOKUP = 32  # 8 FP32 voxel taps per trilinear lookup
 FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
+# NRC settings of the equal-RMSE leg: the reference's network (hash grid, 2 hidden layers, lr 1e-2) with
+# its radiance-scale knob (renderer_main.cpp:979-995) set so the cloud's radiance (~0.03-0.3) sits well
+# above the loss's 0.01 epsilon, and a shorter weight EMA than the reference's 0.99 (the benchmark
+# frame has only ~44 optimiser steps, over which a 0.99 average still lags the training weights)
+NRC_RADIANCE_SCALE = 10.0
+NRC_EMA_DECAY = 0.95
 
 
 def parse():
@@ -256,9 +262,12 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
     mse_plain = cross_mse(x, a, b)
     res = {"reference": f"plain tracer, 2 x {ref_spp} spp with independent seeds (cross-product MSE)",
            "nrc_schedule": "frame 0: 8 training iterations, later frames 1; 4 spp traced into the cache per frame",
+           "nrc_settings": {"encoding": "hash grid", "hidden_layers": 2, "radiance_scale": NRC_RADIANCE_SCALE,
+                            "ema_decay": NRC_EMA_DECAY},
            "spp": spp, "plain_seconds": tx, "plain_rel_rmse": float(np.sqrt(max(mse_plain, 0.0)) / mean)}
     for name, prec in (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16)):
-        run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec, spp_per_frame=4)
+        run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec, spp_per_frame=4,
+                        encoding=_abi.NRC_HASH_GRID, radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY)
         mse = cross_mse(run.image, a, b)
         n_star = spp * mse_plain / mse if mse > 0 else float("inf")
         t_equal = tx * n_star / spp

@@ -61,6 +61,7 @@ struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
     // returns true when the tentative collision is real
     __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu) { return rng.uni() < sigma / mu; }
     __device__ __forceinline__ void after_shadow(Pcg&, float sigma, float mu, float& vis) { vis *= 1.0f - sigma / mu; }
+    __device__ __forceinline__ void skip_shadow(Pcg&) {}
 };
 
 // Local majorants over 8^3 cells, the loops of track_local / ratio_local made resumable.
@@ -121,6 +122,7 @@ struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
         vis *= 1.0f - sigma / mu;
         tau = -rtc8x_logf(1.0f - rng.uni());
     }
+    __device__ __forceinline__ void skip_shadow(Pcg& rng) { tau = -rtc8x_logf(1.0f - rng.uni()); }
 };
 
 enum : uint32_t {
@@ -563,7 +565,13 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             }
         }
         // ---- tentative collisions, with their density lookups, for every stepping lane
-        if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
+        if (COUNT_ONLY && L.mode == kModeShadow) {
+            // The count pass writes no targets, so the shadow ray's visibility is not needed; its random
+            // numbers are consumed exactly as ratio tracking consumes them (they do not depend on the
+            // densities), without the lookups.
+            while (sg.advance(p.grid, med, L.rng, mu)) sg.skip_shadow(L.rng);
+            L.mode = kModeNeeDone;
+        } else if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
             // kSpeculate collisions at once: the positions depend only on the random numbers, so their
             // lookups are issued together.  The numbers are drawn in the sequential algorithm's order
             // (tracking: step, accept, step, accept, ...; ratio tracking: step, step, ...) and the

@@ -175,7 +175,7 @@ class NrcRenderer:
 
     def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
                  precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121,
-                 encoding: int = _abi.NRC_HASH_GRID):
+                 encoding: int = _abi.NRC_HASH_GRID, ema_decay: float = 0.99, first_frame_iterations: int = 8):
         cfg = scene.cfg
         self.scene = scene
         self.stream = stream
@@ -191,7 +191,8 @@ class NrcRenderer:
         self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n))
         self.have_cost = False
         self.cache = NeuralRadianceCache()
-        self.cache.initialize(num_hidden_layers, learning_rate, precision, encoding=encoding)
+        self.cache.initialize(num_hidden_layers, learning_rate, precision, encoding=encoding, ema_decay=ema_decay)
+        self.first_frame_iterations = first_frame_iterations
         self.frame_rng = Mt19937(frame_rng_seed)
         self.buf_idx = 1  # the first iteration toggles to 0 (renderer_main.cpp:1558-1559)
         self.radiance_scale = radiance_scale
@@ -260,7 +261,7 @@ class NrcRenderer:
         self.samples += 1
 
     def render_frame(self, spp: int, train_iterations: int | None = None) -> None:
-        iters = train_iterations if train_iterations is not None else (8 if self.frame == 0 else 1)
+        iters = train_iterations if train_iterations is not None else (self.first_frame_iterations if self.frame == 0 else 1)
         for _ in range(iters):
             self.train_iteration(self.frame == 0)
         for _ in range(spp):

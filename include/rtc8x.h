@@ -94,7 +94,7 @@ rtc8x_status rtc8x_gather_probe(void* stream, const float* leaf_values, uint32_t
 typedef struct rtc8x_root_tile {
     uint64_t key;        /* NanoVDB 64-bit root key of the upper node's origin */
     uint32_t entry;      /* upper-node index, or RTC8X_TILE_BIT | tile value index */
-    uint32_t reserved;
+    uint32_t state;      /* a tile's active state (NanoVDB RootData::Tile::state); not used by the tracer */
 } rtc8x_root_tile;
 
 typedef struct rtc8x_grid {
@@ -134,6 +134,20 @@ typedef struct rtc8x_grid {
     int32_t leaf_map_origin[3];          /* a multiple of 8 */
     int32_t leaf_map_dims[3];
     const uint32_t* leaf_map;
+    /* Activity (host tables only, optional; lookups ignore it): NanoVDB value masks, one bit per voxel of
+     * a leaf (8 words) or per table entry of a lower (64 words) / upper (512 words) node, bit n of word
+     * n / 64 for offset n.  A set bit on an internal table entry marks an active tile.  Decoded grids
+     * carry them; the encoder writes them back, and treats NULL as every leaf voxel active and every
+     * tile inactive.  Never uploaded. */
+    const uint64_t* leaf_masks;          /* num_leaves x 8 */
+    const uint64_t* lower_masks;         /* num_lower x 64 */
+    const uint64_t* upper_masks;         /* num_upper x 512 */
+    /* The index-to-world map in double precision (NanoVDB Map::mMatD, mInvMatD, mVecD; host tables only):
+     * decoded grids keep it so that an encoded copy holds the same map and world bbox.  All zero: the
+     * encoder derives it from inv_mat and vec. */
+    double map_mat_d[9];
+    double map_inv_mat_d[9];
+    double map_vec_d[3];
 } rtc8x_grid;
 
 /* Procedural fBm volume description (the benchmark's synthetic input; the reference loads a .nvdb). */
@@ -162,6 +176,81 @@ rtc8x_status rtc8x_grid_destroy(rtc8x_grid* grid);
  * out_value[n]; all device pointers. */
 rtc8x_status rtc8x_grid_probe(const rtc8x_grid* grid, uint32_t n, const float* world_xyz,
                               int32_t* out_ijk, uint32_t* out_offsets, float* out_value, void* stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * NanoVDB files (.nvdb) and grid buffers, byte-compatible with NanoVDB 32.x (the version rtcamp8
+ * vendors, ext/nanovdb/nanovdb/NanoVDB.h:104-106).  Host code; nothing here touches the GPU except
+ * rtc8x_grid_load_nvdb / rtc8x_grid_upload_nanovdb / rtc8x_grid_download.
+ *
+ * File: one or more segments, each a 16-byte header (magic "NanoVDB0", packed version, grid count,
+ * codec) followed by one 176-byte MetaData record plus the NUL-terminated name per grid, then the grids'
+ * bytes in order (util/IO.h:104-160).  Codec NONE stores a grid buffer verbatim; ZIP stores a u64 byte
+ * count and a zlib stream (util/IO.h:254-344).  BLOSC is reported as RTC8X_ERR_NOT_SUPPORTED.
+ * Grid buffer: GridData (672 B) | TreeData (64 B) | RootData (64 B) + 32-byte root tiles | upper nodes
+ * (270,400 B) | lower nodes (33,856 B) | leaves (2,144 B), child links as signed byte offsets
+ * (NanoVDB.h:2185-2205, 2501-2508, 2687-2760, 3043-3072, 3355-3373).  Only FloatGrid (GridType 1) is
+ * decoded: the reference reads its volume as nanovdb::FloatGrid (renderer/renderer_scene.cpp:491-495).
+ * ---------------------------------------------------------------------------------------------- */
+#define RTC8X_NVDB_CODEC_NONE 0
+#define RTC8X_NVDB_CODEC_ZIP 1
+#define RTC8X_NVDB_CODEC_BLOSC 2
+
+/* One grid of a file: the MetaData record (util/IO.h:144-159) and where its bytes are. */
+typedef struct rtc8x_nvdb_grid_info {
+    uint64_t grid_size;        /* bytes of the decoded grid buffer */
+    uint64_t file_size;        /* bytes the grid takes in the file (compressed size for ZIP) */
+    uint64_t voxel_count;      /* active voxels */
+    uint64_t file_offset;      /* offset of the grid's bytes in the file */
+    uint32_t grid_type;        /* NanoVDB GridType: 1 = Float */
+    uint32_t grid_class;       /* NanoVDB GridClass: 2 = FogVolume */
+    uint32_t codec;            /* RTC8X_NVDB_CODEC_* */
+    uint32_t version;          /* major << 21 | minor << 10 | patch */
+    uint32_t node_count[4];    /* leaves, lower, upper, root (1) */
+    uint32_t tile_count[3];    /* active tiles in lower nodes, upper nodes, the root */
+    uint32_t segment;          /* index of the segment that holds it */
+    double world_bbox[6];      /* min xyz, max xyz */
+    int32_t index_bbox[6];     /* inclusive */
+    double voxel_size[3];
+    char name[256];            /* NUL-terminated; longer names are truncated */
+} rtc8x_nvdb_grid_info;
+
+/* Lists the grids of a file image.  Writes up to `capacity` records to `infos` (may be NULL with
+ * capacity 0) and the total number of grids to *count.  Replaces readGridMetaData (util/IO.h:246-250). */
+rtc8x_status rtc8x_nvdb_list(const void* file, size_t file_size, rtc8x_nvdb_grid_info* infos, uint32_t capacity,
+                             uint32_t* count);
+/* Decodes a FloatGrid buffer (what a GridHandle holds) into host tables of the layout of rtc8x_grid,
+ * in memory the library allocates (release with rtc8x_grid_free_host).  max_value is the majorant as
+ * the reference computes it after loading: the maximum of getValue over the half-open index range
+ * [bbox.min, bbox.max) of the root's index bbox (Scene::allocateVolumeGrid,
+ * renderer/renderer_scene.cpp:497-511); background is the root's background value.  Malformed buffers
+ * (offsets out of range or misaligned, node counts inconsistent with the links) are rejected. */
+rtc8x_status rtc8x_nanovdb_decode(const void* grid_buffer, size_t size, rtc8x_grid* out_host);
+/* Reads grid `n` of a file image (counting across segments, as nanovdb::io::readGrid(fileName, n),
+ * util/IO.h:576-612) and decodes it as rtc8x_nanovdb_decode does. */
+rtc8x_status rtc8x_nvdb_read_grid(const void* file, size_t file_size, uint32_t n, rtc8x_grid* out_host);
+/* Releases host tables returned by rtc8x_nanovdb_decode / rtc8x_nvdb_read_grid / rtc8x_grid_download. */
+rtc8x_status rtc8x_grid_free_host(rtc8x_grid* host);
+/* Device grid from a FloatGrid buffer / from grid n of a .nvdb file image: decode + upload, keeping the
+ * decoded majorant (readGrid + Scene::allocateVolumeGrid, renderer/renderer_scene.cpp:2467-2470,
+ * 491-537).  Release with rtc8x_grid_destroy. */
+rtc8x_status rtc8x_grid_upload_nanovdb(const void* grid_buffer, size_t size, rtc8x_grid* out);
+rtc8x_status rtc8x_grid_load_nvdb(const void* file, size_t file_size, uint32_t n, rtc8x_grid* out);
+/* Host copy of a device grid's tables (library-allocated; release with rtc8x_grid_free_host).
+ * Synchronises the device. */
+rtc8x_status rtc8x_grid_download(const rtc8x_grid* device_grid, rtc8x_grid* out_host);
+/* Encodes host tables as a NanoVDB FloatGrid buffer of class FogVolume named `name` (at most 255 bytes).
+ * Activity comes from the tables' value masks and root-tile states (without masks every leaf voxel is
+ * active and every tile inactive), the map from map_*_d (or inv_mat / vec); node bounding boxes, min/max/average/standard deviation and the world bbox are computed as
+ * nanovdb::gridStats does (util/GridStats.h:530-708) and the checksum as ChecksumMode::Partial, the
+ * default of nanovdb::GridBuilder (util/GridChecksum.h:33-37, 175-203).  Nodes are laid out breadth-first
+ * in traversal order, so a grid built by GridBuilder and decoded re-encodes to the same bytes.  With
+ * out == NULL only *size is set; otherwise capacity must be at least that size. */
+rtc8x_status rtc8x_nanovdb_encode(const rtc8x_grid* host, const char* name, void* out, size_t capacity, size_t* size);
+/* Writes `count` FloatGrid buffers as one segment of a .nvdb file image with the given codec (NONE or
+ * ZIP), as nanovdb::io::writeGrids does (util/IO.h:548-566).  With out == NULL only *size is set (an
+ * upper bound for ZIP; the exact size is returned by the writing call). */
+rtc8x_status rtc8x_nvdb_write(const void* const* grid_buffers, const size_t* grid_sizes, uint32_t count,
+                              uint32_t codec, void* out, size_t capacity, size_t* size);
 
 /* ------------------------------------------------------------------------------------------------
  * Environment light and sun.

@@ -309,7 +309,7 @@ int oracle_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid* g)
     rtc8x_root_tile* tiles = (rtc8x_root_tile*)malloc(sizeof(rtc8x_root_tile));
     tiles[0].key = root_key(0, 0, 0);
     tiles[0].entry = 0;
-    tiles[0].reserved = 0;
+    tiles[0].state = 0;
     float* tile_values = (float*)malloc(sizeof(float));
     tile_values[0] = 0.0f;
 

@@ -34,7 +34,7 @@ TRAIN_BUFFER_SIZE = 2 * 65536
 
 
 class RootTile(C.Structure):
-    _fields_ = [("key", C.c_uint64), ("entry", C.c_uint32), ("reserved", C.c_uint32)]
+    _fields_ = [("key", C.c_uint64), ("entry", C.c_uint32), ("state", C.c_uint32)]
 
 
 class Grid(C.Structure):
@@ -63,6 +63,35 @@ class Grid(C.Structure):
         ("leaf_map_origin", c_i3),
         ("leaf_map_dims", c_i3),
         ("leaf_map", C.c_void_p),
+        ("leaf_masks", C.c_void_p),
+        ("lower_masks", C.c_void_p),
+        ("upper_masks", C.c_void_p),
+        ("map_mat_d", C.c_double * 9),
+        ("map_inv_mat_d", C.c_double * 9),
+        ("map_vec_d", C.c_double * 3),
+    ]
+
+
+NVDB_CODEC_NONE, NVDB_CODEC_ZIP, NVDB_CODEC_BLOSC = 0, 1, 2
+
+
+class NvdbGridInfo(C.Structure):
+    _fields_ = [
+        ("grid_size", C.c_uint64),
+        ("file_size", C.c_uint64),
+        ("voxel_count", C.c_uint64),
+        ("file_offset", C.c_uint64),
+        ("grid_type", C.c_uint32),
+        ("grid_class", C.c_uint32),
+        ("codec", C.c_uint32),
+        ("version", C.c_uint32),
+        ("node_count", C.c_uint32 * 4),
+        ("tile_count", C.c_uint32 * 3),
+        ("segment", C.c_uint32),
+        ("world_bbox", C.c_double * 6),
+        ("index_bbox", C.c_int32 * 6),
+        ("voxel_size", C.c_double * 3),
+        ("name", C.c_char * 256),
     ]
 
 
@@ -279,6 +308,7 @@ STRUCTS = {
     "rtc8x_training_suffix_terminal": TrainingSuffixTerminal,
     "rtc8x_nrc_buffers": NrcBuffers,
     "rtc8x_frame_params": FrameParams,
+    "rtc8x_nvdb_grid_info": NvdbGridInfo,
 }
 
 _P = C.c_void_p
@@ -312,6 +342,16 @@ PROTOTYPES = {
     "rtc8x_grid_upload": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
     "rtc8x_grid_destroy": (_S, [C.POINTER(Grid)]),
     "rtc8x_grid_probe": (_S, [C.POINTER(Grid), C.c_uint32, _P, _P, _P, _P, _P]),
+    "rtc8x_nvdb_list": (_S, [_P, C.c_size_t, C.POINTER(NvdbGridInfo), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "rtc8x_nanovdb_decode": (_S, [_P, C.c_size_t, C.POINTER(Grid)]),
+    "rtc8x_nvdb_read_grid": (_S, [_P, C.c_size_t, C.c_uint32, C.POINTER(Grid)]),
+    "rtc8x_grid_free_host": (_S, [C.POINTER(Grid)]),
+    "rtc8x_grid_upload_nanovdb": (_S, [_P, C.c_size_t, C.POINTER(Grid)]),
+    "rtc8x_grid_load_nvdb": (_S, [_P, C.c_size_t, C.c_uint32, C.POINTER(Grid)]),
+    "rtc8x_grid_download": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
+    "rtc8x_nanovdb_encode": (_S, [C.POINTER(Grid), C.c_char_p, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rtc8x_nvdb_write": (_S, [C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_uint32, C.c_uint32, _P, C.c_size_t,
+                              C.POINTER(C.c_size_t)]),
     "rtc8x_env_build_sampler": (_S, [C.POINTER(EnvLight), _P]),
     "rtc8x_env_release": (_S, [C.POINTER(EnvLight)]),
     "rtc8x_camera_setup": (_S, [C.POINTER(Camera), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float]),

@@ -28,6 +28,7 @@ SOURCES = [
     "rtc8x_nrc_data.hip",
     "rtc8x_nrc_mlp.hip",
     "rtc8x_measure.hip",
+    "rtc8x_nvdb.hip",
 ]
 
 HIPCC_FLAGS = [
@@ -69,7 +70,7 @@ def build_library(jobs: int | None = None) -> Path:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-lz", "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "rtc8x.h"
 
@@ -12,6 +13,19 @@ namespace rtc8x {
 
 void set_error(const std::string& msg);
 rtc8x_status fail(rtc8x_status st, const char* fmt, ...);
+
+// Host copies of grid tables that the library owns until rtc8x_grid_free_host.
+struct OwnedHostGrid {
+    std::vector<rtc8x_root_tile> root;
+    std::vector<uint32_t> upper, lower;
+    std::vector<float> values, tiles, cells;
+    std::vector<int32_t> origins;
+    std::vector<uint64_t> leaf_masks, lower_masks, upper_masks;
+};
+// Takes ownership of `t`, points the table pointers of `g` at its storage and stores the result in *out.
+rtc8x_status adopt_host_grid(OwnedHostGrid&& t, const rtc8x_grid& g, rtc8x_grid* out);
+// rtc8x_grid_upload; with keep_max the host's max_value is kept instead of the brute-force maximum.
+rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_max);
 
 }  // namespace rtc8x
 

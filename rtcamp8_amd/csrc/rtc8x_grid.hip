@@ -355,7 +355,7 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
     rtc8x_root_tile tile;
     tile.key = 0;  // root key of the upper node at the origin
     tile.entry = 0;
-    tile.reserved = 0;
+    tile.state = 0;
     std::vector<rtc8x_root_tile> root(1, tile);
     std::vector<float> tile_values(1, 0.0f);
     std::vector<float> cm;
@@ -407,7 +407,9 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
     return RTC8X_OK;
 }
 
-extern "C" rtc8x_status rtc8x_grid_upload(const rtc8x_grid* h, rtc8x_grid* out)
+namespace rtc8x {
+
+rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_max)
 {
     RTC8X_REQUIRE(h && out, "rtc8x_grid_upload: NULL argument");
     RTC8X_REQUIRE(h->num_root_tiles > 0 && h->root_tiles && h->tile_values, "rtc8x_grid_upload: root missing");
@@ -466,9 +468,10 @@ extern "C" rtc8x_status rtc8x_grid_upload(const rtc8x_grid* h, rtc8x_grid* out)
     }
     for (uint32_t t = 0; t < num_tiles_values; ++t) vmax = std::max(vmax, h->tile_values[t]);
     RTC8X_REQUIRE(std::isfinite(vmax), "rtc8x_grid_upload: non-finite voxel values");
+    RTC8X_REQUIRE(!keep_max || std::isfinite(h->max_value), "rtc8x_grid_upload: non-finite majorant");
 
     rtc8x_grid g = *h;
-    g.max_value = vmax;
+    g.max_value = keep_max ? h->max_value : vmax;
     std::vector<float> cm;
     if (h->num_leaves > 0) {
         int32_t corg[3], cdims[3];
@@ -502,9 +505,62 @@ extern "C" rtc8x_status rtc8x_grid_upload(const rtc8x_grid* h, rtc8x_grid* out)
     g.leaf_origins = d_origins;
     g.cell_majorants = d_cm;
     g.leaf_map = nullptr;
+    g.leaf_masks = g.lower_masks = g.upper_masks = nullptr;  // host-side activity is not uploaded
     if (h->num_root_tiles > 0 && h->num_leaves > 0) RTC8X_TRY(build_leaf_map(g, nullptr));
     *out = g;
     return RTC8X_OK;
+}
+
+}  // namespace rtc8x
+
+extern "C" rtc8x_status rtc8x_grid_upload(const rtc8x_grid* h, rtc8x_grid* out)
+{
+    return rtc8x::upload_host_grid(h, out, false);
+}
+
+extern "C" rtc8x_status rtc8x_grid_download(const rtc8x_grid* d, rtc8x_grid* out)
+{
+    RTC8X_REQUIRE(d && out, "rtc8x_grid_download: NULL argument");
+    RTC8X_REQUIRE(d->num_root_tiles > 0 && d->root_tiles && d->tile_values, "rtc8x_grid_download: grid has no root");
+    rtc8x::OwnedHostGrid t;
+    t.root.resize(d->num_root_tiles);
+    t.upper.resize((size_t)d->num_upper * RTC8X_UPPER_TABLE_SIZE);
+    t.lower.resize((size_t)d->num_lower * RTC8X_LOWER_TABLE_SIZE);
+    t.values.resize((size_t)d->num_leaves * RTC8X_LEAF_SIZE);
+    RTC8X_HIP(hipDeviceSynchronize());
+    RTC8X_HIP(hipMemcpy(t.root.data(), d->root_tiles, t.root.size() * sizeof(rtc8x_root_tile), hipMemcpyDeviceToHost));
+    if (!t.upper.empty())
+        RTC8X_HIP(hipMemcpy(t.upper.data(), d->upper_tables, t.upper.size() * 4, hipMemcpyDeviceToHost));
+    if (!t.lower.empty())
+        RTC8X_HIP(hipMemcpy(t.lower.data(), d->lower_tables, t.lower.size() * 4, hipMemcpyDeviceToHost));
+    if (!t.values.empty())
+        RTC8X_HIP(hipMemcpy(t.values.data(), d->leaf_values, t.values.size() * 4, hipMemcpyDeviceToHost));
+    // the tile-value table is as long as the largest index a table entry names
+    uint32_t num_tiles = 1;
+    auto scan = [&](uint32_t e) {
+        if (e & RTC8X_TILE_BIT) num_tiles = std::max(num_tiles, (e & ~RTC8X_TILE_BIT) + 1);
+    };
+    for (const rtc8x_root_tile& r : t.root) scan(r.entry);
+    for (uint32_t e : t.upper) scan(e);
+    for (uint32_t e : t.lower) scan(e);
+    t.tiles.resize(num_tiles);
+    RTC8X_HIP(hipMemcpy(t.tiles.data(), d->tile_values, num_tiles * 4, hipMemcpyDeviceToHost));
+    if (d->leaf_origins && d->num_leaves) {
+        t.origins.resize((size_t)3 * d->num_leaves);
+        RTC8X_HIP(hipMemcpy(t.origins.data(), d->leaf_origins, t.origins.size() * 4, hipMemcpyDeviceToHost));
+    }
+    const size_t ncells = (size_t)std::max(d->cell_dims[0], 0) * std::max(d->cell_dims[1], 0) * std::max(d->cell_dims[2], 0);
+    if (d->cell_majorants && ncells) {
+        t.cells.resize(ncells);
+        RTC8X_HIP(hipMemcpy(t.cells.data(), d->cell_majorants, ncells * 4, hipMemcpyDeviceToHost));
+    }
+    rtc8x_grid g = *d;
+    g.leaf_map = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        g.leaf_map_origin[a] = 0;
+        g.leaf_map_dims[a] = 0;
+    }
+    return rtc8x::adopt_host_grid(std::move(t), g, out);
 }
 
 extern "C" rtc8x_status rtc8x_grid_probe(const rtc8x_grid* g, uint32_t n, const float* world_xyz, int32_t* out_ijk,

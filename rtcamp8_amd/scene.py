@@ -73,6 +73,8 @@ class SceneConfig:
     fov_y_deg: float = 34.0
     tracking: int = _abi.TRACKING_GLOBAL_MAJORANT
     rng_seed: int = REFERENCE_RNG_SEED
+    volume_file: str = ""               # a .nvdb file to render instead of the procedural volume
+    volume_grid: int = 0                # which grid of the file (nanovdb::io::readGrid's n)
 
     def with_(self, **kw) -> "SceneConfig":
         return replace(self, **kw)
@@ -184,8 +186,13 @@ class DeviceScene:
 
     def __init__(self, cfg: SceneConfig, stream=None):
         self.cfg = cfg
-        self.grid = _abi.Grid()
-        call("rtc8x_grid_build_fbm", C.byref(cfg.fbm.desc()), C.byref(self.grid), C.c_void_p(stream))
+        if cfg.volume_file:
+            from .nvdb import load_grid  # noqa: WPS433
+
+            self.grid = load_grid(cfg.volume_file, cfg.volume_grid)
+        else:
+            self.grid = _abi.Grid()
+            call("rtc8x_grid_build_fbm", C.byref(cfg.fbm.desc()), C.byref(self.grid), C.c_void_p(stream))
         self.env = _abi.EnvLight()
         self._env_buffers = []
         self.env.power_coeff = 1.0

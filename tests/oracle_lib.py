@@ -132,7 +132,7 @@ def download_grid(dg: _abi.Grid) -> HostGrid:
     """Host copy of a device grid built by the library."""
     from rtcamp8_amd.device import download_ptr
 
-    root = np.empty(dg.num_root_tiles, dtype=np.dtype([("key", "<u8"), ("entry", "<u4"), ("reserved", "<u4")]))
+    root = np.empty(dg.num_root_tiles, dtype=np.dtype([("key", "<u8"), ("entry", "<u4"), ("state", "<u4")]))
     raw = download_ptr(dg.root_tiles, np.uint8, root.nbytes)
     root = raw.view(root.dtype).copy()
     arrays = {
@@ -257,7 +257,7 @@ def tables_from_leaves(origins: np.ndarray, values: np.ndarray, voxel_size: floa
         t = tile_of[int(keys[l])]
         li = lower_of[t * UPPER + int(uoff[l])]
         lower[li * LOWER + int(loff[l])] = l
-    root = np.zeros(len(uniq_keys), dtype=np.dtype([("key", "<u8"), ("entry", "<u4"), ("reserved", "<u4")]))
+    root = np.zeros(len(uniq_keys), dtype=np.dtype([("key", "<u8"), ("entry", "<u4"), ("state", "<u4")]))
     root["key"] = np.array(uniq_keys, dtype=np.uint64)
     root["entry"] = np.arange(len(uniq_keys), dtype=np.uint32)
     g = _abi.Grid()

@@ -610,6 +610,58 @@ rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* nrc, void* stream, const float* in, c
  * (renderer/renderer_main.cpp:86-93).  Host-side, then uploads; `states` is a device pointer. */
 rtc8x_status rtc8x_seed_rng(uint64_t* states, uint32_t width, uint32_t height, uint64_t seed);
 
+/* ------------------------------------------------------------------------------------------------
+ * Post-processing and image files.
+ * The batch driver tone-maps the accumulated frame on the device (applyToneMap,
+ * renderer/gpu_kernels/post_process_kernels.cu:9-40, launched at renderer_main.cpp:1646-1647) and
+ * saves it through the SDR conversion (saveImage, common/common_host.cpp:515-570) as PNG; the GUI also
+ * writes EXR (saveImageHDR, common/common_host.cpp:443-513) and BMP (common/common_host.cpp:422-441).
+ * ---------------------------------------------------------------------------------------------- */
+/* SDRImageSaverConfig (common/common_host.h:183-195). */
+typedef struct rtc8x_sdr_config {
+    float alpha_override;       /* >= 0: every pixel's alpha; < 0: the data's alpha (1 without one) */
+    float brightness_scale;     /* multiplies the luminance fed to the tone curve */
+    int32_t apply_tone_map;     /* colour *= (1 - exp(-brightness_scale * L)) / L, after zeroing non-finite
+                                   pixels and clamping negatives */
+    int32_t apply_srgb_gamma;   /* sRGB transfer curve per channel (negatives clamped first) */
+    int32_t flip_y;             /* row y of the output comes from row height-1-y of the input */
+} rtc8x_sdr_config;
+
+typedef struct rtc8x_tone_map_params {
+    const double* accum;        /* width*height*3 FP64 sums over `samples` samples (the tracer's accumulator) */
+    uint32_t width, height;
+    uint32_t samples;           /* > 0 */
+    float exponent;             /* per-channel power before the tone curve: 1/0.6 in the reference */
+    float brightness;           /* tone curve 1 - exp(-brightness * L): 1 in the reference */
+    float* out_rgba32f;         /* optional: the tone-mapped frame, alpha 1 (the reference's output buffer) */
+    uint32_t* out_rgba8;        /* optional: out_rgba32f through the SDR conversion with `sdr`, packed
+                                   R | G << 8 | B << 16 | A << 24 (the saved image's pixels) */
+    rtc8x_sdr_config sdr;
+} rtc8x_tone_map_params;
+
+/* The post-process pass on the device: mean of the accumulator, applyToneMap, and (optionally) the SDR
+ * conversion fused into the same kernel.  Device pointers; asynchronous on `stream`. */
+rtc8x_status rtc8x_tone_map(void* stream, const rtc8x_tone_map_params* params);
+/* The SDR conversion of saveImage on the host: data holds width*height*num_channels floats (1..4
+ * channels), out receives width*height packed RGBA8 pixels.  Channel values are quantised as
+ * min(trunc(255 v), 255), with NaN and values below 0 giving 0. */
+rtc8x_status rtc8x_sdr_convert(const float* data, uint32_t width, uint32_t height, uint32_t num_channels,
+                               const rtc8x_sdr_config* config, uint32_t* out);
+/* Image files in memory (host).  With out == NULL only *size is set (an upper bound for PNG; the exact
+ * size is returned by the writing call).
+ * PNG: 8-bit RGBA, per-row adaptive filters, zlib-deflated (the reference writes PNG with fpng).
+ * BMP: the bytes stb_image_write v1.14's stbi_write_bmp writes for 4-channel data: 24-bit bottom-up BGR,
+ *      alpha composited over (255, 0, 255), rows padded to 4 bytes.
+ * EXR: scanline OpenEXR, no compression, half-float channels A, B, G, R, each value brightness_scale
+ *      times the input (alpha included, as saveImageHDR scales it; 0 where the input has fewer channels),
+ *      rounded to nearest even; flip_y as in rtc8x_sdr_config. */
+rtc8x_status rtc8x_png_encode(const uint32_t* rgba8, uint32_t width, uint32_t height, void* out, size_t capacity,
+                              size_t* size);
+rtc8x_status rtc8x_bmp_encode(const uint32_t* rgba8, uint32_t width, uint32_t height, void* out, size_t capacity,
+                              size_t* size);
+rtc8x_status rtc8x_exr_encode(const float* data, uint32_t width, uint32_t height, uint32_t num_channels,
+                              float brightness_scale, int32_t flip_y, void* out, size_t capacity, size_t* size);
+
 #ifdef __cplusplus
 }
 #endif

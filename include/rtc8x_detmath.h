@@ -21,6 +21,12 @@
  *   atan : |x| > 1 -> pi/2 - atan(1/x); x > 2-sqrt(3) -> pi/6 + atan((sqrt3 x - 1)/(x + sqrt3));
  *          Taylor series to t^15 on |t| <= 2-sqrt(3) (truncation < 2^-27).
  *   atan2, acos are built on atan with the usual quadrant rules.
+ *   exp  : k = floor(x/ln2 + 1/2), r = x - k*ln2_hi - k*ln2_lo (Cody & Waite, |r| <= 0.35),
+ *          Taylor series of e^r to r^8 (truncation < 2^-29 relative), then scaled by 2^k
+ *          built from exponent bits (two factors for subnormal results).
+ *   pow  : x^y = exp(y log x) for x > 0 (relative error grows with |y log x|: about
+ *          (1 + |y log x|) * 2^-22), with the IEEE special cases for 0, +inf and NaN;
+ *          negative bases give NaN.
  *
  * The header is plain C99 (the CPU oracle includes it) and HIP C++ (the kernels
  * include it).  It must be compiled without -ffast-math and with -ffp-contract=off.
@@ -88,6 +94,46 @@ RTC8X_HD float rtc8x_logf(float x)
     lm = lm + lm;
     float fe = (float)e;
     return fe * RTC8X_LN2_HI_F + (lm + fe * RTC8X_LN2_LO_F);
+}
+
+#define RTC8X_INVLN2_F    0x1.715476p+0f   /* float(1/ln 2) */
+
+/* e^x for all float x (NaN in, NaN out; overflow to +inf above 88.72284, underflow to 0). */
+RTC8X_HD float rtc8x_expf(float x)
+{
+    if (x != x) return x;
+    if (x > 88.72284f) return rtc8x_u2f(0x7f800000u);
+    if (x < -103.97208f) return 0.0f;
+    float kf = floorf(x * RTC8X_INVLN2_F + 0.5f);
+    float r = x - kf * RTC8X_LN2_HI_F;      /* exact: |k| <= 150 and ln2_hi has 12 free bits */
+    r = r - kf * RTC8X_LN2_LO_F;
+    float p = 0x1.a01a02p-16f;                /* 1/8! */
+    p = p * r + 0x1.a01a02p-13f;              /* 1/7! */
+    p = p * r + 0x1.6c16c2p-10f;              /* 1/6! */
+    p = p * r + 0x1.111112p-7f;               /* 1/5! */
+    p = p * r + 0x1.555556p-5f;               /* 1/4! */
+    p = p * r + 0x1.555556p-3f;               /* 1/3! */
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int32_t k = (int32_t)kf;
+    if (k >= -126) {
+        if (k > 127) return (p * 2.0f) * rtc8x_u2f((uint32_t)(k - 1 + 127) << 23);
+        return p * rtc8x_u2f((uint32_t)(k + 127) << 23);
+    }
+    /* subnormal result: 2^k = 2^(k+64) * 2^-64, both factors normal */
+    return (p * rtc8x_u2f((uint32_t)(k + 64 + 127) << 23)) * 0x1p-64f;
+}
+
+/* x^y for x >= 0 (NaN for x < 0 or NaN operands; x^0 = 1). */
+RTC8X_HD float rtc8x_powf(float x, float y)
+{
+    if (y == 0.0f) return 1.0f;
+    if (x != x || y != y || x < 0.0f) return rtc8x_u2f(0x7fc00000u);
+    if (x == 0.0f) return y > 0.0f ? 0.0f : rtc8x_u2f(0x7f800000u);
+    if (x == rtc8x_u2f(0x7f800000u)) return y > 0.0f ? x : 0.0f;
+    if (x == 1.0f) return 1.0f;
+    return rtc8x_expf(y * rtc8x_logf(x));
 }
 
 /* sin and cos of x for |x| < 4096 * pi/2 (the phase sampler uses [0, 2pi)). */

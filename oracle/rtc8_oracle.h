@@ -87,6 +87,15 @@ void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float*
 float oracle_round_fp16(float x);
 float oracle_round_bf16(float x);
 
+/* Post-process (rtc8_oracle_post.c): applyToneMap over the accumulator's mean plus the SDR conversion,
+ * and the SDR conversion alone; f64 != 0 evaluates in double with libm. */
+void oracle_tone_map(const double* accum, uint32_t w, uint32_t h, uint32_t samples, float exponent, float brightness,
+                     const rtc8x_sdr_config* cfg, float* out_rgba32f, uint32_t* out_rgba8, int32_t f64);
+void oracle_sdr_convert(const float* data, uint32_t w, uint32_t h, uint32_t nch, const rtc8x_sdr_config* cfg,
+                        uint32_t* out, int32_t f64);
+float oracle_expf(float x);
+float oracle_powf(float x, float y);
+
 #ifdef __cplusplus
 }
 #endif

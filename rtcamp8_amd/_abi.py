@@ -95,6 +95,30 @@ class NvdbGridInfo(C.Structure):
     ]
 
 
+class SdrConfig(C.Structure):
+    _fields_ = [
+        ("alpha_override", C.c_float),
+        ("brightness_scale", C.c_float),
+        ("apply_tone_map", C.c_int32),
+        ("apply_srgb_gamma", C.c_int32),
+        ("flip_y", C.c_int32),
+    ]
+
+
+class ToneMapParams(C.Structure):
+    _fields_ = [
+        ("accum", C.c_void_p),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("samples", C.c_uint32),
+        ("exponent", C.c_float),
+        ("brightness", C.c_float),
+        ("out_rgba32f", C.c_void_p),
+        ("out_rgba8", C.c_void_p),
+        ("sdr", SdrConfig),
+    ]
+
+
 class FbmDesc(C.Structure):
     _fields_ = [
         ("resolution", C.c_uint32),
@@ -309,6 +333,8 @@ STRUCTS = {
     "rtc8x_nrc_buffers": NrcBuffers,
     "rtc8x_frame_params": FrameParams,
     "rtc8x_nvdb_grid_info": NvdbGridInfo,
+    "rtc8x_sdr_config": SdrConfig,
+    "rtc8x_tone_map_params": ToneMapParams,
 }
 
 _P = C.c_void_p
@@ -350,6 +376,12 @@ PROTOTYPES = {
     "rtc8x_grid_load_nvdb": (_S, [_P, C.c_size_t, C.c_uint32, C.POINTER(Grid)]),
     "rtc8x_grid_download": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
     "rtc8x_nanovdb_encode": (_S, [C.POINTER(Grid), C.c_char_p, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rtc8x_tone_map": (_S, [_P, C.POINTER(ToneMapParams)]),
+    "rtc8x_sdr_convert": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(SdrConfig), _P]),
+    "rtc8x_png_encode": (_S, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rtc8x_bmp_encode": (_S, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rtc8x_exr_encode": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int32, _P, C.c_size_t,
+                              C.POINTER(C.c_size_t)]),
     "rtc8x_nvdb_write": (_S, [C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_uint32, C.c_uint32, _P, C.c_size_t,
                               C.POINTER(C.c_size_t)]),
     "rtc8x_env_build_sampler": (_S, [C.POINTER(EnvLight), _P]),

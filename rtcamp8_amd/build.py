@@ -29,6 +29,7 @@ SOURCES = [
     "rtc8x_nrc_mlp.hip",
     "rtc8x_measure.hip",
     "rtc8x_nvdb.hip",
+    "rtc8x_post.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -234,6 +234,8 @@ extern "C" size_t rtc8x_sizeof(const char* type_name)
         {"rtc8x_hosek_band", sizeof(rtc8x_hosek_band)},
         {"rtc8x_hosek_sky", sizeof(rtc8x_hosek_sky)},
         {"rtc8x_nvdb_grid_info", sizeof(rtc8x_nvdb_grid_info)},
+        {"rtc8x_sdr_config", sizeof(rtc8x_sdr_config)},
+        {"rtc8x_tone_map_params", sizeof(rtc8x_tone_map_params)},
     };
     for (const Entry& e : table)
         if (std::strcmp(e.name, type_name) == 0) return e.size;

@@ -69,6 +69,12 @@ def oracle() -> C.CDLL:
         "oracle_nrc_adam": (None, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float,
                                    C.c_float, C.c_float, C.c_float, C.c_uint64]),
         "oracle_round_fp16": (C.c_float, [C.c_float]),
+        "oracle_expf": (C.c_float, [C.c_float]),
+        "oracle_powf": (C.c_float, [C.c_float, C.c_float]),
+        "oracle_tone_map": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                   C.POINTER(_abi.SdrConfig), _P, _P, C.c_int32]),
+        "oracle_sdr_convert": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_abi.SdrConfig), _P,
+                                      C.c_int32]),
         "oracle_round_bf16": (C.c_float, [C.c_float]),
     }
     for name, (res, args) in protos.items():

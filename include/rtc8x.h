@@ -349,6 +349,26 @@ rtc8x_status rtc8x_sky_bake(void* stream, const rtc8x_hosek_sky* sky, const floa
 rtc8x_status rtc8x_sky_eval(void* stream, const rtc8x_hosek_sky* sky, const float sun_dir[3], uint32_t n,
                             const float* phi, const float* theta, float* out_rgb);
 
+/* The spectral Hosek-Wilkie 2013 model for any sun (the reference re-derives the sky state every frame
+ * from the interpolated solar direction and turbidity: AnalyticSkyEnvironment::computeDistribution,
+ * renderer/renderer_scene.cpp:555-602).  rtc8x_hosek_model_create reads the model's tables from the
+ * image of rtcamp8_amd/data/hosek_model.bin (char[8] "RTC8XHW1"; f64 configs[11][1080],
+ * radiances[11][120], solar[11][1800], limb_darkening[11][6]; f32 cmf_x, cmf_y, cmf_z,
+ * center_wavelengths [16 each], integral_cmf; little-endian). */
+typedef struct rtc8x_hosek_model rtc8x_hosek_model;
+rtc8x_status rtc8x_hosek_model_create(const void* data, size_t size, rtc8x_hosek_model** out);
+rtc8x_status rtc8x_hosek_model_destroy(rtc8x_hosek_model* model);
+/* The 16 per-band states for a solar elevation in [0, pi/2] (radians) and turbidity in [1, 10], each band
+ * with ground albedo 0.5 (1 - (b + 0.5) / 16): arhosekskymodelstate_alloc_init
+ * (ext/hosek_wilkie_2013/ArHosekSkyModel.c:147-345) in double, cast to float field by field.  Host. */
+rtc8x_status rtc8x_hosek_bands(const rtc8x_hosek_model* model, float solar_elevation, float turbidity,
+                               rtc8x_hosek_band* out);
+/* Uploads the band states and solar-disc rows for (solar_elevation, turbidity) to device memory the model
+ * owns and fills `sky` for rtc8x_sky_bake / rtc8x_sky_eval.  Synchronises the stream first; the pointers
+ * stay valid until the next call or rtc8x_hosek_model_destroy. */
+rtc8x_status rtc8x_hosek_model_state(rtc8x_hosek_model* model, float solar_elevation, float turbidity,
+                                     rtc8x_hosek_sky* sky, void* stream);
+
 /* ------------------------------------------------------------------------------------------------
  * Camera: PerspectiveCamera (common/common_renderer_types.h:1019-1034) reduced to what the
  * ray generation uses (optix_pathtracing_kernels.cu:43-55).

@@ -60,3 +60,18 @@ void hosek_limb_table(int wl, double* out)
 {
     memcpy(out, limbDarkeningDatasets[wl], sizeof(double) * 6);
 }
+
+extern double* datasets[];
+extern double* datasetsRad[];
+
+/* One wavelength channel of the spectral model's configuration table (1080 = 2 albedos x 10 turbidities
+ * x 6 elevation control points x 9 parameters) and of its radiance table (120 = 2 x 10 x 6). */
+void hosek_config_table(int wl, double* out)
+{
+    memcpy(out, datasets[wl], sizeof(double) * 1080);
+}
+
+void hosek_radiance_table(int wl, double* out)
+{
+    memcpy(out, datasetsRad[wl], sizeof(double) * 120);
+}

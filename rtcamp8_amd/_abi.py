@@ -376,6 +376,10 @@ PROTOTYPES = {
     "rtc8x_grid_load_nvdb": (_S, [_P, C.c_size_t, C.c_uint32, C.POINTER(Grid)]),
     "rtc8x_grid_download": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
     "rtc8x_nanovdb_encode": (_S, [C.POINTER(Grid), C.c_char_p, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rtc8x_hosek_model_create": (_S, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "rtc8x_hosek_model_destroy": (_S, [_P]),
+    "rtc8x_hosek_bands": (_S, [_P, C.c_float, C.c_float, _P]),
+    "rtc8x_hosek_model_state": (_S, [_P, C.c_float, C.c_float, C.POINTER(HosekSky), _P]),
     "rtc8x_tone_map": (_S, [_P, C.POINTER(ToneMapParams)]),
     "rtc8x_sdr_convert": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(SdrConfig), _P]),
     "rtc8x_png_encode": (_S, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "rtc8x_common.hpp"
 #include "rtc8x_device.hpp"
@@ -259,5 +261,175 @@ extern "C" rtc8x_status rtc8x_sky_eval(void* stream, const rtc8x_hosek_sky* sky,
     hipLaunchKernelGGL(dev::sky_eval_kernel, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, *sky,
                        dev::V3{sun_dir[0], sun_dir[1], sun_dir[2]}, n, phi, theta, out_rgb);
     RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// The spectral model's tables and its state for any sun elevation and turbidity (host code).
+// ---------------------------------------------------------------------------------------------------
+
+struct rtc8x_hosek_model {
+    std::vector<double> configs;    // [11][1080]: [albedo 0,1][turbidity 1..10][6 control points][9]
+    std::vector<double> radiances;  // [11][120]:  [albedo][turbidity][6]
+    std::vector<double> solar;      // [11][1800]: [turbidity][45 pieces][4]
+    std::vector<double> limb;       // [11][6]
+    float cmf[4][RTC8X_SKY_BANDS];
+    float integral_cmf;
+    // device copies for the bake (owned; replaced by each rtc8x_hosek_model_state call)
+    rtc8x_hosek_band* d_bands = nullptr;
+    float* d_solar = nullptr;
+    float* d_limb = nullptr;
+};
+
+namespace {
+
+constexpr size_t kHosekConfigs = 11 * 1080, kHosekRadiances = 11 * 120, kHosekSolar = 11 * 1800, kHosekLimb = 11 * 6;
+constexpr size_t kHosekBytes = 8 + 8 * (kHosekConfigs + kHosekRadiances + kHosekSolar + kHosekLimb) +
+                               4 * (4 * RTC8X_SKY_BANDS + 1);
+
+// The quintic Bernstein blend of six control points at x, term by term as the model evaluates it
+// (ArHosekSkyModel.c:171-178).
+double bernstein5(const double* a, size_t stride, double x)
+{
+    return std::pow(1.0 - x, 5.0) * a[0] + 5.0 * std::pow(1.0 - x, 4.0) * x * a[stride] +
+           10.0 * std::pow(1.0 - x, 3.0) * std::pow(x, 2.0) * a[2 * stride] +
+           10.0 * std::pow(1.0 - x, 2.0) * std::pow(x, 3.0) * a[3 * stride] +
+           5.0 * (1.0 - x) * std::pow(x, 4.0) * a[4 * stride] + std::pow(x, 5.0) * a[5 * stride];
+}
+
+// ArHosekSkyModel_CookConfiguration / _CookRadianceConfiguration (ArHosekSkyModel.c:147-289): bilinear
+// in ground albedo (tables 0 and 1) and turbidity (integer levels T and T + 1), Bernstein in the cube
+// root of the normalised solar elevation.  `block` is 9 for the configuration, 1 for the radiance.
+void cook(const double* table, size_t block, double turbidity, double albedo, double elevation, double* out)
+{
+    const int it = (int)turbidity;
+    const double rem = turbidity - (double)it;
+    const double x = std::pow(elevation / (3.14159265358979323846 / 2.0), 1.0 / 3.0);
+    const size_t per_albedo = 6 * block * 10;
+    for (size_t i = 0; i < block; ++i) {
+        const double* lo0 = table + 6 * block * (size_t)(it - 1) + i;
+        const double* lo1 = table + per_albedo + 6 * block * (size_t)(it - 1) + i;
+        out[i] = (1.0 - albedo) * (1.0 - rem) * bernstein5(lo0, block, x);
+        out[i] += albedo * (1.0 - rem) * bernstein5(lo1, block, x);
+    }
+    if (it == 10) return;
+    for (size_t i = 0; i < block; ++i) {
+        const double* hi0 = table + 6 * block * (size_t)it + i;
+        const double* hi1 = table + per_albedo + 6 * block * (size_t)it + i;
+        out[i] += (1.0 - albedo) * rem * bernstein5(hi0, block, x);
+        out[i] += albedo * rem * bernstein5(hi1, block, x);
+    }
+}
+
+void free_device(rtc8x_hosek_model* m)
+{
+    if (m->d_bands) (void)hipFree(m->d_bands);
+    if (m->d_solar) (void)hipFree(m->d_solar);
+    if (m->d_limb) (void)hipFree(m->d_limb);
+    m->d_bands = nullptr;
+    m->d_solar = nullptr;
+    m->d_limb = nullptr;
+}
+
+}  // namespace
+
+extern "C" rtc8x_status rtc8x_hosek_model_create(const void* data, size_t size, rtc8x_hosek_model** out)
+{
+    RTC8X_REQUIRE(data && out, "rtc8x_hosek_model_create: NULL argument");
+    RTC8X_REQUIRE(size == kHosekBytes && std::memcmp(data, "RTC8XHW1", 8) == 0,
+                  "rtc8x_hosek_model_create: not a model table image (%zu bytes, expected %zu)", size, kHosekBytes);
+    auto m = new rtc8x_hosek_model();
+    const uint8_t* p = (const uint8_t*)data + 8;
+    auto take = [&](std::vector<double>& v, size_t n) {
+        v.resize(n);
+        std::memcpy(v.data(), p, 8 * n);
+        p += 8 * n;
+    };
+    take(m->configs, kHosekConfigs);
+    take(m->radiances, kHosekRadiances);
+    take(m->solar, kHosekSolar);
+    take(m->limb, kHosekLimb);
+    std::memcpy(m->cmf, p, sizeof(m->cmf));
+    p += sizeof(m->cmf);
+    std::memcpy(&m->integral_cmf, p, 4);
+    *out = m;
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_hosek_model_destroy(rtc8x_hosek_model* m)
+{
+    if (!m) return RTC8X_OK;
+    free_device(m);
+    delete m;
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_hosek_bands(const rtc8x_hosek_model* m, float solar_elevation, float turbidity,
+                                          rtc8x_hosek_band* out)
+{
+    RTC8X_REQUIRE(m && out, "rtc8x_hosek_bands: NULL argument");
+    RTC8X_REQUIRE(turbidity >= 1.0f && turbidity <= 10.0f, "rtc8x_hosek_bands: turbidity %g outside [1, 10]",
+                  (double)turbidity);
+    RTC8X_REQUIRE(solar_elevation >= 0.0f && solar_elevation <= 1.5707964f,
+                  "rtc8x_hosek_bands: solar elevation %g outside [0, pi/2]", (double)solar_elevation);
+    for (uint32_t b = 0; b < RTC8X_SKY_BANDS; ++b) {
+        // ground albedo per band (renderer_scene.cpp:576), then arhosekskymodelstate_alloc_init
+        // (ArHosekSkyModel.c:308-345) with every field cast to float (renderer_scene.cpp:579-590)
+        const float albedo = 0.5f * (1.0f - ((float)b + 0.5f) / (float)RTC8X_SKY_BANDS);
+        rtc8x_hosek_band& s = out[b];
+        for (int wl = 0; wl < 11; ++wl) {
+            double cfg[9], rad;
+            cook(m->configs.data() + (size_t)wl * 1080, 9, turbidity, albedo, solar_elevation, cfg);
+            cook(m->radiances.data() + (size_t)wl * 120, 1, turbidity, albedo, solar_elevation, &rad);
+            for (int j = 0; j < 9; ++j) s.configs[wl][j] = (float)cfg[j];
+            s.radiances[wl] = (float)rad;
+            s.emission_correction_factor_sky[wl] = 1.0f;
+            s.emission_correction_factor_sun[wl] = 1.0f;
+        }
+        s.turbidity = turbidity;
+        s.solar_radius = (float)((0.51 * (3.14159265358979323846 / 180.0)) / 2.0);
+        s.albedo = albedo;
+        s.elevation = solar_elevation;
+    }
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_hosek_model_state(rtc8x_hosek_model* m, float solar_elevation, float turbidity,
+                                                rtc8x_hosek_sky* sky, void* stream)
+{
+    RTC8X_REQUIRE(m && sky, "rtc8x_hosek_model_state: NULL argument");
+    std::vector<rtc8x_hosek_band> bands(RTC8X_SKY_BANDS);
+    rtc8x_status st = rtc8x_hosek_bands(m, solar_elevation, turbidity, bands.data());
+    if (st != RTC8X_OK) return st;
+    // the solar-disc rows the bake reads: turbidity levels base and base + 1 (ArHosekSkyModel.c:700-707)
+    int base = (int)turbidity - 1;
+    if (base == 9) base = 8;
+    std::vector<float> rows((size_t)11 * 2 * 180), limb(kHosekLimb);
+    for (int wl = 0; wl < 11; ++wl)
+        for (int t = 0; t < 2; ++t)
+            for (int k = 0; k < 180; ++k)
+                rows[((size_t)wl * 2 + t) * 180 + k] = (float)m->solar[(size_t)wl * 1800 + (size_t)(base + t) * 180 + k];
+    for (size_t i = 0; i < kHosekLimb; ++i) limb[i] = (float)m->limb[i];
+    hipStream_t s = (hipStream_t)stream;
+    RTC8X_HIP(hipStreamSynchronize(s));  // the previous state may still be read by a bake on this stream
+    if (!m->d_bands) {
+        RTC8X_HIP(hipMalloc((void**)&m->d_bands, sizeof(rtc8x_hosek_band) * RTC8X_SKY_BANDS));
+        RTC8X_HIP(hipMalloc((void**)&m->d_solar, sizeof(float) * rows.size()));
+        RTC8X_HIP(hipMalloc((void**)&m->d_limb, sizeof(float) * limb.size()));
+    }
+    RTC8X_HIP(hipMemcpy(m->d_bands, bands.data(), sizeof(rtc8x_hosek_band) * RTC8X_SKY_BANDS, hipMemcpyHostToDevice));
+    RTC8X_HIP(hipMemcpy(m->d_solar, rows.data(), sizeof(float) * rows.size(), hipMemcpyHostToDevice));
+    RTC8X_HIP(hipMemcpy(m->d_limb, limb.data(), sizeof(float) * limb.size(), hipMemcpyHostToDevice));
+    sky->bands = m->d_bands;
+    sky->solar_rows = m->d_solar;
+    sky->limb_darkening = m->d_limb;
+    sky->solar_turbidity_base = base;
+    for (uint32_t b = 0; b < RTC8X_SKY_BANDS; ++b) {
+        sky->cmf_x[b] = m->cmf[0][b];
+        sky->cmf_y[b] = m->cmf[1][b];
+        sky->cmf_z[b] = m->cmf[2][b];
+        sky->center_wavelengths[b] = m->cmf[3][b];
+    }
+    sky->integral_cmf = m->integral_cmf;
     return RTC8X_OK;
 }

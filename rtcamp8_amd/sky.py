@@ -1,10 +1,13 @@
-"""Hosek-Wilkie sky of the sparse512 workload: loads the per-band model state at turbidity 3 / solar
-elevation 30 deg (rtcamp8_amd/data/hosek_T3_e30.npz, produced from the Hosek-Wilkie 2013 model by
-tests/golden/make_sky_data.py) and bakes the lat-long environment texture on the GPU
-(rtc8x_sky_bake, the role of generateArHosekSkyEnvironmentalTexture)."""
+"""Hosek-Wilkie skies for any sun: the spectral model's tables (rtcamp8_amd/data/hosek_model.bin,
+written from the Hosek-Wilkie 2013 model by tests/golden/make_sky_model.py) give the per-band state for
+a solar elevation and turbidity (rtc8x_hosek_model_state, the host part of
+AnalyticSkyEnvironment::computeDistribution, renderer/renderer_scene.cpp:555-602), and the lat-long
+environment texture is baked on the GPU (rtc8x_sky_bake, the role of
+generateArHosekSkyEnvironmentalTexture)."""
 from __future__ import annotations
 
 import ctypes as C
+import math
 from pathlib import Path
 
 import numpy as np
@@ -13,28 +16,50 @@ from . import _abi
 from ._lib import call
 from .device import DeviceBuffer
 
-DATA = Path(__file__).resolve().parent / "data" / "hosek_T3_e30.npz"
+MODEL = Path(__file__).resolve().parent / "data" / "hosek_model.bin"
+
+
+class HosekModel:
+    """The model tables, loaded once; ``state`` re-derives the sky block for a sun."""
+
+    def __init__(self, path: Path = MODEL):
+        blob = Path(path).read_bytes()
+        self.handle = C.c_void_p()
+        call("rtc8x_hosek_model_create", blob, len(blob), C.byref(self.handle))
+
+    def bands(self, solar_elevation: float, turbidity: float) -> np.ndarray:
+        """The 16 per-band states as a [16, 136] float32 array (host)."""
+        out = np.zeros((_abi.SKY_BANDS, 136), np.float32)
+        call("rtc8x_hosek_bands", self.handle, float(solar_elevation), float(turbidity), out.ctypes.data)
+        return out
+
+    def state(self, solar_elevation: float, turbidity: float, stream=None) -> _abi.HosekSky:
+        sky = _abi.HosekSky()
+        call("rtc8x_hosek_model_state", self.handle, float(solar_elevation), float(turbidity), C.byref(sky),
+             C.c_void_p(stream))
+        return sky
+
+    def close(self) -> None:
+        if self.handle:
+            call("rtc8x_hosek_model_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+
+def solar_elevation_of(sun_dir) -> float:
+    """0.5 pi - acos(clamp(sun.y)) in float32, as computeDistribution derives it (renderer_scene.cpp:570)."""
+    y = np.float32(min(max(float(sun_dir[1]), -1.0), 1.0))
+    return float(np.float32(0.5) * np.float32(math.pi) - np.float32(math.acos(float(y))))
 
 
 class HosekSky:
-    """Device-resident model tables plus the rtc8x_hosek_sky block that points at them."""
+    """A sky for one sun: the model's state for (solar elevation, turbidity) on the device."""
 
-    def __init__(self, path: Path = DATA):
-        d = np.load(path)
-        self.bands = DeviceBuffer.from_array(np.ascontiguousarray(d["bands"], dtype=np.float32))
-        self.solar = DeviceBuffer.from_array(np.ascontiguousarray(d["solar_rows"], dtype=np.float32))
-        self.limb = DeviceBuffer.from_array(np.ascontiguousarray(d["limb_darkening"], dtype=np.float32))
-        s = _abi.HosekSky()
-        s.bands, s.solar_rows, s.limb_darkening = self.bands.ptr, self.solar.ptr, self.limb.ptr
-        s.solar_turbidity_base = int(d["solar_turbidity_base"])
-        s.cmf_x[:] = d["cmf_x"].tolist()
-        s.cmf_y[:] = d["cmf_y"].tolist()
-        s.cmf_z[:] = d["cmf_z"].tolist()
-        s.center_wavelengths[:] = d["center_wavelengths"].tolist()
-        s.integral_cmf = float(d["integral_cmf"])
-        self.struct = s
-        self.turbidity = float(d["turbidity"])
-        self.elevation = float(d["elevation"])
+    def __init__(self, solar_elevation: float = math.radians(30.0), turbidity: float = 3.0, model: HosekModel | None = None):
+        self.own_model = model is None
+        self.model = model or HosekModel()
+        self.elevation = float(np.float32(solar_elevation))
+        self.turbidity = float(np.float32(turbidity))
+        self.struct = self.model.state(self.elevation, self.turbidity)
 
     def bake(self, sun_dir, width: int, height: int, stream=None) -> DeviceBuffer:
         tex = DeviceBuffer(4 * 4 * width * height)
@@ -48,21 +73,25 @@ class HosekSky:
         dt = DeviceBuffer.from_array(np.ascontiguousarray(theta, dtype=np.float32))
         out = DeviceBuffer(12 * n)
         sd = (C.c_float * 3)(*[float(v) for v in sun_dir])
-        call("rtc8x_sky_eval", C.c_void_p(stream), C.byref(self.struct), sd, n, C.c_void_p(dp.ptr),
-             C.c_void_p(dt.ptr), C.c_void_p(out.ptr))
-        return out.download(np.float32, 3 * n).reshape(n, 3)
+        try:
+            call("rtc8x_sky_eval", C.c_void_p(stream), C.byref(self.struct), sd, n, C.c_void_p(dp.ptr),
+                 C.c_void_p(dt.ptr), C.c_void_p(out.ptr))
+            return out.download(np.float32, 3 * n).reshape(n, 3)
+        finally:
+            for b in (dp, dt, out):
+                b.free()
 
     def close(self) -> None:
-        for b in (self.bands, self.solar, self.limb):
-            b.free()
+        if self.own_model:
+            self.model.close()
 
 
 def bake_hosek_sky(cfg, sun_dir, stream=None) -> DeviceBuffer:
-    sky = HosekSky()
-    assert abs(sky.turbidity - cfg.turbidity) < 1e-6, "the shipped sky data is for turbidity 3"
-    tex = sky.bake(sun_dir, cfg.sky_width, cfg.sky_height, stream)
+    """The workload's sky texture (cfg.sky_width x cfg.sky_height) for its sun and turbidity."""
     from .device import synchronize
 
+    sky = HosekSky(solar_elevation_of(sun_dir), cfg.turbidity)
+    tex = sky.bake(sun_dir, cfg.sky_width, cfg.sky_height, stream)
     synchronize()
     sky.close()
     return tex

@@ -1,18 +1,10 @@
-"""Generates the Hosek-Wilkie inputs of the benchmark sky and the FP64 golden vectors for its bake.
+"""Generates the FP64 golden vectors for the benchmark sky's bake (turbidity 3, solar elevation 30 deg).
 
 Run only where the rtcamp8 tree is mounted: it calls the Hosek-Wilkie 2013 model rtcamp8 vendors
 (ext/hosek_wilkie_2013, 3-clause BSD), compiled into oracle/_ref/libhosek_ref.so by oracle/ref/Makefile,
 and reads the CIE 1931 2-degree colour-matching tables out of common/basic_types.cpp as text.
 
 Outputs
-  rtcamp8_amd/data/hosek_T3_e30.npz  -- the bake's inputs at turbidity 3, solar elevation 30 deg:
-      bands [16, 136] float32  ArHosekSkyModelState per spectral band in float, exactly the block
-                               renderer/renderer_scene.cpp:574-592 uploads (ground albedo per band
-                               0.5 (1 - (b + 0.5) / 16));
-      cmf_x/cmf_y/cmf_z/center_wavelengths [16], integral_cmf  -- computeDiscretizedXyzCmfs
-                               (common/basic_types.cpp:195-248) restated in float32;
-      solar_rows [11, 2, 180] float32 -- the solar-radiance coefficients of turbidity indices 2 and 3;
-      limb_darkening [11, 6] float32.
   tests/golden/hosek_golden.npz -- FP64 sky radiance (RGB after the reference's colour pipeline,
       analytic_sky_kernels.cu:220-276) for a set of view directions including the solar disc and the
       lower hemisphere, evaluated with the vendored FP64 model.
@@ -107,7 +99,7 @@ def band_albedo(b):
     return 0.5 * (1.0 - (b + 0.5) / NUM_BANDS)
 
 
-def sky_rgb_fp64(lib, view, sun, cmf):
+def sky_rgb_fp64(lib, view, sun, cmf, elevation=ELEVATION, turbidity=TURBIDITY):
     """FP64 restatement of generateArHosekSkyEnvironmentalTexture's per-texel pipeline
     (analytic_sky_kernels.cu:227-274) for one view direction given as (phi, theta)."""
     phi, theta = view
@@ -121,9 +113,9 @@ def sky_rgb_fp64(lib, view, sun, cmf):
         wl = min(max(float(centers[b]), 320.0), 720.0)
         alb = band_albedo(b)
         if gamma < (0.51 * math.pi / 180.0) / 2.0 and not lower:
-            v = lib.hosek_solar_radiance(ELEVATION, TURBIDITY, alb, m_theta, gamma, wl)
+            v = lib.hosek_solar_radiance(elevation, turbidity, alb, m_theta, gamma, wl)
         else:
-            v = lib.hosek_radiance(ELEVATION, TURBIDITY, alb, m_theta, gamma, wl)
+            v = lib.hosek_radiance(elevation, turbidity, alb, m_theta, gamma, wl)
         xyz += np.array([xs[b], ys[b], zs[b]], dtype=np.float64) * v
     xyz /= float(integral)
     m = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
@@ -166,27 +158,7 @@ def sky_rgb_fp64(lib, view, sun, cmf):
 
 def main() -> int:
     lib = hosek_lib()
-    n_state = lib.hosek_state_size()
-    assert n_state == 136
-    bands = np.zeros((NUM_BANDS, n_state), dtype=np.float32)
-    buf = np.zeros(n_state, dtype=np.float64)
-    for b in range(NUM_BANDS):
-        lib.hosek_state(ELEVATION, TURBIDITY, band_albedo(b), buf.ctypes.data)
-        bands[b] = buf.astype(np.float32)
-    solar = np.zeros((11, 1800), dtype=np.float64)
-    limb = np.zeros((11, 6), dtype=np.float64)
-    for wl in range(11):
-        lib.hosek_solar_table(wl, solar[wl].ctypes.data)
-        lib.hosek_limb_table(wl, limb[wl].ctypes.data)
-    base = int(TURBIDITY) - 1  # turbidity indices base and base + 1 are the ones the model reads
-    solar_rows = solar.reshape(11, 10, 180)[:, base:base + 2, :].astype(np.float32)
     cmf = discretized_cmfs(*cie_tables(), NUM_BANDS)
-    data_dir = ROOT / "rtcamp8_amd" / "data"
-    data_dir.mkdir(exist_ok=True)
-    np.savez_compressed(data_dir / "hosek_T3_e30.npz", bands=bands, cmf_x=cmf[0], cmf_y=cmf[1], cmf_z=cmf[2],
-                        center_wavelengths=cmf[3], integral_cmf=np.float32(cmf[4]), solar_rows=solar_rows,
-                        solar_turbidity_base=np.int32(base), limb_darkening=limb.astype(np.float32),
-                        turbidity=np.float32(TURBIDITY), elevation=np.float32(ELEVATION))
     # golden directions: a regular grid over the sphere plus points on and around the solar disc
     sun = np.array([math.cos(ELEVATION) * math.sin(SUN_AZIMUTH), math.sin(ELEVATION),
                     math.cos(ELEVATION) * math.cos(SUN_AZIMUTH)])
@@ -206,7 +178,7 @@ def main() -> int:
     rgb = np.array([sky_rgb_fp64(lib, v, sun64, cmf) for v in views])
     np.savez_compressed(ROOT / "tests" / "golden" / "hosek_golden.npz", phi=views[:, 0].astype(np.float32),
                         theta=views[:, 1].astype(np.float32), sun=sun32, rgb=rgb)
-    print(f"wrote hosek_T3_e30.npz and hosek_golden.npz ({len(views)} directions), integral_cmf {cmf[4]}")
+    print(f"wrote hosek_golden.npz ({len(views)} directions), integral_cmf {cmf[4]}")
     return 0
 
 

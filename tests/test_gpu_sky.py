@@ -15,7 +15,7 @@ def test_sky_eval_matches_fp64_model():
     from rtcamp8_amd.sky import HosekSky
 
     g = np.load(GOLDEN)
-    sky = HosekSky()
+    sky = HosekSky(math.radians(30.0), 3.0)
     got = sky.eval(g["sun"], g["phi"], g["theta"]).astype(np.float64)
     ref = g["rgb"]
     sky_part, disc = slice(0, 1152), slice(1152, None)
@@ -46,7 +46,7 @@ def test_bake_matches_eval_at_texel_centres():
     from rtcamp8_amd.sky import HosekSky
 
     g = np.load(GOLDEN)
-    sky = HosekSky()
+    sky = HosekSky(math.radians(30.0), 3.0)
     w, h = 64, 32
     tex = sky.bake(g["sun"], w, h).download(np.float32, 4 * w * h).reshape(h, w, 4)
     y, x = np.mgrid[0:h, 0:w]
@@ -55,4 +55,22 @@ def test_bake_matches_eval_at_texel_centres():
     ev = sky.eval(g["sun"], phi, theta).reshape(h, w, 3)
     assert np.allclose(tex[..., :3], ev, rtol=1e-6, atol=0)
     assert np.all(tex[..., 3] == 1.0)
+    sky.close()
+
+
+def test_second_sun_matches_fp64_model():
+    """A sky re-derived from the model tables for another sun (elevation 70 deg, turbidity 6.5), as the
+    batch driver does per frame, against the vendored FP64 model (tests/golden/hosek_states.npz)."""
+    from rtcamp8_amd.sky import HosekSky
+
+    g = np.load(Path(__file__).resolve().parent / "golden" / "hosek_states.npz")
+    sky = HosekSky(float(g["sun2_elevation"]), float(g["sun2_turbidity"]))
+    got = sky.eval(g["sun2"], g["phi2"], g["theta2"]).astype(np.float64)
+    ref = g["rgb2"]
+    sky_part, disc = slice(0, 1152), slice(1152 + 64, None)
+    rel = np.abs(got[sky_part] - ref[sky_part]) / np.maximum(np.abs(ref[sky_part]), 1e-3)
+    assert np.max(rel) < 2e-4, np.max(rel)
+    peak = ref[disc].max()
+    assert np.max(np.abs(got[disc] - ref[disc])) < 2e-3 * peak
+    assert peak > 1000 * np.median(ref[sky_part])
     sky.close()

@@ -18,6 +18,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "librtc8x.so"
+RENDER = PKG / "rtc8x_render"
+DRIVER_SOURCES = ["driver/scene_file.cpp", "driver/rtc8x_render.cpp"]
 
 SOURCES = [
     "rtc8x_api.hip",
@@ -76,7 +78,21 @@ def build_library(jobs: int | None = None) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver() -> Path:
+    """The batch renderer executable (host C++ over the C ABI, linked against librtc8x.so in place)."""
+    cxx = shutil.which("g++") or "g++"
+    tmp = RENDER.with_suffix(".tmp")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", *[str(CSRC / s) for s in DRIVER_SOURCES],
+           f"-L{PKG}", "-lrtc8x", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, RENDER)
+    return RENDER
 
 
 def main() -> int:

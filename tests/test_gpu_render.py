@@ -1,0 +1,145 @@
+"""The batch renderer end to end on the GPU: its plain-tracer frames are the pixels the library gives for
+the same frame rebuilt through the Python bindings (volume from the .nvdb, sky re-derived for the frame's
+sun, camera at the frame's key interpolation, RNG carried from frame to frame, tone map and SDR
+conversion), and its NRC frames come out as images of the same scene."""
+import ctypes as C
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+EXE = ROOT / "rtcamp8_amd" / "rtc8x_render"
+
+SCENE = """image 96 64
+time 0 1 4
+
+skyenv env
+env-coeff env 2
+skyenv-dir env 30 40
+env-addkey env 0.0
+skyenv-dir env 120 70
+env-rotation env 90
+env-addkey env 1.0
+
+volume "cloud.nvdb"
+
+camera cam
+fovy cam 40
+lookat cam 1.8 1.2 2.5 0.5 0.5 0.5 0 1 0
+cam-addkey cam 0.0 false false
+fovy cam 30
+lookat cam 2.5 0.9 0.8 0.5 0.5 0.5 0 1 0
+cam-addkey cam 1.0 false false
+"""
+
+
+@pytest.fixture(scope="module")
+def scene_dir(tmp_path_factory):
+    from rtcamp8_amd import nvdb
+    from rtcamp8_amd.scene import FbmConfig
+    from oracle_lib import oracle_build_grid
+
+    d = tmp_path_factory.mktemp("render")
+    hg = oracle_build_grid(FbmConfig(resolution=64, seed=5, sparse=True, active_leaf_fraction=0.4, contrast=2.0))
+    (d / "cloud.nvdb").write_bytes(nvdb.write_grids([nvdb.encode_grid(hg.grid, "density")], "zip"))
+    (d / "scene.txt").write_text(SCENE)
+    return d
+
+
+def render(scene_dir, out, *args):
+    print(f"[render] {' '.join(map(str, args))}", flush=True)
+    r = subprocess.run([str(EXE), "-scene", str(scene_dir / "scene.txt"), "-out", str(out), *map(str, args)],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def library_frames(scene_dir, frames, spp):
+    """The same plain-tracer frames through the Python bindings."""
+    from rtcamp8_amd import _abi, image, nvdb
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
+    from rtcamp8_amd.sky import HosekSky
+
+    w, h = 96, 64
+    grid = nvdb.load_grid(scene_dir / "cloud.nvdb")
+    p = _abi.FrameParams()
+    p.image_width, p.image_height = w, h
+    p.grid = grid
+    p.density_coeff, p.scattering_albedo, p.scattering_forwardness = 0.3, 0.99, 0.0
+    p.phase = _abi.PHASE_SCHLICK
+    mn = np.array(grid.world_bbox_min[:], np.float32)
+    mx = np.array(grid.world_bbox_max[:], np.float32)
+    center = (mn + mx) * np.float32(0.5)
+    dd = mx - center
+    r2 = np.float32(0)
+    for a in range(3):
+        r2 = np.float32(r2 + dd[a] * dd[a])
+    p.world_aabb_min[:], p.world_aabb_max[:] = mn.tolist(), mx.tolist()
+    p.world_radius = float(np.sqrt(r2))
+    p.rr_after_path_length, p.max_path_length = 0, 0
+    p.tracking = _abi.TRACKING_GLOBAL_MAJORANT
+    rng, accum, px = DeviceBuffer(8 * w * h), DeviceBuffer(24 * w * h), DeviceBuffer(4 * w * h)
+    tex = DeviceBuffer(16 * 3200 * 1600)
+    call("rtc8x_seed_rng", C.c_void_p(rng.ptr), w, h, C.c_uint64(591842031321323413))
+    p.rng_states, p.accum = rng.ptr, accum.ptr
+    out = []
+    try:
+        for f in frames:
+            pos = (C.c_float * 3)(*f["position"])
+            q = (C.c_float * 4)(*f["orientation"])
+            call("rtc8x_camera_setup", C.byref(p.camera), pos, q, C.c_float(f["fov_y"]), C.c_float(np.float32(w) / np.float32(h)))
+            sky = HosekSky(min(max(f["solar_elevation"], 0.0), 1.5707963), f["turbidity"])
+            sd = (C.c_float * 3)(*f["sun"])
+            call("rtc8x_sky_bake", None, C.byref(sky.struct), sd, 3200, 1600, C.c_void_p(tex.ptr))
+            env = _abi.EnvLight()
+            env.kind, env.sampler = _abi.ENV_LATLONG, _abi.ENV_SAMPLER_HIERARCHICAL
+            env.width, env.height, env.texels = 3200, 1600, tex.ptr
+            call("rtc8x_env_build_sampler", C.byref(env), None)
+            env.power_coeff, env.rotation = f["env_coeff"], f["env_rotation"]
+            p.env = env
+            accum.fill(0)
+            call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE, spp)
+            image.tone_map(accum.ptr, w, h, spp, None, px.ptr, image.sdr_config())
+            out.append(px.download(np.uint32, w * h).reshape(h, w))
+            call("rtc8x_env_release", C.byref(env))
+            sky.close()
+    finally:
+        for b in (rng, accum, px, tex):
+            b.free()
+        call("rtc8x_grid_destroy", C.byref(grid))
+    return out
+
+
+def test_plain_frames_match_the_library(scene_dir, tmp_path):
+    from test_image import decode_png
+
+    render(scene_dir, tmp_path, "-no-nrc", "-spp", 4, "-max-frames", 2)
+    d = json.loads(subprocess.run([str(EXE), "-scene", str(scene_dir / "scene.txt"), "-dump", "-max-frames", "2"],
+                                  capture_output=True, text=True, timeout=60, check=True).stdout)
+    want = library_frames(scene_dir, d["frames"], 4)
+    for k in range(2):
+        got = decode_png((tmp_path / f"{k:03d}.png").read_bytes())
+        assert np.array_equal(got, want[k]), k
+        lum = (got & 255).astype(np.float64)
+        assert 5 < lum.mean() < 250 and lum.std() > 1  # a lit cloud, not a flat image
+
+
+def test_nrc_frames(scene_dir, tmp_path):
+    from test_image import decode_png
+
+    r = render(scene_dir, tmp_path / "nrc", "-spp", 2, "-max-frames", 3)
+    assert r.stdout.count("Frame ") == 3
+    render(scene_dir, tmp_path / "plain", "-no-nrc", "-spp", 16, "-max-frames", 1)
+    a = decode_png((tmp_path / "plain" / "000.png").read_bytes()).view(np.uint8).reshape(64, 96, 4)[..., :3]
+    for k in range(3):
+        img = decode_png((tmp_path / "nrc" / f"{k:03d}.png").read_bytes()).view(np.uint8).reshape(64, 96, 4)
+        assert np.all(img[..., 3] == 255)
+    b = decode_png((tmp_path / "nrc" / "000.png").read_bytes()).view(np.uint8).reshape(64, 96, 4)[..., :3]
+    # the first NRC frame renders the same scene as the plain tracer: mean brightness within 25 %
+    assert abs(b.astype(np.float64).mean() / a.astype(np.float64).mean() - 1) < 0.25

@@ -1,0 +1,168 @@
+"""The batch renderer's scene files and keyframes, without a GPU (rtc8x_render -dump).
+
+The reference's own scene scripts (tests/golden/scenes/reference_test{0,1}.txt, copies of rtcamp8's
+data/test{0,1}.txt) are parsed with the same outcome as the reference: their mesh paths are the author's
+Windows files, so loading fails on the mesh line with the reference's message; with those paths pointing
+at files that exist they parse, and their keyframes evaluate as the reference's interpolation defines
+(keys reproduced exactly at their times, linear field of view, solar direction rotating at a constant
+rate about one axis)."""
+import json
+import math
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+EXE = ROOT / "rtcamp8_amd" / "rtc8x_render"
+SCENES = Path(__file__).resolve().parent / "golden" / "scenes"
+
+pytestmark = pytest.mark.skipif(not EXE.exists(), reason="rtc8x_render not built")
+
+
+def run(*args, cwd=None):
+    return subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=60)
+
+
+def dump(scene: Path, *extra):
+    r = run("-scene", scene, "-dump", *extra)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def polar_y_up(phi_deg, theta_deg):
+    p, t = math.radians(phi_deg), math.radians(theta_deg)
+    return np.array([-math.sin(p) * math.sin(t), math.cos(t), math.cos(p) * math.sin(t)])
+
+
+def rotate(q, v):
+    """Rotation of v by the unit quaternion q = (x, y, z, w)."""
+    x, y, z, w = q
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (z * x + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (z * x - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    return r @ np.asarray(v, dtype=np.float64)
+
+
+def with_local_meshes(src: Path, dst_dir: Path) -> Path:
+    """A copy of a reference scene whose mesh paths name a file that exists."""
+    (dst_dir / "mesh.obj").write_text("# placeholder\n")
+    text = re.sub(r'^(mesh\s+)"[^"]*"', r'\1"mesh.obj"', src.read_text(), flags=re.M)
+    out = dst_dir / src.name
+    out.write_text(text)
+    return out
+
+
+@pytest.mark.parametrize("name,line", [("reference_test0.txt", 22), ("reference_test1.txt", 4)])
+def test_reference_scenes_fail_on_their_missing_meshes(name, line):
+    r = run("-scene", SCENES / name, "-dump")
+    assert r.returncode == 1
+    assert re.search(rf"l\.{line}: Mesh .*(Cornell_box_RB1|sponza)\.obj does not exist\.", r.stderr), r.stderr
+
+
+def test_reference_test0_sky_keyframes(tmp_path):
+    scene = with_local_meshes(SCENES / "reference_test0.txt", tmp_path)
+    r = run("-scene", scene, "-dump")  # the Cornell box instance is a surface
+    assert r.returncode == 1 and "surface meshes" in r.stderr
+    d = dump(scene, "-ignore-surfaces")
+    assert d["image"] == [1920, 1080] and d["time"] == [0, 10] and d["fps"] == 60
+    assert d["environment"] == "analytic_sky" and d["instances"] == 1
+    frames = d["frames"]
+    assert len(frames) == 601  # t = 0, 1/60, ..., 10
+    keys = {0: polar_y_up(85, 85), 300: polar_y_up(0, 25), 600: polar_y_up(-85, 85)}
+    for i, want in keys.items():
+        assert np.allclose(frames[i]["sun"], want, atol=2e-6), i
+    for f in frames:
+        # (1 - t) a + t b of equal keys is a within float rounding
+        assert abs(np.linalg.norm(f["sun"]) - 1) < 1e-5 and abs(f["env_coeff"] - 5) < 1e-6
+        assert abs(f["turbidity"] - 4) < 1e-6
+        assert abs(f["solar_elevation"] - (math.pi / 2 - math.acos(f["sun"][1]))) < 1e-5
+    # between two keys the sun turns at a constant rate about one axis
+    s0, s1 = np.array(frames[0]["sun"]), np.array(frames[300]["sun"])
+    total = math.acos(np.clip(s0 @ s1, -1, 1))
+    for i in (75, 150, 225):
+        si = np.array(frames[i]["sun"])
+        assert abs(math.acos(np.clip(s0 @ si, -1, 1)) - total * i / 300) < 2e-5
+    # the camera: one key at t = 0 (lookat 0 0 3.5 -> origin, fovy 50)
+    for f in (frames[0], frames[-1]):
+        assert np.allclose(f["position"], [0, 0, 3.5]) and abs(f["fov_y"] - math.radians(50)) < 1e-6
+        assert np.allclose(rotate(f["orientation"], [0, 0, -1]), [0, 0, -1], atol=1e-6)
+
+
+def test_reference_test1_camera_keyframes(tmp_path):
+    d = dump(with_local_meshes(SCENES / "reference_test1.txt", tmp_path), "-ignore-surfaces")
+    frames = d["frames"]
+    assert d["environment"] == "none" and d["instances"] == 2 and len(frames) == 601
+    keys = {0: ((0, 10, 0), (1, 3, 0), 70), 120: ((0, 3, 0), (1, 3, 0), 50), 180: ((0, 3, 0), (1, 3, 0), 20),
+            300: ((0, 3, 0), (0, 3, 1), 50), 600: ((0, 3, 0), (-1, 3, 0), 50)}
+    for i, (pos, at, fov) in keys.items():
+        f = frames[i]
+        assert np.allclose(f["position"], pos, atol=1e-5), i
+        assert abs(f["fov_y"] - math.radians(fov)) < 1e-6, i
+        look = np.subtract(at, pos, dtype=np.float64)
+        assert np.allclose(rotate(f["orientation"], [0, 0, -1]), look / np.linalg.norm(look), atol=2e-6), i
+        assert abs(np.linalg.norm(f["orientation"]) - 1) < 1e-6
+    # field of view is linear between keys (t = 2 .. 3 s: 50 -> 20 degrees)
+    for i in range(120, 181, 15):
+        assert abs(frames[i]["fov_y"] - math.radians(50 + (20 - 50) * (i - 120) / 60)) < 2e-6
+    # from t = 3 s every segment's four control points lie on y = 3, and so does the Catmull-Rom curve;
+    # on 2 .. 3 s the tangent at t = 2 is (3 - 10) / 2 per key interval, which gives the cubic
+    # 3 - 3.5 u + 7 u^2 - 3.5 u^3 in y: 2.5625 at the middle
+    assert all(abs(f["position"][1] - 3) < 1e-5 for f in frames[180:])
+    assert abs(frames[150]["position"][1] - 2.5625) < 1e-5
+
+
+def test_active_camera_switching_and_playback_speed(tmp_path):
+    (tmp_path / "v.nvdb").write_bytes(b"")
+    scene = tmp_path / "s.txt"
+    scene.write_text("""image 64 32
+time 0 1 10
+volume "v.nvdb"
+camera b
+lookat b 0 0 5 0 0 0 0 1 0
+cam-addkey b 0 false false
+camera a
+lookat a 5 0 0 0 0 0 0 1 0
+cam-addkey a 0 false false
+active-cam b 0.5
+active-cam a 0
+""")
+    d = dump(scene)
+    names = [f["camera"] for f in d["frames"]]
+    assert names == ["a"] * 5 + ["b"] * 6
+    d2 = dump(scene, "-t-mul", "0.5")
+    assert [f["time"] for f in d2["frames"]] == pytest.approx([0.05 * i for i in range(21)])
+    assert len(dump(scene, "-max-frames", "3")["frames"]) == 3
+    # without active-cam the first camera by name is active
+    scene.write_text(scene.read_text().replace("active-cam b 0.5\nactive-cam a 0\n", ""))
+    assert {f["camera"] for f in dump(scene)["frames"]} == {"a"}
+
+
+@pytest.mark.parametrize("body,message", [
+    ("image 64 32\nfoo 1 2\n", r'l\.2: Unknown token found: "foo"'),
+    ("image 64\n", r'l\.1: failed to parse "image" command: image 64'),
+    ("image 0 32\n", r"l\.1: Invalid image size: 0 x 32"),
+    ("time 2 1 30\n", r"l\.1: Invalid timeBegin/End"),
+    ("fovy cam 40\n", r"l\.1: Camera cam does not exist\."),
+    ("camera c\ncamera c\n", r"l\.2: Camera c has been already created\."),
+    ("camera c\nfovy c -3\n", r"l\.2: Invalid fovY"),
+    ("skyenv sky\nenv-coeff other 1\n", r"l\.2: Environment other does not exist\."),
+    ("skyenv sky\nenv-coeff sky -1\n", r"l\.2: Invalid coeff"),
+    ("volume \"missing.nvdb\"\n", r"l\.1: Volume .*missing\.nvdb does not exist\."),
+    ("// comment\n\n   \nimage 1 1 extra\n", r'l\.4: failed to parse "image" command'),
+])
+def test_parse_errors_name_their_line(tmp_path, body, message):
+    scene = tmp_path / "s.txt"
+    scene.write_text(body)
+    r = run("-scene", scene, "-dump")
+    assert r.returncode == 1 and re.search(message, r.stderr), r.stderr
+
+
+def test_scene_without_camera_or_size_is_rejected(tmp_path):
+    scene = tmp_path / "s.txt"
+    scene.write_text("time 0 1 10\n")
+    r = run("-scene", scene, "-dump")
+    assert r.returncode == 1 and "image size" in r.stderr
+    assert run("-dump").returncode == 2  # -scene is required

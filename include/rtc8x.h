@@ -116,6 +116,8 @@ typedef struct rtc8x_grid {
     const rtc8x_root_tile* root_tiles;   /* num_root_tiles */
     const uint32_t* upper_tables;        /* num_upper * 32768 entries */
     const uint32_t* lower_tables;        /* num_lower * 4096 entries */
+    /* Device grids (from rtc8x_grid_build_fbm / rtc8x_grid_upload*) keep one float of padding after
+     * leaf_values and after tile_values: the tracer reads voxel pairs (z, z+1) without a bounds test. */
     const float* leaf_values;            /* num_leaves * 512 voxel values */
     const float* tile_values;            /* tile_values[0] == background */
     const int32_t* leaf_origins;         /* num_leaves * 3 index-space origins (optional, may be NULL) */
@@ -455,16 +457,22 @@ typedef struct rtc8x_nrc_buffers {
     uint32_t offset_to_select_unbiased_path; /* PerFrame::offsetToSelectUnbiasedPath */
     float radiance_scale;                    /* PerFrame::radianceScale */
     float path_termination_factor;           /* renderer_shared.h:11 (0.01) */
-    uint32_t reserved;
+    /* Samples per PATH_TRACE_WITH_NRC launch (0 counts as 1; the launch's num_samples must equal it):
+     * the render buffers below hold that many image slices.  Sample k of pixel i uses
+     * inference_queries / inferred_radiance entry k * round_up(W*H, 128) + i and inference_terminals /
+     * per_frame_contribution entry k * W*H + i, so one inference covers every sample of a launch;
+     * rtc8x_nrc_accumulate adds the slices in sample order, which gives the sums of as many one-sample
+     * launches. */
+    uint32_t num_samples;
     uint32_t* num_training_data;             /* [2] */
     int32_t* train_image_size;               /* [2][2] */
     float* target_min_max;                   /* [2][2][3]: per buffer index, min RGB then max RGB */
     float* target_avg;                       /* [2][3] */
-    rtc8x_radiance_query* inference_queries; /* image size (rounded up to 128) */
-    rtc8x_terminal_info* inference_terminals;/* image size */
+    rtc8x_radiance_query* inference_queries; /* image size (rounded up to 128) x num_samples */
+    rtc8x_terminal_info* inference_terminals;/* image size x num_samples */
     rtc8x_training_suffix_terminal* train_suffix_terminals; /* max_num_training_suffixes */
-    float* inferred_radiance;                /* image size (rounded up to 128) x 3 */
-    float* per_frame_contribution;           /* image size x 3 */
+    float* inferred_radiance;                /* image size (rounded up to 128) x 3 x num_samples */
+    float* per_frame_contribution;           /* image size x 3 x num_samples */
     rtc8x_radiance_query* train_queries[2];  /* RTC8X_TRAIN_BUFFER_SIZE each */
     float* train_targets[2];                 /* RTC8X_TRAIN_BUFFER_SIZE x 3 each */
     rtc8x_training_vertex_info* train_vertex_info; /* RTC8X_TRAIN_BUFFER_SIZE */

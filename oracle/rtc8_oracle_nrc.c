@@ -170,29 +170,34 @@ void oracle_nrc_shuffle(const rtc8x_frame_params* p)
     }
 }
 
-/* accumulateInferredRadianceValues (nrc_set_up_kernels.cu:176-219), FP64 accumulation. */
+/* accumulateInferredRadianceValues (nrc_set_up_kernels.cu:176-219), FP64 accumulation, once per sample
+ * slice of the launch (nrc.num_samples), in sample order. */
 void oracle_nrc_accumulate(const rtc8x_frame_params* p)
 {
     const rtc8x_nrc_buffers* nb = &p->nrc;
     uint32_t n = p->image_width * p->image_height;
-    for (uint32_t i = 0; i < n; ++i) {
+    uint32_t samples = nb->num_samples ? nb->num_samples : 1u;
+    size_t q_stride = ((size_t)n + 127u) & ~(size_t)127u;
+    for (uint32_t s = 0; s < samples; ++s)
+    for (uint32_t px = 0; px < n; ++px) {
+        size_t i = (size_t)s * n + px, qi = (size_t)s * q_stride + px;
         const rtc8x_terminal_info* ti = &nb->inference_terminals[i];
         float direct[3], rad[3] = {0.0f, 0.0f, 0.0f};
-        rgb_load(nb->per_frame_contribution + 3 * (size_t)i, direct);
+        rgb_load(nb->per_frame_contribution + 3 * i, direct);
         if (ti->has_query & 1u) {
-            rgb_load(nb->inferred_radiance + 3 * (size_t)i, rad);
+            rgb_load(nb->inferred_radiance + 3 * qi, rad);
             for (int k = 0; k < 3; ++k) rad[k] = fmaxf(rad[k], 0.0f);
             if (nb->radiance_scale > 0.0f) {
                 float r = 1.0f / nb->radiance_scale;
                 for (int k = 0; k < 3; ++k) rad[k] = rad[k] * r;
             }
-            const rtc8x_radiance_query* q = &nb->inference_queries[i];
+            const rtc8x_radiance_query* q = &nb->inference_queries[qi];
             for (int k = 0; k < 3; ++k) rad[k] = rad[k] * (q->diffuse_reflectance[k] + q->specular_reflectance[k]);
         }
         for (int k = 0; k < 3; ++k) {
             float indirect = ti->throughput[k] * rad[k];
             float c = direct[k] + indirect;
-            p->accum[3 * (size_t)i + k] += (double)c;
+            p->accum[3 * (size_t)px + k] += (double)c;
         }
     }
 }

@@ -220,7 +220,7 @@ class NrcBuffers(C.Structure):
         ("offset_to_select_unbiased_path", C.c_uint32),
         ("radiance_scale", C.c_float),
         ("path_termination_factor", C.c_float),
-        ("reserved", C.c_uint32),
+        ("num_samples", C.c_uint32),
         ("num_training_data", C.c_void_p),
         ("train_image_size", C.c_void_p),
         ("target_min_max", C.c_void_p),

@@ -12,6 +12,7 @@
 // the plain unbiased path tracer instead.  -dump parses the scene and prints the evaluated cameras and
 // environments per frame as JSON without touching the GPU.  The medium is the reference's batch setting
 // (density coefficient 0.3, albedo 0.99, isotropic Schlick phase, renderer_main.cpp:1463-1467).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -144,18 +145,19 @@ struct NrcBuffers {
         inference_terminals, train_suffix_terminals, inferred_radiance, per_frame_contribution, train_queries[2],
         train_targets[2], train_vertex_info, data_shufflers, scratch;
 
-    NrcBuffers(uint32_t w, uint32_t h)
+    // `samples`: image slices of the render buffers, the samples one render launch traces
+    NrcBuffers(uint32_t w, uint32_t h, uint32_t samples)
     {
-        const size_t n = (size_t)w * h, ninf = round_up(n, 128);
+        const size_t n = (size_t)w * h, ninf = round_up(n, 128) * samples;
         num_training_data = DevBuf(8);
         train_image_size = DevBuf(16);
         target_min_max = DevBuf(48);
         target_avg = DevBuf(24);
         inference_queries = DevBuf(sizeof(rtc8x_radiance_query) * ninf);
-        inference_terminals = DevBuf(sizeof(rtc8x_terminal_info) * n);
+        inference_terminals = DevBuf(sizeof(rtc8x_terminal_info) * n * samples);
         train_suffix_terminals = DevBuf(sizeof(rtc8x_training_suffix_terminal) * n);
         inferred_radiance = DevBuf(12 * ninf);
-        per_frame_contribution = DevBuf(12 * n);
+        per_frame_contribution = DevBuf(12 * n * samples);
         for (int i = 0; i < 2; ++i) {
             train_queries[i] = DevBuf(sizeof(rtc8x_radiance_query) * RTC8X_TRAIN_BUFFER_SIZE);
             train_targets[i] = DevBuf(12 * (size_t)RTC8X_TRAIN_BUFFER_SIZE);
@@ -353,7 +355,8 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
         sky_tex = DevBuf(16 * (size_t)kSkyWidth * kSkyHeight);
     }
 
-    // neural radiance cache
+    // neural radiance cache; a frame's samples go through the cache in launches of up to 16
+    const uint32_t samples_per_launch = std::min<uint32_t>(o.spp, 16);
     rtc8x_nrc* cache = nullptr;
     std::unique_ptr<NrcBuffers> nb;
     if (o.nrc) {
@@ -370,7 +373,7 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
         d.seed = 0x9E3779B97F4A7C15ull;
         d.encoding = o.encoding;
         check(rtc8x_nrc_create(&d, &cache), "nrc create");
-        nb = std::make_unique<NrcBuffers>(W, H);
+        nb = std::make_unique<NrcBuffers>(W, H, samples_per_launch);
         nb->fill(p.nrc, W, H);
     }
     std::mt19937 frame_rng(72139121u);
@@ -445,6 +448,7 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
                 p.nrc.offset_to_select_unbiased_path = frame_rng();
                 p.work_order = nullptr;
                 p.pixel_cost = nullptr;
+                p.nrc.num_samples = 1;
                 check(rtc8x_nrc_prepare(stream, &p, frame == 0 ? 1 : 0), "prepare");
                 check(rtc8x_trace(stream, &p, RTC8X_ENTRY_GENERATE_TRAINING_DATA, 1), "training trace");
                 int32_t tsz[4];
@@ -461,7 +465,9 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
                                           nb->train_targets[1].as<float>() + 3 * (size_t)step * batch, batch, nullptr),
                           "train");
             }
-            for (uint32_t i = 0; i < o.spp; ++i) {
+            for (uint32_t done = 0; done < o.spp;) {
+                const uint32_t k = std::min(samples_per_launch, o.spp - done);
+                p.nrc.num_samples = k;
                 if (have_cost) {
                     check(rtc8x_order_by_cost(stream, cost.as<uint32_t>(), (uint32_t)npx, order.as<uint32_t>(),
                                               order_scratch.ptr), "order");
@@ -471,10 +477,11 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
                 }
                 p.pixel_cost = cost.as<uint32_t>();
                 have_cost = true;
-                check(rtc8x_trace(stream, &p, RTC8X_ENTRY_PATH_TRACE_WITH_NRC, 1), "trace with NRC");
-                check(rtc8x_nrc_infer(cache, stream, (const float*)nb->inference_queries.ptr, (uint32_t)round_up(npx, 128),
-                                      nb->inferred_radiance.as<float>()), "infer");
+                check(rtc8x_trace(stream, &p, RTC8X_ENTRY_PATH_TRACE_WITH_NRC, k), "trace with NRC");
+                check(rtc8x_nrc_infer(cache, stream, (const float*)nb->inference_queries.ptr,
+                                      (uint32_t)(round_up(npx, 128) * k), nb->inferred_radiance.as<float>()), "infer");
                 check(rtc8x_nrc_accumulate(stream, &p), "accumulate");
+                done += k;
             }
         } else {
             if (have_cost) {

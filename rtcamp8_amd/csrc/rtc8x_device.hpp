@@ -152,94 +152,106 @@ struct F2 {
     float x, y;
 };
 
-// Voxels (i,j,k) and (i,j,k+1) of one block (k & 7 != 7): one 8-byte load from a leaf (voxel order
-// puts k innermost), or the tile value twice.
-__device__ __forceinline__ F2 tap_pair(const rtc8x_grid& g, uint32_t e, uint32_t off)
+// SampleFromVoxels<Acc,1,false>::operator() (util/SampleFromVoxels.h:419-460) with Floor (:58-65), for
+// S index-space positions at once.  All loads are issued in two rounds: first every position's leaf-map
+// entries (the stencil's block, and the other seven only for a stencil that crosses a block face), then
+// every position's voxels, so the lookups of a batch overlap instead of each waiting on the one before.
+// Positions outside the map take the tree walk.  The lerps are the reference's, in its order.
+template <uint32_t S>
+__device__ __forceinline__ void trilinear_batch(const rtc8x_grid& g, const V3 (&idx)[S], float (&out)[S])
 {
-    if (e & RTC8X_TILE_BIT) {
-        const float v = g.tile_values[e & ~RTC8X_TILE_BIT];
-        return F2{v, v};
+    const int32_t ob0 = g.leaf_map_origin[0] >> 3, ob1 = g.leaf_map_origin[1] >> 3, ob2 = g.leaf_map_origin[2] >> 3;
+    const int32_t d0 = g.leaf_map_dims[0], d1 = g.leaf_map_dims[1], d2 = g.leaf_map_dims[2];
+    int32_t i[S], j[S], k[S];
+    float x[S], y[S], z[S];
+    bool cross[S];
+    uint32_t e[S][8];  // taps 000, 001, 010, 011, 100, 101, 110, 111 (x y z)
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) {
+        const float fx = floorf(idx[s].x), fy = floorf(idx[s].y), fz = floorf(idx[s].z);
+        x[s] = idx[s].x - fx;
+        y[s] = idx[s].y - fy;
+        z[s] = idx[s].z - fz;
+        i[s] = (int32_t)fx;
+        j[s] = (int32_t)fy;
+        k[s] = (int32_t)fz;
+        const int32_t bx0 = (i[s] >> 3) - ob0, bx1 = ((i[s] + 1) >> 3) - ob0;
+        const int32_t by0 = (j[s] >> 3) - ob1, by1 = ((j[s] + 1) >> 3) - ob1;
+        const int32_t bz0 = (k[s] >> 3) - ob2, bz1 = ((k[s] + 1) >> 3) - ob2;
+        const bool cx = (i[s] & 7) == 7, cy = (j[s] & 7) == 7, cz = (k[s] & 7) == 7;
+        cross[s] = cx || cy || cz;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) e[s][t] = 0u;
+        if (g.leaf_map && bx0 >= 0 && by0 >= 0 && bz0 >= 0 && bx1 < d0 && by1 < d1 && bz1 < d2) {
+            const uint32_t* m = g.leaf_map + ((size_t)bx0 * (uint32_t)d1 + (uint32_t)by0) * (uint32_t)d2 + (uint32_t)bz0;
+            e[s][0] = m[0];
+            if (cross[s]) {
+                const size_t sz = cz ? 1u : 0u, sy = cy ? (size_t)(uint32_t)d2 : 0u,
+                             sx = cx ? (size_t)(uint32_t)d1 * (uint32_t)d2 : 0u;
+                e[s][1] = m[sz];
+                e[s][2] = m[sy];
+                e[s][3] = m[sy + sz];
+                e[s][4] = m[sx];
+                e[s][5] = m[sx + sz];
+                e[s][6] = m[sx + sy];
+                e[s][7] = m[sx + sy + sz];
+            }
+        } else {
+            cross[s] = true;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) e[s][t] = find_leaf(g, i[s] + (t >> 2), j[s] + ((t >> 1) & 1), k[s] + (t & 1));
+        }
     }
-    F2 r;
-    __builtin_memcpy(&r, g.leaf_values + (size_t)e * RTC8X_LEAF_SIZE + off, sizeof(r));
-    return r;
+    // Voxel pairs (z, z+1) of the four (x, y) columns: one 8-byte load per column from the column's
+    // lower block (the grid keeps one float of padding after the leaf and tile values, so the second
+    // word is always readable); a stencil crossing a z face also loads its upper voxels one by one.
+    F2 pr[S][4];
+    float up[S][4];
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) {
+        const bool cz = (k[s] & 7) == 7;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int32_t ii = i[s] + (c >> 1), jj = j[s] + (c & 1);
+            const uint32_t lo = cross[s] ? e[s][2 * c] : e[s][0];
+            const float* pl = (lo & RTC8X_TILE_BIT) ? g.tile_values + (lo & ~RTC8X_TILE_BIT)
+                                                    : g.leaf_values + (size_t)lo * RTC8X_LEAF_SIZE + leaf_offset(ii, jj, k[s]);
+            __builtin_memcpy(&pr[s][c], pl, sizeof(F2));
+            up[s][c] = 0.0f;
+            if (cz) {
+                const uint32_t hi = e[s][2 * c + 1];
+                up[s][c] = *((hi & RTC8X_TILE_BIT) ? g.tile_values + (hi & ~RTC8X_TILE_BIT)
+                                                   : g.leaf_values + (size_t)hi * RTC8X_LEAF_SIZE + leaf_offset(ii, jj, k[s] + 1));
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) {
+        const bool cz = (k[s] & 7) == 7;
+        float v[8];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t lo = cross[s] ? e[s][2 * c] : e[s][0];
+            v[2 * c] = pr[s][c].x;
+            v[2 * c + 1] = cz ? up[s][c] : ((lo & RTC8X_TILE_BIT) ? pr[s][c].x : pr[s][c].y);
+        }
+        const float zz = z[s], yy = y[s];
+        float vy = v[0] + zz * (v[1] - v[0]);
+        float vy1 = v[2] + zz * (v[3] - v[2]);
+        const float vx = vy + yy * (vy1 - vy);
+        vy1 = v[6] + zz * (v[7] - v[6]);
+        vy = v[4] + zz * (v[5] - v[4]);
+        const float vx1 = vy + yy * (vy1 - vy);
+        out[s] = vx + x[s] * (vx1 - vx);
+    }
 }
 
-// SampleFromVoxels<Acc,1,false>::operator() (util/SampleFromVoxels.h:419-460) with Floor (:58-65).
-// Fetches in two dependent rounds: the leaf-map entry of the stencil's block (plus the other blocks
-// only for the lanes whose stencil crosses a block face), then the voxels -- four 8-byte pair loads
-// when the stencil does not cross a z face of its block, else eight single loads.  The lerps are the
-// reference's, in its order.
 __device__ __forceinline__ float trilinear(const rtc8x_grid& g, V3 idx)
 {
-    float fx = floorf(idx.x), fy = floorf(idx.y), fz = floorf(idx.z);
-    float x = idx.x - fx, y = idx.y - fy, z = idx.z - fz;
-    int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
-    const int32_t ob0 = g.leaf_map_origin[0] >> 3, ob1 = g.leaf_map_origin[1] >> 3, ob2 = g.leaf_map_origin[2] >> 3;
-    const int32_t bx0 = (i >> 3) - ob0, bx1 = ((i + 1) >> 3) - ob0;
-    const int32_t by0 = (j >> 3) - ob1, by1 = ((j + 1) >> 3) - ob1;
-    const int32_t bz0 = (k >> 3) - ob2, bz1 = ((k + 1) >> 3) - ob2;
-    const bool cross_x = (i & 7) == 7, cross_y = (j & 7) == 7, cross_z = (k & 7) == 7;
-    uint32_t e000, e001, e010, e011, e100, e101, e110, e111;
-    if (g.leaf_map && bx0 >= 0 && by0 >= 0 && bz0 >= 0 && bx1 < g.leaf_map_dims[0] && by1 < g.leaf_map_dims[1] &&
-        bz1 < g.leaf_map_dims[2]) {
-        const size_t d1 = (uint32_t)g.leaf_map_dims[1], d2 = (uint32_t)g.leaf_map_dims[2];
-        e000 = g.leaf_map[((size_t)bx0 * d1 + by0) * d2 + bz0];
-        e001 = e010 = e011 = e100 = e101 = e110 = e111 = e000;
-        if (cross_x || cross_y || cross_z) {
-            const uint32_t* m00 = g.leaf_map + ((size_t)bx0 * d1 + by0) * d2;
-            const uint32_t* m01 = g.leaf_map + ((size_t)bx0 * d1 + by1) * d2;
-            const uint32_t* m10 = g.leaf_map + ((size_t)bx1 * d1 + by0) * d2;
-            const uint32_t* m11 = g.leaf_map + ((size_t)bx1 * d1 + by1) * d2;
-            e001 = m00[bz1];
-            e010 = m01[bz0];
-            e011 = m01[bz1];
-            e100 = m10[bz0];
-            e101 = m10[bz1];
-            e110 = m11[bz0];
-            e111 = m11[bz1];
-        }
-    } else {
-        e000 = find_leaf(g, i, j, k);
-        e001 = find_leaf(g, i, j, k + 1);
-        e010 = find_leaf(g, i, j + 1, k);
-        e011 = find_leaf(g, i, j + 1, k + 1);
-        e100 = find_leaf(g, i + 1, j, k);
-        e101 = find_leaf(g, i + 1, j, k + 1);
-        e110 = find_leaf(g, i + 1, j + 1, k);
-        e111 = find_leaf(g, i + 1, j + 1, k + 1);
-    }
-    float v000, v001, v010, v011, v100, v101, v110, v111;
-    if (!cross_z) {
-        const F2 p00 = tap_pair(g, e000, leaf_offset(i, j, k));
-        const F2 p01 = tap_pair(g, e010, leaf_offset(i, j + 1, k));
-        const F2 p10 = tap_pair(g, e100, leaf_offset(i + 1, j, k));
-        const F2 p11 = tap_pair(g, e110, leaf_offset(i + 1, j + 1, k));
-        v000 = p00.x;
-        v001 = p00.y;
-        v010 = p01.x;
-        v011 = p01.y;
-        v100 = p10.x;
-        v101 = p10.y;
-        v110 = p11.x;
-        v111 = p11.y;
-    } else {
-        v000 = *voxel_ptr(g, e000, i, j, k);
-        v001 = *voxel_ptr(g, e001, i, j, k + 1);
-        v011 = *voxel_ptr(g, e011, i, j + 1, k + 1);
-        v010 = *voxel_ptr(g, e010, i, j + 1, k);
-        v110 = *voxel_ptr(g, e110, i + 1, j + 1, k);
-        v111 = *voxel_ptr(g, e111, i + 1, j + 1, k + 1);
-        v101 = *voxel_ptr(g, e101, i + 1, j, k + 1);
-        v100 = *voxel_ptr(g, e100, i + 1, j, k);
-    }
-    float vy = v000 + z * (v001 - v000);
-    float vy1 = v010 + z * (v011 - v010);
-    float vx = vy + y * (vy1 - vy);
-    vy1 = v110 + z * (v111 - v110);
-    vy = v100 + z * (v101 - v100);
-    float vx1 = vy + y * (vy1 - vy);
-    return vx + x * (vx1 - vx);
+    const V3 in[1] = {idx};
+    float out[1];
+    trilinear_batch<1>(g, in, out);
+    return out[0];
 }
 
 struct Medium {
@@ -253,6 +265,19 @@ __device__ __forceinline__ float density_at(const rtc8x_grid& g, float coeff, V3
 {
     V3 idx = world_to_index(g, fmaf(t, d.x, o.x), fmaf(t, d.y, o.y), fmaf(t, d.z, o.z));
     return coeff * trilinear(g, idx);
+}
+
+// Density coefficient times the grid value at o + t[s] d for S distances, looked up together.
+template <uint32_t S>
+__device__ __forceinline__ void density_batch(const rtc8x_grid& g, float coeff, V3 o, V3 d, const float (&t)[S],
+                                              float (&sig)[S])
+{
+    V3 idx[S];
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) idx[s] = world_to_index(g, fmaf(t[s], d.x, o.x), fmaf(t[s], d.y, o.y), fmaf(t[s], d.z, o.z));
+    trilinear_batch<S>(g, idx, sig);
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) sig[s] = coeff * sig[s];
 }
 
 __device__ __forceinline__ float sample_density(const rtc8x_grid& g, Medium& m, V3 o, V3 d, float t)

@@ -6,7 +6,8 @@
 //   root_tiles      num_root_tiles x 16 B
 //   upper_tables    num_upper x 32768 x u32   (lower-node index or tile)
 //   lower_tables    num_lower x 4096 x u32    (leaf index or tile)
-//   leaf_values     num_leaves x 512 x f32    (2 KiB per leaf, NanoVDB voxel order x-major)
+//   leaf_values     num_leaves x 512 x f32    (2 KiB per leaf, NanoVDB voxel order x-major), + 1 f32 padding
+//   tile_values     one f32 per tile value, + 1 f32 padding (lookups read voxel pairs z, z+1 unconditionally)
 //   leaf_origins    num_leaves x 3 x i32
 //   cell_majorants  one f32 per 8^3 cell of the index bbox
 //   leaf_map        one u32 per 8^3 block of the index bbox plus a one-block margin (the tree walk's result)
@@ -357,7 +358,7 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
     tile.entry = 0;
     tile.state = 0;
     std::vector<rtc8x_root_tile> root(1, tile);
-    std::vector<float> tile_values(1, 0.0f);
+    std::vector<float> tile_values(2, 0.0f);  // the background tile and the lookup's padding float
     std::vector<float> cm;
     int32_t corg[3], cdims[3];
     cell_majorants(origins, leaf_max, bbmin, bbmax, corg, cdims, cm);
@@ -397,7 +398,8 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
     g.tile_values = d_tiles;
     RTC8X_TRY(upload_vec(cm, &d_cm));
     g.cell_majorants = d_cm;
-    RTC8X_HIP(hipMalloc(&d_values, (size_t)num_leaves * 512 * sizeof(float)));
+    RTC8X_HIP(hipMalloc(&d_values, ((size_t)num_leaves * 512 + 1) * sizeof(float)));
+    RTC8X_HIP(hipMemsetAsync(d_values + (size_t)num_leaves * 512, 0, sizeof(float), s));  // lookup padding
     g.leaf_values = d_values;
     hipLaunchKernelGGL(dev::fill_leaves_kernel, dim3(num_leaves), dim3(512), 0, s, *d, d_origins, sub, inv_range, d_values);
     RTC8X_HIP(hipGetLastError());
@@ -486,6 +488,8 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
     std::vector<uint32_t> lower(h->lower_tables, h->lower_tables + (size_t)h->num_lower * RTC8X_LOWER_TABLE_SIZE);
     std::vector<float> values(h->leaf_values, h->leaf_values + (size_t)h->num_leaves * 512);
     std::vector<float> tiles(h->tile_values, h->tile_values + num_tiles_values);
+    values.push_back(0.0f);  // padding: the lookup reads voxel pairs (z, z+1) at any voxel and tile
+    tiles.push_back(0.0f);
     rtc8x_root_tile* d_root;
     uint32_t *d_upper, *d_lower;
     int32_t* d_origins;

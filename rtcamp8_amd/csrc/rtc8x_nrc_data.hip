@@ -199,24 +199,33 @@ __global__ __launch_bounds__(kShuffleBlocks) void shuffle_stats_kernel(const rtc
     }
 }
 
-__global__ void accumulate_kernel(const rtc8x_nrc_buffers nb, uint32_t n, double* __restrict__ accum)
+// One thread per pixel adds the launch's samples in order, as that many one-sample launches would.
+__global__ void accumulate_kernel(const rtc8x_nrc_buffers nb, uint32_t n, uint32_t samples, double* __restrict__ accum)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const rtc8x_terminal_info ti = nb.inference_terminals[i];
-    const RGB direct = load_rgb(nb.per_frame_contribution + 3 * (size_t)i);
-    RGB radiance{0.0f, 0.0f, 0.0f};
-    if (ti.has_query & 1u) {
-        radiance = max0(load_rgb(nb.inferred_radiance + 3 * (size_t)i));
-        if (nb.radiance_scale > 0.0f) radiance = scale(radiance, 1.0f / nb.radiance_scale);
-        radiance = mul(radiance, reflectance(nb.inference_queries[i]));
-    }
-    const RGB indirect = mul(RGB{ti.throughput[0], ti.throughput[1], ti.throughput[2]}, radiance);
-    const RGB c = add(direct, indirect);
+    const size_t q_stride = ((size_t)n + 127u) & ~(size_t)127u;
     double* a = accum + 3 * (size_t)i;
-    a[0] += (double)c.r;
-    a[1] += (double)c.g;
-    a[2] += (double)c.b;
+    double s0 = a[0], s1 = a[1], s2 = a[2];
+    for (uint32_t k = 0; k < samples; ++k) {
+        const size_t pi = (size_t)k * n + i, qi = (size_t)k * q_stride + i;
+        const rtc8x_terminal_info ti = nb.inference_terminals[pi];
+        const RGB direct = load_rgb(nb.per_frame_contribution + 3 * pi);
+        RGB radiance{0.0f, 0.0f, 0.0f};
+        if (ti.has_query & 1u) {
+            radiance = max0(load_rgb(nb.inferred_radiance + 3 * qi));
+            if (nb.radiance_scale > 0.0f) radiance = scale(radiance, 1.0f / nb.radiance_scale);
+            radiance = mul(radiance, reflectance(nb.inference_queries[qi]));
+        }
+        const RGB indirect = mul(RGB{ti.throughput[0], ti.throughput[1], ti.throughput[2]}, radiance);
+        const RGB c = add(direct, indirect);
+        s0 += (double)c.r;
+        s1 += (double)c.g;
+        s2 += (double)c.b;
+    }
+    a[0] = s0;
+    a[1] = s1;
+    a[2] = s2;
 }
 
 }  // namespace dev
@@ -300,7 +309,9 @@ extern "C" rtc8x_status rtc8x_nrc_accumulate(void* stream, const rtc8x_frame_par
                       p->accum,
                   "rtc8x_nrc_accumulate: buffers missing");
     const uint32_t n = p->image_width * p->image_height;
-    hipLaunchKernelGGL(dev::accumulate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, nb, n, p->accum);
+    const uint32_t samples = nb.num_samples > 1 ? nb.num_samples : 1u;
+    hipLaunchKernelGGL(dev::accumulate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, nb, n, samples,
+                       p->accum);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }

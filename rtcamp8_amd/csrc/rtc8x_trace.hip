@@ -26,7 +26,9 @@ namespace rtc8x {
 namespace dev {
 
 constexpr uint32_t kHardMaxPathLength = 4096u;
-constexpr uint32_t kSpeculate = 2;  // tentative collisions looked up together per lane (global majorant)
+// Tentative collisions looked up together per lane (global majorant).  Deeper batches (3, 4) and a
+// separate depth for ratio tracking or for the training launch measured slower on the 512^3 cloud.
+constexpr uint32_t kSpeculate = 2;
 constexpr uint32_t kCounterSlots = 1024;
 __device__ uint32_t g_work_counter[kCounterSlots];
 
@@ -160,6 +162,74 @@ __device__ __forceinline__ bool start_segment(const rtc8x_grid& g, const Medium&
     return true;
 }
 
+// S tentative collisions of a global-majorant segment at once (track: delta tracking, else ratio
+// tracking).  The positions depend only on the random numbers, so their lookups are issued together.
+// The numbers are drawn in the sequential algorithm's order (tracking: step, accept, step, accept, ...;
+// ratio tracking: step, step, ...) and the results are consumed in order; after an acceptance or the
+// segment's end the PCG state is rewound to where the sequential loop would stand, so later draws and
+// the transmittance product are those of the sequential loop.
+template <uint32_t S>
+__device__ __forceinline__ void speculate_global(const rtc8x_grid& g, Medium& med,
+                                                 Seg<RTC8X_TRACKING_GLOBAL_MAJORANT>& sg, Lane& L, const bool track)
+{
+    Pcg r = L.rng;
+    float tk[S], ua[S];
+    uint64_t st_after[S];
+    uint32_t n = 0;
+    bool ended = false;
+    uint64_t st_end = 0;
+    float t = sg.t;
+#pragma unroll
+    for (uint32_t k = 0; k < S; ++k) {
+        tk[k] = sg.t;
+        ua[k] = 0.0f;
+        st_after[k] = 0;
+        if (!ended) {
+            t += -rtc8x_logf(1.0f - r.uni()) / med.majorant;
+            if (t > sg.t1) {
+                ended = true;
+                st_end = r.state;
+            } else {
+                tk[k] = t;
+                if (track) ua[k] = r.uni();
+                st_after[k] = r.state;
+                n = k + 1;
+            }
+        }
+    }
+    float sig[S];
+    density_batch<S>(g, med.coeff, sg.o, sg.d, tk, sig);
+    bool accepted = false;
+    float t_acc = 0.0f;
+    uint64_t st_acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < S; ++k) {
+        if (!accepted && k < n) {
+            med.lookups++;
+            if (track) {
+                if (ua[k] < sig[k] / med.majorant) {
+                    accepted = true;
+                    t_acc = tk[k];
+                    st_acc = st_after[k];
+                }
+            } else {
+                L.vis *= 1.0f - sig[k] / med.majorant;
+            }
+        }
+    }
+    if (accepted) {
+        sg.t = t_acc;
+        L.rng.state = st_acc;
+        L.mode = kModeScatter;
+    } else if (ended) {
+        L.rng.state = st_end;
+        L.mode = track ? kModeEscape : kModeNeeDone;
+    } else {
+        sg.t = tk[S - 1];
+        L.rng.state = st_after[S - 1];
+    }
+}
+
 // pathTrace_generic<withNRC, genData> (optix_pathtracing_kernels.cu:31-434) for volume-only scenes, as
 // a per-lookup state machine over a persistent grid.  ENTRY selects the ray-generation program;
 // COUNT_ONLY (GENERATE_TRAINING_DATA) traces the paths without writing anything and stores each
@@ -186,6 +256,8 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
     const uint32_t n_first = (gen_data && !p.work_order) ? first_nx * first_ny : 0u;
     const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + 7) / 8) * 64;
     const uint32_t lane_id = threadIdx.x & 63;
+    // per-sample slices of the render buffers (one launch may trace several samples into the cache)
+    const size_t q_stride = ((size_t)W * H + 127u) & ~(size_t)127u, p_stride = (size_t)W * H;
     uint32_t max_len = (ENTRY == RTC8X_ENTRY_PATH_TRACE && p.max_path_length) ? p.max_path_length : kHardMaxPathLength;
     if (max_len > kHardMaxPathLength) max_len = kHardMaxPathLength;
     uint32_t rr_after = p.rr_after_path_length;
@@ -315,7 +387,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                         ti.throughput[1] = L.tp.g;
                         ti.throughput[2] = L.tp.b;
                         ti.has_query = (L.path_length & 0xFFu) << 8;
-                        nb.inference_terminals[L.lin] = ti;
+                        nb.inference_terminals[(size_t)L.sample * p_stride + L.lin] = ti;
                     }
                 }
                 path_end = true;
@@ -364,7 +436,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                     if (ends_with_cache) {
                         if (!COUNT_ONLY) {
                             const V3 vout{-L.dir.x, -L.dir.y, -L.dir.z};
-                            make_query(p, L.pos, vout, &nb.inference_queries[L.lin]);
+                            make_query(p, L.pos, vout, &nb.inference_queries[(size_t)L.sample * q_stride + L.lin]);
                             if (gen_data) {
                                 rtc8x_training_suffix_terminal ti;
                                 ti.prev_vertex_index = L.prev_idx;
@@ -378,7 +450,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                                 ti.throughput[1] = L.tp.g;
                                 ti.throughput[2] = L.tp.b;
                                 ti.has_query = 1u | ((L.path_length & 0xFFu) << 8);
-                                nb.inference_terminals[L.lin] = ti;
+                                nb.inference_terminals[(size_t)L.sample * p_stride + L.lin] = ti;
                             }
                         }
                         path_end = true;
@@ -546,7 +618,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                             a[2] += (double)L.contribution.b;
                         }
                     } else if (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
-                        float* pf = nb.per_frame_contribution + 3 * (size_t)L.lin;
+                        float* pf = nb.per_frame_contribution + 3 * ((size_t)L.sample * p_stride + L.lin);
                         pf[0] = ok ? L.contribution.r : 0.0f;
                         pf[1] = ok ? L.contribution.g : 0.0f;
                         pf[2] = ok ? L.contribution.b : 0.0f;
@@ -572,71 +644,10 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             while (sg.advance(p.grid, med, L.rng, mu)) sg.skip_shadow(L.rng);
             L.mode = kModeNeeDone;
         } else if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
-            // kSpeculate collisions at once: the positions depend only on the random numbers, so their
-            // lookups are issued together.  The numbers are drawn in the sequential algorithm's order
-            // (tracking: step, accept, step, accept, ...; ratio tracking: step, step, ...) and the
-            // results are consumed in order; after an acceptance or the segment's end the PCG state is
-            // rewound to where the sequential loop would stand, so later draws are unaffected.
-            if (L.mode == kModeTrack || L.mode == kModeShadow) {
-                const bool track = L.mode == kModeTrack;
-                Pcg r = L.rng;
-                float tk[kSpeculate], ua[kSpeculate];
-                uint64_t st_after[kSpeculate];
-                uint32_t n = 0;
-                bool ended = false;
-                uint64_t st_end = 0;
-                float t = sg.t;
-#pragma unroll
-                for (uint32_t k = 0; k < kSpeculate; ++k) {
-                    tk[k] = sg.t;
-                    ua[k] = 0.0f;
-                    st_after[k] = 0;
-                    if (!ended) {
-                        t += -rtc8x_logf(1.0f - r.uni()) / med.majorant;
-                        if (t > sg.t1) {
-                            ended = true;
-                            st_end = r.state;
-                        } else {
-                            tk[k] = t;
-                            if (track) ua[k] = r.uni();
-                            st_after[k] = r.state;
-                            n = k + 1;
-                        }
-                    }
-                }
-                float sig[kSpeculate];
-#pragma unroll
-                for (uint32_t k = 0; k < kSpeculate; ++k) sig[k] = density_at(p.grid, med.coeff, sg.o, sg.d, tk[k]);
-                bool accepted = false;
-                float t_acc = 0.0f;
-                uint64_t st_acc = 0;
-#pragma unroll
-                for (uint32_t k = 0; k < kSpeculate; ++k) {
-                    if (!accepted && k < n) {
-                        med.lookups++;
-                        if (track) {
-                            if (ua[k] < sig[k] / med.majorant) {
-                                accepted = true;
-                                t_acc = tk[k];
-                                st_acc = st_after[k];
-                            }
-                        } else {
-                            L.vis *= 1.0f - sig[k] / med.majorant;
-                        }
-                    }
-                }
-                if (accepted) {
-                    sg.t = t_acc;
-                    L.rng.state = st_acc;
-                    L.mode = kModeScatter;
-                } else if (ended) {
-                    L.rng.state = st_end;
-                    L.mode = track ? kModeEscape : kModeNeeDone;
-                } else {
-                    sg.t = tk[kSpeculate - 1];
-                    L.rng.state = st_after[kSpeculate - 1];
-                }
-            }
+            // one copy of the lookup code serves both segment kinds: the kernel's instruction footprint
+            // matters (a copy per kind made the full-image launch half again as slow)
+            if (L.mode == kModeTrack || L.mode == kModeShadow)
+                speculate_global<kSpeculate>(p.grid, med, sg, L, L.mode == kModeTrack);
         } else if (L.mode == kModeTrack || L.mode == kModeShadow) {
             if (sg.advance(p.grid, med, L.rng, mu)) {
                 const float sigma = sample_density(p.grid, med, sg.o, sg.d, sg.t);
@@ -855,8 +866,11 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
                   "rtc8x_trace: bad image size %ux%u", p.image_width, p.image_height);
     RTC8X_REQUIRE(p.rng_states != nullptr, "rtc8x_trace: rng_states is NULL");
     RTC8X_REQUIRE(num_samples >= 1, "rtc8x_trace: num_samples must be >= 1");
-    RTC8X_REQUIRE(entry == RTC8X_ENTRY_PATH_TRACE || num_samples == 1,
-                  "rtc8x_trace: the NRC entry points trace one sample per launch");
+    RTC8X_REQUIRE(entry != RTC8X_ENTRY_GENERATE_TRAINING_DATA || num_samples == 1,
+                  "rtc8x_trace: GENERATE_TRAINING_DATA traces one sample per launch");
+    RTC8X_REQUIRE(entry != RTC8X_ENTRY_PATH_TRACE_WITH_NRC || num_samples == (p.nrc.num_samples ? p.nrc.num_samples : 1u),
+                  "rtc8x_trace: PATH_TRACE_WITH_NRC traces nrc.num_samples (%u) samples per launch, not %u",
+                  p.nrc.num_samples ? p.nrc.num_samples : 1u, num_samples);
     RTC8X_REQUIRE(entry != RTC8X_ENTRY_PATH_TRACE || p.accum != nullptr, "rtc8x_trace: accum is NULL");
     RTC8X_REQUIRE(finite3(p.camera.position) && std::isfinite(p.camera.vh) && std::isfinite(p.camera.vw),
                   "rtc8x_trace: camera is not finite");
@@ -913,7 +927,7 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
                   "rtc8x_trace: NRC buffers missing");
     if (entry == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
         RTC8X_REQUIRE(nb.inference_terminals && nb.per_frame_contribution, "rtc8x_trace: NRC buffers missing");
-        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, false>(p, p.image_width, p.image_height, 1,
+        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, false>(p, p.image_width, p.image_height, num_samples,
                                                                                  nullptr, nullptr, s)));
         return RTC8X_OK;
     }

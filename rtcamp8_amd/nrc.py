@@ -107,19 +107,20 @@ def round_up(x: int, m: int) -> int:
 class NrcBuffers:
     """Screen-size and training buffers of the NRC path (renderer_main.cpp:80-129, 197-230)."""
 
-    def __init__(self, width: int, height: int, shuffler_seed: int = 471313181):
+    def __init__(self, width: int, height: int, shuffler_seed: int = 471313181, samples: int = 1):
+        """`samples`: image slices of the render buffers (samples one PATH_TRACE_WITH_NRC launch traces)."""
         n = width * height
         nb_inf = round_up(n, 128)
-        self.width, self.height = width, height
+        self.width, self.height, self.samples = width, height, samples
         self.num_training_data = DeviceBuffer.zeros(8)
         self.train_image_size = DeviceBuffer.zeros(16)
         self.target_min_max = DeviceBuffer.zeros(4 * 12)
         self.target_avg = DeviceBuffer.zeros(4 * 6)
-        self.inference_queries = DeviceBuffer.zeros(56 * nb_inf)
-        self.inference_terminals = DeviceBuffer.zeros(16 * n)
+        self.inference_queries = DeviceBuffer.zeros(56 * nb_inf * samples)
+        self.inference_terminals = DeviceBuffer.zeros(16 * n * samples)
         self.train_suffix_terminals = DeviceBuffer.zeros(16 * n)
-        self.inferred_radiance = DeviceBuffer.zeros(12 * nb_inf)
-        self.per_frame_contribution = DeviceBuffer.zeros(12 * n)
+        self.inferred_radiance = DeviceBuffer.zeros(12 * nb_inf * samples)
+        self.per_frame_contribution = DeviceBuffer.zeros(12 * n * samples)
         self.train_queries = [DeviceBuffer.zeros(56 * _abi.TRAIN_BUFFER_SIZE) for _ in range(2)]
         self.train_targets = [DeviceBuffer.zeros(12 * _abi.TRAIN_BUFFER_SIZE) for _ in range(2)]
         self.train_vertex_info = DeviceBuffer.zeros(24 * _abi.TRAIN_BUFFER_SIZE)
@@ -175,7 +176,10 @@ class NrcRenderer:
 
     def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
                  precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121,
-                 encoding: int = _abi.NRC_HASH_GRID, ema_decay: float = 0.99, first_frame_iterations: int = 8):
+                 encoding: int = _abi.NRC_HASH_GRID, ema_decay: float = 0.99, first_frame_iterations: int = 8,
+                 samples_per_launch: int = 4):
+        """samples_per_launch: samples traced into the cache by one launch (and inferred and accumulated
+        together); the sums equal those of one-sample launches."""
         cfg = scene.cfg
         self.scene = scene
         self.stream = stream
@@ -184,7 +188,8 @@ class NrcRenderer:
         self.rng = DeviceBuffer(8 * n)
         self.accum = DeviceBuffer(24 * n)
         self.stats_buf = DeviceBuffer.zeros(8 * _abi.STAT_COUNT)
-        self.buffers = NrcBuffers(cfg.width, cfg.height)
+        self.samples_per_launch = max(1, int(samples_per_launch))
+        self.buffers = NrcBuffers(cfg.width, cfg.height, samples=self.samples_per_launch)
         # cost-ordered scheduling of the render launches (the full-image entry)
         self.cost = DeviceBuffer.zeros(4 * n)
         self.order = DeviceBuffer(4 * n)
@@ -244,28 +249,38 @@ class NrcRenderer:
         self.stats.last_loss = loss
         return loss
 
-    def render_sample(self) -> None:
+    def render_samples(self, k: int = 1) -> None:
+        """k samples (k <= samples_per_launch) traced into the cache by one launch, inferred by one
+        network call and accumulated in sample order."""
+        assert 1 <= k <= self.samples_per_launch
         s = C.c_void_p(self.stream)
         p = self.params()
+        p.nrc.num_samples = k
         if self.have_cost:
             call("rtc8x_order_by_cost", s, C.c_void_p(self.cost.ptr), self.width * self.height, C.c_void_p(self.order.ptr),
                  C.c_void_p(self.order_scratch.ptr))
             p.work_order = self.order.ptr
         p.pixel_cost = self.cost.ptr
         self.have_cost = True
-        call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
-        nq = round_up(self.width * self.height, 128)
+        call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, k)
+        nq = round_up(self.width * self.height, 128) * k
         self.cache.infer(self.stream, self.buffers.inference_queries.ptr, nq, self.buffers.inferred_radiance.ptr)
         call("rtc8x_nrc_accumulate", s, C.byref(p))
         self.stats.inference_queries += nq
-        self.samples += 1
+        self.samples += k
+
+    def render_sample(self) -> None:
+        self.render_samples(1)
 
     def render_frame(self, spp: int, train_iterations: int | None = None) -> None:
         iters = train_iterations if train_iterations is not None else (self.first_frame_iterations if self.frame == 0 else 1)
         for _ in range(iters):
             self.train_iteration(self.frame == 0)
-        for _ in range(spp):
-            self.render_sample()
+        done = 0
+        while done < spp:
+            k = min(self.samples_per_launch, spp - done)
+            self.render_samples(k)
+            done += k
         self.frame += 1
 
     def image(self) -> np.ndarray:

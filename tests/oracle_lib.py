@@ -299,6 +299,7 @@ class HostNrc:
         from rtcamp8_amd.nrc import round_up
 
         n = dev.width * dev.height
+        k = getattr(dev, "samples", 1)  # image slices of the render buffers
         ninf = round_up(n, 128)
         self.width, self.height = dev.width, dev.height
         self.a = {
@@ -306,11 +307,11 @@ class HostNrc:
             "train_image_size": dev.train_image_size.download(np.int32, 4),
             "target_min_max": dev.target_min_max.download(np.float32, 12),
             "target_avg": dev.target_avg.download(np.float32, 6),
-            "inference_queries": dev.inference_queries.download(np.float32, 14 * ninf),
-            "inference_terminals": dev.inference_terminals.download(np.uint32, 4 * n),
+            "inference_queries": dev.inference_queries.download(np.float32, 14 * ninf * k),
+            "inference_terminals": dev.inference_terminals.download(np.uint32, 4 * n * k),
             "train_suffix_terminals": dev.train_suffix_terminals.download(np.uint32, 4 * n),
-            "inferred_radiance": dev.inferred_radiance.download(np.float32, 3 * ninf),
-            "per_frame_contribution": dev.per_frame_contribution.download(np.float32, 3 * n),
+            "inferred_radiance": dev.inferred_radiance.download(np.float32, 3 * ninf * k),
+            "per_frame_contribution": dev.per_frame_contribution.download(np.float32, 3 * n * k),
             "train_queries0": dev.train_queries[0].download(np.float32, 14 * _abi.TRAIN_BUFFER_SIZE),
             "train_queries1": dev.train_queries[1].download(np.float32, 14 * _abi.TRAIN_BUFFER_SIZE),
             "train_targets0": dev.train_targets[0].download(np.float32, 3 * _abi.TRAIN_BUFFER_SIZE),

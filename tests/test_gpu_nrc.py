@@ -300,10 +300,73 @@ def test_render_with_nrc_bit_exact(ordered):
     assert np.array_equal(rng, gpu_rng)
     assert np.array_equal(a["inference_terminals"].view(np.uint32), g["inference_terminals"].view(np.uint32))
     assert np.array_equal(a["per_frame_contribution"].view(np.uint32), g["per_frame_contribution"].view(np.uint32))
-    has_q = (a["inference_terminals"].reshape(-1, 4)[:, 3] & 1) == 1
+    has_q = (a["inference_terminals"].reshape(-1, 4)[:n, 3] & 1) == 1
     assert has_q.sum() > 10
     assert np.array_equal(a["inference_queries"].reshape(-1, 14)[:n][has_q].view(np.uint32),
                           g["inference_queries"].reshape(-1, 14)[:n][has_q].view(np.uint32))
     assert np.array_equal(accum.view(np.uint64), gpu_accum.view(np.uint64))
+    r.close()
+    ds.close()
+
+
+def test_render_with_nrc_several_samples_per_launch():
+    """A PATH_TRACE_WITH_NRC launch of 3 samples (slices k * image of the render buffers) matches the
+    oracle bit for bit, and its accumulated sums equal those of three one-sample launches."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd._lib import call
+
+    cfg, ds, r = _nrc_case()
+    assert r.samples_per_launch >= 3
+    b = r.buffers
+    n = cfg.width * cfg.height
+    nq = ((n + 127) // 128) * 128
+    p = r.params()
+    p.nrc.num_samples = 3
+    rng0 = r.rng.download(np.uint64, n)
+    before = HostNrc(b)
+    call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 3)
+    inferred = np.random.default_rng(22).random(b.inferred_radiance.nbytes // 4).astype(np.float32) * 3 - 0.5
+    b.inferred_radiance.upload(inferred)
+    call("rtc8x_nrc_accumulate", None, C.byref(p))
+    after = HostNrc(b)
+    gpu_accum = r.accum.download(np.float64, 3 * n)
+    gpu_rng = r.rng.download(np.uint64, n)
+    with pytest.raises(Exception):  # the launch must trace exactly nrc.num_samples samples
+        call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 2)
+
+    rng = rng0.copy()
+    accum = np.zeros(3 * n)
+    hp, keep = _host_params(cfg, ds, r, before, rng, accum)
+    hp.nrc.num_samples = 3
+    assert oracle().oracle_trace_f32(C.byref(hp), _abi.ENTRY_PATH_TRACE_WITH_NRC, 3, 0, 10 ** 6, 4) == 0
+    before.a["inferred_radiance"][:] = inferred
+    oracle().oracle_nrc_accumulate(C.byref(hp))
+    a, g = before.a, after.a
+    assert np.array_equal(rng, gpu_rng)
+    term_a = a["inference_terminals"].reshape(-1, 4)[:3 * n]
+    term_g = g["inference_terminals"].reshape(-1, 4)[:3 * n]
+    assert np.array_equal(term_a.view(np.uint32), term_g.view(np.uint32))
+    assert np.array_equal(a["per_frame_contribution"][:9 * n].view(np.uint32),
+                          g["per_frame_contribution"][:9 * n].view(np.uint32))
+    qa = a["inference_queries"].reshape(-1, 14)
+    qg = g["inference_queries"].reshape(-1, 14)
+    for k in range(3):
+        has_q = (term_a[k * n:(k + 1) * n, 3] & 1) == 1
+        assert has_q.sum() > 10
+        sl = slice(k * nq, k * nq + n)
+        assert np.array_equal(qa[sl][has_q].view(np.uint32), qg[sl][has_q].view(np.uint32))
+    assert np.array_equal(accum.view(np.uint64), gpu_accum.view(np.uint64))
+
+    # three one-sample launches from the same state give the same sums
+    r.rng.upload(rng0)
+    r.accum.fill(0)
+    for k in range(3):
+        p1 = r.params()
+        p1.nrc.num_samples = 1
+        call("rtc8x_trace", None, C.byref(p1), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
+        b.inferred_radiance.upload(inferred[3 * k * nq: 3 * (k + 1) * nq].copy())
+        call("rtc8x_nrc_accumulate", None, C.byref(p1))
+    assert np.array_equal(r.accum.download(np.float64, 3 * n).view(np.uint64), gpu_accum.view(np.uint64))
+    assert np.array_equal(r.rng.download(np.uint64, n), gpu_rng)
     r.close()
     ds.close()

@@ -215,31 +215,35 @@ def nrc_render_leg(scene, steps: int, stream):
 
 
 def latency_roofline(scene, lookups_per_path: float, trace_lookups_per_s: float, paths_per_s: float,
-                     waves=(4, 8), steps: int = 32):
-    """Memory-latency roofline of the density lookups: the dependent 8-tap gather probe over this grid's
-    leaf values, at the trace kernel's occupancy (4 waves per SIMD at its 116 VGPRs) and at 8."""
+                     waves=(3, 8), widths=(1, 2), steps: int = 32):
+    """Memory-latency roofline of the density lookups: the gather probe runs the tracer's own lookup
+    (leaf-map entry, then the stencil's voxels) at random positions of random leaves of this grid, in
+    serially dependent steps of `width` lookups issued together.  The reference point is the tracer's
+    occupancy (3 waves per SIMD at its 159 VGPRs) and its batch of 2 tentative collisions per lane."""
     from rtcamp8_amd._lib import call
     from rtcamp8_amd.device import DeviceBuffer, EventTimer
 
     g = scene.grid
     threads = 1920 * 1088
     out = DeviceBuffer(4 * threads)
-    res = {"probe": f"{steps} dependent 8-tap stencil gathers per lane at random positions of random leaves "
-                    f"({g.num_leaves} leaves, {g.num_leaves * 2048 / 2**20:.0f} MiB of values), {threads} lanes"}
+    res = {"probe": f"{steps} dependent steps of 1 or 2 trilinear lookups per lane at random positions of random "
+                    f"leaves ({g.num_leaves} leaves, {g.num_leaves * 2048 / 2**20:.0f} MiB of values), {threads} lanes"}
     for w in waves:
-        call("rtc8x_gather_probe", None, C.c_void_p(g.leaf_values), g.num_leaves, threads, steps, w, 7, C.c_void_p(out.ptr))
-        t = EventTimer()
-        t.start(None)
-        call("rtc8x_gather_probe", None, C.c_void_p(g.leaf_values), g.num_leaves, threads, steps, w, 8, C.c_void_p(out.ptr))
-        t.stop(None)
-        ms = t.elapsed_ms()
-        t.destroy()
-        res[f"lookups_per_s_{w}waves"] = threads * steps / (ms * 1e-3)
+        for width in widths:
+            call("rtc8x_gather_probe", None, C.byref(g), threads, steps, width, w, 7, C.c_void_p(out.ptr))
+            t = EventTimer()
+            t.start(None)
+            call("rtc8x_gather_probe", None, C.byref(g), threads, steps, width, w, 8, C.c_void_p(out.ptr))
+            t.stop(None)
+            ms = t.elapsed_ms()
+            t.destroy()
+            res[f"lookups_per_s_{w}waves_x{width}"] = threads * steps * width / (ms * 1e-3)
     out.free()
-    ref = res[f"lookups_per_s_{waves[0]}waves"]
-    res.update({"waves_per_simd": waves[0], "trace_lookups_per_s": trace_lookups_per_s,
-                "frac": trace_lookups_per_s / ref, "paths_per_s": paths_per_s,
-                "roofline_paths_per_s": ref / lookups_per_path,
+    ref = res[f"lookups_per_s_{waves[0]}waves_x2"]
+    res.update({"waves_per_simd": waves[0], "width": 2, "trace_lookups_per_s": trace_lookups_per_s,
+                "frac": trace_lookups_per_s / ref,
+                "frac_vs_width1": trace_lookups_per_s / res[f"lookups_per_s_{waves[0]}waves_x1"],
+                "paths_per_s": paths_per_s, "roofline_paths_per_s": ref / lookups_per_path,
                 "paths_frac": paths_per_s / (ref / lookups_per_path)})
     return res
 
@@ -392,8 +396,9 @@ def main() -> int:
     lat = latency_roofline(scene, c["lookups"] / max(c["paths"], 1), c["lookups"] / sum(kernel_ms) * 1e3,
                            c["paths"] / wall)
     result["latency_roofline"] = lat
-    print(f"[bench] latency roofline: probe {lat['lookups_per_s_4waves']:.3g} lookups/s at 4 waves/SIMD, "
-          f"trace at {lat['frac']:.2f} of it", file=sys.stderr, flush=True)
+    print(f"[bench] latency roofline: probe {lat['lookups_per_s_3waves_x2']:.3g} lookups/s at 3 waves/SIMD, "
+          f"2 per step ({lat['lookups_per_s_3waves_x1']:.3g} at 1); trace at {lat['frac']:.2f} of it",
+          file=sys.stderr, flush=True)
     if args.nrc_steps > 0:
         print("[bench] NRC network leg ...", file=sys.stderr, flush=True)
         mlp = nrc_mlp_leg(scene, args.nrc_steps)

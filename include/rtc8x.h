@@ -72,13 +72,15 @@ rtc8x_status rtc8x_event_elapsed_ms(float* ms, void* start, void* end);
  * rtc8x_copy_async: device-to-device copy by a plain 16-byte-per-lane kernel on `stream` (dst, src and
  *   bytes 16-byte aligned).  It reads and writes exactly `bytes` each, which calibrates HBM byte counters.
  * rtc8x_gather_probe: the memory-latency roofline of density lookups.  `threads` lanes (a multiple of
- *   256) each perform `steps` serially dependent lookups; one lookup reads the 8 taps of a 2x2x2 stencil
- *   at a random position of a random leaf of `leaf_values` (num_leaves x 512 floats), as a trilinear
- *   lookup does.  Occupancy is pinned to `waves_per_simd` (3..8) waves per SIMD through an LDS
- *   reservation.  out[threads] receives a per-lane checksum. */
+ *   256) each perform `steps` serially dependent steps of `width` (1, 2, 4) lookups issued together;
+ *   a lookup is the tracer's trilinear lookup (leaf-map entry, then the 2x2x2 stencil's voxels) at a
+ *   random position inside a random leaf of the device grid `grid`.  Occupancy is pinned to
+ *   `waves_per_simd` (2..8) waves per SIMD through an LDS reservation.  out[threads] receives a per-lane
+ *   checksum. */
+struct rtc8x_grid;
 rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* stream);
-rtc8x_status rtc8x_gather_probe(void* stream, const float* leaf_values, uint32_t num_leaves, uint32_t threads,
-                                uint32_t steps, uint32_t waves_per_simd, uint32_t seed, float* out);
+rtc8x_status rtc8x_gather_probe(void* stream, const struct rtc8x_grid* grid, uint32_t threads, uint32_t steps,
+                                uint32_t width, uint32_t waves_per_simd, uint32_t seed, float* out);
 
 /* ------------------------------------------------------------------------------------------------
  * Sparse density grid in the NanoVDB tree layout (root tiles -> 32^3 upper -> 16^3 lower -> 8^3 leaf;

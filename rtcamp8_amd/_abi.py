@@ -355,7 +355,7 @@ PROTOTYPES = {
     "rtc8x_memcpy_dtod": (_S, [_P, _P, C.c_size_t]),
     "rtc8x_memset": (_S, [_P, C.c_int32, C.c_size_t]),
     "rtc8x_copy_async": (_S, [_P, _P, C.c_size_t, _P]),
-    "rtc8x_gather_probe": (_S, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "rtc8x_gather_probe": (_S, [_P, C.POINTER(Grid), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "rtc8x_device_synchronize": (_S, []),
     "rtc8x_stream_create": (_S, [C.POINTER(C.c_void_p)]),
     "rtc8x_stream_destroy": (_S, [_P]),

@@ -1,11 +1,12 @@
 // rtc8x_measure.hip -- measurement kernels with no counterpart in the renderer: a plain copy (the
-// calibration of the HBM byte counters) and a dependent 8-tap gather over the grid's leaf values (the
-// memory-latency roofline of the density lookups).
+// calibration of the HBM byte counters) and dependent density lookups at random positions of the grid's
+// leaves (the memory-latency roofline of the tracer's lookups).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "rtc8x_common.hpp"
+#include "rtc8x_device.hpp"
 
 namespace rtc8x {
 
@@ -25,31 +26,36 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h)
     return h;
 }
 
-// Each lane performs `steps` lookups.  A lookup reads the 8 corners of a 2x2x2 stencil at a random
-// position inside a random leaf (leaf voxel offset x<<6 | y<<3 | z, the NanoVDB leaf order), exactly
-// the 8 taps of a trilinear density lookup, as four 8-byte loads of z-adjacent pairs (the fewest
-// memory instructions that fetch a stencil); the next lookup's position depends on the values read,
-// so lookups are serially dependent while the taps of one lookup are issued together.
-__global__ void gather_probe_kernel(const float* __restrict__ values, uint32_t num_leaves, uint32_t steps,
-                                    uint32_t seed, float* __restrict__ out)
+// Each lane performs `steps` steps of W density lookups.  A lookup is the tracer's own: the trilinear
+// stencil at a random position inside a random leaf, fetched in two rounds (leaf-map entries, then the
+// voxels) by trilinear_batch, whose W lookups overlap.  The next step's positions depend on the values
+// read, so steps are serially dependent.
+template <uint32_t W>
+__global__ void gather_probe_kernel(const rtc8x_grid g, uint32_t steps, uint32_t seed, float* __restrict__ out)
 {
     extern __shared__ float occupancy_limiter[];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t h = mix32(tid * 0x9e3779b9u ^ seed);
     float acc = 0.0f;
     for (uint32_t s = 0; s < steps; ++s) {
-        const uint32_t leaf = (uint32_t)(((uint64_t)h * num_leaves) >> 32);
-        const uint32_t r = mix32(h + 0x632be5abu);
-        const uint32_t x = (r & 0xffu) % 7u, y = ((r >> 8) & 0xffu) % 7u, z = ((r >> 16) & 0xffu) % 7u;
-        const float* b = values + (size_t)leaf * 512 + (x << 6 | y << 3 | z);
-        float2 q[4];
-        __builtin_memcpy(&q[0], b, 8);
-        __builtin_memcpy(&q[1], b + 8, 8);
-        __builtin_memcpy(&q[2], b + 64, 8);
-        __builtin_memcpy(&q[3], b + 72, 8);
-        const float v = ((q[0].x + q[0].y) + (q[1].x + q[1].y)) + ((q[2].x + q[2].y) + (q[3].x + q[3].y));
-        acc += v;
-        h = mix32(h ^ __float_as_uint(v) ^ s);
+        dev::V3 idx[W];
+#pragma unroll
+        for (uint32_t k = 0; k < W; ++k) {
+            h = mix32(h + 0x632be5abu * (k + 1));
+            const uint32_t leaf = (uint32_t)(((uint64_t)h * g.num_leaves) >> 32);
+            const uint32_t r = mix32(h ^ 0x5bd1e995u);
+            const int32_t* o = g.leaf_origins + 3 * (size_t)leaf;
+            idx[k] = dev::V3{(float)o[0] + (float)(r & 0xffu) * (8.0f / 256.0f),
+                        (float)o[1] + (float)((r >> 8) & 0xffu) * (8.0f / 256.0f),
+                        (float)o[2] + (float)((r >> 16) & 0xffu) * (8.0f / 256.0f)};
+        }
+        float v[W];
+        dev::trilinear_batch<W>(g, idx, v);
+        float sum = 0.0f;
+#pragma unroll
+        for (uint32_t k = 0; k < W; ++k) sum += v[k];
+        acc += sum;
+        h = mix32(h ^ __float_as_uint(sum) ^ s);
     }
     if (acc == -1.0f) occupancy_limiter[threadIdx.x] = acc;  // keeps the dynamic LDS allocation alive
     out[tid] = acc;
@@ -72,14 +78,17 @@ extern "C" rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t byte
     return RTC8X_OK;
 }
 
-extern "C" rtc8x_status rtc8x_gather_probe(void* stream, const float* leaf_values, uint32_t num_leaves,
-                                           uint32_t threads, uint32_t steps, uint32_t waves_per_simd, uint32_t seed,
-                                           float* out)
+extern "C" rtc8x_status rtc8x_gather_probe(void* stream, const rtc8x_grid* grid, uint32_t threads, uint32_t steps,
+                                           uint32_t width, uint32_t waves_per_simd, uint32_t seed, float* out)
 {
-    RTC8X_REQUIRE(leaf_values && out, "rtc8x_gather_probe: NULL pointer");
-    RTC8X_REQUIRE(num_leaves > 0 && threads > 0 && threads % 256 == 0 && steps > 0,
-                  "rtc8x_gather_probe: num_leaves and steps must be > 0, threads a multiple of 256");
-    RTC8X_REQUIRE(waves_per_simd >= 3 && waves_per_simd <= 8, "rtc8x_gather_probe: waves_per_simd must be in [3, 8]");
+    RTC8X_REQUIRE(grid && out, "rtc8x_gather_probe: NULL pointer");
+    const rtc8x_grid& g = *grid;
+    RTC8X_REQUIRE(g.num_leaves > 0 && g.leaf_origins && g.leaf_values && g.tile_values,
+                  "rtc8x_gather_probe: the grid needs leaves with their origins on the device");
+    RTC8X_REQUIRE(threads > 0 && threads % 256 == 0 && steps > 0,
+                  "rtc8x_gather_probe: steps must be > 0, threads a multiple of 256");
+    RTC8X_REQUIRE(width == 1 || width == 2 || width == 4, "rtc8x_gather_probe: width must be 1, 2 or 4");
+    RTC8X_REQUIRE(waves_per_simd >= 2 && waves_per_simd <= 8, "rtc8x_gather_probe: waves_per_simd must be in [2, 8]");
     int dev = 0, lds_cu = 0, lds_block = 0;
     RTC8X_HIP(hipGetDevice(&dev));
     RTC8X_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
@@ -91,8 +100,13 @@ extern "C" rtc8x_status rtc8x_gather_probe(void* stream, const float* leaf_value
     RTC8X_REQUIRE(lds <= (size_t)lds_block && (size_t)lds_cu / lds == waves_per_simd,
                   "rtc8x_gather_probe: %u waves per SIMD cannot be set with %d bytes of LDS per CU", waves_per_simd,
                   lds_cu);
-    hipLaunchKernelGGL(gather_probe_kernel, dim3(threads / 256), dim3(256), lds, (hipStream_t)stream, leaf_values,
-                       num_leaves, steps, seed, out);
+    hipStream_t s = (hipStream_t)stream;
+    if (width == 1)
+        hipLaunchKernelGGL(gather_probe_kernel<1>, dim3(threads / 256), dim3(256), lds, s, g, steps, seed, out);
+    else if (width == 2)
+        hipLaunchKernelGGL(gather_probe_kernel<2>, dim3(threads / 256), dim3(256), lds, s, g, steps, seed, out);
+    else
+        hipLaunchKernelGGL(gather_probe_kernel<4>, dim3(threads / 256), dim3(256), lds, s, g, steps, seed, out);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }

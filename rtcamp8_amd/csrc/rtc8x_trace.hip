@@ -29,6 +29,8 @@ constexpr uint32_t kHardMaxPathLength = 4096u;
 // Tentative collisions looked up together per lane (global majorant).  Deeper batches (3, 4) and a
 // separate depth for ratio tracking or for the training launch measured slower on the 512^3 cloud.
 constexpr uint32_t kSpeculate = 2;
+// A wavefront with at least this many idle lanes evaluates its stepping lanes' segments together.
+constexpr uint32_t kCoopIdle = 48;
 constexpr uint32_t kCounterSlots = 1024;
 __device__ uint32_t g_work_counter[kCounterSlots];
 
@@ -227,6 +229,122 @@ __device__ __forceinline__ void speculate_global(const rtc8x_grid& g, Medium& me
     } else {
         sg.t = tk[S - 1];
         L.rng.state = st_after[S - 1];
+    }
+}
+
+// PCG32 jump-ahead: the state j draws later is mul[j] * state + add[j] (the LCG composed j times).
+struct PcgJump {
+    uint64_t mul[129], add[129];
+};
+constexpr PcgJump make_pcg_jump()
+{
+    PcgJump t{};
+    uint64_t m = 1, a = 0;
+    for (int j = 0; j <= 128; ++j) {
+        t.mul[j] = m;
+        t.add[j] = a;
+        a = a * 6364136223846793005ULL + 1ULL;
+        m = m * 6364136223846793005ULL;
+    }
+    return t;
+}
+__constant__ PcgJump kPcgJump = make_pcg_jump();
+
+__device__ __forceinline__ float lane_f(float v, uint32_t lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, uint32_t lane)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Late in a launch most lanes of a wavefront have no work left while a few paths go on, each one
+// segment at a time.  Then the whole wavefront takes the stepping lanes' segments one after the other:
+// lane j evaluates the segment's j-th next tentative collision, with its random numbers taken from the
+// PCG stream by jump-ahead, so 64 collisions cost one round of lookups.  The distances are summed and the
+// transmittance factors multiplied in the sequential order (a uniform loop), and the source lane gets
+// the state the sequential loop would end in: the same collision, PCG state, transmittance and lookup
+// count.  COUNT_ONLY ratio tracking consumes the numbers without lookups.
+template <bool COUNT_ONLY>
+__device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, Seg<RTC8X_TRACKING_GLOBAL_MAJORANT>& sg,
+                                           Lane& L, uint64_t todo, uint32_t lane)
+{
+    const float maj = med.majorant;
+    while (todo) {
+        const uint32_t src = (uint32_t)__ffsll((long long)todo) - 1u;
+        todo &= todo - 1;
+        const bool track = (uint32_t)__builtin_amdgcn_readlane((int)L.mode, (int)src) == kModeTrack;
+        const uint32_t k = track ? 2u : 1u;  // draws per tentative collision
+        const V3 o{lane_f(sg.o.x, src), lane_f(sg.o.y, src), lane_f(sg.o.z, src)};
+        const V3 d{lane_f(sg.d.x, src), lane_f(sg.d.y, src), lane_f(sg.d.z, src)};
+        const float t1 = lane_f(sg.t1, src);
+        float t = lane_f(sg.t, src);
+        float vis = lane_f(L.vis, src);
+        uint64_t st = lane_u64(L.rng.state, src);
+        uint32_t looked = 0, mode = 0;
+        for (;;) {
+            Pcg r;
+            r.state = kPcgJump.mul[lane * k] * st + kPcgJump.add[lane * k];
+            const float u = r.uni();
+            const uint64_t st_step = r.state;  // after the step's draw
+            const float ua = track ? r.uni() : 0.0f;
+            const uint64_t st_after = r.state;
+            const float dj = -rtc8x_logf(1.0f - u) / maj;
+            // t after collision j, summed in order; the segment ends at the first collision past t1
+            float tj = INFINITY, run = t;
+            uint32_t e = 64;
+            for (uint32_t i = 0; i < 64; ++i) {
+                run += lane_f(dj, i);
+                if (i == lane) tj = run;
+                if (run > t1) {
+                    e = i;
+                    break;
+                }
+            }
+            const bool valid = lane < e;
+            float sig = 0.0f;
+            if (valid && !(COUNT_ONLY && !track)) sig = density_at(g, med.coeff, o, d, tj);
+            if (track) {
+                const uint64_t acc = __ballot(valid && ua < sig / maj);
+                if (acc) {
+                    const uint32_t a = (uint32_t)__ffsll((long long)acc) - 1u;
+                    looked += a + 1;
+                    t = lane_f(tj, a);
+                    st = lane_u64(st_after, a);
+                    mode = kModeScatter;
+                    break;
+                }
+                looked += e;
+                if (e < 64) {
+                    st = lane_u64(st_step, e);
+                    mode = kModeEscape;
+                    break;
+                }
+            } else {
+                if (!COUNT_ONLY) {
+                    const float f = 1.0f - sig / maj;
+                    for (uint32_t i = 0; i < e; ++i) vis *= lane_f(f, i);
+                    looked += e;
+                }
+                if (e < 64) {
+                    st = lane_u64(st_after, e);
+                    mode = kModeNeeDone;
+                    break;
+                }
+            }
+            t = lane_f(tj, 63);
+            st = lane_u64(st_after, 63);
+        }
+        if (lane == src) {
+            if (mode == kModeScatter) sg.t = t;
+            L.rng.state = st;
+            L.vis = vis;
+            L.mode = mode;
+            med.lookups += looked;
+        }
     }
 }
 
@@ -637,7 +755,14 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             }
         }
         // ---- tentative collisions, with their density lookups, for every stepping lane
-        if (COUNT_ONLY && L.mode == kModeShadow) {
+        bool coop = false;
+        if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
+            const uint64_t steppers = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
+            coop = steppers != 0 && __popcll(__ballot(L.mode == kModeIdle)) >= kCoopIdle;
+            if (coop) coop_segments<COUNT_ONLY>(p.grid, med, sg, L, steppers, lane_id);
+        }
+        if (coop) {
+        } else if (COUNT_ONLY && L.mode == kModeShadow) {
             // The count pass writes no targets, so the shadow ray's visibility is not needed; its random
             // numbers are consumed exactly as ratio tracking consumes them (they do not depend on the
             // densities), without the lookups.

@@ -244,7 +244,8 @@ static rtc8x_status check_nrc(const rtc8x_frame_params* p, const char* what)
 
 extern "C" size_t rtc8x_nrc_scratch_bytes(uint32_t width, uint32_t height)
 {
-    return sizeof(uint32_t) * (2 * (size_t)width * height) + sizeof(float) * 9 * dev::kShuffleBlocks;
+    static_assert(dev::kShuffleBlocks == RTC8X_NUM_TRAINING_DATA_PER_FRAME / 256, "scratch layout");
+    return nrc_scratch_layout(width, height).total;
 }
 
 extern "C" rtc8x_status rtc8x_nrc_prepare(void* stream, const rtc8x_frame_params* p, int32_t is_new_sequence)

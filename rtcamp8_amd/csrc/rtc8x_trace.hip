@@ -232,6 +232,18 @@ __device__ __forceinline__ void speculate_global(const rtc8x_grid& g, Medium& me
     }
 }
 
+// Training records of a one-pass GENERATE_TRAINING_DATA launch: each record takes the next slot of a
+// staging area (train_queries[1], train_targets[1], info, tag = launch pixel and the record's index on
+// its path) in whatever order lanes reach it; stage_map/stage_copy then move every record to its slot
+// in pixel order, the slots the count pass would have assigned.  counter == NULL: the two-pass form.
+struct TrainStage {
+    uint32_t* counter;                   // records staged (may exceed RTC8X_TRAIN_BUFFER_SIZE)
+    rtc8x_training_vertex_info* info;    // RTC8X_TRAIN_BUFFER_SIZE
+    uint32_t* tag;                       // 2 x RTC8X_TRAIN_BUFFER_SIZE
+    uint32_t* path_length;               // per launch pixel
+    uint64_t* stats;                     // the launch's RTC8X_STAT_* sums, added once the records are placed
+};
+
 // PCG32 jump-ahead: the state j draws later is mul[j] * state + add[j] (the LCG composed j times).
 struct PcgJump {
     uint64_t mul[129], add[129];
@@ -357,11 +369,13 @@ template <int ENTRY, int TRACKING, bool COUNT_ONLY>
 __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
                                                         uint32_t launch_h, uint32_t num_samples, uint32_t slot,
                                                         const uint32_t* __restrict__ record_offsets,
-                                                        uint32_t* __restrict__ record_counts)
+                                                        uint32_t* __restrict__ record_counts, const TrainStage stage)
 {
     constexpr bool with_nrc = ENTRY != RTC8X_ENTRY_PATH_TRACE;
     constexpr bool gen_data = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA;
     const rtc8x_nrc_buffers& nb = p.nrc;
+    const bool staged = gen_data && !COUNT_ONLY && stage.counter != nullptr;
+    float* const train_targets = staged ? nb.train_targets[1] : nb.train_targets[0];
     const uint32_t W = launch_w, H = launch_h;
     const uint32_t tiles_x = (W + 7) / 8;
     // GENERATE_TRAINING_DATA hands out the unbiased-suffix pixels first (the one pixel of each 4x4 block
@@ -484,7 +498,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                     const RGB implicit_cont = scale(radiance, mis);
                     if (!gen_data) L.contribution = add(L.contribution, mul(L.tp, implicit_cont));
                     if (gen_data && !COUNT_ONLY && L.prev_idx != RTC8X_INVALID_VERTEX_INDEX) {
-                        float* tgt = nb.train_targets[0] + 3 * (size_t)L.prev_idx;
+                        float* tgt = train_targets + 3 * (size_t)L.prev_idx;
                         const RGB a = mul(L.prev_local_tp, implicit_cont);
                         tgt[0] = tgt[0] + a.r;
                         tgt[1] = tgt[1] + a.g;
@@ -677,13 +691,19 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                         bool record_end = false;
                         if (gen_data) {
                             // training vertex record (optix_pathtracing_kernels.cu:358-410)
-                            const uint32_t idx = L.train_base + L.records;
+                            const uint32_t local = L.records;
                             L.records++;
                             if (!COUNT_ONLY) {
                                 rtc8x_radiance_query q;
                                 make_query(p, L.pos, vout, &q);
-                                if (idx < RTC8X_TRAIN_BUFFER_SIZE) {
-                                    nb.train_queries[0][idx] = q;
+                                // the record's slot: in pixel order (two-pass), or the next staging slot
+                                const uint32_t idx = staged ? atomicAdd(stage.counter, 1u) : L.train_base + local;
+                                if (staged && idx >= RTC8X_TRAIN_BUFFER_SIZE) {
+                                    // more records than slots: the launch is redone in two passes, which
+                                    // end paths at the capacity; until then this path goes on unrecorded
+                                    L.prev_idx = RTC8X_INVALID_VERTEX_INDEX;
+                                } else if (idx < RTC8X_TRAIN_BUFFER_SIZE) {
+                                    (staged ? nb.train_queries[1] : nb.train_queries[0])[idx] = q;
                                     rtc8x_training_vertex_info vi;
                                     vi.local_throughput[0] = local_tp.r;
                                     vi.local_throughput[1] = local_tp.g;
@@ -691,8 +711,14 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                                     vi.prev_vertex_index = L.prev_idx;
                                     vi.path_length = L.path_length & 0xFFu;
                                     vi.is_unbiased = L.unbiased ? 1u : 0u;
-                                    nb.train_vertex_info[idx] = vi;
-                                    float* tgt = nb.train_targets[0] + 3 * (size_t)idx;
+                                    if (staged) {
+                                        stage.info[idx] = vi;
+                                        stage.tag[2 * (size_t)idx] = L.lin;
+                                        stage.tag[2 * (size_t)idx + 1] = local;
+                                    } else {
+                                        nb.train_vertex_info[idx] = vi;
+                                    }
+                                    float* tgt = train_targets + 3 * (size_t)idx;
                                     tgt[0] = L.nee_c.r;
                                     tgt[1] = L.nee_c.g;
                                     tgt[2] = L.nee_c.b;
@@ -724,9 +750,9 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             if (path_end) {
                 paths++;
                 vertices += L.path_length;
-                if (COUNT_ONLY) {
-                    record_counts[L.lin] = L.records;
-                } else {
+                if (COUNT_ONLY || staged) record_counts[L.lin] = L.records;
+                if (staged) stage.path_length[L.lin] = L.path_length;
+                if (!COUNT_ONLY) {
                     const bool ok = all_nonneg_finite(L.contribution);
                     if (ENTRY == RTC8X_ENTRY_PATH_TRACE) {
                         if (ok) {
@@ -741,7 +767,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                         pf[1] = ok ? L.contribution.g : 0.0f;
                         pf[2] = ok ? L.contribution.b : 0.0f;
                     }
-                    if (p.path_vertices) p.path_vertices[L.lin] += L.path_length;
+                    if (p.path_vertices && !staged) p.path_vertices[L.lin] += L.path_length;
                 }
                 if (++L.sample < num_samples) {
                     L.mode = kModeStart;
@@ -786,7 +812,8 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             }
         }
     }
-    if (p.stats && !COUNT_ONLY) {
+    uint64_t* const stats = staged ? stage.stats : p.stats;
+    if (stats && !COUNT_ONLY) {
         uint64_t v[4] = {paths, vertices, med.lookups, scatters};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -795,10 +822,10 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             v[i] = x;
         }
         if (lane_id == 0) {
-            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_PATHS], (unsigned long long)v[0]);
-            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_VERTICES], (unsigned long long)v[1]);
-            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_LOOKUPS], (unsigned long long)v[2]);
-            atomicAdd((unsigned long long*)&p.stats[RTC8X_STAT_SCATTER], (unsigned long long)v[3]);
+            atomicAdd((unsigned long long*)&stats[RTC8X_STAT_PATHS], (unsigned long long)v[0]);
+            atomicAdd((unsigned long long*)&stats[RTC8X_STAT_VERTICES], (unsigned long long)v[1]);
+            atomicAdd((unsigned long long*)&stats[RTC8X_STAT_LOOKUPS], (unsigned long long)v[2]);
+            atomicAdd((unsigned long long*)&stats[RTC8X_STAT_SCATTER], (unsigned long long)v[3]);
         }
     }
 }
@@ -831,6 +858,51 @@ __global__ __launch_bounds__(1024) void record_prefix_kernel(const uint32_t* __r
 }
 
 
+
+// One-pass training launch: clear the staging counter and statistics.
+__global__ void stage_init_kernel(TrainStage st)
+{
+    if (threadIdx.x == 0) *st.counter = 0u;
+    if (threadIdx.x < RTC8X_STAT_COUNT) st.stats[threadIdx.x] = 0u;
+}
+
+// Final slot of every staged record: its pixel's offset plus its index on the path.  Nothing to do when
+// the records overflowed the buffer (the launch is then redone in two passes).
+__global__ __launch_bounds__(256) void stage_map_kernel(const uint32_t* __restrict__ total, const uint32_t* __restrict__ tag,
+                                                        const uint32_t* __restrict__ offsets, uint32_t* __restrict__ fmap)
+{
+    const uint32_t n = *total;
+    if (n > RTC8X_TRAIN_BUFFER_SIZE) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        fmap[i] = offsets[tag[2 * (size_t)i]] + tag[2 * (size_t)i + 1];
+}
+
+// Moves the staged records to their final slots (vertex links renamed to final slots), renames the
+// training-suffix terminals' links, adds the path lengths and the launch's statistics.
+__global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_nrc_buffers nb, const uint32_t* __restrict__ total,
+                                                         const TrainStage st, const uint32_t* __restrict__ fmap,
+                                                         uint32_t npx, uint32_t* __restrict__ path_vertices,
+                                                         uint64_t* __restrict__ stats)
+{
+    const uint32_t n = *total;
+    if (n > RTC8X_TRAIN_BUFFER_SIZE) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t f = fmap[i];
+        nb.train_queries[0][f] = nb.train_queries[1][i];
+        rtc8x_training_vertex_info vi = st.info[i];
+        if (vi.prev_vertex_index != RTC8X_INVALID_VERTEX_INDEX) vi.prev_vertex_index = fmap[vi.prev_vertex_index];
+        nb.train_vertex_info[f] = vi;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nb.train_targets[0][3 * (size_t)f + c] = nb.train_targets[1][3 * (size_t)i + c];
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += stride) {
+        const uint32_t prev = nb.train_suffix_terminals[i].prev_vertex_index;
+        if (prev != RTC8X_INVALID_VERTEX_INDEX) nb.train_suffix_terminals[i].prev_vertex_index = fmap[prev];
+        if (path_vertices) path_vertices[i] += st.path_length[i];
+    }
+    if (stats && blockIdx.x == 0 && threadIdx.x < RTC8X_STAT_COUNT) stats[threadIdx.x] += st.stats[threadIdx.x];
+}
 
 // ---------------------------------------------------------------- cost-ordered work lists
 // A stable counting sort of the pixels by class = 31 - floor(log2(cost + 1)) (costliest first):
@@ -924,7 +996,7 @@ __global__ __launch_bounds__(256) void order_scatter_kernel(const uint32_t* __re
 // different slots).
 template <int ENTRY, int TRACKING, bool COUNT_ONLY>
 hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t num_samples, const uint32_t* offsets,
-                  uint32_t* counts, hipStream_t stream)
+                  uint32_t* counts, const TrainStage& stage, hipStream_t stream)
 {
     static std::atomic<uint32_t> next_slot{0};
     const uint32_t slot = next_slot.fetch_add(1) % kCounterSlots;
@@ -938,17 +1010,17 @@ hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t 
     if (per_cu < 1) per_cu = 1;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, slot);
     hipLaunchKernelGGL((wavefront_kernel<ENTRY, TRACKING, COUNT_ONLY>), dim3((uint32_t)(cus * per_cu)), dim3(256), 0,
-                       stream, p, w, h, num_samples, slot, offsets, counts);
+                       stream, p, w, h, num_samples, slot, offsets, counts, stage);
     return hipGetLastError();
 }
 
 template <int ENTRY, bool COUNT_ONLY>
 hipError_t launch_tracking(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t ns, const uint32_t* offsets,
-                           uint32_t* counts, hipStream_t stream)
+                           uint32_t* counts, hipStream_t stream, const TrainStage& stage = TrainStage{})
 {
     if (p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT)
-        return launch<ENTRY, RTC8X_TRACKING_LOCAL_MAJORANT, COUNT_ONLY>(p, w, h, ns, offsets, counts, stream);
-    return launch<ENTRY, RTC8X_TRACKING_GLOBAL_MAJORANT, COUNT_ONLY>(p, w, h, ns, offsets, counts, stream);
+        return launch<ENTRY, RTC8X_TRACKING_LOCAL_MAJORANT, COUNT_ONLY>(p, w, h, ns, offsets, counts, stage, stream);
+    return launch<ENTRY, RTC8X_TRACKING_GLOBAL_MAJORANT, COUNT_ONLY>(p, w, h, ns, offsets, counts, stage, stream);
 }
 
 }  // namespace dev
@@ -1067,11 +1139,45 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
     RTC8X_REQUIRE(ts[0] > 0 && ts[1] > 0 && (uint32_t)ts[0] <= p.image_width && (uint32_t)ts[1] <= p.image_height,
                   "rtc8x_trace: training image size %dx%d out of range", ts[0], ts[1]);
     const uint32_t tw = (uint32_t)ts[0], th = (uint32_t)ts[1], n = tw * th;
-    uint32_t* counts = nb.scratch;
-    uint32_t* offsets = nb.scratch + p.image_width * p.image_height;
+    const NrcScratchLayout lay = nrc_scratch_layout(p.image_width, p.image_height);
+    char* const sb = reinterpret_cast<char*>(nb.scratch);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(sb + lay.counts);
+    uint32_t* offsets = reinterpret_cast<uint32_t*>(sb + lay.offsets);
+    uint32_t* total = nb.num_training_data + nb.nrc_buffer_index;
+    // One pass: records are staged as the paths reach them, then moved to their pixel-order slots.  The
+    // record indices, and with them every result, are those of the two-pass form below (count pass,
+    // prefix sum, recording pass) as long as the records fit the buffer; when they do not, the
+    // training pixels' PCG states are restored and the two passes run.
+    uint64_t* rng_save = reinterpret_cast<uint64_t*>(sb + lay.rng_save);
+    uint64_t* const rng = p.rng_states;
+    RTC8X_HIP(hipMemcpy2DAsync(rng_save, 8 * (size_t)tw, rng, 8 * (size_t)p.image_width, 8 * (size_t)tw, th,
+                               hipMemcpyDeviceToDevice, s));
+    dev::TrainStage stage;
+    stage.counter = reinterpret_cast<uint32_t*>(sb + lay.counter);
+    stage.info = reinterpret_cast<rtc8x_training_vertex_info*>(sb + lay.info);
+    stage.tag = reinterpret_cast<uint32_t*>(sb + lay.tag);
+    stage.path_length = reinterpret_cast<uint32_t*>(sb + lay.path_length);
+    stage.stats = reinterpret_cast<uint64_t*>(sb + lay.stats);
+    uint32_t* fmap = reinterpret_cast<uint32_t*>(sb + lay.fmap);
+    hipLaunchKernelGGL(dev::stage_init_kernel, dim3(1), dim3(64), 0, s, stage);
+    RTC8X_HIP(hipGetLastError());
+    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA, false>(p, tw, th, 1, nullptr, counts, s, stage)));
+    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, counts, offsets, n, total);
+    RTC8X_HIP(hipGetLastError());
+    constexpr uint32_t kCompactBlocks = 512;
+    hipLaunchKernelGGL(dev::stage_map_kernel, dim3(kCompactBlocks), dim3(256), 0, s, total, stage.tag, offsets, fmap);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, nb, total, stage, fmap, n,
+                       p.path_vertices, p.stats);
+    RTC8X_HIP(hipGetLastError());
+    uint32_t records = 0;
+    RTC8X_HIP(hipMemcpyAsync(&records, total, sizeof(records), hipMemcpyDeviceToHost, s));
+    RTC8X_HIP(hipStreamSynchronize(s));
+    if (records <= RTC8X_TRAIN_BUFFER_SIZE) return RTC8X_OK;
+    RTC8X_HIP(hipMemcpy2DAsync(rng, 8 * (size_t)p.image_width, rng_save, 8 * (size_t)tw, 8 * (size_t)tw, th,
+                               hipMemcpyDeviceToDevice, s));
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA, true>(p, tw, th, 1, nullptr, counts, s)));
-    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, counts, offsets, n,
-                       nb.num_training_data + nb.nrc_buffer_index);
+    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, counts, offsets, n, total);
     RTC8X_HIP(hipGetLastError());
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA, false>(p, tw, th, 1, offsets, nullptr, s)));
     return RTC8X_OK;

@@ -173,12 +173,14 @@ def test_training_reduces_loss(precision, encoding):
     c.finalize()
 
 
-def _nrc_case(width=96, height=64, tracking=0):
+def _nrc_case(width=96, height=64, tracking=0, density=None):
     from rtcamp8_amd import scene as sc
     from rtcamp8_amd.nrc import NrcRenderer
 
     cfg = sc.CONFIGS["dense128"].with_(width=width, height=height, tracking=tracking,
                                        fbm=sc.FbmConfig(resolution=64, seed=0x5EED0001))
+    if density is not None:
+        cfg = cfg.with_(density_coeff=density)
     ds = sc.DeviceScene(cfg)
     r = NrcRenderer(ds, num_hidden_layers=2)
     return cfg, ds, r
@@ -203,14 +205,33 @@ def _host_params(cfg, ds, r, host_nrc, rng, accum):
 @pytest.mark.parametrize("tracking", [0, 1])
 def test_training_data_path_bit_exact(tracking):
     """prepare -> GENERATE_TRAINING_DATA trace -> propagate -> shuffle, GPU vs oracle on the same inputs."""
+    _training_data_case(tracking=tracking)
+
+
+def test_training_data_overflow_bit_exact():
+    """More training records than the buffer holds (long paths: a thick medium, a large path-termination
+    factor and Russian roulette only after 64 vertices): the one-pass launch is redone in two passes,
+    which end paths at the buffer's capacity, and the results are still the oracle's."""
+    from rtcamp8_amd import _abi
+
+    num = _training_data_case(width=512, height=384, termination=1.0e4, rr_after=64, density=400.0)
+    assert num > _abi.TRAIN_BUFFER_SIZE
+
+
+def _training_data_case(width=96, height=64, tracking=0, termination=None, rr_after=None, density=None):
     from rtcamp8_amd import _abi
     from rtcamp8_amd._lib import call
 
-    cfg, ds, r = _nrc_case(tracking=tracking)
+    print(f"[training data] {width}x{height} tracking {tracking} termination {termination}", flush=True)
+    cfg, ds, r = _nrc_case(width=width, height=height, tracking=tracking, density=density)
     b = r.buffers
     r.buf_idx = 0
     p = r.params()
     p.nrc.offset_to_select_unbiased_path = 5
+    if termination is not None:
+        p.nrc.path_termination_factor = termination
+    if rr_after is not None:
+        p.rr_after_path_length = rr_after
     # snapshot of the inputs, then the GPU steps
     before = HostNrc(b)
     rng0 = r.rng.download(np.uint64, cfg.width * cfg.height)
@@ -233,6 +254,8 @@ def test_training_data_path_bit_exact(tracking):
     hp, keep = _host_params(cfg, ds, r, before, rng, accum)
     hp.nrc.nrc_buffer_index = 0
     hp.nrc.offset_to_select_unbiased_path = 5
+    hp.nrc.path_termination_factor = p.nrc.path_termination_factor
+    hp.rr_after_path_length = p.rr_after_path_length
     oracle().oracle_nrc_prepare(C.byref(hp), 1)
     assert oracle().oracle_trace_f32(C.byref(hp), _abi.ENTRY_GENERATE_TRAINING_DATA, 1, 0, 10 ** 6, 4) == 0
     a, g = before.a, after_trace.a
@@ -261,6 +284,7 @@ def test_training_data_path_bit_exact(tracking):
         assert np.array_equal(a2.a[key].view(np.uint32), after_shuf.a[key].view(np.uint32)), key
     r.close()
     ds.close()
+    return num
 
 
 @pytest.mark.parametrize("ordered", [False, True])

@@ -152,30 +152,31 @@ __device__ __forceinline__ void encode(const rtc8x_radiance_query& q, float* e)
     e[63] = 0.0f;
 }
 
-// Input encoding with the hash grid: [grid 32 | OneBlob 20 | identity 6 | 1 | 0 x 5].
-__device__ __forceinline__ void encode_hash(const rtc8x_radiance_query& q, const float* __restrict__ tables,
-                                            const HashGeom& gm, float* e)
+// The grid part of the hash encoding: level l's two features, trilinear over the cell's 8 corners.
+__device__ __forceinline__ float2 hash_level(const float p[3], const float* __restrict__ tables, const HashGeom& gm,
+                                             uint32_t l)
 {
-    float p[3];
-    query_position01(q, p);
-    for (uint32_t l = 0; l < kHashLevels; ++l) {
-        uint32_t gi[3];
-        float f[3];
-        hash_cell(gm, l, p, gi, f);
-        float a0 = 0.0f, a1 = 0.0f;
+    uint32_t gi[3];
+    float f[3];
+    hash_cell(gm, l, p, gi, f);
+    float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            uint32_t idx;
-            float w;
-            hash_corner(gm, l, gi, f, c, idx, w);
-            float2 v;
-            __builtin_memcpy(&v, tables + idx, sizeof(v));
-            a0 += w * v.x;
-            a1 += w * v.y;
-        }
-        e[2 * l] = a0;
-        e[2 * l + 1] = a1;
+    for (uint32_t c = 0; c < 8; ++c) {
+        uint32_t idx;
+        float w;
+        hash_corner(gm, l, gi, f, c, idx, w);
+        float2 v;
+        __builtin_memcpy(&v, tables + idx, sizeof(v));
+        a0 += w * v.x;
+        a1 += w * v.y;
     }
+    return float2{a0, a1};
+}
+
+// Features 32..63 of the hash encoding (OneBlob of the angles and roughness, the reflectances, 1, zeros),
+// into t[0..31].
+__device__ __forceinline__ void encode_hash_tail(const rtc8x_radiance_query& q, float* t)
+{
     const float inv_pi = 1.0f / RTC8X_PI_F, inv_2pi = 1.0f / (2.0f * RTC8X_PI_F);
     const float a[5] = {q.normal_phi * inv_pi, q.normal_theta * inv_2pi, q.vout_phi * inv_pi, q.vout_theta * inv_2pi,
                         q.roughness};
@@ -184,17 +185,63 @@ __device__ __forceinline__ void encode_hash(const rtc8x_radiance_query& q, const
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             float d = a[j] - ((float)b + 0.5f) * 0.25f;
-            e[32 + 4 * j + b] = expf(-8.0f * d * d);
+            t[4 * j + b] = expf(-8.0f * d * d);
         }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        e[52 + c] = q.diffuse_reflectance[c];
-        e[55 + c] = q.specular_reflectance[c];
+        t[20 + c] = q.diffuse_reflectance[c];
+        t[23 + c] = q.specular_reflectance[c];
     }
-    e[58] = 1.0f;
+    t[26] = 1.0f;
 #pragma unroll
-    for (int j = 59; j < 64; ++j) e[j] = 0.0f;
+    for (int j = 27; j < 32; ++j) t[j] = 0.0f;
+}
+
+// Input encoding with the hash grid: [grid 32 | OneBlob 20 | identity 6 | 1 | 0 x 5].
+__device__ __forceinline__ void encode_hash(const rtc8x_radiance_query& q, const float* __restrict__ tables,
+                                            const HashGeom& gm, float* e)
+{
+    float p[3];
+    query_position01(q, p);
+    for (uint32_t l = 0; l < kHashLevels; ++l) {
+        const float2 a = hash_level(p, tables, gm, l);
+        e[2 * l] = a.x;
+        e[2 * l + 1] = a.y;
+    }
+    encode_hash_tail(q, e + 32);
+}
+
+// The hash encoding of query q (NULL: zeros) straight into an activation row: the grid features go
+// to the row level by level, so no per-lane array is indexed at run time.
+template <typename T>
+__device__ __forceinline__ void encode_hash_row(const rtc8x_radiance_query* q, const float* __restrict__ tables,
+                                                const HashGeom& gm, T* row)
+{
+    using V = typename Vec8<T>::type;
+    float t[32];
+    if (q) {
+        float p[3];
+        query_position01(*q, p);
+        for (uint32_t l = 0; l < kHashLevels; ++l) {
+            const float2 a = hash_level(p, tables, gm, l);
+            row[2 * l] = (T)a.x;
+            row[2 * l + 1] = (T)a.y;
+        }
+        encode_hash_tail(*q, t);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) row[j] = (T)0.0f;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) t[j] = 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        V v;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = (T)t[j + k];
+        *reinterpret_cast<V*>(&row[32 + j]) = v;
+    }
 }
 
 template <int ENC>
@@ -221,7 +268,14 @@ __global__ __launch_bounds__(256) void forward_kernel(const T* __restrict__ w, u
     const uint32_t s0 = (blockIdx.x * 4 + wave) * 64;
     const uint32_t s = s0 + lane;
     T* A = act[wave];
-    {
+    if constexpr (ENC == kEncHash) {
+        encode_hash_row<T>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
+        if (TRAIN && s < n) {
+#pragma unroll
+            for (int j = 0; j < 64; j += 8)
+                *reinterpret_cast<V*>(&hbuf[(size_t)s * 64 + j]) = *reinterpret_cast<const V*>(&A[lane * kActStride + j]);
+        }
+    } else {
         float e[64];
         if (s < n) {
             encode_any<ENC>(in[s], tables, geom, e);

@@ -27,30 +27,50 @@ rtc8x_status adopt_host_grid(OwnedHostGrid&& t, const rtc8x_grid& g, rtc8x_grid*
 // rtc8x_grid_upload; with keep_max the host's max_value is kept instead of the brute-force maximum.
 rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_max);
 
+// Records the one-pass training launch can stage: beyond them, records find no slot and their pixels
+// are traced again.
+constexpr size_t kStageCapacity = 4 * (size_t)RTC8X_TRAIN_BUFFER_SIZE;
+
+// A staged record's path state: what the path's end would be if it were cut at this record.
+struct StageSnapshot {
+    uint64_t rng;          // PCG state after the record's vertex
+    uint32_t path_length;  // vertices so far
+    uint32_t lookups;      // density lookups so far
+    uint32_t scatters;     // real collisions so far
+    uint32_t pad;
+};
+
 // Byte offsets of the parts of rtc8x_nrc_buffers.scratch for a width x height image: record counts and
 // offsets of the training launch (one u32 per pixel each), the shuffle's per-block target statistics,
-// then the one-pass training launch's staging: saved PCG states, path lengths, record info, tags and
-// final slots, the staging counter and its statistics.
+// then the one-pass training launch's staging: saved PCG states, per-pixel path lengths, lookups, real
+// collisions and lost-record flags, the staged records (query, target, vertex info, tag, snapshot,
+// final slot) and the staging counter.
 struct NrcScratchLayout {
-    size_t counts, offsets, shuffle, rng_save, path_length, info, tag, fmap, counter, stats, total;
+    size_t counts, offsets, shuffle, rng_save, path_length, lookups, scatters, lost;
+    size_t queries, targets, info, tag, snap, fmap, counter, total;
 };
 inline NrcScratchLayout nrc_scratch_layout(uint32_t width, uint32_t height)
 {
     constexpr size_t kShuffleFloats = 9 * (RTC8X_NUM_TRAINING_DATA_PER_FRAME / 256);
-    const size_t px = (size_t)width * height, cap = RTC8X_TRAIN_BUFFER_SIZE;
-    const auto up8 = [](size_t v) { return (v + 7) & ~(size_t)7; };
+    const size_t px = (size_t)width * height, cap = kStageCapacity;
+    const auto up16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
     NrcScratchLayout l;
     l.counts = 0;
     l.offsets = 4 * px;
     l.shuffle = 8 * px;
-    l.rng_save = up8(l.shuffle + 4 * kShuffleFloats);
+    l.rng_save = up16(l.shuffle + 4 * kShuffleFloats);
     l.path_length = l.rng_save + 8 * px;
-    l.info = up8(l.path_length + 4 * px);
+    l.lookups = l.path_length + 4 * px;
+    l.scatters = l.lookups + 4 * px;
+    l.lost = l.scatters + 4 * px;
+    l.queries = up16(l.lost + 4 * px);
+    l.targets = l.queries + sizeof(rtc8x_radiance_query) * cap;
+    l.info = up16(l.targets + 12 * cap);
     l.tag = l.info + sizeof(rtc8x_training_vertex_info) * cap;
-    l.fmap = l.tag + 8 * cap;
-    l.counter = up8(l.fmap + 4 * cap);
-    l.stats = l.counter + 8;
-    l.total = l.stats + 8 * RTC8X_STAT_COUNT;
+    l.snap = l.tag + 8 * cap;
+    l.fmap = l.snap + sizeof(StageSnapshot) * cap;
+    l.counter = up16(l.fmap + 4 * cap);
+    l.total = l.counter + 16;
     return l;
 }
 

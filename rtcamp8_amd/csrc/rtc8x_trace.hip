@@ -35,6 +35,7 @@ constexpr uint32_t kCounterSlots = 1024;
 __device__ uint32_t g_work_counter[kCounterSlots];
 
 __global__ void reset_counter_kernel(uint32_t slot) { g_work_counter[slot] = 0u; }
+__global__ void reset_value_kernel(uint32_t* v) { *v = 0u; }
 
 template <int TRACKING>
 struct Seg;
@@ -65,7 +66,6 @@ struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
     // returns true when the tentative collision is real
     __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu) { return rng.uni() < sigma / mu; }
     __device__ __forceinline__ void after_shadow(Pcg&, float sigma, float mu, float& vis) { vis *= 1.0f - sigma / mu; }
-    __device__ __forceinline__ void skip_shadow(Pcg&) {}
 };
 
 // Local majorants over 8^3 cells, the loops of track_local / ratio_local made resumable.
@@ -126,7 +126,6 @@ struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
         vis *= 1.0f - sigma / mu;
         tau = -rtc8x_logf(1.0f - rng.uni());
     }
-    __device__ __forceinline__ void skip_shadow(Pcg& rng) { tau = -rtc8x_logf(1.0f - rng.uni()); }
 };
 
 enum : uint32_t {
@@ -141,7 +140,7 @@ enum : uint32_t {
 };
 
 struct Lane {
-    uint32_t lx, ly, lin, sample, cost0;
+    uint32_t lx, ly, lin, sample, cost0, scat0;
     Pcg rng;
     V3 org, dir, pos;
     RGB tp, contribution, nee_f;
@@ -152,6 +151,7 @@ struct Lane {
     bool unbiased;
     float primary_spread, cur_sqrt_spread;
     uint32_t prev_idx, records, train_base;
+    bool lost;
     RGB prev_local_tp, nee_c;
 };
 
@@ -232,16 +232,29 @@ __device__ __forceinline__ void speculate_global(const rtc8x_grid& g, Medium& me
     }
 }
 
-// Training records of a one-pass GENERATE_TRAINING_DATA launch: each record takes the next slot of a
-// staging area (train_queries[1], train_targets[1], info, tag = launch pixel and the record's index on
-// its path) in whatever order lanes reach it; stage_map/stage_copy then move every record to its slot
-// in pixel order, the slots the count pass would have assigned.  counter == NULL: the two-pass form.
+// GENERATE_TRAINING_DATA in one pass.  Each record takes the next slot of a staging area (query,
+// target, vertex info, tag = launch pixel and the record's index on its path, and a snapshot of the path
+// at the record) in whatever order lanes reach it, and each pixel leaves its record count; a prefix sum
+// of the counts then gives every record its slot in pixel order, and stage_map/stage_copy move the
+// records there.  A path whose records run past the buffer ends, in pixel-order indexing, at its first
+// record beyond it: the copy rebuilds that end from the record and its snapshot.  When more records come
+// than the staging area holds, the last ones find no slot; their pixels are traced again from their
+// saved PCG states by a redo launch that records with the known offsets.
 struct TrainStage {
-    uint32_t* counter;                   // records staged (may exceed RTC8X_TRAIN_BUFFER_SIZE)
-    rtc8x_training_vertex_info* info;    // RTC8X_TRAIN_BUFFER_SIZE
-    uint32_t* tag;                       // 2 x RTC8X_TRAIN_BUFFER_SIZE
-    uint32_t* path_length;               // per launch pixel
-    uint64_t* stats;                     // the launch's RTC8X_STAT_* sums, added once the records are placed
+    // staging launch (counter != NULL)
+    uint32_t* counter;                   // records staged (may exceed kStageCapacity)
+    rtc8x_radiance_query* queries;       // kStageCapacity each
+    float* targets;
+    rtc8x_training_vertex_info* info;
+    uint32_t* tag;
+    StageSnapshot* snap;
+    uint32_t* path_length;               // per launch pixel: vertices, lookups and real collisions of its
+    uint32_t* lookups;                   //   whole path
+    uint32_t* scatters;
+    uint32_t* lost;                      // per launch pixel: 1 when a record found no staging slot
+    // redo launch (redo_lost != NULL): the pixels with a lost record, from their saved states
+    const uint32_t* redo_lost;
+    const uint64_t* rng_save;            // launch pixel order
 };
 
 // PCG32 jump-ahead: the state j draws later is mul[j] * state + add[j] (the LCG composed j times).
@@ -279,8 +292,7 @@ __device__ __forceinline__ uint64_t lane_u64(uint64_t v, uint32_t lane)
 // PCG stream by jump-ahead, so 64 collisions cost one round of lookups.  The distances are summed and the
 // transmittance factors multiplied in the sequential order (a uniform loop), and the source lane gets
 // the state the sequential loop would end in: the same collision, PCG state, transmittance and lookup
-// count.  COUNT_ONLY ratio tracking consumes the numbers without lookups.
-template <bool COUNT_ONLY>
+// count.
 __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, Seg<RTC8X_TRACKING_GLOBAL_MAJORANT>& sg,
                                            Lane& L, uint64_t todo, uint32_t lane)
 {
@@ -318,7 +330,7 @@ __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, 
             }
             const bool valid = lane < e;
             float sig = 0.0f;
-            if (valid && !(COUNT_ONLY && !track)) sig = density_at(g, med.coeff, o, d, tj);
+            if (valid) sig = density_at(g, med.coeff, o, d, tj);
             if (track) {
                 const uint64_t acc = __ballot(valid && ua < sig / maj);
                 if (acc) {
@@ -336,11 +348,9 @@ __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, 
                     break;
                 }
             } else {
-                if (!COUNT_ONLY) {
-                    const float f = 1.0f - sig / maj;
-                    for (uint32_t i = 0; i < e; ++i) vis *= lane_f(f, i);
-                    looked += e;
-                }
+                const float f = 1.0f - sig / maj;
+                for (uint32_t i = 0; i < e; ++i) vis *= lane_f(f, i);
+                looked += e;
                 if (e < 64) {
                     st = lane_u64(st_after, e);
                     mode = kModeNeeDone;
@@ -362,10 +372,9 @@ __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, 
 
 // pathTrace_generic<withNRC, genData> (optix_pathtracing_kernels.cu:31-434) for volume-only scenes, as
 // a per-lookup state machine over a persistent grid.  ENTRY selects the ray-generation program;
-// COUNT_ONLY (GENERATE_TRAINING_DATA) traces the paths without writing anything and stores each
-// pixel's number of training-record attempts, from which the records' indices are assigned in pixel
-// order (record_offsets) before the writing pass.
-template <int ENTRY, int TRACKING, bool COUNT_ONLY>
+// GENERATE_TRAINING_DATA runs as the staging launch or the redo launch (TrainStage; the redo records at
+// record_offsets).
+template <int ENTRY, int TRACKING>
 __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
                                                         uint32_t launch_h, uint32_t num_samples, uint32_t slot,
                                                         const uint32_t* __restrict__ record_offsets,
@@ -374,8 +383,9 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
     constexpr bool with_nrc = ENTRY != RTC8X_ENTRY_PATH_TRACE;
     constexpr bool gen_data = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA;
     const rtc8x_nrc_buffers& nb = p.nrc;
-    const bool staged = gen_data && !COUNT_ONLY && stage.counter != nullptr;
-    float* const train_targets = staged ? nb.train_targets[1] : nb.train_targets[0];
+    const bool staged = gen_data && stage.counter != nullptr;
+    const bool redo = gen_data && stage.redo_lost != nullptr;
+    float* const train_targets = staged ? stage.targets : nb.train_targets[0];
     const uint32_t W = launch_w, H = launch_h;
     const uint32_t tiles_x = (W + 7) / 8;
     // GENERATE_TRAINING_DATA hands out the unbiased-suffix pixels first (the one pixel of each 4x4 block
@@ -446,8 +456,10 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                     if (gen_data && (L.lx & 3u) == first_rx && (L.ly & 3u) == first_ry) continue;  // taken first
                 }
                 L.lin = L.ly * W + L.lx;
+                if (redo && !stage.redo_lost[L.lin]) continue;  // all of its records were staged
                 L.cost0 = med.lookups;
-                L.rng.state = p.rng_states[(size_t)L.ly * p.image_width + L.lx];
+                L.scat0 = scatters;
+                L.rng.state = redo ? stage.rng_save[L.lin] : p.rng_states[(size_t)L.ly * p.image_width + L.lx];
                 L.sample = 0;
                 if (gen_data) L.train_base = record_offsets ? record_offsets[L.lin] : 0u;
                 L.mode = kModeStart;
@@ -480,6 +492,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                     L.prev_idx = RTC8X_INVALID_VERTEX_INDEX;
                     L.prev_local_tp = RGB{0.0f, 0.0f, 0.0f};
                     L.records = 0;
+                    L.lost = false;
                 }
                 L.mode = start_segment<TRACKING>(p.grid, med, sg, L.rng, L.org, L.dir, 1e10f) ? kModeTrack : kModeEscape;
             }
@@ -497,7 +510,7 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                     }
                     const RGB implicit_cont = scale(radiance, mis);
                     if (!gen_data) L.contribution = add(L.contribution, mul(L.tp, implicit_cont));
-                    if (gen_data && !COUNT_ONLY && L.prev_idx != RTC8X_INVALID_VERTEX_INDEX) {
+                    if (gen_data && L.prev_idx != RTC8X_INVALID_VERTEX_INDEX) {
                         float* tgt = train_targets + 3 * (size_t)L.prev_idx;
                         const RGB a = mul(L.prev_local_tp, implicit_cont);
                         tgt[0] = tgt[0] + a.r;
@@ -505,22 +518,20 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                         tgt[2] = tgt[2] + a.b;
                     }
                 }
-                if (!COUNT_ONLY) {
-                    if (gen_data) {
-                        rtc8x_training_suffix_terminal ti;
-                        ti.prev_vertex_index = L.prev_idx;
-                        ti.has_query = 0;
-                        ti.path_length = L.path_length & 0xFFu;
-                        ti.is_unbiased = L.unbiased ? 1u : 0u;
-                        nb.train_suffix_terminals[L.lin] = ti;
-                    } else if (with_nrc) {
-                        rtc8x_terminal_info ti;
-                        ti.throughput[0] = L.tp.r;
-                        ti.throughput[1] = L.tp.g;
-                        ti.throughput[2] = L.tp.b;
-                        ti.has_query = (L.path_length & 0xFFu) << 8;
-                        nb.inference_terminals[(size_t)L.sample * p_stride + L.lin] = ti;
-                    }
+                if (gen_data) {
+                    rtc8x_training_suffix_terminal ti;
+                    ti.prev_vertex_index = L.prev_idx;
+                    ti.has_query = 0;
+                    ti.path_length = L.path_length & 0xFFu;
+                    ti.is_unbiased = L.unbiased ? 1u : 0u;
+                    nb.train_suffix_terminals[L.lin] = ti;
+                } else if (with_nrc) {
+                    rtc8x_terminal_info ti;
+                    ti.throughput[0] = L.tp.r;
+                    ti.throughput[1] = L.tp.g;
+                    ti.throughput[2] = L.tp.b;
+                    ti.has_query = (L.path_length & 0xFFu) << 8;
+                    nb.inference_terminals[(size_t)L.sample * p_stride + L.lin] = ti;
                 }
                 path_end = true;
             } else if (L.mode == kModeNeeDone) {
@@ -566,24 +577,22 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                         ends_with_cache |= L.cur_sqrt_spread * L.cur_sqrt_spread > pt * L.primary_spread;
                     }
                     if (ends_with_cache) {
-                        if (!COUNT_ONLY) {
-                            const V3 vout{-L.dir.x, -L.dir.y, -L.dir.z};
-                            make_query(p, L.pos, vout, &nb.inference_queries[(size_t)L.sample * q_stride + L.lin]);
-                            if (gen_data) {
-                                rtc8x_training_suffix_terminal ti;
-                                ti.prev_vertex_index = L.prev_idx;
-                                ti.has_query = 1;
-                                ti.path_length = L.path_length & 0xFFu;
-                                ti.is_unbiased = L.unbiased ? 1u : 0u;
-                                nb.train_suffix_terminals[L.lin] = ti;
-                            } else {
-                                rtc8x_terminal_info ti;
-                                ti.throughput[0] = L.tp.r;
-                                ti.throughput[1] = L.tp.g;
-                                ti.throughput[2] = L.tp.b;
-                                ti.has_query = 1u | ((L.path_length & 0xFFu) << 8);
-                                nb.inference_terminals[(size_t)L.sample * p_stride + L.lin] = ti;
-                            }
+                        const V3 vout{-L.dir.x, -L.dir.y, -L.dir.z};
+                        make_query(p, L.pos, vout, &nb.inference_queries[(size_t)L.sample * q_stride + L.lin]);
+                        if (gen_data) {
+                            rtc8x_training_suffix_terminal ti;
+                            ti.prev_vertex_index = L.prev_idx;
+                            ti.has_query = 1;
+                            ti.path_length = L.path_length & 0xFFu;
+                            ti.is_unbiased = L.unbiased ? 1u : 0u;
+                            nb.train_suffix_terminals[L.lin] = ti;
+                        } else {
+                            rtc8x_terminal_info ti;
+                            ti.throughput[0] = L.tp.r;
+                            ti.throughput[1] = L.tp.g;
+                            ti.throughput[2] = L.tp.b;
+                            ti.has_query = 1u | ((L.path_length & 0xFFu) << 8);
+                            nb.inference_terminals[(size_t)L.sample * p_stride + L.lin] = ti;
                         }
                         path_end = true;
                     }
@@ -693,47 +702,52 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                             // training vertex record (optix_pathtracing_kernels.cu:358-410)
                             const uint32_t local = L.records;
                             L.records++;
-                            if (!COUNT_ONLY) {
-                                rtc8x_radiance_query q;
-                                make_query(p, L.pos, vout, &q);
-                                // the record's slot: in pixel order (two-pass), or the next staging slot
-                                const uint32_t idx = staged ? atomicAdd(stage.counter, 1u) : L.train_base + local;
-                                if (staged && idx >= RTC8X_TRAIN_BUFFER_SIZE) {
-                                    // more records than slots: the launch is redone in two passes, which
-                                    // end paths at the capacity; until then this path goes on unrecorded
-                                    L.prev_idx = RTC8X_INVALID_VERTEX_INDEX;
-                                } else if (idx < RTC8X_TRAIN_BUFFER_SIZE) {
-                                    (staged ? nb.train_queries[1] : nb.train_queries[0])[idx] = q;
-                                    rtc8x_training_vertex_info vi;
-                                    vi.local_throughput[0] = local_tp.r;
-                                    vi.local_throughput[1] = local_tp.g;
-                                    vi.local_throughput[2] = local_tp.b;
-                                    vi.prev_vertex_index = L.prev_idx;
-                                    vi.path_length = L.path_length & 0xFFu;
-                                    vi.is_unbiased = L.unbiased ? 1u : 0u;
-                                    if (staged) {
-                                        stage.info[idx] = vi;
-                                        stage.tag[2 * (size_t)idx] = L.lin;
-                                        stage.tag[2 * (size_t)idx + 1] = local;
-                                    } else {
-                                        nb.train_vertex_info[idx] = vi;
-                                    }
-                                    float* tgt = train_targets + 3 * (size_t)idx;
-                                    tgt[0] = L.nee_c.r;
-                                    tgt[1] = L.nee_c.g;
-                                    tgt[2] = L.nee_c.b;
-                                    L.prev_local_tp = local_tp;
-                                    L.prev_idx = idx;
+                            rtc8x_radiance_query q;
+                            make_query(p, L.pos, vout, &q);
+                            // the record's slot: the next staging slot, or in pixel order (redo launch)
+                            const uint32_t idx = staged ? atomicAdd(stage.counter, 1u) : L.train_base + local;
+                            if (staged && idx >= kStageCapacity) {
+                                // no staging slot left: the redo launch traces this pixel again
+                                L.prev_idx = RTC8X_INVALID_VERTEX_INDEX;
+                                L.lost = true;
+                            } else if (staged || idx < RTC8X_TRAIN_BUFFER_SIZE) {
+                                (staged ? stage.queries : nb.train_queries[0])[idx] = q;
+                                rtc8x_training_vertex_info vi;
+                                vi.local_throughput[0] = local_tp.r;
+                                vi.local_throughput[1] = local_tp.g;
+                                vi.local_throughput[2] = local_tp.b;
+                                vi.prev_vertex_index = L.prev_idx;
+                                vi.path_length = L.path_length & 0xFFu;
+                                vi.is_unbiased = L.unbiased ? 1u : 0u;
+                                if (staged) {
+                                    stage.info[idx] = vi;
+                                    stage.tag[2 * (size_t)idx] = L.lin;
+                                    stage.tag[2 * (size_t)idx + 1] = local;
+                                    StageSnapshot sn;
+                                    sn.rng = L.rng.state;
+                                    sn.path_length = L.path_length;
+                                    sn.lookups = med.lookups - L.cost0;
+                                    sn.scatters = scatters - L.scat0;
+                                    sn.pad = 0;
+                                    stage.snap[idx] = sn;
                                 } else {
-                                    nb.inference_queries[L.lin] = q;
-                                    rtc8x_training_suffix_terminal ti;
-                                    ti.prev_vertex_index = L.prev_idx;
-                                    ti.has_query = 1;
-                                    ti.path_length = L.path_length & 0xFFu;
-                                    ti.is_unbiased = L.unbiased ? 1u : 0u;
-                                    nb.train_suffix_terminals[L.lin] = ti;
-                                    record_end = true;
+                                    nb.train_vertex_info[idx] = vi;
                                 }
+                                float* tgt = train_targets + 3 * (size_t)idx;
+                                tgt[0] = L.nee_c.r;
+                                tgt[1] = L.nee_c.g;
+                                tgt[2] = L.nee_c.b;
+                                L.prev_local_tp = local_tp;
+                                L.prev_idx = idx;
+                            } else {
+                                nb.inference_queries[L.lin] = q;
+                                rtc8x_training_suffix_terminal ti;
+                                ti.prev_vertex_index = L.prev_idx;
+                                ti.has_query = 1;
+                                ti.path_length = L.path_length & 0xFFu;
+                                ti.is_unbiased = L.unbiased ? 1u : 0u;
+                                nb.train_suffix_terminals[L.lin] = ti;
+                                record_end = true;
                             }
                         }
                         if (record_end) {
@@ -750,32 +764,33 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             if (path_end) {
                 paths++;
                 vertices += L.path_length;
-                if (COUNT_ONLY || staged) record_counts[L.lin] = L.records;
-                if (staged) stage.path_length[L.lin] = L.path_length;
-                if (!COUNT_ONLY) {
-                    const bool ok = all_nonneg_finite(L.contribution);
-                    if (ENTRY == RTC8X_ENTRY_PATH_TRACE) {
-                        if (ok) {
-                            double* a = p.accum + 3 * (size_t)L.lin;
-                            a[0] += (double)L.contribution.r;
-                            a[1] += (double)L.contribution.g;
-                            a[2] += (double)L.contribution.b;
-                        }
-                    } else if (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
-                        float* pf = nb.per_frame_contribution + 3 * ((size_t)L.sample * p_stride + L.lin);
-                        pf[0] = ok ? L.contribution.r : 0.0f;
-                        pf[1] = ok ? L.contribution.g : 0.0f;
-                        pf[2] = ok ? L.contribution.b : 0.0f;
-                    }
-                    if (p.path_vertices && !staged) p.path_vertices[L.lin] += L.path_length;
+                if (staged) record_counts[L.lin] = L.records;
+                if (staged) {
+                    stage.path_length[L.lin] = L.path_length;
+                    stage.lookups[L.lin] = med.lookups - L.cost0;
+                    stage.scatters[L.lin] = scatters - L.scat0;
+                    stage.lost[L.lin] = L.lost ? 1u : 0u;
                 }
+                const bool ok = all_nonneg_finite(L.contribution);
+                if (ENTRY == RTC8X_ENTRY_PATH_TRACE) {
+                    if (ok) {
+                        double* a = p.accum + 3 * (size_t)L.lin;
+                        a[0] += (double)L.contribution.r;
+                        a[1] += (double)L.contribution.g;
+                        a[2] += (double)L.contribution.b;
+                    }
+                } else if (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
+                    float* pf = nb.per_frame_contribution + 3 * ((size_t)L.sample * p_stride + L.lin);
+                    pf[0] = ok ? L.contribution.r : 0.0f;
+                    pf[1] = ok ? L.contribution.g : 0.0f;
+                    pf[2] = ok ? L.contribution.b : 0.0f;
+                }
+                if (p.path_vertices && !staged) p.path_vertices[L.lin] += L.path_length;
                 if (++L.sample < num_samples) {
                     L.mode = kModeStart;
                 } else {
-                    if (!COUNT_ONLY) {
-                        p.rng_states[(size_t)L.ly * p.image_width + L.lx] = L.rng.state;
-                        if (p.pixel_cost) p.pixel_cost[L.lin] = med.lookups - L.cost0;
-                    }
+                    p.rng_states[(size_t)L.ly * p.image_width + L.lx] = L.rng.state;
+                    if (p.pixel_cost) p.pixel_cost[L.lin] = med.lookups - L.cost0;
                     L.mode = kModeWork;
                 }
             }
@@ -785,15 +800,9 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
         if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
             const uint64_t steppers = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
             coop = steppers != 0 && __popcll(__ballot(L.mode == kModeIdle)) >= kCoopIdle;
-            if (coop) coop_segments<COUNT_ONLY>(p.grid, med, sg, L, steppers, lane_id);
+            if (coop) coop_segments(p.grid, med, sg, L, steppers, lane_id);
         }
         if (coop) {
-        } else if (COUNT_ONLY && L.mode == kModeShadow) {
-            // The count pass writes no targets, so the shadow ray's visibility is not needed; its random
-            // numbers are consumed exactly as ratio tracking consumes them (they do not depend on the
-            // densities), without the lookups.
-            while (sg.advance(p.grid, med, L.rng, mu)) sg.skip_shadow(L.rng);
-            L.mode = kModeNeeDone;
         } else if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {
             // one copy of the lookup code serves both segment kinds: the kernel's instruction footprint
             // matters (a copy per kind made the full-image launch half again as slow)
@@ -812,8 +821,8 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
             }
         }
     }
-    uint64_t* const stats = staged ? stage.stats : p.stats;
-    if (stats && !COUNT_ONLY) {
+    uint64_t* const stats = staged ? nullptr : p.stats;  // staged: from the per-pixel sums
+    if (stats) {
         uint64_t v[4] = {paths, vertices, med.lookups, scatters};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -859,49 +868,79 @@ __global__ __launch_bounds__(1024) void record_prefix_kernel(const uint32_t* __r
 
 
 
-// One-pass training launch: clear the staging counter and statistics.
-__global__ void stage_init_kernel(TrainStage st)
-{
-    if (threadIdx.x == 0) *st.counter = 0u;
-    if (threadIdx.x < RTC8X_STAT_COUNT) st.stats[threadIdx.x] = 0u;
-}
-
-// Final slot of every staged record: its pixel's offset plus its index on the path.  Nothing to do when
-// the records overflowed the buffer (the launch is then redone in two passes).
+// Final slot of every staged record: its pixel's offset plus its index on the path.
 __global__ __launch_bounds__(256) void stage_map_kernel(const uint32_t* __restrict__ total, const uint32_t* __restrict__ tag,
                                                         const uint32_t* __restrict__ offsets, uint32_t* __restrict__ fmap)
 {
-    const uint32_t n = *total;
-    if (n > RTC8X_TRAIN_BUFFER_SIZE) return;
+    const uint32_t n = (uint32_t)min((size_t)*total, kStageCapacity);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         fmap[i] = offsets[tag[2 * (size_t)i]] + tag[2 * (size_t)i + 1];
 }
 
-// Moves the staged records to their final slots (vertex links renamed to final slots), renames the
-// training-suffix terminals' links, adds the path lengths and the launch's statistics.
-__global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_nrc_buffers nb, const uint32_t* __restrict__ total,
+// Moves the staged records to their final slots (vertex links renamed), and finishes every pixel whose
+// records were all staged: a pixel whose records all fit keeps its path's end (its suffix terminal's
+// link renamed, its statistics added); a pixel whose records run past the buffer ends at its first
+// record past it (pixel-order slot max(offset, capacity)), as in recordGeneration's full-buffer branch:
+// that record's query becomes the suffix query, and the PCG state, path length and statistics are the
+// record's snapshot.  Pixels with a lost record are left to the redo launch.
+__global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_frame_params p, const uint32_t* __restrict__ total,
                                                          const TrainStage st, const uint32_t* __restrict__ fmap,
-                                                         uint32_t npx, uint32_t* __restrict__ path_vertices,
-                                                         uint64_t* __restrict__ stats)
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ counts, uint32_t tw, uint32_t npx)
 {
-    const uint32_t n = *total;
-    if (n > RTC8X_TRAIN_BUFFER_SIZE) return;
+    const rtc8x_nrc_buffers& nb = p.nrc;
+    const uint32_t n = (uint32_t)min((size_t)*total, kStageCapacity);
     const uint32_t stride = gridDim.x * blockDim.x;
+    uint64_t sum[4] = {0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t f = fmap[i];
-        nb.train_queries[0][f] = nb.train_queries[1][i];
-        rtc8x_training_vertex_info vi = st.info[i];
-        if (vi.prev_vertex_index != RTC8X_INVALID_VERTEX_INDEX) vi.prev_vertex_index = fmap[vi.prev_vertex_index];
-        nb.train_vertex_info[f] = vi;
+        if (f < RTC8X_TRAIN_BUFFER_SIZE) {
+            nb.train_queries[0][f] = st.queries[i];
+            rtc8x_training_vertex_info vi = st.info[i];
+            if (vi.prev_vertex_index != RTC8X_INVALID_VERTEX_INDEX) vi.prev_vertex_index = fmap[vi.prev_vertex_index];
+            nb.train_vertex_info[f] = vi;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nb.train_targets[0][3 * (size_t)f + c] = nb.train_targets[1][3 * (size_t)i + c];
+            for (int c = 0; c < 3; ++c) nb.train_targets[0][3 * (size_t)f + c] = st.targets[3 * (size_t)i + c];
+            continue;
+        }
+        const uint32_t pix = st.tag[2 * (size_t)i], local = st.tag[2 * (size_t)i + 1];
+        if (st.lost[pix] || f != max(offsets[pix], RTC8X_TRAIN_BUFFER_SIZE)) continue;
+        // the cut: this record's vertex ends the path with a cache query
+        const rtc8x_training_vertex_info vi = st.info[i];
+        const StageSnapshot sn = st.snap[i];
+        nb.inference_queries[pix] = st.queries[i];
+        rtc8x_training_suffix_terminal ti;
+        ti.prev_vertex_index = local ? f - 1u : RTC8X_INVALID_VERTEX_INDEX;
+        ti.has_query = 1;
+        ti.path_length = vi.path_length;
+        ti.is_unbiased = vi.is_unbiased;
+        nb.train_suffix_terminals[pix] = ti;
+        p.rng_states[(size_t)(pix / tw) * p.image_width + pix % tw] = sn.rng;
+        if (p.path_vertices) p.path_vertices[pix] += sn.path_length;
+        sum[0] += 1;
+        sum[1] += sn.path_length;
+        sum[2] += sn.lookups;
+        sum[3] += sn.scatters;
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += stride) {
+        if (st.lost[i] || (uint64_t)offsets[i] + counts[i] > RTC8X_TRAIN_BUFFER_SIZE) continue;
         const uint32_t prev = nb.train_suffix_terminals[i].prev_vertex_index;
         if (prev != RTC8X_INVALID_VERTEX_INDEX) nb.train_suffix_terminals[i].prev_vertex_index = fmap[prev];
-        if (path_vertices) path_vertices[i] += st.path_length[i];
+        if (p.path_vertices) p.path_vertices[i] += st.path_length[i];
+        sum[0] += 1;
+        sum[1] += st.path_length[i];
+        sum[2] += st.lookups[i];
+        sum[3] += st.scatters[i];
     }
-    if (stats && blockIdx.x == 0 && threadIdx.x < RTC8X_STAT_COUNT) stats[threadIdx.x] += st.stats[threadIdx.x];
+    if (p.stats) {
+        const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint64_t x = sum[q];
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            if (lane == 0 && x) atomicAdd((unsigned long long*)&p.stats[q], (unsigned long long)x);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- cost-ordered work lists
@@ -994,7 +1033,7 @@ __global__ __launch_bounds__(256) void order_scatter_kernel(const uint32_t* __re
 // One persistent launch: as many 256-lane workgroups as fit on the device at once, a fresh work
 // counter slot (cleared by a one-lane kernel on the same stream; launches on different streams take
 // different slots).
-template <int ENTRY, int TRACKING, bool COUNT_ONLY>
+template <int ENTRY, int TRACKING>
 hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t num_samples, const uint32_t* offsets,
                   uint32_t* counts, const TrainStage& stage, hipStream_t stream)
 {
@@ -1005,22 +1044,22 @@ hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t 
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wavefront_kernel<ENTRY, TRACKING, COUNT_ONLY>, 256, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wavefront_kernel<ENTRY, TRACKING>, 256, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, slot);
-    hipLaunchKernelGGL((wavefront_kernel<ENTRY, TRACKING, COUNT_ONLY>), dim3((uint32_t)(cus * per_cu)), dim3(256), 0,
+    hipLaunchKernelGGL((wavefront_kernel<ENTRY, TRACKING>), dim3((uint32_t)(cus * per_cu)), dim3(256), 0,
                        stream, p, w, h, num_samples, slot, offsets, counts, stage);
     return hipGetLastError();
 }
 
-template <int ENTRY, bool COUNT_ONLY>
+template <int ENTRY>
 hipError_t launch_tracking(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t ns, const uint32_t* offsets,
                            uint32_t* counts, hipStream_t stream, const TrainStage& stage = TrainStage{})
 {
     if (p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT)
-        return launch<ENTRY, RTC8X_TRACKING_LOCAL_MAJORANT, COUNT_ONLY>(p, w, h, ns, offsets, counts, stage, stream);
-    return launch<ENTRY, RTC8X_TRACKING_GLOBAL_MAJORANT, COUNT_ONLY>(p, w, h, ns, offsets, counts, stage, stream);
+        return launch<ENTRY, RTC8X_TRACKING_LOCAL_MAJORANT>(p, w, h, ns, offsets, counts, stage, stream);
+    return launch<ENTRY, RTC8X_TRACKING_GLOBAL_MAJORANT>(p, w, h, ns, offsets, counts, stage, stream);
 }
 
 }  // namespace dev
@@ -1115,7 +1154,7 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
                   "rtc8x_trace: work_order / pixel_cost are for the full-image entries (the training size varies)");
     hipStream_t s = (hipStream_t)stream;
     if (entry == RTC8X_ENTRY_PATH_TRACE) {
-        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE, false>(p, p.image_width, p.image_height, num_samples,
+        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE>(p, p.image_width, p.image_height, num_samples,
                                                                         nullptr, nullptr, s)));
         return RTC8X_OK;
     }
@@ -1124,7 +1163,7 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
                   "rtc8x_trace: NRC buffers missing");
     if (entry == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
         RTC8X_REQUIRE(nb.inference_terminals && nb.per_frame_contribution, "rtc8x_trace: NRC buffers missing");
-        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, false>(p, p.image_width, p.image_height, num_samples,
+        RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE_WITH_NRC>(p, p.image_width, p.image_height, num_samples,
                                                                                  nullptr, nullptr, s)));
         return RTC8X_OK;
     }
@@ -1144,41 +1183,39 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
     uint32_t* counts = reinterpret_cast<uint32_t*>(sb + lay.counts);
     uint32_t* offsets = reinterpret_cast<uint32_t*>(sb + lay.offsets);
     uint32_t* total = nb.num_training_data + nb.nrc_buffer_index;
-    // One pass: records are staged as the paths reach them, then moved to their pixel-order slots.  The
-    // record indices, and with them every result, are those of the two-pass form below (count pass,
-    // prefix sum, recording pass) as long as the records fit the buffer; when they do not, the
-    // training pixels' PCG states are restored and the two passes run.
+    // One pass: records are staged as the paths reach them, then moved to their pixel-order slots, paths
+    // whose records run past the buffer are cut there, and the pixels with a record that found no
+    // staging slot are traced again.  Record indices, and with them every result, are those of
+    // pixel-order assignment with paths ending at the buffer's capacity.
     uint64_t* rng_save = reinterpret_cast<uint64_t*>(sb + lay.rng_save);
-    uint64_t* const rng = p.rng_states;
-    RTC8X_HIP(hipMemcpy2DAsync(rng_save, 8 * (size_t)tw, rng, 8 * (size_t)p.image_width, 8 * (size_t)tw, th,
+    RTC8X_HIP(hipMemcpy2DAsync(rng_save, 8 * (size_t)tw, p.rng_states, 8 * (size_t)p.image_width, 8 * (size_t)tw, th,
                                hipMemcpyDeviceToDevice, s));
-    dev::TrainStage stage;
+    dev::TrainStage stage{};
     stage.counter = reinterpret_cast<uint32_t*>(sb + lay.counter);
+    stage.queries = reinterpret_cast<rtc8x_radiance_query*>(sb + lay.queries);
+    stage.targets = reinterpret_cast<float*>(sb + lay.targets);
     stage.info = reinterpret_cast<rtc8x_training_vertex_info*>(sb + lay.info);
     stage.tag = reinterpret_cast<uint32_t*>(sb + lay.tag);
+    stage.snap = reinterpret_cast<StageSnapshot*>(sb + lay.snap);
     stage.path_length = reinterpret_cast<uint32_t*>(sb + lay.path_length);
-    stage.stats = reinterpret_cast<uint64_t*>(sb + lay.stats);
+    stage.lookups = reinterpret_cast<uint32_t*>(sb + lay.lookups);
+    stage.scatters = reinterpret_cast<uint32_t*>(sb + lay.scatters);
+    stage.lost = reinterpret_cast<uint32_t*>(sb + lay.lost);
     uint32_t* fmap = reinterpret_cast<uint32_t*>(sb + lay.fmap);
-    hipLaunchKernelGGL(dev::stage_init_kernel, dim3(1), dim3(64), 0, s, stage);
+    hipLaunchKernelGGL(dev::reset_value_kernel, dim3(1), dim3(1), 0, s, stage.counter);
     RTC8X_HIP(hipGetLastError());
-    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA, false>(p, tw, th, 1, nullptr, counts, s, stage)));
+    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, tw, th, 1, nullptr, counts, s, stage)));
     hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, counts, offsets, n, total);
     RTC8X_HIP(hipGetLastError());
     constexpr uint32_t kCompactBlocks = 512;
     hipLaunchKernelGGL(dev::stage_map_kernel, dim3(kCompactBlocks), dim3(256), 0, s, total, stage.tag, offsets, fmap);
     RTC8X_HIP(hipGetLastError());
-    hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, nb, total, stage, fmap, n,
-                       p.path_vertices, p.stats);
+    hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, p, total, stage, fmap, offsets,
+                       counts, tw, n);
     RTC8X_HIP(hipGetLastError());
-    uint32_t records = 0;
-    RTC8X_HIP(hipMemcpyAsync(&records, total, sizeof(records), hipMemcpyDeviceToHost, s));
-    RTC8X_HIP(hipStreamSynchronize(s));
-    if (records <= RTC8X_TRAIN_BUFFER_SIZE) return RTC8X_OK;
-    RTC8X_HIP(hipMemcpy2DAsync(rng, 8 * (size_t)p.image_width, rng_save, 8 * (size_t)tw, 8 * (size_t)tw, th,
-                               hipMemcpyDeviceToDevice, s));
-    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA, true>(p, tw, th, 1, nullptr, counts, s)));
-    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, counts, offsets, n, total);
-    RTC8X_HIP(hipGetLastError());
-    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA, false>(p, tw, th, 1, offsets, nullptr, s)));
+    dev::TrainStage redo{};
+    redo.redo_lost = stage.lost;
+    redo.rng_save = rng_save;
+    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, tw, th, 1, offsets, nullptr, s, redo)));
     return RTC8X_OK;
 }

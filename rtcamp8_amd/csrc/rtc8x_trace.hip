@@ -418,13 +418,13 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
     Seg<TRACKING> sg;
     float mu = 0.0f;
     for (;;) {
-        // Vertex events are batched: a lane that reaches one parks until a quarter of the wavefront's active
+        // Vertex events are batched: a lane that reaches one parks until half of the wavefront's active
         // lanes have, then all of them run their events together (each event block is issued once for
         // many lanes instead of once per lane), while the others keep stepping through lookups.
         const uint64_t parked = __ballot(L.mode != kModeTrack && L.mode != kModeShadow && L.mode != kModeIdle);
         const uint64_t stepping = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
         if (parked == 0 && stepping == 0) break;
-        const bool event_pass = stepping == 0 || 4 * __popcll(parked) >= __popcll(parked | stepping);
+        const bool event_pass = stepping == 0 || 2 * __popcll(parked) >= __popcll(parked | stepping);
         // ---- events until this lane has a tracking or shadow segment under way, or no work left
         while (event_pass && L.mode != kModeIdle && L.mode != kModeTrack && L.mode != kModeShadow) {
             if (L.mode == kModeWork) {

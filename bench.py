@@ -30,6 +30,7 @@ METRIC = "path vertices/sec + NRC queries/sec; wall time to equal RMSE vs FP64 o
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LOOKUP = 32  # 8 FP32 voxel taps per trilinear lookup
 FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
+FP8_PEAK_TFLOPS = 5000.0  # MI355X dense FP8 MFMA
 # NRC settings of the equal-RMSE leg: the reference's network (hash grid, 2 hidden layers, lr 1e-2) with
 # its radiance-scale knob (renderer_main.cpp:979-995) set so the cloud's radiance (~0.03-0.3) sits well
 # above the loss's 0.01 epsilon, and a shorter weight EMA than the reference's 0.99 (the benchmark
@@ -128,10 +129,11 @@ def random_leaf_queries(scene, n: int, seed: int) -> np.ndarray:
 
 
 def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 21, n_train: int = 1 << 16,
-                encoding: int = 0):
+                encoding: int = 0, precision: int | None = None):
     """Standalone network step: train on 2^16 samples, then infer 2^21 queries (FP16 storage, FP32
-    accumulation); the benchmark configuration is 5 hidden layers of 64 with the frequency (triangle
-    wave) encoding, the renderer's default 2 hidden layers with the hash grid."""
+    accumulation, or BF16 training with E4M3 inference); the benchmark configuration is 5 hidden layers
+    of 64 with the frequency (triangle wave) encoding, the renderer's default 2 hidden layers with the
+    hash grid."""
     from rtcamp8_amd import _abi
     from rtcamp8_amd.device import DeviceBuffer, EventTimer
     from rtcamp8_amd.nrc import NeuralRadianceCache
@@ -140,7 +142,9 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     t = (0.2 + np.random.default_rng(99).random((n_train, 3))).astype(np.float32)
     dq, dt, dout = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t), DeviceBuffer(12 * n_infer)
     cache = NeuralRadianceCache()
-    cache.initialize(num_hidden_layers=hidden_layers, learning_rate=1e-2, precision=_abi.NRC_FP16, encoding=encoding)
+    precision = _abi.NRC_FP16 if precision is None else precision
+    fp8 = precision == _abi.NRC_BF16_FP8
+    cache.initialize(num_hidden_layers=hidden_layers, learning_rate=1e-2, precision=precision, encoding=encoding)
     for _ in range(2):
         cache.train(None, dq.ptr, dt.ptr, n_train)
         cache.infer(None, dq.ptr, n_infer, dout.ptr)
@@ -164,7 +168,8 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     fwd_flop = 2.0 * (hidden_layers * 64 * 64 + 64 * 3)
     train_flop = 3.0 * fwd_flop  # forward + backward (input and weight gradients)
     return {
-        "hidden_layers": hidden_layers, "precision": "fp16 storage, fp32 accumulate",
+        "hidden_layers": hidden_layers,
+        "precision": "bf16 training, e4m3 inference, fp32 accumulate" if fp8 else "fp16 storage, fp32 accumulate",
         "encoding": "hash grid (16 levels x 2, 2^15)" if encoding == 1 else "triangle waves (12 frequencies)",
         "infer_queries": n_infer, "train_samples": n_train,
         "infer_ms": infer_ms, "train_ms": train_ms,
@@ -172,8 +177,8 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
         "train_samples_per_s": n_train / (train_ms * 1e-3),
         "infer_tflops": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12,
         "train_tflops": n_train * train_flop / (train_ms * 1e-3) / 1e12,
-        "mfma_peak_tflops": FP16_PEAK_TFLOPS,
-        "infer_frac": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12 / FP16_PEAK_TFLOPS,
+        "mfma_peak_tflops": FP8_PEAK_TFLOPS if fp8 else FP16_PEAK_TFLOPS,
+        "infer_frac": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12 / (FP8_PEAK_TFLOPS if fp8 else FP16_PEAK_TFLOPS),
         "finite": ok,
     }
 
@@ -269,7 +274,7 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
            "nrc_settings": {"encoding": "hash grid", "hidden_layers": 2, "radiance_scale": NRC_RADIANCE_SCALE,
                             "ema_decay": NRC_EMA_DECAY},
            "spp": spp, "plain_seconds": tx, "plain_rel_rmse": float(np.sqrt(max(mse_plain, 0.0)) / mean)}
-    for name, prec in (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16)):
+    for name, prec in (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16), ("bf16_fp8", _abi.NRC_BF16_FP8)):
         run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec, spp_per_frame=4,
                         encoding=_abi.NRC_HASH_GRID, radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY)
         mse = cross_mse(run.image, a, b)
@@ -403,19 +408,22 @@ def main() -> int:
         print("[bench] NRC network leg ...", file=sys.stderr, flush=True)
         mlp = nrc_mlp_leg(scene, args.nrc_steps)
         mlp_hash = nrc_mlp_leg(scene, args.nrc_steps, hidden_layers=2, encoding=1)
-        print(f"[bench] NRC network: infer {mlp['infer_ms']:.2f} ms / 2^21, train {mlp['train_ms']:.2f} ms / 2^16",
-              file=sys.stderr, flush=True)
+        mlp_fp8 = nrc_mlp_leg(scene, args.nrc_steps, precision=_abi.NRC_BF16_FP8)
+        print(f"[bench] NRC network: infer {mlp['infer_ms']:.2f} ms / 2^21, train {mlp['train_ms']:.2f} ms / 2^16; "
+              f"e4m3 inference {mlp_fp8['infer_ms']:.2f} ms", file=sys.stderr, flush=True)
         print("[bench] NRC render leg ...", file=sys.stderr, flush=True)
         rend = nrc_render_leg(scene, args.nrc_steps, stream)
         print(f"[bench] NRC render: {rend['ms_per_spp']:.2f} ms/spp", file=sys.stderr, flush=True)
         result["nrc_queries_per_s"] = mlp["queries_per_s"]
-        result["nrc"] = {"network": mlp, "network_hash_grid": mlp_hash, "render": rend}
+        result["nrc"] = {"network": mlp, "network_hash_grid": mlp_hash, "network_fp8": mlp_fp8, "render": rend}
     if args.rmse_ref_spp > 0:
         print("[bench] equal-RMSE leg ...", file=sys.stderr, flush=True)
         eq = equal_rmse_leg(scene, args.rmse_ref_spp, cfg.spp, stream)
         result["equal_rmse"] = eq
         print(f"[bench] equal RMSE: plain rel RMSE {eq['plain_rel_rmse']:.4f} at {cfg.spp} spp; NRC fp16 "
-              f"{eq['fp16']['nrc_rel_rmse']:.4f} -> speedup {eq['fp16']['speedup']:.2f}x", file=sys.stderr, flush=True)
+              f"{eq['fp16']['nrc_rel_rmse']:.4f} -> speedup {eq['fp16']['speedup']:.2f}x; bf16 "
+              f"{eq['bf16']['speedup']:.2f}x, bf16 + e4m3 inference {eq['bf16_fp8']['speedup']:.2f}x "
+              f"(rel RMSE {eq['bf16_fp8']['nrc_rel_rmse']:.4f})", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)

@@ -567,7 +567,11 @@ typedef struct rtc8x_nrc rtc8x_nrc;
 
 typedef enum rtc8x_nrc_precision {
     RTC8X_NRC_FP16 = 0,
-    RTC8X_NRC_BF16 = 1
+    RTC8X_NRC_BF16 = 1,
+    /* BF16 training; inference on E4M3 operands (v_mfma_f32_16x16x32_fp8_fp8, FP32 accumulation), the
+     * EMA weights quantised with one power-of-two scale per layer and the activations with one per
+     * 64-query tile and layer, each mapping the largest magnitude to at most 240 */
+    RTC8X_NRC_BF16_FP8 = 2
 } rtc8x_nrc_precision;
 
 typedef enum rtc8x_nrc_encoding {

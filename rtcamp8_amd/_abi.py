@@ -281,7 +281,7 @@ class HosekBand(C.Structure):
 
 SKY_BANDS = 16
 
-NRC_FP16, NRC_BF16 = 0, 1
+NRC_FP16, NRC_BF16, NRC_BF16_FP8 = 0, 1, 2
 NRC_TRIANGLE_WAVE, NRC_HASH_GRID = 0, 1
 
 

@@ -366,6 +366,156 @@ __global__ __launch_bounds__(256) void forward_kernel(const T* __restrict__ w, u
     if (lane == 0) loss_part[blockIdx.x * 4 + wave] = loss;
 }
 
+// ---------------------------------------------------------------- FP8 inference
+// E4M3 operands on v_mfma_f32_16x16x32_fp8_fp8 with FP32 accumulation.  Weights carry one power-of-two
+// scale per layer (quantize_kernel, from the BF16 EMA weights); activations one power-of-two scale per
+// wavefront tile and layer, from the tile's largest magnitude.  Both map the largest value to at most
+// kFp8Max, inside the range of the E4M3 formats, and are divided out of the FP32 accumulators.
+constexpr float kFp8Max = 240.0f;
+
+__device__ __forceinline__ float pow2_scale(float maxabs)
+{
+    if (!(maxabs > 0.0f) || !isfinite(maxabs)) return 1.0f;
+    return exp2f(floorf(log2f(kFp8Max / maxabs)));
+}
+
+// 8 floats -> 8 E4M3 bytes (round to nearest even), element k in byte k
+__device__ __forceinline__ long pack_fp8(const float (&x)[8])
+{
+    int lo = 0, hi = 0;
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[0], x[1], lo, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[2], x[3], lo, true);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[4], x[5], hi, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[6], x[7], hi, true);
+    return (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// One workgroup per layer: the layer's scale and its E4M3 weights ([out][in], like the BF16 copy).
+__global__ __launch_bounds__(256) void quantize_kernel(const __bf16* __restrict__ winf, uint32_t L,
+                                                       uint8_t* __restrict__ w8, float* __restrict__ inv_scale)
+{
+    const uint32_t l = blockIdx.x;
+    const uint32_t count = l < L ? kLayer : kOutPad * kWidth;
+    const __bf16* src = winf + (size_t)l * kLayer;
+    __shared__ float red[256];
+    float m = 0.0f;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) m = fmaxf(m, fabsf((float)src[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const float sc = pow2_scale(red[0]);
+    if (threadIdx.x == 0) inv_scale[l] = 1.0f / sc;
+    for (uint32_t i = 8 * threadIdx.x; i < count; i += 8 * 256) {
+        float x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = (float)src[i + k] * sc;
+        *reinterpret_cast<long*>(w8 + (size_t)l * kLayer + i) = pack_fp8(x);
+    }
+}
+
+// Inference with E4M3 operands: the BF16 forward's structure (encoding into the LDS tile, then per
+// layer 4 x 4 x 2 MFMAs per wavefront), with each layer's activations scaled and quantised on the way
+// from LDS into the A operands.
+template <int ENC>
+__global__ __launch_bounds__(256) void forward_fp8_kernel(const uint8_t* __restrict__ w8, const float* __restrict__ w_inv,
+                                                          uint32_t L, const rtc8x_radiance_query* __restrict__ in,
+                                                          uint32_t n, float* __restrict__ out,
+                                                          const float* __restrict__ tables, const HashGeom geom)
+{
+    using T = __bf16;
+    using V = typename Vec8<T>::type;
+    __shared__ __align__(16) T act[4][64 * kActStride];
+    __shared__ float pred[4][64][3];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t s = (blockIdx.x * 4 + wave) * 64 + lane;
+    T* A = act[wave];
+    if constexpr (ENC == kEncHash) {
+        encode_hash_row<T>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
+    } else {
+        float e[64];
+        if (s < n) {
+            encode(in[s], e);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 64; ++j) e[j] = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 64; j += 8) {
+            V v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (T)e[j + k];
+            *reinterpret_cast<V*>(&A[lane * kActStride + j]) = v;
+        }
+    }
+    const uint32_t r16 = lane & 15, kq = 8 * (lane >> 4);
+    for (uint32_t l = 0; l <= L; ++l) {
+        const bool last = l == L;
+        const uint8_t* W = w8 + (size_t)l * kLayer;
+        const int ntiles = last ? 1 : 4;
+        // the tile's largest activation (every lane scans its own row)
+        float m = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 64; j += 8) {
+            const V v = *reinterpret_cast<const V*>(&A[lane * kActStride + j]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) m = fmaxf(m, fabsf((float)v[k]));
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        const float sa = pow2_scale(m);
+        const float unscale = w_inv[l] / sa;
+        f4 acc[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            long a[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const V v = *reinterpret_cast<const V*>(&A[(mt * 16 + r16) * kActStride + ks * 32 + kq]);
+                float x[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) x[k] = (float)v[k] * sa;
+                a[mt] = pack_fp8(x);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if (nt < ntiles) {
+                    const long b = *reinterpret_cast<const long*>(&W[(nt * 16 + r16) * kWidth + ks * 32 + kq]);
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[mt], b, acc[mt][nt], 0, 0, 0);
+                }
+            }
+        }
+        if (!last) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        A[(mt * 16 + 4 * (lane >> 4) + i) * kActStride + nt * 16 + r16] =
+                            (T)fmaxf(acc[mt][nt][i] * unscale, 0.0f);
+        } else {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (r16 < 3) pred[wave][mt * 16 + 4 * (lane >> 4) + i][r16] = acc[mt][0][i] * unscale;
+        }
+    }
+    if (s < n) {
+        out[3 * (size_t)s] = pred[wave][lane][0];
+        out[3 * (size_t)s + 1] = pred[wave][lane][1];
+        out[3 * (size_t)s + 2] = pred[wave][lane][2];
+    }
+}
+
 template <typename T, int ENC>
 __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt, uint32_t L, uint32_t n,
                                                        const T* __restrict__ hbuf, const T* __restrict__ dout,
@@ -591,6 +741,8 @@ struct rtc8x_nrc {
     float* grid_inf = nullptr;  // EMA (debiased) hash tables for inference
     uint32_t* bad = nullptr;
     void *w = nullptr, *wt = nullptr, *winf = nullptr, *hbuf = nullptr, *dout = nullptr;
+    uint8_t* w8 = nullptr;      // E4M3 inference weights (RTC8X_NRC_BF16_FP8)
+    float* w8_inv = nullptr;    // per layer: 1 / the layer's weight scale
 };
 
 namespace {
@@ -654,7 +806,23 @@ hipError_t launch_convert(rtc8x_nrc* h, hipStream_t s)
     return hipGetLastError();
 }
 
-bool is_bf16(const rtc8x_nrc* h) { return h->desc.precision == RTC8X_NRC_BF16; }
+// training precision: BF16 also under FP8 inference
+bool is_bf16(const rtc8x_nrc* h) { return h->desc.precision != RTC8X_NRC_FP16; }
+bool is_fp8(const rtc8x_nrc* h) { return h->desc.precision == RTC8X_NRC_BF16_FP8; }
+
+hipError_t launch_forward_fp8(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n, float* out)
+{
+    hipLaunchKernelGGL(nrc::quantize_kernel, dim3(h->L + 1), dim3(256), 0, s, (const __bf16*)h->winf, h->L, h->w8,
+                       h->w8_inv);
+    const uint32_t blocks = (n + 255) / 256;
+    if (h->enc == RTC8X_NRC_HASH_GRID)
+        hipLaunchKernelGGL((nrc::forward_fp8_kernel<RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, h->w8,
+                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom);
+    else
+        hipLaunchKernelGGL((nrc::forward_fp8_kernel<RTC8X_NRC_TRIANGLE_WAVE>), dim3(blocks), dim3(256), 0, s, h->w8,
+                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom);
+    return hipGetLastError();
+}
 
 rtc8x_status reset_from_master(rtc8x_nrc* h)
 {
@@ -689,7 +857,7 @@ extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
 {
     if (!h) return RTC8X_OK;
     void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout,
-                    h->grid_inf};
+                    h->grid_inf, h->w8, h->w8_inv};
     for (void* p : ptrs)
         if (p) RTC8X_HIP(hipFree(p));
     delete h;
@@ -701,7 +869,8 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
     RTC8X_REQUIRE(d && out, "rtc8x_nrc_create: NULL argument");
     RTC8X_REQUIRE(d->num_hidden_layers >= 1 && d->num_hidden_layers <= nrc::kMaxLayers,
                   "rtc8x_nrc_create: num_hidden_layers must be in [1, %u]", nrc::kMaxLayers);
-    RTC8X_REQUIRE(d->precision == RTC8X_NRC_FP16 || d->precision == RTC8X_NRC_BF16, "rtc8x_nrc_create: bad precision");
+    RTC8X_REQUIRE(d->precision == RTC8X_NRC_FP16 || d->precision == RTC8X_NRC_BF16 || d->precision == RTC8X_NRC_BF16_FP8,
+                  "rtc8x_nrc_create: bad precision");
     RTC8X_REQUIRE(d->learning_rate > 0.0f && d->beta1 >= 0.0f && d->beta1 < 1.0f && d->beta2 >= 0.0f && d->beta2 < 1.0f &&
                       d->epsilon > 0.0f && d->ema_decay >= 0.0f && d->ema_decay < 1.0f && d->l2_reg >= 0.0f,
                   "rtc8x_nrc_create: bad optimiser parameters");
@@ -725,7 +894,9 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
               alloc((void**)&h->bad, 4) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
               alloc(&h->winf, h->tsize * P_mlp) && (h->P_grid == 0 || alloc((void**)&h->grid_inf, 4 * (size_t)h->P_grid)) &&
               alloc(&h->hbuf, h->tsize * 64 * (size_t)(h->L + 1) * nrc::kMaxTrain) &&
-              alloc(&h->dout, h->tsize * nrc::kOutPad * (size_t)nrc::kMaxTrain);
+              alloc(&h->dout, h->tsize * nrc::kOutPad * (size_t)nrc::kMaxTrain) &&
+              (d->precision != RTC8X_NRC_BF16_FP8 ||
+               (alloc((void**)&h->w8, P_mlp) && alloc((void**)&h->w8_inv, 4 * (size_t)(h->L + 1))));
     if (!ok) {
         rtc8x_nrc_destroy(h);
         return rtc8x::fail(RTC8X_ERR_OUT_OF_MEMORY, "rtc8x_nrc_create: out of device memory");
@@ -791,8 +962,11 @@ extern "C" rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* h, void* stream, const float*
     RTC8X_REQUIRE(h && in && out, "rtc8x_nrc_infer: NULL argument");
     if (n == 0) return RTC8X_OK;
     hipStream_t s = (hipStream_t)stream;
-    RTC8X_HIP(is_bf16(h) ? launch_forward<__bf16>(h, false, s, in, n, out, nullptr)
-                         : launch_forward<_Float16>(h, false, s, in, n, out, nullptr));
+    if (is_fp8(h))
+        RTC8X_HIP(launch_forward_fp8(h, s, in, n, out));
+    else
+        RTC8X_HIP(is_bf16(h) ? launch_forward<__bf16>(h, false, s, in, n, out, nullptr)
+                             : launch_forward<_Float16>(h, false, s, in, n, out, nullptr));
     return RTC8X_OK;
 }
 

@@ -80,6 +80,35 @@ def test_inference_matches_oracle(L, precision, mode, tol, encoding):
 
 
 @pytest.mark.parametrize("encoding", ENCODINGS)
+@pytest.mark.parametrize("L", [2, 5])
+def test_fp8_inference_matches_oracle(L, encoding):
+    """E4M3 inference (BF16 training) against the exact FP64 forward of the same weights.  E4M3 keeps 3
+    mantissa bits: each operand carries up to 2^-4 relative rounding (RMS about 3.6 %), and a 64-term
+    dot product of random-sign terms keeps about 5 % of it per layer.  Measured: relative RMS error
+    0.057-0.065 at L = 2 and 0.096-0.119 at L = 5, max 0.20-0.60 of the RMS.  Tolerance: relative RMS
+    error 0.04 per layer (L + 1 layers), max error 0.8 of the RMS."""
+    from rtcamp8_amd.device import DeviceBuffer
+
+    n = 1000
+    q = random_queries(n, seed=10)
+    w = random_weights(L, seed=11, scale=1.2, encoding=encoding, table_scale=1.0)
+    c = _cache(L, 2, encoding)
+    c.set_weights(w)
+    dq, do = DeviceBuffer.from_array(q), DeviceBuffer(12 * n)
+    c.infer(None, dq.ptr, n, do.ptr)
+    got = do.download(np.float32, 3 * n).astype(np.float64)
+    ref = np.zeros(3 * n)
+    oracle().oracle_nrc_forward_ex(ptr(w), L, encoding, ptr(q), n, ptr(ref), 0)
+    rms = np.sqrt(np.mean(ref ** 2))
+    err = np.sqrt(np.mean((got - ref) ** 2)) / rms
+    print(f"[fp8] L={L} encoding={encoding}: relative RMS error {err:.4f}, max {np.max(np.abs(got - ref)) / rms:.4f}")
+    assert rms > 1e-3 and np.all(np.isfinite(got))
+    assert err < 0.04 * (L + 1), err
+    assert np.max(np.abs(got - ref)) < 0.8 * rms
+    c.finalize()
+
+
+@pytest.mark.parametrize("encoding", ENCODINGS)
 @pytest.mark.parametrize("L,precision,mode,tol", [(2, 0, 1, 5e-3), (5, 0, 1, 5e-3), (3, 1, 2, 3e-2)])
 def test_gradient_matches_oracle(L, precision, mode, tol, encoding):
     from rtcamp8_amd.device import DeviceBuffer

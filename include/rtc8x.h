@@ -528,6 +528,16 @@ rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t
 rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, uint32_t n, uint32_t* order, void* scratch);
 size_t rtc8x_order_scratch_bytes(uint32_t n);
 
+/* Several instances -- each its own volume, camera, sky, image buffers and, for the NRC entry, cache
+ * buffers -- in one persistent launch of a full-image entry (PATH_TRACE or PATH_TRACE_WITH_NRC);
+ * instance i traces num_samples[i] samples per pixel exactly as rtc8x_trace(params[i]) would.  The
+ * device's workgroups are split among the instances in proportion to shares[i] (NULL: pixels x
+ * samples; every instance gets at least one workgroup); each instance keeps its own work counter.
+ * count <= 16; the instances share one tracking mode.  No counterpart in the reference (it launches
+ * one image per OptiX launch). */
+rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
+                               const float* shares, uint32_t count, int32_t entry);
+
 /* ------------------------------------------------------------------------------------------------
  * NRC data-path kernels (renderer/gpu_kernels/nrc_set_up_kernels.cu).  They read the same
  * rtc8x_frame_params as rtc8x_trace (buffers in params->nrc, image size, accum).

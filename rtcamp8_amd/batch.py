@@ -1,0 +1,79 @@
+"""Several instances in one persistent launch (rtc8x_trace_batch) and the autotuner of their shares of
+the device.
+
+An instance is a PathTracer (plain entry) or an NrcRenderer (the NRC render entry), each with its own
+scene, image and buffers.  `autotune_shares` times every instance's own launch and returns those times
+as the workgroup shares of the batched launch, so that the instances finish together."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+from . import _abi
+from ._lib import call
+from .device import EventTimer, synchronize
+from .nrc import NrcRenderer, round_up
+from .tracer import PathTracer
+
+
+def _launch(params: Sequence[_abi.FrameParams], samples: Sequence[int], entry: int, shares, stream) -> None:
+    n = len(params)
+    arr = (_abi.FrameParams * n)(*params)
+    ns = (C.c_uint32 * n)(*[int(k) for k in samples])
+    sh = None if shares is None else (C.c_float * n)(*[float(x) for x in shares])
+    call("rtc8x_trace_batch", C.c_void_p(stream), arr, ns, sh, n, entry)
+
+
+def trace_plain(tracers: Sequence[PathTracer], samples: Sequence[int], shares=None, stream=None) -> None:
+    """PathTracer.trace(samples[i]) for every tracer, in one launch."""
+    params = []
+    for t in tracers:
+        p = t.params()
+        t._schedule(p)
+        params.append(p)
+    _launch(params, samples, _abi.ENTRY_PATH_TRACE, shares, stream)
+    for t, k in zip(tracers, samples):
+        t.samples += int(k)
+
+
+def render_nrc(renderers: Sequence[NrcRenderer], samples: Sequence[int], shares=None, stream=None) -> None:
+    """NrcRenderer.render_samples(samples[i]) for every renderer, with the traces in one launch (the
+    inference and accumulation stay per renderer: every instance has its own cache)."""
+    params = []
+    for r, k in zip(renderers, samples):
+        assert 1 <= k <= r.samples_per_launch
+        p = r.params()
+        p.nrc.num_samples = int(k)
+        if r.have_cost:
+            call("rtc8x_order_by_cost", C.c_void_p(stream), C.c_void_p(r.cost.ptr), r.width * r.height,
+                 C.c_void_p(r.order.ptr), C.c_void_p(r.order_scratch.ptr))
+            p.work_order = r.order.ptr
+        p.pixel_cost = r.cost.ptr
+        r.have_cost = True
+        params.append(p)
+    _launch(params, samples, _abi.ENTRY_PATH_TRACE_WITH_NRC, shares, stream)
+    for r, p, k in zip(renderers, params, samples):
+        nq = round_up(r.width * r.height, 128) * int(k)
+        r.cache.infer(stream, r.buffers.inference_queries.ptr, nq, r.buffers.inferred_radiance.ptr)
+        call("rtc8x_nrc_accumulate", C.c_void_p(stream), C.byref(p))
+        r.samples += int(k)
+        r.stats.inference_queries += nq
+
+
+def autotune_shares(launch_one, count: int, repeats: int = 2, stream=None) -> list[float]:
+    """Times launch_one(i) for every instance i (the best of `repeats`, after one untimed launch) and
+    returns the times: the shares under which the instances of one batched launch finish together."""
+    times = []
+    for i in range(count):
+        launch_one(i)
+        best = float("inf")
+        for _ in range(repeats):
+            t = EventTimer()
+            t.start(stream)
+            launch_one(i)
+            t.stop(stream)
+            synchronize()
+            best = min(best, t.elapsed_ms())
+            t.destroy()
+        times.append(best)
+    return times

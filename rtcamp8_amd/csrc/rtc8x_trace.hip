@@ -15,9 +15,11 @@
 // pixel's paths one after another.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "rtc8x_common.hpp"
 #include "rtc8x_device.hpp"
@@ -32,6 +34,7 @@ constexpr uint32_t kSpeculate = 2;
 // A wavefront with at least this many idle lanes evaluates its stepping lanes' segments together.
 constexpr uint32_t kCoopIdle = 48;
 constexpr uint32_t kCounterSlots = 1024;
+constexpr uint32_t kMaxBatch = 16;  // instances of one batched launch
 __device__ uint32_t g_work_counter[kCounterSlots];
 
 __global__ void reset_counter_kernel(uint32_t slot) { g_work_counter[slot] = 0u; }
@@ -375,10 +378,10 @@ __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, 
 // GENERATE_TRAINING_DATA runs as the staging launch or the redo launch (TrainStage; the redo records at
 // record_offsets).
 template <int ENTRY, int TRACKING>
-__global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
-                                                        uint32_t launch_h, uint32_t num_samples, uint32_t slot,
-                                                        const uint32_t* __restrict__ record_offsets,
-                                                        uint32_t* __restrict__ record_counts, const TrainStage stage)
+__device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t launch_w, uint32_t launch_h,
+                                           uint32_t num_samples, uint32_t slot,
+                                           const uint32_t* __restrict__ record_offsets,
+                                           uint32_t* __restrict__ record_counts, const TrainStage& stage)
 {
     constexpr bool with_nrc = ENTRY != RTC8X_ENTRY_PATH_TRACE;
     constexpr bool gen_data = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA;
@@ -839,6 +842,51 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
     }
 }
 
+// Work counters of a launch: a block of kMaxBatch slots, taken in turn by the launches in flight (so
+// launches on different streams count in different slots).
+inline uint32_t take_counter_slots()
+{
+    static std::atomic<uint32_t> next{0};
+    return (next.fetch_add(1) % (kCounterSlots / kMaxBatch)) * kMaxBatch;
+}
+
+// One launch of one image.
+template <int ENTRY, int TRACKING>
+__global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
+                                                        uint32_t launch_h, uint32_t num_samples, uint32_t slot,
+                                                        const uint32_t* __restrict__ record_offsets,
+                                                        uint32_t* __restrict__ record_counts, const TrainStage stage)
+{
+    trace_body<ENTRY, TRACKING>(p, launch_w, launch_h, num_samples, slot, record_offsets, record_counts, stage);
+}
+
+// Several images (instances: their own volume, camera, sky, buffers and, with the NRC entry, cache
+// buffers) in one persistent launch.  Workgroup b traces the instance whose workgroup range
+// [wg_end[i-1], wg_end[i]) holds b, from that instance's work counter; the ranges are the caller's
+// shares of the device (rtc8x_trace_batch).  The instance is uniform over a workgroup, so its frame
+// parameters are read with scalar loads, as a single launch reads its kernel arguments.
+struct BatchArgs {
+    const rtc8x_frame_params* params;  // device copy, count entries
+    uint32_t count, slot;              // instance i counts its work at g_work_counter[slot + i]
+    uint32_t wg_end[kMaxBatch];
+    uint32_t num_samples[kMaxBatch];
+};
+
+template <int ENTRY, int TRACKING>
+__global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
+{
+    uint32_t inst = 0;
+    while (inst + 1 < b.count && blockIdx.x >= b.wg_end[inst]) ++inst;
+    const rtc8x_frame_params* __restrict__ pp = b.params + inst;
+    trace_body<ENTRY, TRACKING>(*pp, pp->image_width, pp->image_height, b.num_samples[inst], b.slot + inst, nullptr,
+                                nullptr, TrainStage{});
+}
+
+__global__ void reset_counters_kernel(uint32_t slot, uint32_t count)
+{
+    if (threadIdx.x < count) g_work_counter[slot + threadIdx.x] = 0u;
+}
+
 // Exclusive prefix sum of the per-path record counts (one workgroup, a few hundred thousand entries).
 __global__ __launch_bounds__(1024) void record_prefix_kernel(const uint32_t* __restrict__ counts,
                                                              uint32_t* __restrict__ offsets, uint32_t n,
@@ -1037,8 +1085,7 @@ template <int ENTRY, int TRACKING>
 hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t num_samples, const uint32_t* offsets,
                   uint32_t* counts, const TrainStage& stage, hipStream_t stream)
 {
-    static std::atomic<uint32_t> next_slot{0};
-    const uint32_t slot = next_slot.fetch_add(1) % kCounterSlots;
+    const uint32_t slot = take_counter_slots();
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
@@ -1050,6 +1097,43 @@ hipError_t launch(const rtc8x_frame_params& p, uint32_t w, uint32_t h, uint32_t 
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, slot);
     hipLaunchKernelGGL((wavefront_kernel<ENTRY, TRACKING>), dim3((uint32_t)(cus * per_cu)), dim3(256), 0,
                        stream, p, w, h, num_samples, slot, offsets, counts, stage);
+    return hipGetLastError();
+}
+
+// One persistent launch over several instances; shares[i] is instance i's part of the device's
+// workgroups (each gets at least one).
+template <int ENTRY, int TRACKING>
+hipError_t launch_batch(const rtc8x_frame_params* d_params, const uint32_t* num_samples, const double* shares,
+                        uint32_t count, hipStream_t stream)
+{
+    const uint32_t slot = take_counter_slots();
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<ENTRY, TRACKING>, 256, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const uint32_t total = std::max<uint32_t>((uint32_t)(cus * per_cu), count);
+    BatchArgs b{};
+    b.params = d_params;
+    b.count = count;
+    b.slot = slot;
+    double sum = 0.0;
+    for (uint32_t i = 0; i < count; ++i) sum += shares[i];
+    uint32_t end = 0;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < count; ++i) {
+        acc += shares[i];
+        // cumulative rounding keeps the ranges summing to `total`; every instance keeps one workgroup
+        uint32_t e_i = (uint32_t)std::llround(acc / sum * (total - count)) + i + 1;
+        e_i = std::max(e_i, end + 1);
+        b.wg_end[i] = end = std::min(e_i, total - (count - 1 - i));
+        b.num_samples[i] = num_samples[i];
+    }
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxBatch), 0, stream, slot, count);
+    hipLaunchKernelGGL((batch_kernel<ENTRY, TRACKING>), dim3(end), dim3(256), 0, stream, b);
     return hipGetLastError();
 }
 
@@ -1092,10 +1176,11 @@ extern "C" rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, 
 
 static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
-extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t entry, uint32_t num_samples)
+namespace {
+
+// The checks every launch of an entry point makes on its frame parameters.
+rtc8x_status check_launch(const rtc8x_frame_params& p, int32_t entry, uint32_t num_samples)
 {
-    RTC8X_REQUIRE(params != nullptr, "rtc8x_trace: params is NULL");
-    const rtc8x_frame_params& p = *params;
     RTC8X_REQUIRE(entry >= RTC8X_ENTRY_PATH_TRACE && entry <= RTC8X_ENTRY_PATH_TRACE_WITH_NRC,
                   "rtc8x_trace: unknown entry %d", entry);
     RTC8X_REQUIRE(p.image_width > 0 && p.image_height > 0 && p.image_width <= 16384 && p.image_height <= 16384,
@@ -1152,6 +1237,19 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
     }
     RTC8X_REQUIRE(entry != RTC8X_ENTRY_GENERATE_TRAINING_DATA || (!p.work_order && !p.pixel_cost),
                   "rtc8x_trace: work_order / pixel_cost are for the full-image entries (the training size varies)");
+    return RTC8X_OK;
+}
+
+}  // namespace
+
+extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t entry, uint32_t num_samples)
+{
+    RTC8X_REQUIRE(params != nullptr, "rtc8x_trace: params is NULL");
+    const rtc8x_frame_params& p = *params;
+    {
+        const rtc8x_status st = check_launch(p, entry, num_samples);
+        if (st != RTC8X_OK) return st;
+    }
     hipStream_t s = (hipStream_t)stream;
     if (entry == RTC8X_ENTRY_PATH_TRACE) {
         RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_PATH_TRACE>(p, p.image_width, p.image_height, num_samples,
@@ -1217,5 +1315,47 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
     redo.redo_lost = stage.lost;
     redo.rng_save = rng_save;
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, tw, th, 1, offsets, nullptr, s, redo)));
+    return RTC8X_OK;
+}
+
+extern "C" rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
+                                          const float* shares, uint32_t count, int32_t entry)
+{
+    RTC8X_REQUIRE(params && num_samples, "rtc8x_trace_batch: NULL argument");
+    RTC8X_REQUIRE(count >= 1 && count <= dev::kMaxBatch, "rtc8x_trace_batch: count %u must be in [1, %u]", count,
+                  dev::kMaxBatch);
+    RTC8X_REQUIRE(entry == RTC8X_ENTRY_PATH_TRACE || entry == RTC8X_ENTRY_PATH_TRACE_WITH_NRC,
+                  "rtc8x_trace_batch: entry %d: only the full-image entries run in a batch", entry);
+    double w[dev::kMaxBatch];
+    for (uint32_t i = 0; i < count; ++i) {
+        const rtc8x_status st = check_launch(params[i], entry, num_samples[i]);
+        if (st != RTC8X_OK) {
+            const std::string why = rtc8x_last_error();
+            return fail(st, "rtc8x_trace_batch: instance %u: %s", i, why.c_str());
+        }
+        RTC8X_REQUIRE(params[i].tracking == params[0].tracking, "rtc8x_trace_batch: instances must share the tracking mode");
+        if (shares) {
+            RTC8X_REQUIRE(std::isfinite(shares[i]) && shares[i] > 0.0f, "rtc8x_trace_batch: shares must be positive");
+            w[i] = shares[i];
+        } else {
+            w[i] = (double)params[i].image_width * params[i].image_height * num_samples[i];
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    rtc8x_frame_params* d = nullptr;
+    RTC8X_HIP(hipMallocAsync((void**)&d, sizeof(rtc8x_frame_params) * count, s));
+    RTC8X_HIP(hipMemcpyAsync(d, params, sizeof(rtc8x_frame_params) * count, hipMemcpyHostToDevice, s));
+    if (params[0].tracking == RTC8X_TRACKING_LOCAL_MAJORANT) {
+        RTC8X_HIP(entry == RTC8X_ENTRY_PATH_TRACE
+                      ? (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w, count, s))
+                      : (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w,
+                                                                                                           count, s)));
+    } else {
+        RTC8X_HIP(entry == RTC8X_ENTRY_PATH_TRACE
+                      ? (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w, count, s))
+                      : (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w,
+                                                                                                            count, s)));
+    }
+    RTC8X_HIP(hipFreeAsync(d, s));
     return RTC8X_OK;
 }

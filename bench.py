@@ -183,6 +183,82 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     }
 
 
+MULTI_INSTANCES = [  # (fBm resolution, sparse, sigma_t max, image width, height)
+    (64, False, 5.0, 256, 256), (96, False, 10.0, 320, 240), (128, False, 20.0, 512, 512),
+    (192, True, 40.0, 640, 360), (256, True, 60.0, 800, 600), (384, True, 100.0, 1024, 576),
+    (512, True, 150.0, 1280, 720), (1024, True, 200.0, 1920, 1080)]
+
+
+def multi_instance_leg(steps: int, stream):
+    """Uneven multi-instance batch: eight fBm volumes from 64^3 to 1024^3 (sigma_t max 5 to 200) at image
+    sizes from 256^2 to 1920x1080, each with its own NRC (hash grid, BF16 training, E4M3 inference),
+    after the first frame's training.  One NRC render launch per instance against all eight in one
+    persistent launch, with the device's workgroups shared by the autotuner (each instance's own launch
+    time) or by pixels."""
+    from rtcamp8_amd import _abi, batch
+    from rtcamp8_amd import scene as sc
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import EventTimer
+    from rtcamp8_amd.nrc import NrcRenderer
+
+    base = sc.CONFIGS["sparse512"]
+    scenes, rs = [], []
+    for res, sparse, sigma, w, h in MULTI_INSTANCES:
+        fbm = sc.FbmConfig(resolution=res, seed=0x5EED0100 + res, sparse=sparse, active_leaf_fraction=0.25,
+                           contrast=4.0 if sparse else 1.0)
+        cfg = base.with_(name=f"fbm{res}", width=w, height=h, density_coeff=sigma, fbm=fbm)
+        scenes.append(sc.DeviceScene(cfg, stream.handle))
+        rs.append(NrcRenderer(scenes[-1], num_hidden_layers=2, precision=_abi.NRC_BF16_FP8, stream=stream.handle,
+                              radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY, samples_per_launch=1))
+    for r in rs:
+        r.render_frame(spp=1)
+    stream.synchronize()
+
+    def params(r):
+        p = r.params()
+        p.nrc.num_samples = 1
+        return p
+
+    def one(i):
+        call("rtc8x_trace", C.c_void_p(stream.handle), C.byref(params(rs[i])), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
+
+    shares = batch.autotune_shares(one, len(rs), stream=stream.handle)
+
+    def timed(fn):
+        fn()
+        ms = []
+        for _ in range(steps):
+            t = EventTimer()
+            t.start(stream.handle)
+            fn()
+            t.stop(stream.handle)
+            stream.synchronize()
+            ms.append(t.elapsed_ms())
+            t.destroy()
+        return float(np.median(ms))
+
+    ones = [1] * len(rs)
+    seq_ms = timed(lambda: [one(i) for i in range(len(rs))])
+    tuned_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, shares,
+                                           stream.handle))
+    pixel_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, None,
+                                           stream.handle))
+    batch.render_nrc(rs, ones, shares, stream.handle)
+    stream.synchronize()
+    finite = all(bool(np.all(np.isfinite(r.image()))) for r in rs)
+    for r in rs:
+        r.close()
+    for s_ in scenes:
+        s_.close()
+    return {"instances": [{"fbm_resolution": res, "sparse": sparse, "sigma_t_max": sigma, "image": [w, h],
+                           "own_launch_ms": t}
+                          for (res, sparse, sigma, w, h), t in zip(MULTI_INSTANCES, shares)],
+            "nrc": "hash grid, 2 hidden layers, bf16 training, e4m3 inference; first frame (8 training iterations) done",
+            "entry": "PATH_TRACE_WITH_NRC, 1 spp per instance",
+            "one_launch_per_instance_ms": seq_ms, "batched_autotuned_ms": tuned_ms, "batched_by_pixels_ms": pixel_ms,
+            "speedup_autotuned": seq_ms / tuned_ms, "finite": finite}
+
+
 def nrc_render_leg(scene, steps: int, stream):
     """NRC-integrated rendering of the same scene: per step one training iteration and one sample per
     pixel traced into the cache (2 hidden layers, FP16), after the first frame's 8 training iterations."""
@@ -416,6 +492,13 @@ def main() -> int:
         print(f"[bench] NRC render: {rend['ms_per_spp']:.2f} ms/spp", file=sys.stderr, flush=True)
         result["nrc_queries_per_s"] = mlp["queries_per_s"]
         result["nrc"] = {"network": mlp, "network_hash_grid": mlp_hash, "network_fp8": mlp_fp8, "render": rend}
+    if args.nrc_steps > 0:
+        print("[bench] multi-instance leg ...", file=sys.stderr, flush=True)
+        multi = multi_instance_leg(args.nrc_steps, stream)
+        result["multi_instance"] = multi
+        print(f"[bench] multi-instance: 8 NRC render launches {multi['one_launch_per_instance_ms']:.2f} ms, one "
+              f"batched launch {multi['batched_autotuned_ms']:.2f} ms (autotuned), {multi['batched_by_pixels_ms']:.2f} ms "
+              f"(by pixels)", file=sys.stderr, flush=True)
     if args.rmse_ref_spp > 0:
         print("[bench] equal-RMSE leg ...", file=sys.stderr, flush=True)
         eq = equal_rmse_leg(scene, args.rmse_ref_spp, cfg.spp, stream)

@@ -861,10 +861,12 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
 }
 
 // Several images (instances: their own volume, camera, sky, buffers and, with the NRC entry, cache
-// buffers) in one persistent launch.  Workgroup b traces the instance whose workgroup range
-// [wg_end[i-1], wg_end[i]) holds b, from that instance's work counter; the ranges are the caller's
-// shares of the device (rtc8x_trace_batch).  The instance is uniform over a workgroup, so its frame
-// parameters are read with scalar loads, as a single launch reads its kernel arguments.
+// buffers) in one persistent launch.  Workgroup b starts on the instance whose workgroup range
+// [wg_end[i-1], wg_end[i]) holds b (the ranges are the caller's shares of the device,
+// rtc8x_trace_batch) and takes pixels from that instance's work counter; a wavefront that has
+// finished its instance's last pixels moves on to the following instances and helps with what is left
+// of them.  The instance is uniform over a wavefront, so its frame parameters are read with scalar
+// loads, as a single launch reads its kernel arguments.
 struct BatchArgs {
     const rtc8x_frame_params* params;  // device copy, count entries
     uint32_t count, slot;              // instance i counts its work at g_work_counter[slot + i]
@@ -875,11 +877,14 @@ struct BatchArgs {
 template <int ENTRY, int TRACKING>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
 {
-    uint32_t inst = 0;
-    while (inst + 1 < b.count && blockIdx.x >= b.wg_end[inst]) ++inst;
-    const rtc8x_frame_params* __restrict__ pp = b.params + inst;
-    trace_body<ENTRY, TRACKING>(*pp, pp->image_width, pp->image_height, b.num_samples[inst], b.slot + inst, nullptr,
-                                nullptr, TrainStage{});
+    uint32_t first = 0;
+    while (first + 1 < b.count && blockIdx.x >= b.wg_end[first]) ++first;
+    for (uint32_t k = 0; k < b.count; ++k) {
+        const uint32_t inst = first + k < b.count ? first + k : first + k - b.count;
+        const rtc8x_frame_params* __restrict__ pp = b.params + inst;
+        trace_body<ENTRY, TRACKING>(*pp, pp->image_width, pp->image_height, b.num_samples[inst], b.slot + inst,
+                                    nullptr, nullptr, TrainStage{});
+    }
 }
 
 __global__ void reset_counters_kernel(uint32_t slot, uint32_t count)

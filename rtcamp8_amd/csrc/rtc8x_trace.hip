@@ -421,15 +421,33 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     Seg<TRACKING> sg;
     float mu = 0.0f;
     for (;;) {
-        // Vertex events are batched: a lane that reaches one parks until half of the wavefront's active
-        // lanes have, then all of them run their events together (each event block is issued once for
-        // many lanes instead of once per lane), while the others keep stepping through lookups.
+        // Vertex events are batched: a lane that reaches one parks, and once enough of the wavefront's
+        // active lanes have parked they run their events together (each event block is issued once for
+        // many lanes instead of once per lane), while the others keep stepping through lookups.  The
+        // plain and training entries then run only the larger of the two vertex events (a real
+        // collision: Russian roulette and NEE set-up; an NEE segment's end: phase sampling and the next
+        // segment) together with the path starts and ends, from 3/8 of the active lanes; the other
+        // event's lanes wait for a later pass.  The NRC render entry, whose paths are short, runs every
+        // parked lane from half of the active lanes.
         const uint64_t parked = __ballot(L.mode != kModeTrack && L.mode != kModeShadow && L.mode != kModeIdle);
         const uint64_t stepping = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
         if (parked == 0 && stepping == 0) break;
-        const bool event_pass = stepping == 0 || 2 * __popcll(parked) >= __popcll(parked | stepping);
+        const uint32_t n_parked = __popcll(parked), n_active = __popcll(parked | stepping);
+        bool run_events;
+        if constexpr (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
+            run_events = stepping == 0 || 2 * n_parked >= n_active;
+        } else {
+            run_events = stepping == 0;
+            if (!run_events && 8 * n_parked >= 3 * n_active) {
+                const uint64_t at_collision = __ballot(L.mode == kModeScatter), at_nee_end = __ballot(L.mode == kModeNeeDone);
+                const uint64_t others = parked & ~(at_collision | at_nee_end);
+                const uint64_t sel =
+                    (__popcll(at_collision) >= __popcll(at_nee_end) ? at_collision : at_nee_end) | others;
+                run_events = ((sel >> lane_id) & 1ull) != 0;
+            }
+        }
         // ---- events until this lane has a tracking or shadow segment under way, or no work left
-        while (event_pass && L.mode != kModeIdle && L.mode != kModeTrack && L.mode != kModeShadow) {
+        while (run_events && L.mode != kModeIdle && L.mode != kModeTrack && L.mode != kModeShadow) {
             if (L.mode == kModeWork) {
                 // the next pixel from the work counter, one atomic per wavefront
                 const uint64_t ask = __ballot(1);

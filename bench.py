@@ -31,6 +31,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_LOOKUP = 32  # 8 FP32 voxel taps per trilinear lookup
 FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 FP8_PEAK_TFLOPS = 5000.0  # MI355X dense FP8 MFMA
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 MFMA
 # NRC settings of the equal-RMSE leg: the reference's network (hash grid, 2 hidden layers, lr 1e-2) with
 # its radiance-scale knob (renderer_main.cpp:979-995) set so the cloud's radiance (~0.03-0.3) sits well
 # above the loss's 0.01 epsilon, and a shorter weight EMA than the reference's 0.99 (the benchmark
@@ -131,7 +132,7 @@ def random_leaf_queries(scene, n: int, seed: int) -> np.ndarray:
 def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 21, n_train: int = 1 << 16,
                 encoding: int = 0, precision: int | None = None):
     """Standalone network step: train on 2^16 samples, then infer 2^21 queries (FP16 storage, FP32
-    accumulation, or BF16 training with E4M3 inference); the benchmark configuration is 5 hidden layers
+    accumulation; BF16 training with E4M3 inference; or FP32 throughout); the benchmark configuration is 5 hidden layers
     of 64 with the frequency (triangle wave) encoding, the renderer's default 2 hidden layers with the
     hash grid."""
     from rtcamp8_amd import _abi
@@ -143,7 +144,10 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     dq, dt, dout = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t), DeviceBuffer(12 * n_infer)
     cache = NeuralRadianceCache()
     precision = _abi.NRC_FP16 if precision is None else precision
-    fp8 = precision == _abi.NRC_BF16_FP8
+    label, peak = {_abi.NRC_FP16: ("fp16 storage, fp32 accumulate", FP16_PEAK_TFLOPS),
+                   _abi.NRC_BF16: ("bf16 storage, fp32 accumulate", FP16_PEAK_TFLOPS),
+                   _abi.NRC_BF16_FP8: ("bf16 training, e4m3 inference, fp32 accumulate", FP8_PEAK_TFLOPS),
+                   _abi.NRC_FP32: ("fp32 throughout", FP32_PEAK_TFLOPS)}[precision]
     cache.initialize(num_hidden_layers=hidden_layers, learning_rate=1e-2, precision=precision, encoding=encoding)
     for _ in range(2):
         cache.train(None, dq.ptr, dt.ptr, n_train)
@@ -169,7 +173,7 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
     train_flop = 3.0 * fwd_flop  # forward + backward (input and weight gradients)
     return {
         "hidden_layers": hidden_layers,
-        "precision": "bf16 training, e4m3 inference, fp32 accumulate" if fp8 else "fp16 storage, fp32 accumulate",
+        "precision": label,
         "encoding": "hash grid (16 levels x 2, 2^15)" if encoding == 1 else "triangle waves (12 frequencies)",
         "infer_queries": n_infer, "train_samples": n_train,
         "infer_ms": infer_ms, "train_ms": train_ms,
@@ -177,8 +181,8 @@ def nrc_mlp_leg(scene, steps: int, hidden_layers: int = 5, n_infer: int = 1 << 2
         "train_samples_per_s": n_train / (train_ms * 1e-3),
         "infer_tflops": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12,
         "train_tflops": n_train * train_flop / (train_ms * 1e-3) / 1e12,
-        "mfma_peak_tflops": FP8_PEAK_TFLOPS if fp8 else FP16_PEAK_TFLOPS,
-        "infer_frac": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12 / (FP8_PEAK_TFLOPS if fp8 else FP16_PEAK_TFLOPS),
+        "mfma_peak_tflops": peak,
+        "infer_frac": n_infer * fwd_flop / (infer_ms * 1e-3) / 1e12 / peak,
         "finite": ok,
     }
 
@@ -329,7 +333,7 @@ def latency_roofline(scene, lookups_per_path: float, trace_lookups_per_s: float,
     return res
 
 
-def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
+def equal_rmse_leg(scene, ref_spp: int, spp: int, stream, precisions=None, encoding=None):
     """Wall time to equal RMSE: NRC-accelerated rendering vs the plain tracer on the same frame.
 
     NRC runs the reference's batch schedule (training iterations per frame, then the frame's samples) with
@@ -340,6 +344,9 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
     from rtcamp8_amd import _abi
     from rtcamp8_amd.convergence import cross_mse, nrc_image, plain_image
 
+    if precisions is None:
+        precisions = (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16), ("bf16_fp8", _abi.NRC_BF16_FP8))
+    encoding = _abi.NRC_HASH_GRID if encoding is None else encoding
     a, ta = plain_image(scene, ref_spp, seed=0xA11CE, stream=stream.handle)
     b, _ = plain_image(scene, ref_spp, seed=0xB0B, stream=stream.handle)
     x, tx = plain_image(scene, spp, seed=0xC0FFEE, stream=stream.handle)
@@ -347,12 +354,13 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
     mse_plain = cross_mse(x, a, b)
     res = {"reference": f"plain tracer, 2 x {ref_spp} spp with independent seeds (cross-product MSE)",
            "nrc_schedule": "frame 0: 8 training iterations, later frames 1; 4 spp traced into the cache per frame",
-           "nrc_settings": {"encoding": "hash grid", "hidden_layers": 2, "radiance_scale": NRC_RADIANCE_SCALE,
+           "nrc_settings": {"encoding": "hash grid" if encoding == _abi.NRC_HASH_GRID else "triangle waves",
+                            "hidden_layers": 2, "radiance_scale": NRC_RADIANCE_SCALE,
                             "ema_decay": NRC_EMA_DECAY},
            "spp": spp, "plain_seconds": tx, "plain_rel_rmse": float(np.sqrt(max(mse_plain, 0.0)) / mean)}
-    for name, prec in (("fp16", _abi.NRC_FP16), ("bf16", _abi.NRC_BF16), ("bf16_fp8", _abi.NRC_BF16_FP8)):
+    for name, prec in precisions:
         run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec, spp_per_frame=4,
-                        encoding=_abi.NRC_HASH_GRID, radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY)
+                        encoding=encoding, radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY)
         mse = cross_mse(run.image, a, b)
         n_star = spp * mse_plain / mse if mse > 0 else float("inf")
         t_equal = tx * n_star / spp
@@ -360,6 +368,32 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream):
                      "plain_spp_for_equal_rmse": n_star, "plain_seconds_for_equal_rmse": t_equal,
                      "speedup": t_equal / run.seconds}
     return res
+
+
+def dense128_equal_rmse_leg(ref_spp: int, stream, spp: int = 16) -> dict:
+    """The NRC on the dense 128^3 cloud (256 x 256, <= 16 bounces, constant sky and sun) against the
+    plain tracer at equal RMSE, with the network in BF16 and in FP32.  The reference images are the
+    plain tracer at 2 x 4 ref_spp; the plain tracer converges to the FP64 oracle on this scene (GPU
+    test test_converges_to_fp64_oracle), so they stand in for it."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd import scene as sc
+
+    small = sc.DeviceScene(sc.CONFIGS["dense128"])
+    out = {"workload": "dense128: dense 128^3 fBm, sigma_t max 20, albedo 0.99, HG g=0.8, constant white sky + sun, "
+                       f"256x256, <= 16 bounces, {spp} spp"}
+    try:
+        for enc_name, enc in (("hash_grid", _abi.NRC_HASH_GRID), ("triangle_waves", _abi.NRC_TRIANGLE_WAVE)):
+            out[enc_name] = equal_rmse_leg(small, 4 * ref_spp, spp, stream, encoding=enc,
+                                           precisions=(("bf16", _abi.NRC_BF16), ("fp32", _abi.NRC_FP32)))
+            r = out[enc_name]
+            print(f"[bench] dense128 equal RMSE ({enc_name}): plain rel RMSE {r['plain_rel_rmse']:.4f} in "
+                  f"{r['plain_seconds'] * 1e3:.2f} ms; NRC bf16 {r['bf16']['nrc_rel_rmse']:.4f} in "
+                  f"{r['bf16']['nrc_seconds'] * 1e3:.2f} ms -> {r['bf16']['speedup']:.2f}x; fp32 "
+                  f"{r['fp32']['nrc_rel_rmse']:.4f} in {r['fp32']['nrc_seconds'] * 1e3:.2f} ms -> "
+                  f"{r['fp32']['speedup']:.2f}x", file=sys.stderr, flush=True)
+    finally:
+        small.close()
+    return out
 
 
 def calibrate_copy(mib: int, steps: int) -> int:
@@ -485,13 +519,15 @@ def main() -> int:
         mlp = nrc_mlp_leg(scene, args.nrc_steps)
         mlp_hash = nrc_mlp_leg(scene, args.nrc_steps, hidden_layers=2, encoding=1)
         mlp_fp8 = nrc_mlp_leg(scene, args.nrc_steps, precision=_abi.NRC_BF16_FP8)
+        mlp_fp32 = nrc_mlp_leg(scene, args.nrc_steps, precision=_abi.NRC_FP32)
         print(f"[bench] NRC network: infer {mlp['infer_ms']:.2f} ms / 2^21, train {mlp['train_ms']:.2f} ms / 2^16; "
               f"e4m3 inference {mlp_fp8['infer_ms']:.2f} ms", file=sys.stderr, flush=True)
         print("[bench] NRC render leg ...", file=sys.stderr, flush=True)
         rend = nrc_render_leg(scene, args.nrc_steps, stream)
         print(f"[bench] NRC render: {rend['ms_per_spp']:.2f} ms/spp", file=sys.stderr, flush=True)
         result["nrc_queries_per_s"] = mlp["queries_per_s"]
-        result["nrc"] = {"network": mlp, "network_hash_grid": mlp_hash, "network_fp8": mlp_fp8, "render": rend}
+        result["nrc"] = {"network": mlp, "network_hash_grid": mlp_hash, "network_fp8": mlp_fp8, "network_fp32": mlp_fp32,
+                         "render": rend}
     if args.nrc_steps > 0:
         print("[bench] multi-instance leg ...", file=sys.stderr, flush=True)
         multi = multi_instance_leg(args.nrc_steps, stream)
@@ -507,6 +543,8 @@ def main() -> int:
               f"{eq['fp16']['nrc_rel_rmse']:.4f} -> speedup {eq['fp16']['speedup']:.2f}x; bf16 "
               f"{eq['bf16']['speedup']:.2f}x, bf16 + e4m3 inference {eq['bf16_fp8']['speedup']:.2f}x "
               f"(rel RMSE {eq['bf16_fp8']['nrc_rel_rmse']:.4f})", file=sys.stderr, flush=True)
+        print("[bench] equal-RMSE leg on the 128^3 dense cloud ...", file=sys.stderr, flush=True)
+        result["equal_rmse_dense128"] = dense128_equal_rmse_leg(args.rmse_ref_spp, stream)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)

@@ -581,7 +581,10 @@ typedef enum rtc8x_nrc_precision {
     /* BF16 training; inference on E4M3 operands (v_mfma_f32_16x16x32_fp8_fp8, FP32 accumulation), the
      * EMA weights quantised with one power-of-two scale per layer and the activations with one per
      * 64-query tile and layer, each mapping the largest magnitude to at most 240 */
-    RTC8X_NRC_BF16_FP8 = 2
+    RTC8X_NRC_BF16_FP8 = 2,
+    /* FP32 weights and activations in training and inference (v_mfma_f32_16x16x4_f32, FP32
+     * accumulation): the full-precision network the reduced-precision modes are compared against */
+    RTC8X_NRC_FP32 = 3
 } rtc8x_nrc_precision;
 
 typedef enum rtc8x_nrc_encoding {

@@ -281,7 +281,7 @@ class HosekBand(C.Structure):
 
 SKY_BANDS = 16
 
-NRC_FP16, NRC_BF16, NRC_BF16_FP8 = 0, 1, 2
+NRC_FP16, NRC_BF16, NRC_BF16_FP8, NRC_FP32 = 0, 1, 2, 3
 NRC_TRIANGLE_WAVE, NRC_HASH_GRID = 0, 1
 
 

@@ -5,7 +5,7 @@
 // Forward (inference and training): a 256-lane workgroup = 4 wavefronts x 64 samples.  Each lane
 // encodes one query into 64 features and writes them to its wavefront's LDS tile; every layer is then
 // 64x64x64 per wavefront = 4 (samples) x 4 (outputs) x 2 (K) v_mfma_f32_16x16x32 {f16,bf16}
-// instructions with FP32 accumulation.  A operands (activations) come from LDS, B operands (weights,
+// instructions with FP32 accumulation (FP32 weights: 8 v_mfma_f32_16x16x4_f32 per 16x16x32 step).  A operands (activations) come from LDS, B operands (weights,
 // [out][in] rows) straight from global memory, where the ~45 KB of weights stay L1/L2-resident for
 // the whole launch.  ReLU and the down-conversion are fused into the write-back to LDS.
 //
@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "rtc8x_common.hpp"
@@ -29,6 +30,7 @@ namespace nrc {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
@@ -41,9 +43,23 @@ template <>
 struct Vec8<__bf16> {
     using type = b8;
 };
+template <>
+struct Vec8<float> {
+    using type = f8;
+};
 
 __device__ __forceinline__ f4 mfma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+// A 16x16x32 step in FP32.  In the 16x16x32 layouts lane l holds row (l & 15) and the eight K indices
+// 8 (l >> 4) + j; v_mfma_f32_16x16x4_f32 takes one K index per lane, (l >> 4), with the same row and
+// output layouts.  Instruction j therefore covers the K indices 8 g + j (g = 0..3), and the eight of
+// them together cover all 32.
+__device__ __forceinline__ f4 mfma(f8 a, f8 b, f4 c)
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
+    return c;
+}
 
 constexpr uint32_t kWidth = 64;
 constexpr uint32_t kLayer = kWidth * kWidth;             // 4096 weights per hidden layer
@@ -523,8 +539,10 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
                                                        const HashGeom geom, float* __restrict__ ggrid)
 {
     using V = typename Vec8<T>::type;
-    __shared__ __align__(16) T dzs[256 * kActStride];   // dZ, sample-major [sample][out]
-    __shared__ __align__(16) T dzf[64 * kColStride];    // dZ, feature-major [out][sample]
+    // FP32 keeps dZ only feature-major, so that the three tiles fit the LDS
+    constexpr bool kSampleMajor = sizeof(T) == 2;
+    __shared__ __align__(16) T dzs[kSampleMajor ? 256 * kActStride : 8];   // dZ, sample-major [sample][out]
+    __shared__ __align__(16) T dzf[64 * kColStride];                       // dZ, feature-major [out][sample]
     __shared__ __align__(16) T hf[64 * kColStride];     // layer input H, feature-major [in][sample]
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t r16 = lane & 15, kq = 8 * (lane >> 4);
@@ -534,7 +552,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
     for (uint32_t idx = tid; idx < 256 * 64; idx += 256) {
         const uint32_t s = idx >> 6, c = idx & 63;
         const T v = (c < kOutPad && b0 + s < n) ? dout[(size_t)(b0 + s) * kOutPad + c] : (T)0.0f;
-        dzs[s * kActStride + c] = v;
+        if constexpr (kSampleMajor) dzs[s * kActStride + c] = v;
         dzf[c * kColStride + s] = v;
     }
     __syncthreads();
@@ -574,8 +592,14 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
         for (int ks = 0; ks < ksteps; ++ks) {
             V a[4];
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-                a[mt] = *reinterpret_cast<const V*>(&dzs[(wave * 64 + mt * 16 + r16) * kActStride + ks * 32 + kq]);
+            for (int mt = 0; mt < 4; ++mt) {
+                if constexpr (kSampleMajor) {
+                    a[mt] = *reinterpret_cast<const V*>(&dzs[(wave * 64 + mt * 16 + r16) * kActStride + ks * 32 + kq]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) a[mt][j] = dzf[(ks * 32 + kq + j) * kColStride + wave * 64 + mt * 16 + r16];
+                }
+            }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 const V b = *reinterpret_cast<const V*>(&WT[(nt * 16 + r16) * ldk + ks * 32 + kq]);
@@ -626,7 +650,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
                     const uint32_t s = wave * 64 + mt * 16 + 4 * (lane >> 4) + i, c = nt * 16 + r16;
                     const float hv = (float)hf[c * kColStride + s];
                     const T v = (T)(hv > 0.0f ? acc[mt][nt][i] : 0.0f);
-                    dzs[s * kActStride + c] = v;
+                    if constexpr (kSampleMajor) dzs[s * kActStride + c] = v;
                     dzf[c * kColStride + s] = v;
                 }
         __syncthreads();
@@ -806,8 +830,18 @@ hipError_t launch_convert(rtc8x_nrc* h, hipStream_t s)
     return hipGetLastError();
 }
 
-// training precision: BF16 also under FP8 inference
-bool is_bf16(const rtc8x_nrc* h) { return h->desc.precision != RTC8X_NRC_FP16; }
+// Calls f with a null pointer of the network's working type: FP16, FP32, or BF16 (also under FP8
+// inference, whose training runs in BF16).
+template <typename F>
+hipError_t with_type(const rtc8x_nrc* h, F&& f)
+{
+    switch (h->desc.precision) {
+    case RTC8X_NRC_FP16: return f(static_cast<_Float16*>(nullptr));
+    case RTC8X_NRC_FP32: return f(static_cast<float*>(nullptr));
+    default: return f(static_cast<__bf16*>(nullptr));
+    }
+}
+#define RTC8X_WITH_TYPE(h, call) with_type(h, [&](auto tag) { using T = std::remove_pointer_t<decltype(tag)>; return call; })
 bool is_fp8(const rtc8x_nrc* h) { return h->desc.precision == RTC8X_NRC_BF16_FP8; }
 
 hipError_t launch_forward_fp8(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n, float* out)
@@ -830,7 +864,7 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
     RTC8X_HIP(hipMemset(h->v, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->ema, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->wt, 0, h->tsize * h->PT));
-    RTC8X_HIP(is_bf16(h) ? launch_convert<__bf16>(h, nullptr) : launch_convert<_Float16>(h, nullptr));
+    RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_convert<T>(h, nullptr)));
     RTC8X_HIP(hipDeviceSynchronize());
     h->step = 0;
     return RTC8X_OK;
@@ -839,11 +873,9 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
 // forward (train) + backward + ordered reduction into h->grad; returns the loss sum in *loss_sum when set
 rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const float* target, uint32_t n)
 {
-    const bool bf = is_bf16(h);
-    RTC8X_HIP(bf ? launch_forward<__bf16>(h, true, s, in, n, nullptr, target)
-                 : launch_forward<_Float16>(h, true, s, in, n, nullptr, target));
+    RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, true, s, in, n, nullptr, target)));
     if (h->P_grid) RTC8X_HIP(hipMemsetAsync(h->grad + h->P_mlp, 0, sizeof(float) * h->P_grid, s));
-    RTC8X_HIP(bf ? launch_backward<__bf16>(h, s, in, n) : launch_backward<_Float16>(h, s, in, n));
+    RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_backward<T>(h, s, in, n)));
     RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->gpart, (n + 255) / 256,
                        h->P_mlp, h->P, h->grad, h->bad);
@@ -869,7 +901,8 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
     RTC8X_REQUIRE(d && out, "rtc8x_nrc_create: NULL argument");
     RTC8X_REQUIRE(d->num_hidden_layers >= 1 && d->num_hidden_layers <= nrc::kMaxLayers,
                   "rtc8x_nrc_create: num_hidden_layers must be in [1, %u]", nrc::kMaxLayers);
-    RTC8X_REQUIRE(d->precision == RTC8X_NRC_FP16 || d->precision == RTC8X_NRC_BF16 || d->precision == RTC8X_NRC_BF16_FP8,
+    RTC8X_REQUIRE(d->precision == RTC8X_NRC_FP16 || d->precision == RTC8X_NRC_BF16 || d->precision == RTC8X_NRC_BF16_FP8 ||
+                      d->precision == RTC8X_NRC_FP32,
                   "rtc8x_nrc_create: bad precision");
     RTC8X_REQUIRE(d->learning_rate > 0.0f && d->beta1 >= 0.0f && d->beta1 < 1.0f && d->beta2 >= 0.0f && d->beta2 < 1.0f &&
                       d->epsilon > 0.0f && d->ema_decay >= 0.0f && d->ema_decay < 1.0f && d->l2_reg >= 0.0f,
@@ -879,6 +912,7 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
     *out = nullptr;
     rtc8x_nrc* h = new rtc8x_nrc();
     h->desc = *d;
+    h->tsize = d->precision == RTC8X_NRC_FP32 ? 4 : 2;
     h->enc = d->encoding;
     h->L = d->num_hidden_layers;
     h->P_mlp = nrc::num_params(h->L);
@@ -965,8 +999,7 @@ extern "C" rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* h, void* stream, const float*
     if (is_fp8(h))
         RTC8X_HIP(launch_forward_fp8(h, s, in, n, out));
     else
-        RTC8X_HIP(is_bf16(h) ? launch_forward<__bf16>(h, false, s, in, n, out, nullptr)
-                             : launch_forward<_Float16>(h, false, s, in, n, out, nullptr));
+        RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, false, s, in, n, out, nullptr)));
     return RTC8X_OK;
 }
 
@@ -1006,7 +1039,7 @@ extern "C" rtc8x_status rtc8x_nrc_train(rtc8x_nrc* h, void* stream, const float*
     a.P = h->P;
     a.P_mlp = h->P_mlp;
     a.L = h->L;
-    RTC8X_HIP(is_bf16(h) ? launch_adam<__bf16>(h, s, a) : launch_adam<_Float16>(h, s, a));
+    RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_adam<T>(h, s, a)));
     if (loss_on_cpu) {
         const uint32_t nw = 4 * ((n + 255) / 256);
         std::vector<float> parts(nw);

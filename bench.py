@@ -516,10 +516,13 @@ def main() -> int:
           file=sys.stderr, flush=True)
     if args.nrc_steps > 0:
         print("[bench] NRC network leg ...", file=sys.stderr, flush=True)
-        mlp = nrc_mlp_leg(scene, args.nrc_steps)
-        mlp_hash = nrc_mlp_leg(scene, args.nrc_steps, hidden_layers=2, encoding=1)
-        mlp_fp8 = nrc_mlp_leg(scene, args.nrc_steps, precision=_abi.NRC_BF16_FP8)
-        mlp_fp32 = nrc_mlp_leg(scene, args.nrc_steps, precision=_abi.NRC_FP32)
+        # the network's inputs are drawn over the active leaves of the dense 128^3 cloud
+        net_scene = sc.DeviceScene(sc.CONFIGS["dense128"])
+        mlp = nrc_mlp_leg(net_scene, args.nrc_steps)
+        mlp_hash = nrc_mlp_leg(net_scene, args.nrc_steps, hidden_layers=2, encoding=1)
+        mlp_fp8 = nrc_mlp_leg(net_scene, args.nrc_steps, precision=_abi.NRC_BF16_FP8)
+        mlp_fp32 = nrc_mlp_leg(net_scene, args.nrc_steps, precision=_abi.NRC_FP32)
+        net_scene.close()
         print(f"[bench] NRC network: infer {mlp['infer_ms']:.2f} ms / 2^21, train {mlp['train_ms']:.2f} ms / 2^16; "
               f"e4m3 inference {mlp_fp8['infer_ms']:.2f} ms", file=sys.stderr, flush=True)
         print("[bench] NRC render leg ...", file=sys.stderr, flush=True)

@@ -1,11 +1,14 @@
 """The batch renderer's scene files and keyframes, without a GPU (rtc8x_render -dump).
 
-The reference's own scene scripts (tests/golden/scenes/reference_test{0,1}.txt, copies of rtcamp8's
-data/test{0,1}.txt) are parsed with the same outcome as the reference: their mesh paths are the author's
-Windows files, so loading fails on the mesh line with the reference's message; with those paths pointing
-at files that exist they parse, and their keyframes evaluate as the reference's interpolation defines
-(keys reproduced exactly at their times, linear field of view, solar direction rotating at a constant
-rate about one axis)."""
+The scene files under tests/golden/scenes/ use the commands of rtcamp8's scene language
+(renderer/renderer_scene.cpp:1732-2290): a sweeping analytic sky with a surface mesh instance
+(sky_keys.txt) and a camera that descends, zooms and pans (camera_keys.txt). Their mesh paths name files
+that do not exist, so loading fails on the mesh line with the reference's message; with those paths
+pointing at files that exist they parse, and their keyframes evaluate as the reference's interpolation
+defines (keys reproduced exactly at their times, linear field of view, solar direction rotating at a
+constant rate about one axis, Catmull-Rom positions). When the reference tree is mounted, its own
+scene scripts data/test{0,1}.txt are read from there and fail on their missing meshes at the
+reference's lines."""
 import json
 import math
 import re
@@ -55,48 +58,62 @@ def with_local_meshes(src: Path, dst_dir: Path) -> Path:
     return out
 
 
-@pytest.mark.parametrize("name,line", [("reference_test0.txt", 22), ("reference_test1.txt", 4)])
-def test_reference_scenes_fail_on_their_missing_meshes(name, line):
+@pytest.mark.parametrize("name,line,mesh", [("sky_keys.txt", 14, "missing_box"),
+                                            ("camera_keys.txt", 4, "missing_hall")])
+def test_scenes_fail_on_their_missing_meshes(name, line, mesh):
     r = run("-scene", SCENES / name, "-dump")
+    assert r.returncode == 1
+    assert re.search(rf"l\.{line}: Mesh .*{mesh}\.obj does not exist\.", r.stderr), r.stderr
+
+
+REFERENCE_DATA = Path("/root/reference/data")
+
+
+@pytest.mark.skipif(not REFERENCE_DATA.is_dir(), reason="reference tree not mounted")
+@pytest.mark.parametrize("name,line", [("test0.txt", 22), ("test1.txt", 4)])
+def test_reference_scenes_fail_on_their_missing_meshes(name, line):
+    r = run("-scene", REFERENCE_DATA / name, "-dump")
     assert r.returncode == 1
     assert re.search(rf"l\.{line}: Mesh .*(Cornell_box_RB1|sponza)\.obj does not exist\.", r.stderr), r.stderr
 
 
-def test_reference_test0_sky_keyframes(tmp_path):
-    scene = with_local_meshes(SCENES / "reference_test0.txt", tmp_path)
-    r = run("-scene", scene, "-dump")  # the Cornell box instance is a surface
+def test_sky_keyframes(tmp_path):
+    scene = with_local_meshes(SCENES / "sky_keys.txt", tmp_path)
+    r = run("-scene", scene, "-dump")  # the box instance is a surface
     assert r.returncode == 1 and "surface meshes" in r.stderr
     d = dump(scene, "-ignore-surfaces")
-    assert d["image"] == [1920, 1080] and d["time"] == [0, 10] and d["fps"] == 60
+    assert d["image"] == [1280, 720] and d["time"] == [0, 8] and d["fps"] == 30
     assert d["environment"] == "analytic_sky" and d["instances"] == 1
     frames = d["frames"]
-    assert len(frames) == 601  # t = 0, 1/60, ..., 10
-    keys = {0: polar_y_up(85, 85), 300: polar_y_up(0, 25), 600: polar_y_up(-85, 85)}
+    assert len(frames) == 241  # t = 0, 1/30, ..., 8
+    keys = {0: polar_y_up(60, 70), 120: polar_y_up(-20, 40), 240: polar_y_up(-100, 75)}
     for i, want in keys.items():
         assert np.allclose(frames[i]["sun"], want, atol=2e-6), i
     for f in frames:
         # (1 - t) a + t b of equal keys is a within float rounding
-        assert abs(np.linalg.norm(f["sun"]) - 1) < 1e-5 and abs(f["env_coeff"] - 5) < 1e-6
+        assert abs(np.linalg.norm(f["sun"]) - 1) < 1e-5 and abs(f["env_coeff"] - 3) < 1e-6
         assert abs(f["turbidity"] - 4) < 1e-6
         assert abs(f["solar_elevation"] - (math.pi / 2 - math.acos(f["sun"][1]))) < 1e-5
     # between two keys the sun turns at a constant rate about one axis
-    s0, s1 = np.array(frames[0]["sun"]), np.array(frames[300]["sun"])
-    total = math.acos(np.clip(s0 @ s1, -1, 1))
-    for i in (75, 150, 225):
-        si = np.array(frames[i]["sun"])
-        assert abs(math.acos(np.clip(s0 @ si, -1, 1)) - total * i / 300) < 2e-5
-    # the camera: one key at t = 0 (lookat 0 0 3.5 -> origin, fovy 50)
+    for a, b in ((0, 120), (120, 240)):
+        s0, s1 = np.array(frames[a]["sun"]), np.array(frames[b]["sun"])
+        total = math.acos(np.clip(s0 @ s1, -1, 1))
+        for i in (a + 30, a + 60, a + 90):
+            si = np.array(frames[i]["sun"])
+            assert abs(math.acos(np.clip(s0 @ si, -1, 1)) - total * (i - a) / 120) < 2e-5
+    # the camera: one key at t = 0 (lookat 0 1 4 -> origin, fovy 45)
+    look = np.array([0.0, -1.0, -4.0]) / math.sqrt(17.0)
     for f in (frames[0], frames[-1]):
-        assert np.allclose(f["position"], [0, 0, 3.5]) and abs(f["fov_y"] - math.radians(50)) < 1e-6
-        assert np.allclose(rotate(f["orientation"], [0, 0, -1]), [0, 0, -1], atol=1e-6)
+        assert np.allclose(f["position"], [0, 1, 4]) and abs(f["fov_y"] - math.radians(45)) < 1e-6
+        assert np.allclose(rotate(f["orientation"], [0, 0, -1]), look, atol=1e-6)
 
 
-def test_reference_test1_camera_keyframes(tmp_path):
-    d = dump(with_local_meshes(SCENES / "reference_test1.txt", tmp_path), "-ignore-surfaces")
+def test_camera_keyframes(tmp_path):
+    d = dump(with_local_meshes(SCENES / "camera_keys.txt", tmp_path), "-ignore-surfaces")
     frames = d["frames"]
-    assert d["environment"] == "none" and d["instances"] == 2 and len(frames) == 601
-    keys = {0: ((0, 10, 0), (1, 3, 0), 70), 120: ((0, 3, 0), (1, 3, 0), 50), 180: ((0, 3, 0), (1, 3, 0), 20),
-            300: ((0, 3, 0), (0, 3, 1), 50), 600: ((0, 3, 0), (-1, 3, 0), 50)}
+    assert d["environment"] == "none" and d["instances"] == 2 and len(frames) == 121
+    keys = {0: ((0, 6, 0), (2, 2, 0), 60), 30: ((0, 2, 0), (2, 2, 0), 40), 50: ((0, 2, 0), (2, 2, 0), 30),
+            80: ((0, 2, 0), (0, 2, 2), 40), 120: ((0, 2, 0), (-2, 2, 0), 40)}
     for i, (pos, at, fov) in keys.items():
         f = frames[i]
         assert np.allclose(f["position"], pos, atol=1e-5), i
@@ -104,14 +121,14 @@ def test_reference_test1_camera_keyframes(tmp_path):
         look = np.subtract(at, pos, dtype=np.float64)
         assert np.allclose(rotate(f["orientation"], [0, 0, -1]), look / np.linalg.norm(look), atol=2e-6), i
         assert abs(np.linalg.norm(f["orientation"]) - 1) < 1e-6
-    # field of view is linear between keys (t = 2 .. 3 s: 50 -> 20 degrees)
-    for i in range(120, 181, 15):
-        assert abs(frames[i]["fov_y"] - math.radians(50 + (20 - 50) * (i - 120) / 60)) < 2e-6
-    # from t = 3 s every segment's four control points lie on y = 3, and so does the Catmull-Rom curve;
-    # on 2 .. 3 s the tangent at t = 2 is (3 - 10) / 2 per key interval, which gives the cubic
-    # 3 - 3.5 u + 7 u^2 - 3.5 u^3 in y: 2.5625 at the middle
-    assert all(abs(f["position"][1] - 3) < 1e-5 for f in frames[180:])
-    assert abs(frames[150]["position"][1] - 2.5625) < 1e-5
+    # field of view is linear between keys (t = 1.5 .. 2.5 s: 40 -> 30 degrees)
+    for i in range(30, 51, 5):
+        assert abs(frames[i]["fov_y"] - math.radians(40 + (30 - 40) * (i - 30) / 20)) < 2e-6
+    # from t = 2.5 s every segment's four control points lie on y = 2, and so does the Catmull-Rom curve;
+    # on 1.5 .. 2.5 s the tangent at t = 1.5 is (2 - 6) / 2 per key interval and the one at 2.5 is 0, which
+    # gives the cubic 2 - 2 u + 4 u^2 - 2 u^3 in y: 1.75 at the middle (t = 2, frame 40)
+    assert all(abs(f["position"][1] - 2) < 1e-5 for f in frames[50:])
+    assert abs(frames[40]["position"][1] - 1.75) < 1e-5
 
 
 def test_active_camera_switching_and_playback_speed(tmp_path):

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -91,3 +93,20 @@ inline NrcScratchLayout nrc_scratch_layout(uint32_t width, uint32_t height)
     do {                                                                      \
         if (!(cond)) return rtc8x::fail(RTC8X_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
     } while (0)
+
+// The handler of an entry point's function-try-block: no C++ exception crosses the C ABI. A failed host
+// allocation (a std::vector sized from a file header, say) becomes RTC8X_ERR_OUT_OF_MEMORY, anything else
+// RTC8X_ERR_INVALID_ARGUMENT, with the entry point's name in rtc8x_last_error().
+#define RTC8X_CATCH_ALL(name)                                                                     \
+    catch (const std::bad_alloc&)                                                                 \
+    {                                                                                             \
+        return rtc8x::fail(RTC8X_ERR_OUT_OF_MEMORY, "%s: out of host memory", name);               \
+    }                                                                                             \
+    catch (const std::exception& e_)                                                              \
+    {                                                                                             \
+        return rtc8x::fail(RTC8X_ERR_INVALID_ARGUMENT, "%s: %s", name, e_.what());                 \
+    }                                                                                             \
+    catch (...)                                                                                   \
+    {                                                                                             \
+        return rtc8x::fail(RTC8X_ERR_INVALID_ARGUMENT, "%s: unknown C++ exception", name);         \
+    }

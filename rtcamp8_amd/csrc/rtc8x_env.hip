@@ -109,7 +109,7 @@ void build_alias(const double* w, uint32_t n, rtc8x_alias_entry* out)
 }  // namespace
 
 extern "C" rtc8x_status rtc8x_env_release(rtc8x_env_light* env)
-{
+try {
     RTC8X_REQUIRE(env != nullptr, "rtc8x_env_release: env is NULL");
     const void* ptrs[] = {env->mips, env->alias_rows, env->alias_cols, env->alias_weights};
     for (const void* p : ptrs)
@@ -123,9 +123,10 @@ extern "C" rtc8x_status rtc8x_env_release(rtc8x_env_light* env)
     env->alias_integral = 0.0f;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_env_release")
 
 extern "C" rtc8x_status rtc8x_env_build_sampler(rtc8x_env_light* env, void* stream)
-{
+try {
     RTC8X_REQUIRE(env != nullptr, "rtc8x_env_build_sampler: env is NULL");
     if (env->kind != RTC8X_ENV_LATLONG) return RTC8X_OK;
     RTC8X_REQUIRE(env->texels && env->width > 0 && env->height > 0 && env->width <= 16384 && env->height <= 16384,
@@ -206,3 +207,4 @@ extern "C" rtc8x_status rtc8x_env_build_sampler(rtc8x_env_light* env, void* stre
     env->alias_weights = d_w;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_env_build_sampler")

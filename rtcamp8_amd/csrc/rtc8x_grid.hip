@@ -246,7 +246,7 @@ rtc8x_status build_leaf_map(rtc8x_grid& g, hipStream_t s)
 using namespace rtc8x;
 
 extern "C" rtc8x_status rtc8x_grid_destroy(rtc8x_grid* g)
-{
+try {
     RTC8X_REQUIRE(g != nullptr, "rtc8x_grid_destroy: grid is NULL");
     const void* ptrs[] = {g->root_tiles, g->upper_tables, g->lower_tables, g->leaf_values,
                           g->tile_values, g->leaf_origins, g->cell_majorants, g->leaf_map};
@@ -255,9 +255,10 @@ extern "C" rtc8x_status rtc8x_grid_destroy(rtc8x_grid* g)
     std::memset(g, 0, sizeof(*g));
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_grid_destroy")
 
 extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid* out, void* stream)
-{
+try {
     RTC8X_REQUIRE(d && out, "rtc8x_grid_build_fbm: NULL argument");
     const uint32_t n = d->resolution;
     RTC8X_REQUIRE(n >= 8 && n <= 4096 && (n % 8) == 0, "rtc8x_grid_build_fbm: resolution %u must be a multiple of 8 in [8, 4096]", n);
@@ -408,6 +409,7 @@ extern "C" rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid
     *out = g;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_grid_build_fbm")
 
 namespace rtc8x {
 
@@ -518,12 +520,13 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
 }  // namespace rtc8x
 
 extern "C" rtc8x_status rtc8x_grid_upload(const rtc8x_grid* h, rtc8x_grid* out)
-{
+try {
     return rtc8x::upload_host_grid(h, out, false);
 }
+RTC8X_CATCH_ALL("rtc8x_grid_upload")
 
 extern "C" rtc8x_status rtc8x_grid_download(const rtc8x_grid* d, rtc8x_grid* out)
-{
+try {
     RTC8X_REQUIRE(d && out, "rtc8x_grid_download: NULL argument");
     RTC8X_REQUIRE(d->num_root_tiles > 0 && d->root_tiles && d->tile_values, "rtc8x_grid_download: grid has no root");
     rtc8x::OwnedHostGrid t;
@@ -566,10 +569,11 @@ extern "C" rtc8x_status rtc8x_grid_download(const rtc8x_grid* d, rtc8x_grid* out
     }
     return rtc8x::adopt_host_grid(std::move(t), g, out);
 }
+RTC8X_CATCH_ALL("rtc8x_grid_download")
 
 extern "C" rtc8x_status rtc8x_grid_probe(const rtc8x_grid* g, uint32_t n, const float* world_xyz, int32_t* out_ijk,
                                          uint32_t* out_offsets, float* out_value, void* stream)
-{
+try {
     RTC8X_REQUIRE(g && world_xyz && out_ijk && out_offsets && out_value, "rtc8x_grid_probe: NULL argument");
     RTC8X_REQUIRE(g->num_root_tiles > 0 && g->root_tiles && g->tile_values, "rtc8x_grid_probe: grid has no root");
     if (n == 0) return RTC8X_OK;
@@ -578,3 +582,4 @@ extern "C" rtc8x_status rtc8x_grid_probe(const rtc8x_grid* g, uint32_t n, const 
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_grid_probe")

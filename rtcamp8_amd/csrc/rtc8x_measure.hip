@@ -66,7 +66,7 @@ __global__ void gather_probe_kernel(const rtc8x_grid g, uint32_t steps, uint32_t
 using namespace rtc8x;
 
 extern "C" rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* stream)
-{
+try {
     if (bytes == 0) return RTC8X_OK;
     RTC8X_REQUIRE(dst && src, "rtc8x_copy_async: NULL pointer");
     RTC8X_REQUIRE(((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0, "rtc8x_copy_async: needs 16-byte alignment");
@@ -77,10 +77,11 @@ extern "C" rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t byte
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_copy_async")
 
 extern "C" rtc8x_status rtc8x_gather_probe(void* stream, const rtc8x_grid* grid, uint32_t threads, uint32_t steps,
                                            uint32_t width, uint32_t waves_per_simd, uint32_t seed, float* out)
-{
+try {
     RTC8X_REQUIRE(grid && out, "rtc8x_gather_probe: NULL pointer");
     const rtc8x_grid& g = *grid;
     RTC8X_REQUIRE(g.num_leaves > 0 && g.leaf_origins && g.leaf_values && g.tile_values,
@@ -110,3 +111,4 @@ extern "C" rtc8x_status rtc8x_gather_probe(void* stream, const rtc8x_grid* grid,
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_gather_probe")

@@ -249,7 +249,7 @@ extern "C" size_t rtc8x_nrc_scratch_bytes(uint32_t width, uint32_t height)
 }
 
 extern "C" rtc8x_status rtc8x_nrc_prepare(void* stream, const rtc8x_frame_params* p, int32_t is_new_sequence)
-{
+try {
     rtc8x_status st = check_nrc(p, "rtc8x_nrc_prepare");
     if (st != RTC8X_OK) return st;
     const rtc8x_nrc_buffers& nb = p->nrc;
@@ -262,9 +262,10 @@ extern "C" rtc8x_status rtc8x_nrc_prepare(void* stream, const rtc8x_frame_params
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_prepare")
 
 extern "C" rtc8x_status rtc8x_nrc_propagate(void* stream, const rtc8x_frame_params* p)
-{
+try {
     rtc8x_status st = check_nrc(p, "rtc8x_nrc_propagate");
     if (st != RTC8X_OK) return st;
     const rtc8x_nrc_buffers& nb = p->nrc;
@@ -282,9 +283,10 @@ extern "C" rtc8x_status rtc8x_nrc_propagate(void* stream, const rtc8x_frame_para
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_propagate")
 
 extern "C" rtc8x_status rtc8x_nrc_shuffle(void* stream, const rtc8x_frame_params* p)
-{
+try {
     rtc8x_status st = check_nrc(p, "rtc8x_nrc_shuffle");
     if (st != RTC8X_OK) return st;
     const rtc8x_nrc_buffers& nb = p->nrc;
@@ -300,9 +302,10 @@ extern "C" rtc8x_status rtc8x_nrc_shuffle(void* stream, const rtc8x_frame_params
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_shuffle")
 
 extern "C" rtc8x_status rtc8x_nrc_accumulate(void* stream, const rtc8x_frame_params* p)
-{
+try {
     rtc8x_status st = check_nrc(p, "rtc8x_nrc_accumulate");
     if (st != RTC8X_OK) return st;
     const rtc8x_nrc_buffers& nb = p->nrc;
@@ -316,3 +319,4 @@ extern "C" rtc8x_status rtc8x_nrc_accumulate(void* stream, const rtc8x_frame_par
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_accumulate")

@@ -508,7 +508,7 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
             for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt)
-                    a[ks][mt] = *reinterpret_cast<const long*>(&A8[(mt * 16 + r16) * kRow8 + ks * 32 + kq]);
+                    __builtin_memcpy(&a[ks][mt], &A8[(mt * 16 + r16) * kRow8 + ks * 32 + kq], sizeof(long));
         }
         const float unscale = w_inv[l] / sa;
         // Weights as the A operand, activations as B: the product is the layer's output transposed, so
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
                     int v = 0;
                     v = __builtin_amdgcn_cvt_pk_fp8_f32(acc[mt][nt][0] * sa, acc[mt][nt][1] * sa, v, false);
                     v = __builtin_amdgcn_cvt_pk_fp8_f32(acc[mt][nt][2] * sa, acc[mt][nt][3] * sa, v, true);
-                    *reinterpret_cast<int*>(&A8[(mt * 16 + r16) * kRow8 + nt * 16 + 4 * (lane >> 4)]) = v;
+                    __builtin_memcpy(&A8[(mt * 16 + r16) * kRow8 + nt * 16 + 4 * (lane >> 4)], &v, sizeof(int));
                 }
         } else if ((lane >> 4) == 0) {
 #pragma unroll
@@ -923,7 +923,7 @@ rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const f
 }  // namespace
 
 extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
-{
+try {
     if (!h) return RTC8X_OK;
     void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout,
                     h->grid_inf, h->w8, h->w8_inv};
@@ -932,9 +932,10 @@ extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
     delete h;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_destroy")
 
 extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** out)
-{
+try {
     RTC8X_REQUIRE(d && out, "rtc8x_nrc_create: NULL argument");
     RTC8X_REQUIRE(d->num_hidden_layers >= 1 && d->num_hidden_layers <= nrc::kMaxLayers,
                   "rtc8x_nrc_create: num_hidden_layers must be in [1, %u]", nrc::kMaxLayers);
@@ -1004,11 +1005,12 @@ extern "C" rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* d, rtc8x_nrc** ou
     *out = h;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_create")
 
 extern "C" uint32_t rtc8x_nrc_num_params(const rtc8x_nrc* h) { return h ? h->P : 0; }
 
 extern "C" rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* h, int32_t which, float* host, size_t count)
-{
+try {
     RTC8X_REQUIRE(h && host && count == h->P, "rtc8x_nrc_get_weights: bad arguments");
     if (which == 0 || h->step == 0) {
         RTC8X_HIP(hipMemcpy(host, h->master, 4 * h->P, hipMemcpyDeviceToHost));
@@ -1020,16 +1022,18 @@ extern "C" rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* h, int32_t which,
     for (size_t i = 0; i < count; ++i) host[i] = host[i] / fbc;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_get_weights")
 
 extern "C" rtc8x_status rtc8x_nrc_set_weights(rtc8x_nrc* h, const float* host, size_t count)
-{
+try {
     RTC8X_REQUIRE(h && host && count == h->P, "rtc8x_nrc_set_weights: bad arguments");
     RTC8X_HIP(hipMemcpy(h->master, host, 4 * h->P, hipMemcpyHostToDevice));
     return reset_from_master(h);
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_set_weights")
 
 extern "C" rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* h, void* stream, const float* in, uint32_t n, float* out)
-{
+try {
     RTC8X_REQUIRE(h && in && out, "rtc8x_nrc_infer: NULL argument");
     if (n == 0) return RTC8X_OK;
     hipStream_t s = (hipStream_t)stream;
@@ -1039,10 +1043,11 @@ extern "C" rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* h, void* stream, const float*
         RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, false, s, in, n, out, nullptr)));
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_infer")
 
 extern "C" rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* h, void* stream, const float* in, const float* target, uint32_t n,
                                            float* grad)
-{
+try {
     RTC8X_REQUIRE(h && in && target && grad, "rtc8x_nrc_gradient: NULL argument");
     RTC8X_REQUIRE(n > 0 && n <= nrc::kMaxTrain, "rtc8x_nrc_gradient: n must be in [1, %u]", nrc::kMaxTrain);
     hipStream_t s = (hipStream_t)stream;
@@ -1051,10 +1056,11 @@ extern "C" rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* h, void* stream, const flo
     RTC8X_HIP(hipMemcpyAsync(grad, h->grad, 4 * h->P, hipMemcpyDeviceToDevice, s));
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_gradient")
 
 extern "C" rtc8x_status rtc8x_nrc_train(rtc8x_nrc* h, void* stream, const float* in, const float* target, uint32_t n,
                                         float* loss_on_cpu)
-{
+try {
     RTC8X_REQUIRE(h && in && target, "rtc8x_nrc_train: NULL argument");
     RTC8X_REQUIRE(n > 0 && n <= nrc::kMaxTrain, "rtc8x_nrc_train: n must be in [1, %u]", nrc::kMaxTrain);
     hipStream_t s = (hipStream_t)stream;
@@ -1088,9 +1094,10 @@ extern "C" rtc8x_status rtc8x_nrc_train(rtc8x_nrc* h, void* stream, const float*
     }
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_train")
 
 extern "C" rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t n, float* out)
-{
+try {
     RTC8X_REQUIRE(in && out, "rtc8x_nrc_encode: NULL argument");
     if (n == 0) return RTC8X_OK;
     nrc::HashGeom g{};
@@ -1099,10 +1106,11 @@ extern "C" rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_encode")
 
 extern "C" rtc8x_status rtc8x_nrc_encode_net(const rtc8x_nrc* h, void* stream, const float* in, uint32_t n, float* out,
                                              int32_t which)
-{
+try {
     RTC8X_REQUIRE(h && in && out && (which == 0 || which == 1), "rtc8x_nrc_encode_net: bad arguments");
     if (n == 0) return RTC8X_OK;
     const float* tables = which == 0 ? h->master + h->P_mlp : h->grid_inf;
@@ -1115,3 +1123,4 @@ extern "C" rtc8x_status rtc8x_nrc_encode_net(const rtc8x_nrc* h, void* stream, c
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nrc_encode_net")

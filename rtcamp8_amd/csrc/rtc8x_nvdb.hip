@@ -474,6 +474,11 @@ rtc8x_status grid_bytes(const uint8_t* f, const rtc8x_nvdb_grid_info& info, std:
         if (info.file_size < 8) return fail(RTC8X_ERR_INVALID_ARGUMENT, "nvdb: truncated ZIP block");
         const uint64_t zbytes = rd<uint64_t>(f + info.file_offset);
         if (zbytes > info.file_size - 8) return fail(RTC8X_ERR_INVALID_ARGUMENT, "nvdb: ZIP block overruns the grid");
+        // deflate expands at most 1032:1 (zlib's maximum ratio), so a header claiming more than that
+        // is corrupt; it is rejected before the buffer is sized from it
+        if (info.grid_size < kGridBytes + kTreeBytes + kRootBytes || info.grid_size / 1032 > zbytes)
+            return fail(RTC8X_ERR_INVALID_ARGUMENT, "nvdb: grid \"%s\" claims %llu bytes from a %llu-byte ZIP block",
+                        info.name, (unsigned long long)info.grid_size, (unsigned long long)zbytes);
         buf.resize(info.grid_size);
         uLongf n = (uLongf)info.grid_size;
         const int zs = uncompress(buf.data(), &n, f + info.file_offset + 8, (uLong)zbytes);
@@ -945,7 +950,7 @@ rtc8x_status adopt_host_grid(OwnedHostGrid&& t, const rtc8x_grid& g0, rtc8x_grid
 using namespace rtc8x;
 
 extern "C" rtc8x_status rtc8x_grid_free_host(rtc8x_grid* g)
-{
+try {
     RTC8X_REQUIRE(g != nullptr, "rtc8x_grid_free_host: grid is NULL");
     {
         std::lock_guard<std::mutex> lock(g_owned_mutex);
@@ -956,15 +961,17 @@ extern "C" rtc8x_status rtc8x_grid_free_host(rtc8x_grid* g)
     std::memset(g, 0, sizeof(*g));
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_grid_free_host")
 
 extern "C" rtc8x_status rtc8x_nanovdb_decode(const void* buf, size_t size, rtc8x_grid* out)
-{
+try {
     return decode_buffer(buf, size, out);
 }
+RTC8X_CATCH_ALL("rtc8x_nanovdb_decode")
 
 extern "C" rtc8x_status rtc8x_nvdb_list(const void* file, size_t size, rtc8x_nvdb_grid_info* infos, uint32_t capacity,
                                         uint32_t* count)
-{
+try {
     RTC8X_REQUIRE(file && count, "rtc8x_nvdb_list: NULL argument");
     RTC8X_REQUIRE(capacity == 0 || infos, "rtc8x_nvdb_list: NULL infos with capacity %u", capacity);
     std::vector<rtc8x_nvdb_grid_info> v;
@@ -974,9 +981,10 @@ extern "C" rtc8x_status rtc8x_nvdb_list(const void* file, size_t size, rtc8x_nvd
     *count = (uint32_t)v.size();
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nvdb_list")
 
 extern "C" rtc8x_status rtc8x_nvdb_read_grid(const void* file, size_t size, uint32_t n, rtc8x_grid* out)
-{
+try {
     RTC8X_REQUIRE(out, "rtc8x_nvdb_read_grid: NULL output");
     std::vector<uint8_t> buf;
     const uint8_t* data = nullptr;
@@ -985,9 +993,10 @@ extern "C" rtc8x_status rtc8x_nvdb_read_grid(const void* file, size_t size, uint
     if (st != RTC8X_OK) return st;
     return decode_buffer(data, grid_size, out);
 }
+RTC8X_CATCH_ALL("rtc8x_nvdb_read_grid")
 
 extern "C" rtc8x_status rtc8x_grid_upload_nanovdb(const void* buf, size_t size, rtc8x_grid* out)
-{
+try {
     RTC8X_REQUIRE(out, "rtc8x_grid_upload_nanovdb: NULL output");
     rtc8x_grid host;
     rtc8x_status st = decode_buffer(buf, size, &host);
@@ -996,9 +1005,10 @@ extern "C" rtc8x_status rtc8x_grid_upload_nanovdb(const void* buf, size_t size, 
     rtc8x_grid_free_host(&host);
     return st;
 }
+RTC8X_CATCH_ALL("rtc8x_grid_upload_nanovdb")
 
 extern "C" rtc8x_status rtc8x_grid_load_nvdb(const void* file, size_t size, uint32_t n, rtc8x_grid* out)
-{
+try {
     RTC8X_REQUIRE(out, "rtc8x_grid_load_nvdb: NULL output");
     rtc8x_grid host;
     rtc8x_status st = rtc8x_nvdb_read_grid(file, size, n, &host);
@@ -1007,10 +1017,11 @@ extern "C" rtc8x_status rtc8x_grid_load_nvdb(const void* file, size_t size, uint
     rtc8x_grid_free_host(&host);
     return st;
 }
+RTC8X_CATCH_ALL("rtc8x_grid_load_nvdb")
 
 extern "C" rtc8x_status rtc8x_nanovdb_encode(const rtc8x_grid* h, const char* name, void* out, size_t capacity,
                                              size_t* size)
-{
+try {
     RTC8X_REQUIRE(h && size, "rtc8x_nanovdb_encode: NULL argument");
     const char* nm = name ? name : "";
     RTC8X_REQUIRE(std::strlen(nm) < 256, "rtc8x_nanovdb_encode: grid name longer than 255 bytes");
@@ -1027,10 +1038,11 @@ extern "C" rtc8x_status rtc8x_nanovdb_encode(const rtc8x_grid* h, const char* na
                   (unsigned long long)e.total);
     return e.write((uint8_t*)out, nm);
 }
+RTC8X_CATCH_ALL("rtc8x_nanovdb_encode")
 
 extern "C" rtc8x_status rtc8x_nvdb_write(const void* const* grids, const size_t* sizes, uint32_t count, uint32_t codec,
                                          void* out, size_t capacity, size_t* size)
-{
+try {
     RTC8X_REQUIRE(grids && sizes && size && count > 0, "rtc8x_nvdb_write: NULL argument or no grids");
     RTC8X_REQUIRE(count <= 0xFFFFu, "rtc8x_nvdb_write: %u grids exceed a segment's 65535", count);
     RTC8X_REQUIRE(codec == RTC8X_NVDB_CODEC_NONE || codec == RTC8X_NVDB_CODEC_ZIP,
@@ -1111,3 +1123,4 @@ extern "C" rtc8x_status rtc8x_nvdb_write(const void* const* grids, const size_t*
     }
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_nvdb_write")

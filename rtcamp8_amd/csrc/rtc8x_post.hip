@@ -225,7 +225,7 @@ using namespace rtc8x;
 using namespace rtc8x::post;
 
 extern "C" rtc8x_status rtc8x_tone_map(void* stream, const rtc8x_tone_map_params* p)
-{
+try {
     RTC8X_REQUIRE(p && p->accum, "rtc8x_tone_map: NULL argument");
     RTC8X_REQUIRE(p->samples > 0, "rtc8x_tone_map: samples must be positive");
     RTC8X_REQUIRE(p->out_rgba32f || p->out_rgba8, "rtc8x_tone_map: no output");
@@ -238,10 +238,11 @@ extern "C" rtc8x_status rtc8x_tone_map(void* stream, const rtc8x_tone_map_params
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_tone_map")
 
 extern "C" rtc8x_status rtc8x_sdr_convert(const float* data, uint32_t w, uint32_t h, uint32_t nch,
                                           const rtc8x_sdr_config* cfg, uint32_t* out)
-{
+try {
     RTC8X_REQUIRE(cfg && out && (data || (size_t)w * h == 0), "rtc8x_sdr_convert: NULL argument");
     RTC8X_REQUIRE(nch >= 1 && nch <= 4, "rtc8x_sdr_convert: %u channels (1..4 supported)", nch);
     for (uint32_t y = 0; y < h; ++y) {
@@ -255,10 +256,11 @@ extern "C" rtc8x_status rtc8x_sdr_convert(const float* data, uint32_t w, uint32_
     }
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_sdr_convert")
 
 extern "C" rtc8x_status rtc8x_png_encode(const uint32_t* px, uint32_t w, uint32_t h, void* out, size_t cap,
                                          size_t* size)
-{
+try {
     RTC8X_REQUIRE(size && (px || (size_t)w * h == 0), "rtc8x_png_encode: NULL argument");
     RTC8X_REQUIRE(w > 0 && h > 0 && w <= (1u << 24) && h <= (1u << 24), "rtc8x_png_encode: bad size %ux%u", w, h);
     const size_t raw = ((size_t)w * 4 + 1) * h;
@@ -285,10 +287,11 @@ extern "C" rtc8x_status rtc8x_png_encode(const uint32_t* px, uint32_t w, uint32_
     *size = f.size();
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_png_encode")
 
 extern "C" rtc8x_status rtc8x_bmp_encode(const uint32_t* px, uint32_t w, uint32_t h, void* out, size_t cap,
                                          size_t* size)
-{
+try {
     RTC8X_REQUIRE(size && (px || (size_t)w * h == 0), "rtc8x_bmp_encode: NULL argument");
     RTC8X_REQUIRE(w <= (1u << 16) && h <= (1u << 16), "rtc8x_bmp_encode: bad size %ux%u", w, h);
     const uint32_t pad = (uint32_t)(-(int64_t)w * 3) & 3u;
@@ -324,10 +327,11 @@ extern "C" rtc8x_status rtc8x_bmp_encode(const uint32_t* px, uint32_t w, uint32_
     }
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_bmp_encode")
 
 extern "C" rtc8x_status rtc8x_exr_encode(const float* data, uint32_t w, uint32_t h, uint32_t nch, float scale,
                                          int32_t flip_y, void* out, size_t cap, size_t* size)
-{
+try {
     RTC8X_REQUIRE(size && data, "rtc8x_exr_encode: NULL argument");
     RTC8X_REQUIRE(nch >= 1 && nch <= 4, "rtc8x_exr_encode: %u channels (1..4 supported)", nch);
     RTC8X_REQUIRE(w > 0 && h > 0 && w <= (1u << 24) && h <= (1u << 24), "rtc8x_exr_encode: bad size %ux%u", w, h);
@@ -422,3 +426,4 @@ extern "C" rtc8x_status rtc8x_exr_encode(const float* data, uint32_t w, uint32_t
     }
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_exr_encode")

@@ -240,7 +240,7 @@ static rtc8x_status check_sky(const rtc8x_hosek_sky* sky, const float* sun)
 
 extern "C" rtc8x_status rtc8x_sky_bake(void* stream, const rtc8x_hosek_sky* sky, const float sun_dir[3], uint32_t width,
                                        uint32_t height, float* out_rgba)
-{
+try {
     rtc8x_status st = check_sky(sky, sun_dir);
     if (st != RTC8X_OK) return st;
     RTC8X_REQUIRE(out_rgba && width > 0 && height > 0 && width <= 16384 && height <= 16384, "rtc8x_sky_bake: bad output");
@@ -250,10 +250,11 @@ extern "C" rtc8x_status rtc8x_sky_bake(void* stream, const rtc8x_hosek_sky* sky,
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_sky_bake")
 
 extern "C" rtc8x_status rtc8x_sky_eval(void* stream, const rtc8x_hosek_sky* sky, const float sun_dir[3], uint32_t n,
                                        const float* phi, const float* theta, float* out_rgb)
-{
+try {
     rtc8x_status st = check_sky(sky, sun_dir);
     if (st != RTC8X_OK) return st;
     RTC8X_REQUIRE(phi && theta && out_rgb, "rtc8x_sky_eval: NULL buffer");
@@ -263,6 +264,7 @@ extern "C" rtc8x_status rtc8x_sky_eval(void* stream, const rtc8x_hosek_sky* sky,
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_sky_eval")
 
 // ---------------------------------------------------------------------------------------------------
 // The spectral model's tables and its state for any sun elevation and turbidity (host code).
@@ -334,7 +336,7 @@ void free_device(rtc8x_hosek_model* m)
 }  // namespace
 
 extern "C" rtc8x_status rtc8x_hosek_model_create(const void* data, size_t size, rtc8x_hosek_model** out)
-{
+try {
     RTC8X_REQUIRE(data && out, "rtc8x_hosek_model_create: NULL argument");
     RTC8X_REQUIRE(size == kHosekBytes && std::memcmp(data, "RTC8XHW1", 8) == 0,
                   "rtc8x_hosek_model_create: not a model table image (%zu bytes, expected %zu)", size, kHosekBytes);
@@ -355,18 +357,20 @@ extern "C" rtc8x_status rtc8x_hosek_model_create(const void* data, size_t size, 
     *out = m;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_hosek_model_create")
 
 extern "C" rtc8x_status rtc8x_hosek_model_destroy(rtc8x_hosek_model* m)
-{
+try {
     if (!m) return RTC8X_OK;
     free_device(m);
     delete m;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_hosek_model_destroy")
 
 extern "C" rtc8x_status rtc8x_hosek_bands(const rtc8x_hosek_model* m, float solar_elevation, float turbidity,
                                           rtc8x_hosek_band* out)
-{
+try {
     RTC8X_REQUIRE(m && out, "rtc8x_hosek_bands: NULL argument");
     RTC8X_REQUIRE(turbidity >= 1.0f && turbidity <= 10.0f, "rtc8x_hosek_bands: turbidity %g outside [1, 10]",
                   (double)turbidity);
@@ -393,10 +397,11 @@ extern "C" rtc8x_status rtc8x_hosek_bands(const rtc8x_hosek_model* m, float sola
     }
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_hosek_bands")
 
 extern "C" rtc8x_status rtc8x_hosek_model_state(rtc8x_hosek_model* m, float solar_elevation, float turbidity,
                                                 rtc8x_hosek_sky* sky, void* stream)
-{
+try {
     RTC8X_REQUIRE(m && sky, "rtc8x_hosek_model_state: NULL argument");
     std::vector<rtc8x_hosek_band> bands(RTC8X_SKY_BANDS);
     rtc8x_status st = rtc8x_hosek_bands(m, solar_elevation, turbidity, bands.data());
@@ -433,3 +438,4 @@ extern "C" rtc8x_status rtc8x_hosek_model_state(rtc8x_hosek_model* m, float sola
     sky->integral_cmf = m->integral_cmf;
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_hosek_model_state")

@@ -860,8 +860,10 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     }
 }
 
-// Work counters of a launch: a block of kMaxBatch slots, taken in turn by the launches in flight (so
-// launches on different streams count in different slots).
+// Work counters of a launch: a block of kMaxBatch slots, taken in turn by successive launches. Launches
+// on different streams count in different slots as long as fewer than kCounterSlots / kMaxBatch of
+// them are in flight at once; the slots wrap after that many launches, so a caller that keeps more
+// launches in flight concurrently must order them on one stream.
 inline uint32_t take_counter_slots()
 {
     static std::atomic<uint32_t> next{0};
@@ -1182,7 +1184,7 @@ extern "C" size_t rtc8x_order_scratch_bytes(uint32_t n)
 
 extern "C" rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, uint32_t n, uint32_t* order,
                                             void* scratch)
-{
+try {
     RTC8X_REQUIRE(cost && order && scratch, "rtc8x_order_by_cost: NULL argument");
     if (n == 0) return RTC8X_OK;
     const uint32_t blocks = (n + dev::kOrderBlock - 1) / dev::kOrderBlock;
@@ -1196,6 +1198,7 @@ extern "C" rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, 
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_order_by_cost")
 
 static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
@@ -1266,7 +1269,7 @@ rtc8x_status check_launch(const rtc8x_frame_params& p, int32_t entry, uint32_t n
 }  // namespace
 
 extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t entry, uint32_t num_samples)
-{
+try {
     RTC8X_REQUIRE(params != nullptr, "rtc8x_trace: params is NULL");
     const rtc8x_frame_params& p = *params;
     {
@@ -1340,10 +1343,11 @@ extern "C" rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* para
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, tw, th, 1, offsets, nullptr, s, redo)));
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_trace")
 
 extern "C" rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
                                           const float* shares, uint32_t count, int32_t entry)
-{
+try {
     RTC8X_REQUIRE(params && num_samples, "rtc8x_trace_batch: NULL argument");
     RTC8X_REQUIRE(count >= 1 && count <= dev::kMaxBatch, "rtc8x_trace_batch: count %u must be in [1, %u]", count,
                   dev::kMaxBatch);
@@ -1368,17 +1372,22 @@ extern "C" rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params
     rtc8x_frame_params* d = nullptr;
     RTC8X_HIP(hipMallocAsync((void**)&d, sizeof(rtc8x_frame_params) * count, s));
     RTC8X_HIP(hipMemcpyAsync(d, params, sizeof(rtc8x_frame_params) * count, hipMemcpyHostToDevice, s));
+    // the parameter copy is freed on every path, also when the launch fails
+    hipError_t launched;
     if (params[0].tracking == RTC8X_TRACKING_LOCAL_MAJORANT) {
-        RTC8X_HIP(entry == RTC8X_ENTRY_PATH_TRACE
-                      ? (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w, count, s))
-                      : (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w,
-                                                                                                           count, s)));
+        launched = entry == RTC8X_ENTRY_PATH_TRACE
+                       ? dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w, count, s)
+                       : dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w,
+                                                                                                          count, s);
     } else {
-        RTC8X_HIP(entry == RTC8X_ENTRY_PATH_TRACE
-                      ? (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w, count, s))
-                      : (dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w,
-                                                                                                            count, s)));
+        launched = entry == RTC8X_ENTRY_PATH_TRACE
+                       ? dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w, count, s)
+                       : dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w,
+                                                                                                           count, s);
     }
-    RTC8X_HIP(hipFreeAsync(d, s));
+    const hipError_t freed = hipFreeAsync(d, s);
+    RTC8X_HIP(launched);
+    RTC8X_HIP(freed);
     return RTC8X_OK;
 }
+RTC8X_CATCH_ALL("rtc8x_trace_batch")

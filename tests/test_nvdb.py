@@ -210,6 +210,25 @@ def test_malformed_files_are_rejected(fx):
         nvdb.decode(b"\0" * 1024)
 
 
+def test_zip_header_claiming_a_huge_grid_is_an_error(fx):
+    """A ZIP grid's record claims its inflated size; a claim beyond deflate's 1032:1 bound (2^62 bytes here)
+    is rejected before any buffer is sized from it, and so is a claim that disagrees with the stream."""
+    from rtcamp8_amd import nvdb
+    from rtcamp8_amd._lib import Rtc8xError
+
+    with nvdb.read_grid(fx["file_none"].tobytes(), 0) as h:
+        g0 = nvdb.encode_grid(h.grid, "first")
+    zipped = nvdb.write_grids([g0], "zip")
+    assert nvdb.list_grids(zipped)[0].grid_size == len(g0)
+    for claim, status in ((1 << 62, _abi.RTC8X_ERR_INVALID_ARGUMENT), (len(g0) + 4096, _abi.RTC8X_ERR_INVALID_ARGUMENT),
+                          (100, _abi.RTC8X_ERR_INVALID_ARGUMENT)):
+        bad = _corrupt(zipped, 16, struct.pack("<Q", claim))  # the first record's grid size
+        for reader in (lambda d: nvdb.read_grid(d, 0).close(), lambda d: nvdb.load_grid(d, 0)):
+            with pytest.raises(Rtc8xError) as e:
+                reader(bad)
+            assert e.value.status == status, claim
+
+
 def test_free_host_rejects_tables_it_did_not_allocate():
     from rtcamp8_amd._lib import Rtc8xError, call
     from rtcamp8_amd.scene import FbmConfig

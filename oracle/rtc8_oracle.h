@@ -53,6 +53,19 @@ int oracle_trace_f32(const rtc8x_frame_params* p, int32_t entry, uint32_t num_sa
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
 int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
+/* One segment's free-flight (transmittance = 0: out = collision distance or -1) or ratio-tracked
+ * transmittance (transmittance = 1), one PCG state per ray; and the phase functions' sampling and value. */
+void oracle_track_batch_f32(const rtc8x_frame_params* p, int32_t transmittance, uint32_t n, uint64_t* rng,
+                            const float* o, const float* d, float tmax, double* out, uint64_t* lookups);
+void oracle_track_batch_f64(const rtc8x_frame_params* p, int32_t transmittance, uint32_t n, uint64_t* rng,
+                            const float* o, const float* d, float tmax, double* out, uint64_t* lookups);
+void oracle_phase_sample_batch_f32(int32_t kind, float k, uint32_t n, const double* u, double* dir, double* value);
+void oracle_phase_sample_batch_f64(int32_t kind, float k, uint32_t n, const double* u, double* dir, double* value);
+double oracle_phase_value_f32(int32_t kind, float k, double cos_theta);
+double oracle_phase_value_f64(int32_t kind, float k, double cos_theta);
+/* The hierarchical importance map built from the texels alone (compute_light_probs.cu:161-211). */
+uint32_t oracle_env_build_mips(const float* texels, uint32_t w, uint32_t h, float* mips, uint32_t* dim_x,
+                               uint32_t* dim_y, float* integral);
 /* EnvironmentalLight::sample / evaluate (common_renderer_types.h:988-1015, 2419-2458) in FP64. */
 void oracle_env_sample_f64(const rtc8x_env_light* e, double u0, double u1, double dir[3], double* pdf, double rgb[3]);
 void oracle_env_evaluate_f64(const rtc8x_env_light* e, const double dir[3], double* pdf, double rgb[3]);

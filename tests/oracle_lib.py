@@ -27,7 +27,13 @@ def oracle() -> C.CDLL:
         return _orc
     if not ORACLE_SO.exists():
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
-    lib = C.CDLL(str(ORACLE_SO))
+    _orc = load_oracle(ORACLE_SO)
+    return _orc
+
+
+def load_oracle(path) -> C.CDLL:
+    """An oracle library at `path` with its prototypes set (the built one, or a copy built elsewhere)."""
+    lib = C.CDLL(str(path))
     protos = {
         "oracle_pcg_next": (C.c_uint32, [C.POINTER(C.c_uint64)]),
         "oracle_pcg_float": (C.c_float, [C.POINTER(C.c_uint64)]),
@@ -76,12 +82,20 @@ def oracle() -> C.CDLL:
         "oracle_sdr_convert": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_abi.SdrConfig), _P,
                                       C.c_int32]),
         "oracle_round_bf16": (C.c_float, [C.c_float]),
+        "oracle_track_batch_f32": (None, [C.POINTER(_abi.FrameParams), C.c_int32, C.c_uint32, _P, _P, _P, C.c_float,
+                                          _P, _P]),
+        "oracle_track_batch_f64": (None, [C.POINTER(_abi.FrameParams), C.c_int32, C.c_uint32, _P, _P, _P, C.c_float,
+                                          _P, _P]),
+        "oracle_phase_sample_batch_f32": (None, [C.c_int32, C.c_float, C.c_uint32, _P, _P, _P]),
+        "oracle_phase_sample_batch_f64": (None, [C.c_int32, C.c_float, C.c_uint32, _P, _P, _P]),
+        "oracle_phase_value_f32": (C.c_double, [C.c_int32, C.c_float, C.c_double]),
+        "oracle_phase_value_f64": (C.c_double, [C.c_int32, C.c_float, C.c_double]),
+        "oracle_env_build_mips": (C.c_uint32, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _orc = lib
     return lib
 
 

@@ -82,6 +82,20 @@ rtc8x_status rtc8x_copy_async(void* dst, const void* src, size_t bytes, void* st
 rtc8x_status rtc8x_gather_probe(void* stream, const struct rtc8x_grid* grid, uint32_t threads, uint32_t steps,
                                 uint32_t width, uint32_t waves_per_simd, uint32_t seed, float* out);
 
+struct rtc8x_frame_params;
+/*
+ * rtc8x_march_probe: the memory-latency roofline of the tracer's own lookups (no counterpart in the
+ * reference).  `lanes` lanes (a multiple of 256) walk paths of the frame's pixels (lane i starts at
+ * pixel i mod width x height, with that pixel's PCG state, which is read and not written): jittered
+ * camera rays, delta tracking at the global majorant's spacing with two tentative collisions looked up
+ * together, and at every real collision a ratio-tracked shadow segment towards `light_dir` and a sample
+ * of the frame's phase function, with none of the tracer's vertex shading.  Every lane stops after
+ * `budget` lookups (<= 2^20).  The launch runs at `waves_per_simd` waves per SIMD (2..8, set with dynamic
+ * LDS).  stats[0] += lookups made, stats[1] += paths ended; stats is a device array of two uint64.
+ */
+rtc8x_status rtc8x_march_probe(void* stream, const struct rtc8x_frame_params* params, const float light_dir[3],
+                               uint32_t lanes, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats);
+
 /* ------------------------------------------------------------------------------------------------
  * Sparse density grid in the NanoVDB tree layout (root tiles -> 32^3 upper -> 16^3 lower -> 8^3 leaf;
  * ext/nanovdb/nanovdb/NanoVDB.h).  Node and voxel offsets follow NanoVDB exactly
@@ -633,6 +647,11 @@ rtc8x_status rtc8x_nrc_create(const rtc8x_nrc_desc* desc, rtc8x_nrc** out);
 rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* nrc);
 /* infer() (network_interface.cu:143-149): out[3n] from in[14n], EMA weights. */
 rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* nrc, void* stream, const float* in, uint32_t n, float* out);
+/* The inference step of a training iteration (renderer_main.cpp:1579-1587: infer the training suffixes
+ * queries): p->nrc.inference_queries -> p->nrc.inferred_radiance for round_up(w h, 128) queries, where
+ * w x h is the current buffer's training size as prepareNRC left it on the device.  The count is read on
+ * the device, so unlike the reference the host does not wait for it. */
+rtc8x_status rtc8x_nrc_infer_training(rtc8x_nrc* nrc, void* stream, const struct rtc8x_frame_params* p);
 /* train() (network_interface.cu:151-159): one optimiser step on n samples (n <= 65536); if loss_on_cpu
  * is not NULL the mean loss is copied to it (synchronises the stream). */
 rtc8x_status rtc8x_nrc_train(rtc8x_nrc* nrc, void* stream, const float* in, const float* target, uint32_t n,

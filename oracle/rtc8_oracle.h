@@ -53,6 +53,9 @@ int oracle_trace_f32(const rtc8x_frame_params* p, int32_t entry, uint32_t num_sa
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
 int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
                      uint32_t y0, uint32_t y1, uint32_t nthreads);
+/* PATH_TRACE of the pixels [x0, x1) x [y0, y1) only, FP64 (a crop of a large frame). */
+int oracle_trace_window_f64(const rtc8x_frame_params* p, uint32_t num_samples, uint32_t x0, uint32_t x1,
+                            uint32_t y0, uint32_t y1, uint32_t nthreads);
 /* One segment's free-flight (transmittance = 0: out = collision distance or -1) or ratio-tracked
  * transmittance (transmittance = 1), one PCG state per ray; and the phase functions' sampling and value. */
 void oracle_track_batch_f32(const rtc8x_frame_params* p, int32_t transmittance, uint32_t n, uint64_t* rng,

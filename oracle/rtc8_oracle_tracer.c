@@ -61,13 +61,19 @@ static void sincos_f64(double x, double* s, double* c) { *s = sin(x); *c = cos(x
 int oracle_trace_f32(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
                      uint32_t y0, uint32_t y1, uint32_t nthreads)
 {
-    return oracle_trace_impl_f32(p, entry, num_samples, y0, y1, nthreads);
+    return oracle_trace_impl_f32(p, entry, num_samples, 0, 0xFFFFFFFFu, y0, y1, nthreads);
 }
 
 int oracle_trace_f64(const rtc8x_frame_params* p, int32_t entry, uint32_t num_samples,
                      uint32_t y0, uint32_t y1, uint32_t nthreads)
 {
-    return oracle_trace_impl_f64(p, entry, num_samples, y0, y1, nthreads);
+    return oracle_trace_impl_f64(p, entry, num_samples, 0, 0xFFFFFFFFu, y0, y1, nthreads);
+}
+
+int oracle_trace_window_f64(const rtc8x_frame_params* p, uint32_t num_samples, uint32_t x0, uint32_t x1,
+                            uint32_t y0, uint32_t y1, uint32_t nthreads)
+{
+    return oracle_trace_impl_f64(p, RTC8X_ENTRY_PATH_TRACE, num_samples, x0, x1, y0, y1, nthreads);
 }
 
 /* Environment sampling hooks for the tests (FP64 instance). */

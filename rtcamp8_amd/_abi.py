@@ -356,6 +356,7 @@ PROTOTYPES = {
     "rtc8x_memset": (_S, [_P, C.c_int32, C.c_size_t]),
     "rtc8x_copy_async": (_S, [_P, _P, C.c_size_t, _P]),
     "rtc8x_gather_probe": (_S, [_P, C.POINTER(Grid), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "rtc8x_march_probe": (_S, [_P, C.POINTER(FrameParams), _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "rtc8x_device_synchronize": (_S, []),
     "rtc8x_stream_create": (_S, [C.POINTER(C.c_void_p)]),
     "rtc8x_stream_destroy": (_S, [_P]),
@@ -406,6 +407,7 @@ PROTOTYPES = {
     "rtc8x_nrc_create": (_S, [C.POINTER(NrcDesc), C.POINTER(C.c_void_p)]),
     "rtc8x_nrc_destroy": (_S, [_P]),
     "rtc8x_nrc_infer": (_S, [_P, _P, _P, C.c_uint32, _P]),
+    "rtc8x_nrc_infer_training": (_S, [_P, _P, C.POINTER(FrameParams)]),
     "rtc8x_nrc_train": (_S, [_P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
     "rtc8x_nrc_num_params": (C.c_uint32, [_P]),
     "rtc8x_nrc_get_weights": (_S, [_P, C.c_int32, _P, C.c_size_t]),

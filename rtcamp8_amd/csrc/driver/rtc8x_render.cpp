@@ -451,12 +451,8 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
                 p.nrc.num_samples = 1;
                 check(rtc8x_nrc_prepare(stream, &p, frame == 0 ? 1 : 0), "prepare");
                 check(rtc8x_trace(stream, &p, RTC8X_ENTRY_GENERATE_TRAINING_DATA, 1), "training trace");
-                int32_t tsz[4];
-                check(rtc8x_stream_synchronize(stream), "sync");
-                check(rtc8x_memcpy_dtoh(tsz, nb->train_image_size.ptr, sizeof(tsz)), "train size");
-                const uint32_t nq = (uint32_t)round_up((size_t)tsz[2 * buf_idx] * tsz[2 * buf_idx + 1], 128);
-                check(rtc8x_nrc_infer(cache, stream, (const float*)nb->inference_queries.ptr, nq,
-                                      nb->inferred_radiance.as<float>()), "infer");
+                // the training size stays on the device: no host synchronisation in the iteration
+                check(rtc8x_nrc_infer_training(cache, stream, &p), "infer");
                 check(rtc8x_nrc_propagate(stream, &p), "propagate");
                 check(rtc8x_nrc_shuffle(stream, &p), "shuffle");
                 const uint32_t batch = RTC8X_NUM_TRAINING_DATA_PER_FRAME / 4;

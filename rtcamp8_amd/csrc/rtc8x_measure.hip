@@ -1,6 +1,7 @@
 // rtc8x_measure.hip -- measurement kernels with no counterpart in the renderer: a plain copy (the
-// calibration of the HBM byte counters) and dependent density lookups at random positions of the grid's
-// leaves (the memory-latency roofline of the tracer's lookups).
+// calibration of the HBM byte counters), dependent density lookups at random positions of the grid's
+// leaves, and the tracer's own stepping pattern with its shading taken out (the memory-latency roofline
+// of the tracer's lookups).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,6 +62,145 @@ __global__ void gather_probe_kernel(const rtc8x_grid g, uint32_t steps, uint32_t
     out[tid] = acc;
 }
 
+// The tracer's stepping pattern with its shading taken out.  Lane i walks paths of pixel i % (W H) of the
+// frame's camera: a jittered camera ray (optix_pathtracing_kernels.cu:43-55), delta tracking at the
+// global majorant's spacing, and at every real collision a ratio-tracked shadow segment towards `light`
+// (the length the tracer gives it, 2 x the world radius) and a phase-function sample of the next
+// direction, until the path escapes.  Tentative collisions are taken two at a time and their lookups
+// issued together (density_batch<2>), as the tracer does, and one copy of the lookup code serves both
+// segment kinds.  Every lane stops after `budget` lookups, so the launch has no tail of long paths;
+// what it leaves out is the tracer's vertex work: NEE light sampling and MIS, the environment, Russian
+// roulette, NRC records and accumulation.  stats[0] += lookups, stats[1] += paths that escaped.
+__global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, uint32_t budget,
+                                   unsigned long long* __restrict__ stats)
+{
+    extern __shared__ float occupancy_limiter[];
+    const uint32_t W = p.image_width, H = p.image_height, N = W * H;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, lanes = gridDim.x * blockDim.x;
+    uint32_t lin = tid % N;
+    dev::Pcg rng;
+    rng.state = p.rng_states[lin];
+    const float maj = p.density_coeff * p.grid.max_value;
+    const float coeff = p.density_coeff;
+    const float trace_len = 2.0f * p.world_radius;
+    dev::V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
+    float t = 0.0f, t1 = -1.0f, vis = 1.0f;
+    uint32_t mode = 0;  // 0: start a path, 1: tracking, 2: shadow
+    uint32_t looked = 0, paths = 0;
+    dev::V3 pos{0.0f, 0.0f, 0.0f}, dir{0.0f, 0.0f, 1.0f};
+    for (uint32_t iter = 0; iter < 4u * budget + 64u && looked < budget; ++iter) {
+        if (mode == 0) {
+            const float px = ((float)(lin % W) + rng.uni()) / (float)W, py = ((float)(lin / W) + rng.uni()) / (float)H;
+            const dev::V3 ldir = dev::normalize(dev::V3{p.camera.vw * (-0.5f + px), p.camera.vh * (0.5f - py), -1.0f});
+            const float* M = p.camera.orientation;
+            dir = dev::normalize(dev::V3{dev::dot(dev::V3{M[0], M[1], M[2]}, ldir), dev::dot(dev::V3{M[3], M[4], M[5]}, ldir),
+                                         dev::dot(dev::V3{M[6], M[7], M[8]}, ldir)});
+            pos = dev::V3{p.camera.position[0], p.camera.position[1], p.camera.position[2]};
+            float t0;
+            if (dev::clip(p.grid, pos, dir, 1e10f, t0, t1)) {
+                o = pos;
+                d = dir;
+                t = fmaxf(0.0f, t0);
+                mode = 1;
+            } else {  // the camera ray misses the volume: the lane moves on to another pixel
+                ++paths;
+                lin = (lin + lanes) % N;
+                rng.state = p.rng_states[lin];
+            }
+            continue;
+        }
+        // two tentative collisions (a collision past the segment's end is looked up at the current
+        // position, as the tracer does); a tracking segment stops at the first real one
+        float tk[2], tl[2], u[2];
+        tk[0] = t - rtc8x_logf(1.0f - rng.uni()) / maj;
+        tk[1] = tk[0] - rtc8x_logf(1.0f - rng.uni()) / maj;
+        u[0] = rng.uni();
+        u[1] = rng.uni();
+        tl[0] = tk[0] > t1 ? t : tk[0];
+        tl[1] = tk[1] > t1 ? t : tk[1];
+        float sig[2];
+        dev::density_batch<2>(p.grid, coeff, o, d, tl, sig);
+        uint32_t event = 0;  // 1: real collision at t, 2: segment end
+        for (int k = 0; k < 2 && event == 0; ++k) {
+            if (tk[k] > t1) {
+                event = 2;
+                break;
+            }
+            ++looked;
+            t = tk[k];
+            if (mode == 1) {
+                if (u[k] < sig[k] / maj) event = 1;
+            } else {
+                vis *= 1.0f - sig[k] / maj;
+            }
+        }
+        if (event == 1) {  // real collision: the shadow segment, ending at the collision
+            pos = dev::V3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
+            const dev::V3 lp{pos.x + trace_len * light.x, pos.y + trace_len * light.y, pos.z + trace_len * light.z};
+            const dev::V3 sd{-light.x, -light.y, -light.z};
+            float t0;
+            vis = 1.0f;
+            mode = 2;
+            if (dev::clip(p.grid, lp, sd, trace_len, t0, t1)) {
+                o = lp;
+                d = sd;
+                t = fmaxf(0.0f, t0);
+            } else {
+                event = 2;
+            }
+        }
+        if (event == 2) {
+            if (mode == 1) {  // escaped: the pixel's next path
+                ++paths;
+                mode = 0;
+            } else {  // shadow segment done: phase sample and the next tracking segment
+                dev::V3 dl;
+                const float us0 = rng.uni(), us1 = rng.uni();
+                dev::phase_sample(p.phase, p.scattering_forwardness, us0, us1, dl);
+                const dev::Frame fr = dev::make_frame(dev::V3{-dir.x, -dir.y, -dir.z});
+                dir = dev::from_local(fr, dl);
+                float t0;
+                if (dev::clip(p.grid, pos, dir, 1e10f, t0, t1)) {
+                    o = pos;
+                    d = dir;
+                    t = fmaxf(0.0f, t0);
+                    mode = 1;
+                } else {
+                    ++paths;
+                    mode = 0;
+                }
+            }
+        }
+    }
+    if (vis == -1.0f) occupancy_limiter[threadIdx.x] = vis;  // keeps the dynamic LDS allocation alive
+    unsigned long long a = looked, b = paths;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], a);
+        atomicAdd(&stats[1], b);
+    }
+}
+
+// The dynamic LDS per 256-lane workgroup that admits `waves_per_simd` workgroups per CU and not one more
+// (a 256-lane workgroup puts one wave on each of the CU's 4 SIMDs).
+static rtc8x_status occupancy_lds(uint32_t waves_per_simd, size_t* lds_out, const char* who)
+{
+    int dev = 0, lds_cu = 0, lds_block = 0;
+    RTC8X_HIP(hipGetDevice(&dev));
+    RTC8X_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    RTC8X_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    size_t lds = (size_t)lds_cu / waves_per_simd;
+    lds = lds > 1024 ? lds - 1024 : lds;
+    if (!(lds <= (size_t)lds_block && (size_t)lds_cu / lds == waves_per_simd))
+        return fail(RTC8X_ERR_INVALID_ARGUMENT, "%s: %u waves per SIMD cannot be set with %d bytes of LDS per CU", who,
+                    waves_per_simd, lds_cu);
+    *lds_out = lds;
+    return RTC8X_OK;
+}
+
 }  // namespace rtc8x
 
 using namespace rtc8x;
@@ -90,17 +230,11 @@ try {
                   "rtc8x_gather_probe: steps must be > 0, threads a multiple of 256");
     RTC8X_REQUIRE(width == 1 || width == 2 || width == 4, "rtc8x_gather_probe: width must be 1, 2 or 4");
     RTC8X_REQUIRE(waves_per_simd >= 2 && waves_per_simd <= 8, "rtc8x_gather_probe: waves_per_simd must be in [2, 8]");
-    int dev = 0, lds_cu = 0, lds_block = 0;
-    RTC8X_HIP(hipGetDevice(&dev));
-    RTC8X_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    RTC8X_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    // a 256-lane workgroup puts one wave on each of the CU's 4 SIMDs, so W workgroups per CU give W
-    // waves per SIMD; the LDS reservation admits W of them and not W + 1
-    size_t lds = (size_t)lds_cu / waves_per_simd;
-    lds = lds > 1024 ? lds - 1024 : lds;
-    RTC8X_REQUIRE(lds <= (size_t)lds_block && (size_t)lds_cu / lds == waves_per_simd,
-                  "rtc8x_gather_probe: %u waves per SIMD cannot be set with %d bytes of LDS per CU", waves_per_simd,
-                  lds_cu);
+    size_t lds = 0;
+    {
+        const rtc8x_status st = occupancy_lds(waves_per_simd, &lds, "rtc8x_gather_probe");
+        if (st != RTC8X_OK) return st;
+    }
     hipStream_t s = (hipStream_t)stream;
     if (width == 1)
         hipLaunchKernelGGL(gather_probe_kernel<1>, dim3(threads / 256), dim3(256), lds, s, g, steps, seed, out);
@@ -112,3 +246,31 @@ try {
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_gather_probe")
+
+extern "C" rtc8x_status rtc8x_march_probe(void* stream, const rtc8x_frame_params* params, const float light_dir[3],
+                                          uint32_t lanes, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats)
+try {
+    RTC8X_REQUIRE(params && light_dir && stats, "rtc8x_march_probe: NULL pointer");
+    const rtc8x_frame_params& p = *params;
+    RTC8X_REQUIRE(p.image_width > 0 && p.image_height > 0 && p.rng_states,
+                  "rtc8x_march_probe: the frame needs an image and its PCG states");
+    RTC8X_REQUIRE(p.grid.leaf_values && p.density_coeff * p.grid.max_value > 0.0f,
+                  "rtc8x_march_probe: the grid must be on the device with a positive majorant");
+    RTC8X_REQUIRE(lanes > 0 && lanes % 256 == 0 && budget > 0 && budget <= (1u << 20),
+                  "rtc8x_march_probe: lanes must be a positive multiple of 256 and budget in [1, 2^20]");
+    RTC8X_REQUIRE(waves_per_simd >= 2 && waves_per_simd <= 8, "rtc8x_march_probe: waves_per_simd must be in [2, 8]");
+    size_t lds = 0;
+    {
+        const rtc8x_status st = occupancy_lds(waves_per_simd, &lds, "rtc8x_march_probe");
+        if (st != RTC8X_OK) return st;
+    }
+    const float n = std::sqrt(light_dir[0] * light_dir[0] + light_dir[1] * light_dir[1] + light_dir[2] * light_dir[2]);
+    RTC8X_REQUIRE(n > 0.0f, "rtc8x_march_probe: light direction is zero");
+    const dev::V3 l{light_dir[0] / n, light_dir[1] / n, light_dir[2] / n};
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(march_probe_kernel, dim3(lanes / 256), dim3(256), lds, s, p, l, budget,
+                       (unsigned long long*)stats);
+    RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_march_probe")

@@ -80,10 +80,13 @@ __global__ void prepare_kernel(const rtc8x_nrc_buffers nb, int32_t img_w, int32_
     nb.train_suffix_terminals[i] = t;
 }
 
-__global__ void propagate_kernel(const rtc8x_nrc_buffers nb, uint32_t n)
+// Launched over every suffix slot; the training size comes from the device (prepareNRC wrote it).
+__global__ void propagate_kernel(const rtc8x_nrc_buffers nb, uint32_t max_n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int32_t* ts = nb.train_image_size + 2 * nb.nrc_buffer_index;
+    const uint64_t n = (uint64_t)max(ts[0], 0) * (uint64_t)max(ts[1], 0);
+    if (i >= max_n || i >= n) return;
     const rtc8x_training_suffix_terminal ti = nb.train_suffix_terminals[i];
     if (ti.prev_vertex_index == RTC8X_INVALID_VERTEX_INDEX) return;
     RGB contribution{0.0f, 0.0f, 0.0f};
@@ -272,13 +275,8 @@ try {
     RTC8X_REQUIRE(nb.train_suffix_terminals && nb.train_vertex_info && nb.train_targets[0] && nb.train_queries[0] &&
                       nb.inference_queries && nb.inferred_radiance && nb.train_image_size,
                   "rtc8x_nrc_propagate: buffers missing");
-    int32_t ts[2];
-    RTC8X_HIP(hipMemcpyAsync(ts, nb.train_image_size + 2 * nb.nrc_buffer_index, sizeof(ts), hipMemcpyDeviceToHost,
-                             (hipStream_t)stream));
-    RTC8X_HIP(hipStreamSynchronize((hipStream_t)stream));
-    RTC8X_REQUIRE(ts[0] > 0 && ts[1] > 0 && (uint64_t)ts[0] * ts[1] <= nb.max_num_training_suffixes,
-                  "rtc8x_nrc_propagate: training size out of range");
-    const uint32_t n = (uint32_t)ts[0] * (uint32_t)ts[1];
+    RTC8X_REQUIRE(nb.max_num_training_suffixes > 0, "rtc8x_nrc_propagate: no suffix slots");
+    const uint32_t n = nb.max_num_training_suffixes;
     hipLaunchKernelGGL(dev::propagate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, nb, n);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;

@@ -32,6 +32,24 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+
+// four values of the working type (one 8- or 16-byte store)
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<_Float16> {
+    using type = h4;
+};
+template <>
+struct Vec4<__bf16> {
+    using type = b4;
+};
+template <>
+struct Vec4<float> {
+    using type = f4;
+};
 
 template <typename T>
 struct Vec8;
@@ -269,6 +287,14 @@ __device__ __forceinline__ void encode_any(const rtc8x_radiance_query& q, const 
         encode(q, e);
 }
 
+// With count_dims (a training size w x h on the device), round_up(w h, 128) queries, at most n.
+__device__ __forceinline__ uint32_t device_count(uint32_t n, const int32_t* count_dims)
+{
+    if (!count_dims) return n;
+    const uint64_t m = ((uint64_t)max(count_dims[0], 0) * (uint64_t)max(count_dims[1], 0) + 127u) & ~(uint64_t)127u;
+    return m < n ? (uint32_t)m : n;
+}
+
 template <typename T, bool TRAIN, int ENC>
 // (256, 3): the register budget of three workgroups per CU, under which the FP16/BF16 accumulators stay
 // in VGPRs and the kernel runs four waves per SIMD (0.19 ms per 2^21 queries at L = 5, against 0.21 ms
@@ -278,9 +304,11 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
                                                       float* __restrict__ out, const float* __restrict__ target,
                                                       T* __restrict__ hbuf, T* __restrict__ dout,
                                                       float* __restrict__ loss_part, const float* __restrict__ tables,
-                                                      const HashGeom geom)
+                                                      const HashGeom geom, const int32_t* __restrict__ count_dims)
 {
     using V = typename Vec8<T>::type;
+    n = device_count(n, count_dims);
+    if (blockIdx.x * 256u >= n) return;
     __shared__ __align__(16) T act[4][64 * kActStride];
     __shared__ float pred[4][64][3];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -331,31 +359,35 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
             for (int nt = 0; nt < 4; ++nt) {
                 if (nt < ntiles) {
                     const V b = *reinterpret_cast<const V*>(&W[(nt * 16 + r16) * kWidth + ks * 32 + kq]);
+                    // weights as the A operand, activations as B: acc[mt][nt][i] is feature
+                    // nt * 16 + 4 (lane >> 4) + i of query mt * 16 + (lane & 15)
 #pragma unroll
-                    for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(a[mt], b, acc[mt][nt]);
+                    for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(b, a[mt], acc[mt][nt]);
                 }
             }
         }
         if (!last) {
+            // ReLU and four consecutive features of one query per store
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
+                for (int nt = 0; nt < 4; ++nt) {
+                    typename Vec4<T>::type v;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        A[(mt * 16 + 4 * (lane >> 4) + i) * kActStride + nt * 16 + r16] = (T)fmaxf(acc[mt][nt][i], 0.0f);
+                    for (int i = 0; i < 4; ++i) v[i] = (T)fmaxf(acc[mt][nt][i], 0.0f);
+                    *reinterpret_cast<typename Vec4<T>::type*>(&A[(mt * 16 + r16) * kActStride + nt * 16 + 4 * (lane >> 4)]) = v;
+                }
             if (TRAIN && s < n) {
                 T* dst = hbuf + ((size_t)(l + 1) * n + s) * 64;
 #pragma unroll
                 for (int j = 0; j < 64; j += 8)
                     *reinterpret_cast<V*>(&dst[j]) = *reinterpret_cast<const V*>(&A[lane * kActStride + j]);
             }
-        } else {
+        } else if ((lane >> 4) == 0) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (r16 < 3) pred[wave][mt * 16 + 4 * (lane >> 4) + i][r16] = acc[mt][0][i];
+                for (int i = 0; i < 3; ++i) pred[wave][mt * 16 + r16][i] = acc[mt][0][i];
         }
     }
     if (!TRAIN) {
@@ -444,10 +476,13 @@ template <int ENC>
 __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __restrict__ w8, const float* __restrict__ w_inv,
                                                           uint32_t L, const rtc8x_radiance_query* __restrict__ in,
                                                           uint32_t n, float* __restrict__ out,
-                                                          const float* __restrict__ tables, const HashGeom geom)
+                                                          const float* __restrict__ tables, const HashGeom geom,
+                                                          const int32_t* __restrict__ count_dims)
 {
     using T = __bf16;
     using V = typename Vec8<T>::type;
+    n = device_count(n, count_dims);
+    if (blockIdx.x * 256u >= n) return;
     __shared__ __align__(16) T act[4][64 * kActStride];
     __shared__ float pred[4][64][3];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -810,30 +845,30 @@ namespace {
 
 template <typename T, int ENC>
 hipError_t launch_forward_enc(rtc8x_nrc* h, bool train, const void* w, const float* tables, hipStream_t s, const float* in,
-                              uint32_t n, float* out, const float* target)
+                              uint32_t n, float* out, const float* target, const int32_t* count_dims)
 {
     const uint32_t blocks = (n + 255) / 256;
     if (train)
         hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
                            (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part, tables,
-                           h->geom);
+                           h->geom, (const int32_t*)nullptr);
     else
         hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
                            (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr, (float*)nullptr,
-                           tables, h->geom);
+                           tables, h->geom, count_dims);
     return hipGetLastError();
 }
 
 // train: the working weights and the FP32 master tables; inference: the EMA weights and tables
 template <typename T>
 hipError_t launch_forward(rtc8x_nrc* h, bool train, hipStream_t s, const float* in, uint32_t n, float* out,
-                          const float* target)
+                          const float* target, const int32_t* count_dims = nullptr)
 {
     const void* w = train ? h->w : h->winf;
     const float* tables = train ? h->master + h->P_mlp : h->grid_inf;
     if (h->enc == RTC8X_NRC_HASH_GRID)
-        return launch_forward_enc<T, RTC8X_NRC_HASH_GRID>(h, train, w, tables, s, in, n, out, target);
-    return launch_forward_enc<T, RTC8X_NRC_TRIANGLE_WAVE>(h, train, w, tables, s, in, n, out, target);
+        return launch_forward_enc<T, RTC8X_NRC_HASH_GRID>(h, train, w, tables, s, in, n, out, target, count_dims);
+    return launch_forward_enc<T, RTC8X_NRC_TRIANGLE_WAVE>(h, train, w, tables, s, in, n, out, target, count_dims);
 }
 
 template <typename T>
@@ -881,17 +916,18 @@ hipError_t with_type(const rtc8x_nrc* h, F&& f)
 #define RTC8X_WITH_TYPE(h, call) with_type(h, [&](auto tag) { using T = std::remove_pointer_t<decltype(tag)>; return call; })
 bool is_fp8(const rtc8x_nrc* h) { return h->desc.precision == RTC8X_NRC_BF16_FP8; }
 
-hipError_t launch_forward_fp8(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n, float* out)
+hipError_t launch_forward_fp8(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n, float* out,
+                              const int32_t* count_dims = nullptr)
 {
     hipLaunchKernelGGL(nrc::quantize_kernel, dim3(h->L + 1), dim3(256), 0, s, (const __bf16*)h->winf, h->L, h->w8,
                        h->w8_inv);
     const uint32_t blocks = (n + 255) / 256;
     if (h->enc == RTC8X_NRC_HASH_GRID)
         hipLaunchKernelGGL((nrc::forward_fp8_kernel<RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, h->w8,
-                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom);
+                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom, count_dims);
     else
         hipLaunchKernelGGL((nrc::forward_fp8_kernel<RTC8X_NRC_TRIANGLE_WAVE>), dim3(blocks), dim3(256), 0, s, h->w8,
-                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom);
+                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom, count_dims);
     return hipGetLastError();
 }
 
@@ -1044,6 +1080,26 @@ try {
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_nrc_infer")
+
+extern "C" rtc8x_status rtc8x_nrc_infer_training(rtc8x_nrc* h, void* stream, const rtc8x_frame_params* p)
+try {
+    RTC8X_REQUIRE(h && p, "rtc8x_nrc_infer_training: NULL argument");
+    const rtc8x_nrc_buffers& nb = p->nrc;
+    RTC8X_REQUIRE(nb.inference_queries && nb.inferred_radiance && nb.train_image_size && nb.nrc_buffer_index < 2 &&
+                      p->image_width > 0 && p->image_height > 0,
+                  "rtc8x_nrc_infer_training: NRC buffers missing");
+    // at most the image's queries; the training size itself is read on the device
+    const uint32_t n = (uint32_t)(((size_t)p->image_width * p->image_height + 127u) & ~(size_t)127u);
+    const int32_t* dims = nb.train_image_size + 2 * nb.nrc_buffer_index;
+    hipStream_t s = (hipStream_t)stream;
+    const float* in = reinterpret_cast<const float*>(nb.inference_queries);
+    if (is_fp8(h))
+        RTC8X_HIP(launch_forward_fp8(h, s, in, n, nb.inferred_radiance, dims));
+    else
+        RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, false, s, in, n, nb.inferred_radiance, nullptr, dims)));
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_nrc_infer_training")
 
 extern "C" rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* h, void* stream, const float* in, const float* target, uint32_t n,
                                            float* grad)

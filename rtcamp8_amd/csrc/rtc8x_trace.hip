@@ -373,6 +373,15 @@ __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, 
     }
 }
 
+// The training launch's size, which prepareNRC left on the device for the current buffer, clamped to
+// the image (nrc_set_up_kernels.cu:4-44 computes it; reading it here spares the host a synchronisation).
+__device__ __forceinline__ void train_dims(const rtc8x_frame_params& p, uint32_t& tw, uint32_t& th)
+{
+    const int32_t* ts = p.nrc.train_image_size + 2 * p.nrc.nrc_buffer_index;
+    tw = (uint32_t)min(max(ts[0], 0), (int32_t)p.image_width);
+    th = (uint32_t)min(max(ts[1], 0), (int32_t)p.image_height);
+}
+
 // pathTrace_generic<withNRC, genData> (optix_pathtracing_kernels.cu:31-434) for volume-only scenes, as
 // a per-lookup state machine over a persistent grid.  ENTRY selects the ray-generation program;
 // GENERATE_TRAINING_DATA runs as the staging launch or the redo launch (TrainStage; the redo records at
@@ -389,7 +398,9 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     const bool staged = gen_data && stage.counter != nullptr;
     const bool redo = gen_data && stage.redo_lost != nullptr;
     float* const train_targets = staged ? stage.targets : nb.train_targets[0];
-    const uint32_t W = launch_w, H = launch_h;
+    // GENERATE_TRAINING_DATA is launched with a zero size and takes the training size from the device
+    uint32_t W = launch_w, H = launch_h;
+    if (gen_data && W == 0) train_dims(p, W, H);
     const uint32_t tiles_x = (W + 7) / 8;
     // GENERATE_TRAINING_DATA hands out the unbiased-suffix pixels first (the one pixel of each 4x4 block
     // with ((ly % 4) * 4 + lx % 4 + offset) % 16 == 0): their paths run to full length, so starting
@@ -913,11 +924,13 @@ __global__ void reset_counters_kernel(uint32_t slot, uint32_t count)
 }
 
 // Exclusive prefix sum of the per-path record counts (one workgroup, a few hundred thousand entries).
-__global__ __launch_bounds__(1024) void record_prefix_kernel(const uint32_t* __restrict__ counts,
-                                                             uint32_t* __restrict__ offsets, uint32_t n,
-                                                             uint32_t* __restrict__ total_out)
+__global__ __launch_bounds__(1024) void record_prefix_kernel(const rtc8x_frame_params p, const uint32_t* __restrict__ counts,
+                                                             uint32_t* __restrict__ offsets, uint32_t* __restrict__ total_out)
 {
     __shared__ uint64_t part[1024];
+    uint32_t tw, th;
+    train_dims(p, tw, th);
+    const uint32_t n = tw * th;
     const uint32_t tid = threadIdx.x;
     const uint32_t chunk = (n + 1023) / 1024;
     const uint32_t b = tid * chunk, e = min(n, b + chunk);
@@ -959,9 +972,12 @@ __global__ __launch_bounds__(256) void stage_map_kernel(const uint32_t* __restri
 __global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_frame_params p, const uint32_t* __restrict__ total,
                                                          const TrainStage st, const uint32_t* __restrict__ fmap,
                                                          const uint32_t* __restrict__ offsets,
-                                                         const uint32_t* __restrict__ counts, uint32_t tw, uint32_t npx)
+                                                         const uint32_t* __restrict__ counts)
 {
     const rtc8x_nrc_buffers& nb = p.nrc;
+    uint32_t tw, th;
+    train_dims(p, tw, th);
+    const uint32_t npx = tw * th;
     const uint32_t n = (uint32_t)min((size_t)*total, kStageCapacity);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint64_t sum[4] = {0, 0, 0, 0};
@@ -1014,6 +1030,16 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_frame_param
             if (lane == 0 && x) atomicAdd((unsigned long long*)&p.stats[q], (unsigned long long)x);
         }
     }
+}
+
+// The training pixels' PCG states before the launch, in launch-pixel order (the redo launch's start).
+__global__ __launch_bounds__(256) void save_rng_kernel(const rtc8x_frame_params p, uint64_t* __restrict__ out)
+{
+    uint32_t tw, th;
+    train_dims(p, tw, th);
+    const uint32_t n = tw * th;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = p.rng_states[(size_t)(i / tw) * p.image_width + i % tw];
 }
 
 // ---------------------------------------------------------------- cost-ordered work lists
@@ -1291,17 +1317,12 @@ try {
                                                                                  nullptr, nullptr, s)));
         return RTC8X_OK;
     }
-    // GENERATE_TRAINING_DATA: the launch size lives on the device (prepareNRC writes it); read it.
+    // GENERATE_TRAINING_DATA: the launch size lives on the device (prepareNRC writes it); the kernels read
+    // it there, so the host never waits for it
     RTC8X_REQUIRE(nb.train_image_size && nb.num_training_data && nb.train_queries[0] && nb.train_targets[0] &&
                       nb.train_vertex_info && nb.train_suffix_terminals && nb.scratch,
                   "rtc8x_trace: NRC training buffers missing");
     RTC8X_REQUIRE(nb.nrc_buffer_index < 2, "rtc8x_trace: nrc_buffer_index must be 0 or 1");
-    int32_t ts[2];
-    RTC8X_HIP(hipMemcpyAsync(ts, nb.train_image_size + 2 * nb.nrc_buffer_index, sizeof(ts), hipMemcpyDeviceToHost, s));
-    RTC8X_HIP(hipStreamSynchronize(s));
-    RTC8X_REQUIRE(ts[0] > 0 && ts[1] > 0 && (uint32_t)ts[0] <= p.image_width && (uint32_t)ts[1] <= p.image_height,
-                  "rtc8x_trace: training image size %dx%d out of range", ts[0], ts[1]);
-    const uint32_t tw = (uint32_t)ts[0], th = (uint32_t)ts[1], n = tw * th;
     const NrcScratchLayout lay = nrc_scratch_layout(p.image_width, p.image_height);
     char* const sb = reinterpret_cast<char*>(nb.scratch);
     uint32_t* counts = reinterpret_cast<uint32_t*>(sb + lay.counts);
@@ -1312,8 +1333,9 @@ try {
     // staging slot are traced again.  Record indices, and with them every result, are those of
     // pixel-order assignment with paths ending at the buffer's capacity.
     uint64_t* rng_save = reinterpret_cast<uint64_t*>(sb + lay.rng_save);
-    RTC8X_HIP(hipMemcpy2DAsync(rng_save, 8 * (size_t)tw, p.rng_states, 8 * (size_t)p.image_width, 8 * (size_t)tw, th,
-                               hipMemcpyDeviceToDevice, s));
+    constexpr uint32_t kCompactBlocks = 512;
+    hipLaunchKernelGGL(dev::save_rng_kernel, dim3(kCompactBlocks), dim3(256), 0, s, p, rng_save);
+    RTC8X_HIP(hipGetLastError());
     dev::TrainStage stage{};
     stage.counter = reinterpret_cast<uint32_t*>(sb + lay.counter);
     stage.queries = reinterpret_cast<rtc8x_radiance_query*>(sb + lay.queries);
@@ -1328,64 +1350,83 @@ try {
     uint32_t* fmap = reinterpret_cast<uint32_t*>(sb + lay.fmap);
     hipLaunchKernelGGL(dev::reset_value_kernel, dim3(1), dim3(1), 0, s, stage.counter);
     RTC8X_HIP(hipGetLastError());
-    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, tw, th, 1, nullptr, counts, s, stage)));
-    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, counts, offsets, n, total);
+    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, 0, 0, 1, nullptr, counts, s, stage)));
+    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, p, counts, offsets, total);
     RTC8X_HIP(hipGetLastError());
-    constexpr uint32_t kCompactBlocks = 512;
     hipLaunchKernelGGL(dev::stage_map_kernel, dim3(kCompactBlocks), dim3(256), 0, s, total, stage.tag, offsets, fmap);
     RTC8X_HIP(hipGetLastError());
     hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, p, total, stage, fmap, offsets,
-                       counts, tw, n);
+                       counts);
     RTC8X_HIP(hipGetLastError());
     dev::TrainStage redo{};
     redo.redo_lost = stage.lost;
     redo.rng_save = rng_save;
-    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, tw, th, 1, offsets, nullptr, s, redo)));
+    RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, 0, 0, 1, offsets, nullptr, s, redo)));
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_trace")
 
-extern "C" rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
-                                          const float* shares, uint32_t count, int32_t entry)
-try {
-    RTC8X_REQUIRE(params && num_samples, "rtc8x_trace_batch: NULL argument");
-    RTC8X_REQUIRE(count >= 1 && count <= dev::kMaxBatch, "rtc8x_trace_batch: count %u must be in [1, %u]", count,
-                  dev::kMaxBatch);
+namespace {
+
+// Checks a batch and turns its shares into the launch's weights.
+rtc8x_status check_batch(const char* who, const rtc8x_frame_params* params, const uint32_t* num_samples,
+                         const float* shares, uint32_t count, int32_t entry, double* w)
+{
+    RTC8X_REQUIRE(params && num_samples, "%s: NULL argument", who);
+    RTC8X_REQUIRE(count >= 1 && count <= dev::kMaxBatch, "%s: count %u must be in [1, %u]", who, count, dev::kMaxBatch);
     RTC8X_REQUIRE(entry == RTC8X_ENTRY_PATH_TRACE || entry == RTC8X_ENTRY_PATH_TRACE_WITH_NRC,
-                  "rtc8x_trace_batch: entry %d: only the full-image entries run in a batch", entry);
-    double w[dev::kMaxBatch];
+                  "%s: entry %d: only the full-image entries run in a batch", who, entry);
     for (uint32_t i = 0; i < count; ++i) {
         const rtc8x_status st = check_launch(params[i], entry, num_samples[i]);
         if (st != RTC8X_OK) {
             const std::string why = rtc8x_last_error();
-            return fail(st, "rtc8x_trace_batch: instance %u: %s", i, why.c_str());
+            return fail(st, "%s: instance %u: %s", who, i, why.c_str());
         }
-        RTC8X_REQUIRE(params[i].tracking == params[0].tracking, "rtc8x_trace_batch: instances must share the tracking mode");
+        RTC8X_REQUIRE(params[i].tracking == params[0].tracking, "%s: instances must share the tracking mode", who);
         if (shares) {
-            RTC8X_REQUIRE(std::isfinite(shares[i]) && shares[i] > 0.0f, "rtc8x_trace_batch: shares must be positive");
+            RTC8X_REQUIRE(std::isfinite(shares[i]) && shares[i] > 0.0f, "%s: shares must be positive", who);
             w[i] = shares[i];
         } else {
             w[i] = (double)params[i].image_width * params[i].image_height * num_samples[i];
         }
     }
+    return RTC8X_OK;
+}
+
+// The batched launch over a device copy `d` of the instances' frame parameters.
+hipError_t launch_batch_any(const rtc8x_frame_params* d, int32_t tracking, int32_t entry, const uint32_t* num_samples,
+                            const double* w, uint32_t count, hipStream_t s)
+{
+    if (tracking == RTC8X_TRACKING_LOCAL_MAJORANT)
+        return entry == RTC8X_ENTRY_PATH_TRACE
+                   ? dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w, count, s)
+                   : dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w,
+                                                                                                      count, s);
+    return entry == RTC8X_ENTRY_PATH_TRACE
+               ? dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w, count, s)
+               : dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w,
+                                                                                                   count, s);
+}
+
+}  // namespace
+
+extern "C" rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
+                                          const float* shares, uint32_t count, int32_t entry)
+try {
+    double w[dev::kMaxBatch];
+    {
+        const rtc8x_status st = check_batch("rtc8x_trace_batch", params, num_samples, shares, count, entry, w);
+        if (st != RTC8X_OK) return st;
+    }
     hipStream_t s = (hipStream_t)stream;
     rtc8x_frame_params* d = nullptr;
     RTC8X_HIP(hipMallocAsync((void**)&d, sizeof(rtc8x_frame_params) * count, s));
-    RTC8X_HIP(hipMemcpyAsync(d, params, sizeof(rtc8x_frame_params) * count, hipMemcpyHostToDevice, s));
-    // the parameter copy is freed on every path, also when the launch fails
-    hipError_t launched;
-    if (params[0].tracking == RTC8X_TRACKING_LOCAL_MAJORANT) {
-        launched = entry == RTC8X_ENTRY_PATH_TRACE
-                       ? dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w, count, s)
-                       : dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_LOCAL_MAJORANT>(d, num_samples, w,
-                                                                                                          count, s);
-    } else {
-        launched = entry == RTC8X_ENTRY_PATH_TRACE
-                       ? dev::launch_batch<RTC8X_ENTRY_PATH_TRACE, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w, count, s)
-                       : dev::launch_batch<RTC8X_ENTRY_PATH_TRACE_WITH_NRC, RTC8X_TRACKING_GLOBAL_MAJORANT>(d, num_samples, w,
-                                                                                                           count, s);
-    }
+    const hipError_t copied = hipMemcpyAsync(d, params, sizeof(rtc8x_frame_params) * count, hipMemcpyHostToDevice, s);
+    // the parameter copy is freed on every path, also when the copy or the launch fails
+    const hipError_t launched =
+        copied == hipSuccess ? launch_batch_any(d, params[0].tracking, entry, num_samples, w, count, s) : hipSuccess;
     const hipError_t freed = hipFreeAsync(d, s);
+    RTC8X_HIP(copied);
     RTC8X_HIP(launched);
     RTC8X_HIP(freed);
     return RTC8X_OK;

@@ -73,6 +73,10 @@ class NeuralRadianceCache:
         call("rtc8x_nrc_infer", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr), int(num),
              C.c_void_p(output_ptr))
 
+    def infer_training(self, stream, params: _abi.FrameParams) -> None:
+        """Inference of a training iteration's suffix queries, sized on the device (no host wait)."""
+        call("rtc8x_nrc_infer_training", C.c_void_p(self.handle), C.c_void_p(stream), C.byref(params))
+
     def train(self, stream, input_ptr: int, target_ptr: int, num: int, want_loss: bool = False):
         loss = C.c_float(0.0)
         call("rtc8x_nrc_train", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr),
@@ -232,10 +236,8 @@ class NrcRenderer:
         p.nrc.offset_to_select_unbiased_path = self.frame_rng()
         call("rtc8x_nrc_prepare", s, C.byref(p), 1 if is_new_sequence else 0)
         call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_GENERATE_TRAINING_DATA, 1)
-        ts = self.buffers.train_image_size.download(np.int32, 4)
-        tw, th = int(ts[2 * self.buf_idx]), int(ts[2 * self.buf_idx + 1])
-        self.cache.infer(self.stream, self.buffers.inference_queries.ptr, round_up(tw * th, 128),
-                         self.buffers.inferred_radiance.ptr)
+        # the training size stays on the device: no host synchronisation in the iteration
+        self.cache.infer_training(self.stream, p)
         call("rtc8x_nrc_propagate", s, C.byref(p))
         call("rtc8x_nrc_shuffle", s, C.byref(p))
         loss = None

@@ -6,9 +6,12 @@ workloads the path is measured on are described directly:
 
 * ``dense128``  -- dense 128^3 fBm, sigma_t max 20, albedo 0.99, HG g = 0.8, constant white sky plus a
   directional sun, 256x256, 4 spp, at most 16 bounces.
-* ``sparse512`` -- sparse 512^3 thresholded fBm cloud (25 % of the 8^3 leaf slots active), sigma_t max 50,
-  albedo 0.999, HG g = 0.85, Hosek-Wilkie sky (turbidity 3, sun elevation 30 deg) importance-sampled with
-  alias tables, 1920x1080, 16 spp, at most 256 bounces, Russian roulette after bounce 8.
+* ``sparse512`` -- sparse 512^3 thresholded fBm cloud (25 % of the 8^3 leaf slots active; density =
+  (g - threshold) / (g_max - threshold), contrast 1), sigma_t max 50, albedo 0.999, HG g = 0.85,
+  Hosek-Wilkie sky (turbidity 3, sun elevation 30 deg) importance-sampled with alias tables, 1920x1080,
+  16 spp, at most 256 bounces, Russian roulette after bounce 8.
+* ``sparse512_saturated`` -- the same cloud with contrast 4: density = min(4 (g - threshold) /
+  (g_max - threshold), 1), a saturated interior at the majorant (denser, more vertices per path).
 """
 from __future__ import annotations
 
@@ -89,12 +92,15 @@ CONFIGS = {
         env_kind="constant", env_radiance=(1.0, 1.0, 1.0), sun_enabled=True),
     "sparse512": SceneConfig(
         name="sparse512",
-        fbm=FbmConfig(resolution=512, seed=0x5EED0002, sparse=True, active_leaf_fraction=0.25, contrast=4.0),
+        fbm=FbmConfig(resolution=512, seed=0x5EED0002, sparse=True, active_leaf_fraction=0.25, contrast=1.0),
         density_coeff=50.0, albedo=0.999, phase=_abi.PHASE_HENYEY_GREENSTEIN, forwardness=0.85,
         width=1920, height=1080, spp=16, max_path_length=256, rr_after_path_length=8,
         env_kind="hosek", env_sampler=_abi.ENV_SAMPLER_ALIAS, sun_enabled=False,
         camera_position=(1.27, 0.95, 2.0), fov_y_deg=32.0),
 }
+CONFIGS["sparse512_saturated"] = CONFIGS["sparse512"].with_(
+    name="sparse512_saturated",
+    fbm=FbmConfig(resolution=512, seed=0x5EED0002, sparse=True, active_leaf_fraction=0.25, contrast=4.0))
 
 
 def look_at_quaternion(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:

@@ -57,6 +57,8 @@ def load_oracle(path) -> C.CDLL:
                                        C.c_uint32]),
         "oracle_trace_f64": (C.c_int, [C.POINTER(_abi.FrameParams), C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_uint32]),
+        "oracle_trace_window_f64": (C.c_int, [C.POINTER(_abi.FrameParams), C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.c_uint32, C.c_uint32, C.c_uint32]),
         "oracle_env_sample_f64": (None, [C.POINTER(_abi.EnvLight), C.c_double, C.c_double, _P, _P, _P]),
         "oracle_env_evaluate_f64": (None, [C.POINTER(_abi.EnvLight), _P, _P, _P]),
         "oracle_nrc_prepare": (None, [C.POINTER(_abi.FrameParams), C.c_int32]),
@@ -167,8 +169,20 @@ def download_grid(dg: _abi.Grid) -> HostGrid:
     return host_grid_from_arrays(dg, arrays)
 
 
+def oracle_env_mips(texels: np.ndarray, w: int, h: int):
+    """The oracle's own hierarchical importance map of a texture: (mips, dim_x, dim_y, levels, integral)."""
+    dx, dy = C.c_uint32(), C.c_uint32()
+    levels = oracle().oracle_env_build_mips(ptr(texels), w, h, None, C.byref(dx), C.byref(dy), None)
+    assert levels > 0, "map must be 1:1 or 2:1"
+    total = sum(max(dx.value >> l, 1) * max(dy.value >> l, 1) for l in range(levels))
+    mips, integral = np.zeros(total, np.float32), np.zeros(1, np.float32)
+    oracle().oracle_env_build_mips(ptr(texels), w, h, ptr(mips), None, None, ptr(integral))
+    return mips, dx.value, dy.value, levels, float(integral[0])
+
+
 def download_env(de: _abi.EnvLight):
-    """Host copy of a device environment light (texels, mips, alias tables)."""
+    """Host copy of a device environment light (texels, alias tables).  The hierarchical importance map is
+    the oracle's own, built from the texels, and must equal the library's bit for bit."""
     from rtcamp8_amd.device import download_ptr
 
     e = _abi.EnvLight()
@@ -178,8 +192,12 @@ def download_env(de: _abi.EnvLight):
         n = de.width * de.height
         keep["texels"] = download_ptr(de.texels, np.float32, 4 * n)
         e.texels = keep["texels"].ctypes.data
-        total = sum(max(de.map_dim_x >> l, 1) * max(de.map_dim_y >> l, 1) for l in range(de.num_mip_levels))
-        keep["mips"] = download_ptr(de.mips, np.float32, total)
+        mips, dx, dy, levels, integral = oracle_env_mips(keep["texels"], de.width, de.height)
+        assert (dx, dy, levels) == (de.map_dim_x, de.map_dim_y, de.num_mip_levels)
+        gpu_mips = download_ptr(de.mips, np.float32, mips.size)
+        assert np.array_equal(gpu_mips.view(np.uint32), mips.view(np.uint32)), "importance map differs from the oracle's"
+        assert np.float32(integral).view(np.uint32) == np.float32(de.map_integral).view(np.uint32)
+        keep["mips"] = mips
         e.mips = keep["mips"].ctypes.data
         keep["rows"] = download_ptr(de.alias_rows, np.uint32, 2 * de.height)
         keep["cols"] = download_ptr(de.alias_cols, np.uint32, 2 * n)

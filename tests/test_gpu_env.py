@@ -127,3 +127,18 @@ def test_tracer_with_textured_sky_bit_exact(sampler):
     assert res["rng_equal"] and res["vertices_equal"]
     assert np.array_equal(res["gpu_accum"].view(np.uint64), res["cpu_accum"].view(np.uint64))
     assert res["lookups"] == int(res["cpu_stats"][2])
+
+
+def test_importance_map_at_the_benchmark_sky_size_equals_the_oracle():
+    """The 3200 x 1600 Hosek-Wilkie sky of the sparse 512^3 workload: the library's 4096 x 2048, 13-level map
+    and its integral equal the oracle's own build from the same texels bit for bit (download_env checks)."""
+    from rtcamp8_amd import scene as sc
+
+    cfg = sc.CONFIGS["sparse512"].with_(width=16, height=16, fbm=sc.FbmConfig(resolution=64, seed=7, sparse=True))
+    ds = sc.DeviceScene(cfg)
+    try:
+        he, keep = download_env(ds.env)
+        assert (he.map_dim_x, he.map_dim_y, he.num_mip_levels) == (4096, 2048, 13)
+        assert he.map_integral > 0
+    finally:
+        ds.close()

@@ -85,16 +85,17 @@ rtc8x_status rtc8x_gather_probe(void* stream, const struct rtc8x_grid* grid, uin
 struct rtc8x_frame_params;
 /*
  * rtc8x_march_probe: the memory-latency roofline of the tracer's own lookups (no counterpart in the
- * reference).  `lanes` lanes (a multiple of 256) walk paths of the frame's pixels (lane i starts at
- * pixel i mod width x height, with that pixel's PCG state, which is read and not written): jittered
- * camera rays, delta tracking at the global majorant's spacing with two tentative collisions looked up
- * together, and at every real collision a ratio-tracked shadow segment towards `light_dir` and a sample
- * of the frame's phase function, with none of the tracer's vertex shading.  Every lane stops after
- * `budget` lookups (<= 2^20).  The launch runs at `waves_per_simd` waves per SIMD (2..8, set with dynamic
- * LDS).  stats[0] += lookups made, stats[1] += paths ended; stats is a device array of two uint64.
+ * reference).  A persistent launch, `waves_per_simd` (2..8, set with dynamic LDS) 256-lane workgroups per
+ * CU, whose lanes take the frame's pixels in 8x8 tiles from a work counter as the tracer's do (the
+ * tiles wrapping around the image; a pixel's PCG state is read, never written) and trace `spp` paths per
+ * pixel: jittered camera rays, delta tracking at the global majorant's spacing with two tentative
+ * collisions looked up together, and at every real collision a ratio-tracked shadow segment towards
+ * `light_dir` and a sample of the frame's phase function, with none of the tracer's vertex shading.
+ * Every lane stops after `budget` lookups (<= 2^24), so the launch has no tail.  stats[0] += lookups
+ * made, stats[1] += paths ended; stats is a device array of two uint64.
  */
 rtc8x_status rtc8x_march_probe(void* stream, const struct rtc8x_frame_params* params, const float light_dir[3],
-                               uint32_t lanes, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats);
+                               uint32_t spp, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats);
 
 /* ------------------------------------------------------------------------------------------------
  * Sparse density grid in the NanoVDB tree layout (root tiles -> 32^3 upper -> 16^3 lower -> 8^3 leaf;

@@ -29,6 +29,12 @@ rtc8x_status adopt_host_grid(OwnedHostGrid&& t, const rtc8x_grid& g, rtc8x_grid*
 // rtc8x_grid_upload; with keep_max the host's max_value is kept instead of the brute-force maximum.
 rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_max);
 
+// rtc8x_trace_batch with the instances' frame parameters copied into `table`, a device array of at least
+// `count` entries that the caller provides (`who` names the entry point in error messages).
+rtc8x_status trace_batch_into(const char* who, void* stream, const rtc8x_frame_params* params,
+                              const uint32_t* num_samples, const float* shares, uint32_t count, int32_t entry,
+                              rtc8x_frame_params* table);
+
 // Records the one-pass training launch can stage: beyond them, records find no slot and their pixels
 // are traced again.
 constexpr size_t kStageCapacity = 4 * (size_t)RTC8X_TRAIN_BUFFER_SIZE;

@@ -62,35 +62,57 @@ __global__ void gather_probe_kernel(const rtc8x_grid g, uint32_t steps, uint32_t
     out[tid] = acc;
 }
 
-// The tracer's stepping pattern with its shading taken out.  Lane i walks paths of pixel i % (W H) of the
-// frame's camera: a jittered camera ray (optix_pathtracing_kernels.cu:43-55), delta tracking at the
-// global majorant's spacing, and at every real collision a ratio-tracked shadow segment towards `light`
-// (the length the tracer gives it, 2 x the world radius) and a phase-function sample of the next
-// direction, until the path escapes.  Tentative collisions are taken two at a time and their lookups
-// issued together (density_batch<2>), as the tracer does, and one copy of the lookup code serves both
-// segment kinds.  Every lane stops after `budget` lookups, so the launch has no tail of long paths;
-// what it leaves out is the tracer's vertex work: NEE light sampling and MIS, the environment, Russian
-// roulette, NRC records and accumulation.  stats[0] += lookups, stats[1] += paths that escaped.
-__global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, uint32_t budget,
+// The tracer's stepping pattern with its shading taken out, as a persistent launch like the tracer's:
+// lanes take pixels in 8x8 tiles from a work counter (one atomic per wavefront, the tiles wrapping around
+// the image) and trace every one of a pixel's `spp` paths before the next: a jittered camera ray
+// (optix_pathtracing_kernels.cu:43-55), delta tracking at the global majorant's spacing, and at every real
+// collision a ratio-tracked shadow segment towards `light` (the length the tracer gives it, 2 x the
+// world radius) and a phase-function sample of the next direction, until the path escapes.  Tentative
+// collisions are taken two at a time and their lookups issued together (density_batch<2>), as the tracer
+// does, and one copy of the lookup code serves both segment kinds.  Every lane stops after `budget`
+// lookups, so the launch has no tail; what it leaves out is the tracer's vertex work: NEE light sampling
+// and MIS, the environment, Russian roulette, NRC records and accumulation.
+// stats[0] += lookups, stats[1] += paths that ended.
+__device__ uint32_t g_probe_counter;
+__global__ void reset_probe_kernel() { g_probe_counter = 0u; }
+
+__global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, uint32_t spp, uint32_t budget,
                                    unsigned long long* __restrict__ stats)
 {
     extern __shared__ float occupancy_limiter[];
-    const uint32_t W = p.image_width, H = p.image_height, N = W * H;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, lanes = gridDim.x * blockDim.x;
-    uint32_t lin = tid % N;
-    dev::Pcg rng;
-    rng.state = p.rng_states[lin];
+    const uint32_t W = p.image_width, H = p.image_height;
+    const uint32_t tiles_x = (W + 7) / 8, tiles = tiles_x * ((H + 7) / 8);
+    const uint32_t lane_id = threadIdx.x & 63;
     const float maj = p.density_coeff * p.grid.max_value;
     const float coeff = p.density_coeff;
     const float trace_len = 2.0f * p.world_radius;
+    dev::Pcg rng;
+    rng.state = 0;
     dev::V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
     float t = 0.0f, t1 = -1.0f, vis = 1.0f;
-    uint32_t mode = 0;  // 0: start a path, 1: tracking, 2: shadow
-    uint32_t looked = 0, paths = 0;
+    uint32_t mode = 3;  // 0: start a path, 1: tracking, 2: shadow, 3: needs a pixel
+    uint32_t looked = 0, paths = 0, lx = 0, ly = 0, sample = 0;
     dev::V3 pos{0.0f, 0.0f, 0.0f}, dir{0.0f, 0.0f, 1.0f};
-    for (uint32_t iter = 0; iter < 4u * budget + 64u && looked < budget; ++iter) {
+    for (uint32_t iter = 0; iter < 8u * budget + 256u && looked < budget; ++iter) {
+        if (mode == 3) {
+            // the next pixel, one atomic per wavefront
+            const uint64_t ask = __ballot(1);
+            const uint32_t leader = (uint32_t)__ffsll((long long)ask) - 1u;
+            uint32_t base = 0;
+            if (lane_id == leader) base = atomicAdd(&g_probe_counter, (uint32_t)__popcll(ask));
+            base = __shfl(base, (int)leader, 64);
+            const uint32_t w = base + (uint32_t)__popcll(ask & ((1ull << lane_id) - 1ull));
+            const uint32_t tile = (w >> 6) % tiles, within = w & 63u;
+            lx = (tile % tiles_x) * 8 + (within & 7u);
+            ly = (tile / tiles_x) * 8 + (within >> 3);
+            if (lx >= W || ly >= H) continue;
+            rng.state = p.rng_states[(size_t)ly * W + lx] ^ ((uint64_t)(w / (64u * tiles)) << 48);
+            sample = 0;
+            mode = 0;
+            continue;
+        }
         if (mode == 0) {
-            const float px = ((float)(lin % W) + rng.uni()) / (float)W, py = ((float)(lin / W) + rng.uni()) / (float)H;
+            const float px = ((float)lx + rng.uni()) / (float)W, py = ((float)ly + rng.uni()) / (float)H;
             const dev::V3 ldir = dev::normalize(dev::V3{p.camera.vw * (-0.5f + px), p.camera.vh * (0.5f - py), -1.0f});
             const float* M = p.camera.orientation;
             dir = dev::normalize(dev::V3{dev::dot(dev::V3{M[0], M[1], M[2]}, ldir), dev::dot(dev::V3{M[3], M[4], M[5]}, ldir),
@@ -102,10 +124,9 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
                 d = dir;
                 t = fmaxf(0.0f, t0);
                 mode = 1;
-            } else {  // the camera ray misses the volume: the lane moves on to another pixel
+            } else {  // the camera ray misses the volume: the path ends at once
                 ++paths;
-                lin = (lin + lanes) % N;
-                rng.state = p.rng_states[lin];
+                mode = ++sample < spp ? 0u : 3u;
             }
             continue;
         }
@@ -150,10 +171,8 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
             }
         }
         if (event == 2) {
-            if (mode == 1) {  // escaped: the pixel's next path
-                ++paths;
-                mode = 0;
-            } else {  // shadow segment done: phase sample and the next tracking segment
+            bool ended = mode == 1;  // escaped
+            if (!ended) {  // shadow segment done: phase sample and the next tracking segment
                 dev::V3 dl;
                 const float us0 = rng.uni(), us1 = rng.uni();
                 dev::phase_sample(p.phase, p.scattering_forwardness, us0, us1, dl);
@@ -166,9 +185,12 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
                     t = fmaxf(0.0f, t0);
                     mode = 1;
                 } else {
-                    ++paths;
-                    mode = 0;
+                    ended = true;
                 }
+            }
+            if (ended) {
+                ++paths;
+                mode = ++sample < spp ? 0u : 3u;
             }
         }
     }
@@ -248,7 +270,7 @@ try {
 RTC8X_CATCH_ALL("rtc8x_gather_probe")
 
 extern "C" rtc8x_status rtc8x_march_probe(void* stream, const rtc8x_frame_params* params, const float light_dir[3],
-                                          uint32_t lanes, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats)
+                                          uint32_t spp, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats)
 try {
     RTC8X_REQUIRE(params && light_dir && stats, "rtc8x_march_probe: NULL pointer");
     const rtc8x_frame_params& p = *params;
@@ -256,8 +278,7 @@ try {
                   "rtc8x_march_probe: the frame needs an image and its PCG states");
     RTC8X_REQUIRE(p.grid.leaf_values && p.density_coeff * p.grid.max_value > 0.0f,
                   "rtc8x_march_probe: the grid must be on the device with a positive majorant");
-    RTC8X_REQUIRE(lanes > 0 && lanes % 256 == 0 && budget > 0 && budget <= (1u << 20),
-                  "rtc8x_march_probe: lanes must be a positive multiple of 256 and budget in [1, 2^20]");
+    RTC8X_REQUIRE(spp > 0 && budget > 0 && budget <= (1u << 24), "rtc8x_march_probe: spp must be > 0, budget in [1, 2^24]");
     RTC8X_REQUIRE(waves_per_simd >= 2 && waves_per_simd <= 8, "rtc8x_march_probe: waves_per_simd must be in [2, 8]");
     size_t lds = 0;
     {
@@ -267,8 +288,14 @@ try {
     const float n = std::sqrt(light_dir[0] * light_dir[0] + light_dir[1] * light_dir[1] + light_dir[2] * light_dir[2]);
     RTC8X_REQUIRE(n > 0.0f, "rtc8x_march_probe: light direction is zero");
     const dev::V3 l{light_dir[0] / n, light_dir[1] / n, light_dir[2] / n};
+    // persistent: waves_per_simd 256-lane workgroups per CU, all resident at once
+    int dev = 0, cus = 0;
+    RTC8X_HIP(hipGetDevice(&dev));
+    RTC8X_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(march_probe_kernel, dim3(lanes / 256), dim3(256), lds, s, p, l, budget,
+    hipLaunchKernelGGL(reset_probe_kernel, dim3(1), dim3(1), 0, s);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(march_probe_kernel, dim3((uint32_t)cus * waves_per_simd), dim3(256), lds, s, p, l, spp, budget,
                        (unsigned long long*)stats);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;

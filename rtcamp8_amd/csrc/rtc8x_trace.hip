@@ -1410,24 +1410,38 @@ hipError_t launch_batch_any(const rtc8x_frame_params* d, int32_t tracking, int32
 
 }  // namespace
 
+namespace rtc8x {
+
+rtc8x_status trace_batch_into(const char* who, void* stream, const rtc8x_frame_params* params,
+                              const uint32_t* num_samples, const float* shares, uint32_t count, int32_t entry,
+                              rtc8x_frame_params* table)
+{
+    double w[dev::kMaxBatch];
+    {
+        const rtc8x_status st = check_batch(who, params, num_samples, shares, count, entry, w);
+        if (st != RTC8X_OK) return st;
+    }
+    RTC8X_REQUIRE(table != nullptr, "%s: no device table for the instances", who);
+    hipStream_t s = (hipStream_t)stream;
+    RTC8X_HIP(hipMemcpyAsync(table, params, sizeof(rtc8x_frame_params) * count, hipMemcpyHostToDevice, s));
+    RTC8X_HIP(launch_batch_any(table, params[0].tracking, entry, num_samples, w, count, s));
+    return RTC8X_OK;
+}
+
+}  // namespace rtc8x
+
 extern "C" rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
                                           const float* shares, uint32_t count, int32_t entry)
 try {
-    double w[dev::kMaxBatch];
-    {
-        const rtc8x_status st = check_batch("rtc8x_trace_batch", params, num_samples, shares, count, entry, w);
-        if (st != RTC8X_OK) return st;
-    }
+    RTC8X_REQUIRE(count >= 1 && count <= dev::kMaxBatch, "rtc8x_trace_batch: count %u must be in [1, %u]", count,
+                  dev::kMaxBatch);
     hipStream_t s = (hipStream_t)stream;
     rtc8x_frame_params* d = nullptr;
     RTC8X_HIP(hipMallocAsync((void**)&d, sizeof(rtc8x_frame_params) * count, s));
-    const hipError_t copied = hipMemcpyAsync(d, params, sizeof(rtc8x_frame_params) * count, hipMemcpyHostToDevice, s);
-    // the parameter copy is freed on every path, also when the copy or the launch fails
-    const hipError_t launched =
-        copied == hipSuccess ? launch_batch_any(d, params[0].tracking, entry, num_samples, w, count, s) : hipSuccess;
+    // the instance table is freed on every path, also when the checks, the copy or the launch fail
+    const rtc8x_status st = trace_batch_into("rtc8x_trace_batch", stream, params, num_samples, shares, count, entry, d);
     const hipError_t freed = hipFreeAsync(d, s);
-    RTC8X_HIP(copied);
-    RTC8X_HIP(launched);
+    if (st != RTC8X_OK) return st;
     RTC8X_HIP(freed);
     return RTC8X_OK;
 }

@@ -21,6 +21,16 @@ def device_name() -> str:
     return buf.value.decode()
 
 
+def device_cus() -> int:
+    """Compute units of the current device (rtc8x_device_name reports them as "..., N CUs)")."""
+    import re
+
+    m = re.search(r"(\d+) CUs\)", device_name())
+    if not m:
+        raise RuntimeError(f"no CU count in the device name {device_name()!r}")
+    return int(m.group(1))
+
+
 def synchronize() -> None:
     call("rtc8x_device_synchronize")
 

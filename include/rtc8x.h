@@ -553,6 +553,16 @@ size_t rtc8x_order_scratch_bytes(uint32_t n);
 rtc8x_status rtc8x_trace_batch(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
                                const float* shares, uint32_t count, int32_t entry);
 
+/* rtc8x_trace_batch for callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  No counterpart in the reference.  The same
+ * launch and results as rtc8x_trace_batch, but the instances' frame parameters are staged in a fixed
+ * device array of 16 entries (16 x sizeof(rtc8x_frame_params) bytes per GPU, held until the process
+ * ends) instead of a per-call allocation: count above 16 is an error, and nothing is allocated or freed.
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one GPU (different streams, or
+ * threads) race: order them on one stream. */
+rtc8x_status rtc8x_trace_batch_static(void* stream, const rtc8x_frame_params* params, const uint32_t* num_samples,
+                                      const float* shares, uint32_t count, int32_t entry);
+
 /* ------------------------------------------------------------------------------------------------
  * NRC data-path kernels (renderer/gpu_kernels/nrc_set_up_kernels.cu).  They read the same
  * rtc8x_frame_params as rtc8x_trace (buffers in params->nrc, image size, accum).

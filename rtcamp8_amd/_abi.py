@@ -394,6 +394,7 @@ PROTOTYPES = {
     "rtc8x_camera_setup": (_S, [C.POINTER(Camera), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float]),
     "rtc8x_trace": (_S, [_P, C.POINTER(FrameParams), C.c_int32, C.c_uint32]),
     "rtc8x_trace_batch": (_S, [_P, _P, _P, _P, C.c_uint32, C.c_int32]),
+    "rtc8x_trace_batch_static": (_S, [_P, _P, _P, _P, C.c_uint32, C.c_int32]),
     "rtc8x_order_by_cost": (_S, [_P, _P, C.c_uint32, _P, _P]),
     "rtc8x_order_scratch_bytes": (C.c_size_t, [C.c_uint32]),
     "rtc8x_seed_rng": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint64]),

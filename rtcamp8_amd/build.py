@@ -32,6 +32,7 @@ SOURCES = [
     "rtc8x_measure.hip",
     "rtc8x_nvdb.hip",
     "rtc8x_post.hip",
+    "rtc8x_noalloc.hip",
 ]
 
 HIPCC_FLAGS = [

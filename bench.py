@@ -274,7 +274,9 @@ def multi_instance_leg(steps: int, stream):
     def one(i):
         call("rtc8x_trace", C.c_void_p(stream.handle), C.byref(params(rs[i])), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
 
-    shares = batch.autotune_shares(one, len(rs), stream=stream.handle)
+    def one_with(i, ev, shape):
+        call("rtc8x_trace", C.c_void_p(stream.handle), C.byref(batch.apply_knobs(params(rs[i]), (ev, shape))),
+             _abi.ENTRY_PATH_TRACE_WITH_NRC, 1)
 
     def timed(fn):
         fn()
@@ -291,9 +293,15 @@ def multi_instance_leg(steps: int, stream):
 
     ones = [1] * len(rs)
     seq_ms = timed(lambda: [one(i) for i in range(len(rs))])
-    tuned_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, shares,
-                                           stream.handle))
     pixel_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, None,
+                                           stream.handle))
+    t_tune = time.perf_counter()
+    shares, knobs = batch.autotune(one_with, len(rs), stream=stream.handle)
+    tune_s = time.perf_counter() - t_tune
+    for r, k in zip(rs, knobs):
+        r.knobs = k
+    seq_tuned_ms = timed(lambda: [one(i) for i in range(len(rs))])
+    tuned_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, shares,
                                            stream.handle))
     batch.render_nrc(rs, ones, shares, stream.handle)
     stream.synchronize()
@@ -307,7 +315,11 @@ def multi_instance_leg(steps: int, stream):
                           for (res, sparse, sigma, w, h), t in zip(MULTI_INSTANCES, shares)],
             "nrc": "hash grid, 2 hidden layers, bf16 training, e4m3 inference; first frame (8 training iterations) done",
             "entry": "PATH_TRACE_WITH_NRC, 1 spp per instance",
-            "one_launch_per_instance_ms": seq_ms, "batched_autotuned_ms": tuned_ms, "batched_by_pixels_ms": pixel_ms,
+            "one_launch_per_instance_ms": seq_ms, "one_launch_per_instance_tuned_ms": seq_tuned_ms,
+            "batched_autotuned_ms": tuned_ms, "batched_by_pixels_ms": pixel_ms,
+            "autotuned": [{"event_eighths": k[0], "tile": batch.TILE_SHAPES[k[1]], "share_ms": sh}
+                          for k, sh in zip(knobs, shares)],
+            "autotune_seconds": tune_s,
             "speedup_autotuned": seq_ms / tuned_ms, "finite": finite}
 
 

@@ -167,7 +167,21 @@ typedef struct rtc8x_grid {
     double map_mat_d[9];
     double map_inv_mat_d[9];
     double map_vec_d[3];
+    /* Region majorants (library-built with the cell majorants, optional; used by
+     * RTC8X_TRACKING_LOCAL_MAJORANT, which walks regions, not cells): the cell grid is grouped into regions
+     * of region_cells^3 cells from cell_origin (region_cells a power of two, the least >= 1 that keeps
+     * region_dims[0] * region_dims[1] * region_dims[2] <= RTC8X_MAX_REGIONS, so the tracer can stage the
+     * table in LDS).  One byte q per region, x-major: the region's majorant is (float)q * region_unit,
+     * q the least value >= the region's largest cell majorant (q = 0: every cell of the region is 0).
+     * region_unit is the largest cell majorant / 255, raised to the next float until 255 * region_unit
+     * covers it. */
+    int32_t region_dims[3];
+    uint32_t region_cells;
+    float region_unit;
+    const uint8_t* region_majorants;
 } rtc8x_grid;
+
+#define RTC8X_MAX_REGIONS 32768
 
 /* Procedural fBm volume description (the benchmark's synthetic input; the reference loads a .nvdb). */
 typedef struct rtc8x_fbm_desc {
@@ -522,11 +536,18 @@ typedef struct rtc8x_frame_params {
     /* Optional scheduling (results do not depend on it): pixel_cost[i] receives the density lookups of
      * launch pixel i (row-major over the launch) in this launch; work_order, a permutation of the
      * launch's pixels, is the order in which lanes take them (see rtc8x_order_by_cost).  Without
-     * work_order lanes take 8x8 pixel tiles in order (GENERATE_TRAINING_DATA: the unbiased-suffix
-     * pixels first). */
+     * work_order lanes take pixel tiles (tile_shape below) in order (GENERATE_TRAINING_DATA: the
+     * unbiased-suffix pixels first). */
     uint32_t* pixel_cost;
     const uint32_t* work_order;
     rtc8x_nrc_buffers nrc;
+    /* Scheduling knobs (no counterpart in the reference; results do not depend on them), 0 = default:
+     * event_eighths, 1..8: a wavefront runs its parked lanes' vertex events once that many eighths of its
+     * active lanes are parked (default 3 for PATH_TRACE and GENERATE_TRAINING_DATA, 4 for the NRC render
+     * entry); tile_shape: the pixel tiles lanes take without a work order, 0 = 8x8 (default), 1 = 16x4,
+     * 2 = 32x2, 3 = 4x16. */
+    uint32_t event_eighths;
+    uint32_t tile_shape;
 } rtc8x_frame_params;
 
 /* One launch of an entry point over the image (optixPipeline.launch(stream, plp, w, h, 1),

@@ -347,6 +347,7 @@ void oracle_grid_free(rtc8x_grid* g)
     free((void*)g->tile_values);
     free((void*)g->leaf_origins);
     free((void*)g->cell_majorants);
+    free((void*)g->region_majorants);
     memset(g, 0, sizeof(*g));
 }
 
@@ -477,6 +478,60 @@ void oracle_compute_cell_majorants(rtc8x_grid* g)
             }
     free(lm);
     g->cell_majorants = cm;
+    oracle_compute_region_majorants(g);
+}
+
+/* Region majorants over groups of region_cells^3 cells (the grid's definition in rtc8x.h): the least
+ * power-of-two region_cells >= 1 with at most RTC8X_MAX_REGIONS regions; region_unit = the largest cell
+ * majorant / 255, raised float by float until 255 * region_unit covers it; per region the least q with
+ * q * region_unit >= the region's largest cell majorant. */
+void oracle_compute_region_majorants(rtc8x_grid* g)
+{
+    free((void*)g->region_majorants);
+    g->region_majorants = NULL;
+    g->region_cells = 0;
+    g->region_unit = 0.0f;
+    for (int a = 0; a < 3; ++a) g->region_dims[a] = 0;
+    if (!g->cell_majorants) return;
+    const int32_t* cd = g->cell_dims;
+    uint32_t rc = 1;
+    int32_t rd[3];
+    for (;;) {
+        size_t n = 1;
+        for (int a = 0; a < 3; ++a) {
+            rd[a] = (cd[a] + (int32_t)rc - 1) / (int32_t)rc;
+            n *= (size_t)rd[a];
+        }
+        if (n <= RTC8X_MAX_REGIONS) break;
+        rc *= 2;
+    }
+    size_t nr = (size_t)rd[0] * rd[1] * rd[2];
+    float* rmax = (float*)calloc(nr, sizeof(float));
+    float top = 0.0f;
+    for (int32_t x = 0; x < cd[0]; ++x)
+        for (int32_t y = 0; y < cd[1]; ++y)
+            for (int32_t z = 0; z < cd[2]; ++z) {
+                float m = g->cell_majorants[((size_t)x * cd[1] + y) * cd[2] + z];
+                size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + y / (int32_t)rc) * rd[2] + z / (int32_t)rc;
+                if (m > rmax[r]) rmax[r] = m;
+                if (m > top) top = m;
+            }
+    float unit = top / 255.0f;
+    while (255.0f * unit < top) unit = nextafterf(unit, INFINITY);
+    uint8_t* q = (uint8_t*)malloc(nr);
+    for (size_t r = 0; r < nr; ++r) {
+        uint32_t v = 0;
+        if (rmax[r] > 0.0f) {
+            v = 1;
+            while (v < 255 && (float)v * unit < rmax[r]) ++v;
+        }
+        q[r] = (uint8_t)v;
+    }
+    free(rmax);
+    for (int a = 0; a < 3; ++a) g->region_dims[a] = rd[a];
+    g->region_cells = rc;
+    g->region_unit = unit;
+    g->region_majorants = q;
 }
 
 /* nanovdb::Ray::intersects(const BBox<Vec3f>&, t0, t1) and clip (util/Ray.h:63-73, 427-455, 512-520):

@@ -31,6 +31,7 @@ STAT_PATHS, STAT_VERTICES, STAT_LOOKUPS, STAT_SCATTER, STAT_COUNT = 0, 1, 2, 3, 
 INVALID_VERTEX_INDEX = 0x003FFFFF
 NUM_TRAINING_DATA_PER_FRAME = 65536
 TRAIN_BUFFER_SIZE = 2 * 65536
+MAX_REGIONS = 32768
 
 
 class RootTile(C.Structure):
@@ -69,6 +70,10 @@ class Grid(C.Structure):
         ("map_mat_d", C.c_double * 9),
         ("map_inv_mat_d", C.c_double * 9),
         ("map_vec_d", C.c_double * 3),
+        ("region_dims", c_i3),
+        ("region_cells", C.c_uint32),
+        ("region_unit", C.c_float),
+        ("region_majorants", C.c_void_p),
     ]
 
 
@@ -263,6 +268,8 @@ class FrameParams(C.Structure):
         ("pixel_cost", C.c_void_p),
         ("work_order", C.c_void_p),
         ("nrc", NrcBuffers),
+        ("event_eighths", C.c_uint32),
+        ("tile_shape", C.c_uint32),
     ]
 
 

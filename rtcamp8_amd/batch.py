@@ -1,9 +1,11 @@
-"""Several instances in one persistent launch (rtc8x_trace_batch) and the autotuner of their shares of
-the device.
+"""Several instances in one persistent launch (rtc8x_trace_batch) and the autotuner of their scheduling.
 
 An instance is a PathTracer (plain entry) or an NrcRenderer (the NRC render entry), each with its own
-scene, image and buffers.  `autotune_shares` times every instance's own launch and returns those times
-as the workgroup shares of the batched launch, so that the instances finish together."""
+scene, image and buffers.  `autotune` times every instance's own launch under each candidate setting of
+the tracer's scheduling knobs (the event threshold: how many eighths of a wavefront's active lanes park
+before their vertex events run; and the shape of the 64-pixel tiles lanes take), keeps each instance's
+fastest setting, and returns the fastest times as the workgroup shares of the batched launch, so that
+the instances finish together.  The knobs change the schedule, never the results."""
 from __future__ import annotations
 
 import ctypes as C
@@ -60,20 +62,43 @@ def render_nrc(renderers: Sequence[NrcRenderer], samples: Sequence[int], shares=
         r.stats.inference_queries += nq
 
 
+# (event_eighths, tile_shape) candidates: events from 2/8 to 5/8 of the active lanes; 8x8, 16x4, 4x16 tiles
+KNOB_CANDIDATES = [(ev, shape) for ev in (2, 3, 4, 5) for shape in (0, 1, 3)]
+TILE_SHAPES = {0: "8x8", 1: "16x4", 2: "32x2", 3: "4x16"}
+
+
+def _time(fn, repeats: int, stream) -> float:
+    fn()
+    best = float("inf")
+    for _ in range(repeats):
+        t = EventTimer()
+        t.start(stream)
+        fn()
+        t.stop(stream)
+        synchronize()
+        best = min(best, t.elapsed_ms())
+        t.destroy()
+    return best
+
+
+def autotune(launch_one, count: int, candidates=KNOB_CANDIDATES, repeats: int = 2, stream=None):
+    """launch_one(i, event_eighths, tile_shape) launches instance i alone with those knobs.  Returns
+    (shares, knobs): per instance the time of its fastest candidate and that candidate."""
+    shares, knobs = [], []
+    for i in range(count):
+        timed = [(_time(lambda: launch_one(i, ev, shape), repeats, stream), (ev, shape)) for ev, shape in candidates]
+        best = min(timed)
+        shares.append(best[0])
+        knobs.append(best[1])
+    return shares, knobs
+
+
+def apply_knobs(params: _abi.FrameParams, knobs) -> _abi.FrameParams:
+    params.event_eighths, params.tile_shape = int(knobs[0]), int(knobs[1])
+    return params
+
+
 def autotune_shares(launch_one, count: int, repeats: int = 2, stream=None) -> list[float]:
     """Times launch_one(i) for every instance i (the best of `repeats`, after one untimed launch) and
     returns the times: the shares under which the instances of one batched launch finish together."""
-    times = []
-    for i in range(count):
-        launch_one(i)
-        best = float("inf")
-        for _ in range(repeats):
-            t = EventTimer()
-            t.start(stream)
-            launch_one(i)
-            t.stop(stream)
-            synchronize()
-            best = min(best, t.elapsed_ms())
-            t.destroy()
-        times.append(best)
-    return times
+    return [_time(lambda: launch_one(i), repeats, stream) for i in range(count)]

@@ -21,6 +21,7 @@ struct OwnedHostGrid {
     std::vector<rtc8x_root_tile> root;
     std::vector<uint32_t> upper, lower;
     std::vector<float> values, tiles, cells;
+    std::vector<uint8_t> regions;
     std::vector<int32_t> origins;
     std::vector<uint64_t> leaf_masks, lower_masks, upper_masks;
 };

@@ -345,9 +345,11 @@ __device__ __forceinline__ float ratio_global(const rtc8x_grid& g, Medium& m, Pc
     return vis;
 }
 
-// 3D DDA over the 8^3 majorant cells in index space; boundaries recomputed from the cell index.
-// Per-axis state is held in named scalars and selected by axis (no indexed arrays, so it stays in
-// registers).
+// 3D DDA over the majorant regions (region_cells^3 cells of 8^3 voxels) in index space; boundaries
+// recomputed from the region index.  Per-axis state is held in named scalars and selected by axis (no
+// indexed arrays, so it stays in registers).
+__device__ __forceinline__ int32_t region_size(const rtc8x_grid& g) { return 8 * (int32_t)g.region_cells; }
+
 struct Dda {
     int32_t c0, c1, c2;      // cell
     int32_t s0, s1, s2;      // step per axis (-1, 0, +1)
@@ -361,20 +363,21 @@ struct Dda {
     }
     __device__ __forceinline__ int32_t cell(int a) const { return seli(a, c0, c1, c2); }
     __device__ __forceinline__ float tnext(int a) const { return self(a, tn0, tn1, tn2); }
-    static __device__ __forceinline__ float boundary_of(int32_t origin, int32_t c, int32_t st, float io, float iv)
+    static __device__ __forceinline__ float boundary_of(int32_t origin, int32_t size, int32_t c, int32_t st, float io,
+                                                        float iv)
     {
-        int32_t b = origin + 8 * (c + (st > 0 ? 1 : 0));
+        int32_t b = origin + size * (c + (st > 0 ? 1 : 0));
         return ((float)b - io) * iv;
     }
-    static __device__ __forceinline__ void axis_init(float t, float io, float id, int32_t origin, int32_t dims,
-                                                     int32_t& c, int32_t& st, float& iv, float& tn)
+    static __device__ __forceinline__ void axis_init(float t, float io, float id, int32_t origin, int32_t size,
+                                                     int32_t dims, int32_t& c, int32_t& st, float& iv, float& tn)
     {
         float pa = fmaf(t, id, io);
-        float cf = floorf((pa - (float)origin) * 0.125f);
+        float cf = floorf((pa - (float)origin) * (1.0f / (float)size));
         c = cf < 0.0f ? 0 : (cf > (float)(dims - 1) ? dims - 1 : (int32_t)cf);
         st = id > 0.0f ? 1 : (id < 0.0f ? -1 : 0);
         iv = 1.0f / id;
-        tn = st != 0 ? boundary_of(origin, c, st, io, iv) : __builtin_huge_valf();
+        tn = st != 0 ? boundary_of(origin, size, c, st, io, iv) : __builtin_huge_valf();
     }
     __device__ __forceinline__ void init(const rtc8x_grid& g, V3 o, V3 d, float t)
     {
@@ -386,9 +389,10 @@ struct Dda {
         const float id0 = fmaf(d.x, mm[0], fmaf(d.y, mm[1], d.z * mm[2]));
         const float id1 = fmaf(d.x, mm[3], fmaf(d.y, mm[4], d.z * mm[5]));
         const float id2 = fmaf(d.x, mm[6], fmaf(d.y, mm[7], d.z * mm[8]));
-        axis_init(t, io0, id0, g.cell_origin[0], g.cell_dims[0], c0, s0, iv0, tn0);
-        axis_init(t, io1, id1, g.cell_origin[1], g.cell_dims[1], c1, s1, iv1, tn1);
-        axis_init(t, io2, id2, g.cell_origin[2], g.cell_dims[2], c2, s2, iv2, tn2);
+        const int32_t size = region_size(g);
+        axis_init(t, io0, id0, g.cell_origin[0], size, g.region_dims[0], c0, s0, iv0, tn0);
+        axis_init(t, io1, id1, g.cell_origin[1], size, g.region_dims[1], c1, s1, iv1, tn1);
+        axis_init(t, io2, id2, g.cell_origin[2], size, g.region_dims[2], c2, s2, iv2, tn2);
     }
     __device__ __forceinline__ int axis() const
     {
@@ -408,9 +412,9 @@ struct Dda {
     {
         const int32_t st = seli(a, s0, s1, s2);
         const int32_t c = seli(a, c0, c1, c2) + st;
-        if (c < 0 || c >= seli(a, g.cell_dims[0], g.cell_dims[1], g.cell_dims[2])) return false;
-        const float tn = boundary_of(seli(a, g.cell_origin[0], g.cell_origin[1], g.cell_origin[2]), c, st,
-                                     self(a, io0, io1, io2), self(a, iv0, iv1, iv2));
+        if (c < 0 || c >= seli(a, g.region_dims[0], g.region_dims[1], g.region_dims[2])) return false;
+        const float tn = boundary_of(seli(a, g.cell_origin[0], g.cell_origin[1], g.cell_origin[2]), region_size(g), c,
+                                     st, self(a, io0, io1, io2), self(a, iv0, iv1, iv2));
         c0 = a == 0 ? c : c0;
         c1 = a == 1 ? c : c1;
         c2 = a == 2 ? c : c2;
@@ -421,10 +425,14 @@ struct Dda {
     }
 };
 
-__device__ __forceinline__ float cell_majorant(const rtc8x_grid& g, const Medium& m, const Dda& s)
+// The region's majorant, densityCoeff x (q x region_unit), with q from `table`: the grid's region bytes,
+// or a copy of them staged in LDS.
+__device__ __forceinline__ float cell_majorant(const rtc8x_grid& g, const Medium& m, const Dda& s,
+                                              const uint8_t* __restrict__ table)
 {
-    size_t idx = ((size_t)s.c0 * (size_t)g.cell_dims[1] + (size_t)s.c1) * (size_t)g.cell_dims[2] + (size_t)s.c2;
-    return m.coeff * g.cell_majorants[idx];
+    const uint32_t idx = ((uint32_t)s.c0 * (uint32_t)g.region_dims[1] + (uint32_t)s.c1) * (uint32_t)g.region_dims[2] +
+                         (uint32_t)s.c2;
+    return m.coeff * ((float)table[idx] * g.region_unit);
 }
 
 __device__ __forceinline__ bool track_local(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0, float t1,
@@ -438,7 +446,7 @@ __device__ __forceinline__ bool track_local(const rtc8x_grid& g, Medium& m, Pcg&
         int a = s.axis();
         const float tna = s.tnext(a);
         float tend = tna < t1 ? tna : t1;
-        float mu = cell_majorant(g, m, s);
+        float mu = cell_majorant(g, m, s, g.region_majorants);
         if (mu > 0.0f) {
             for (;;) {
                 float tc = t + tau / mu;
@@ -473,7 +481,7 @@ __device__ __forceinline__ float ratio_local(const rtc8x_grid& g, Medium& m, Pcg
         int a = s.axis();
         const float tna = s.tnext(a);
         float tend = tna < t1 ? tna : t1;
-        float mu = cell_majorant(g, m, s);
+        float mu = cell_majorant(g, m, s, g.region_majorants);
         if (mu > 0.0f) {
             for (;;) {
                 float tc = t + tau / mu;

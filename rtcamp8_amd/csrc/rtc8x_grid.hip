@@ -197,6 +197,49 @@ void cell_majorants(const std::vector<int32_t>& origins, const std::vector<float
             }
 }
 
+// Region majorants from the cell majorants (the layout rtc8x.h defines for rtc8x_grid): the least
+// power-of-two region_cells with at most RTC8X_MAX_REGIONS regions, a unit of (largest cell majorant) / 255
+// raised until 255 units cover it, and per region the least q whose q units cover its largest cell.
+void region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3], int32_t region_dims[3],
+                      uint32_t* region_cells, float* region_unit, std::vector<uint8_t>& out)
+{
+    uint32_t rc = 1;
+    for (;;) {
+        size_t n = 1;
+        for (int a = 0; a < 3; ++a) {
+            region_dims[a] = (cell_dims[a] + (int32_t)rc - 1) / (int32_t)rc;
+            n *= (size_t)region_dims[a];
+        }
+        if (n <= RTC8X_MAX_REGIONS) break;
+        rc *= 2;
+    }
+    std::vector<float> rmax((size_t)region_dims[0] * region_dims[1] * region_dims[2], 0.0f);
+    float top = 0.0f;
+    for (int32_t x = 0; x < cell_dims[0]; ++x)
+        for (int32_t y = 0; y < cell_dims[1]; ++y)
+            for (int32_t z = 0; z < cell_dims[2]; ++z) {
+                const float m = cells[((size_t)x * cell_dims[1] + y) * cell_dims[2] + z];
+                float& r = rmax[((size_t)(x / (int32_t)rc) * region_dims[1] + y / (int32_t)rc) * region_dims[2] +
+                                z / (int32_t)rc];
+                r = std::max(r, m);
+                top = std::max(top, m);
+            }
+    float unit = top / 255.0f;
+    while (255.0f * unit < top) unit = std::nextafter(unit, INFINITY);
+    // a multiple of 16 bytes: the tracer stages the table in LDS 16 bytes at a time
+    out.assign((rmax.size() + 15) & ~(size_t)15, 0);
+    for (size_t r = 0; r < rmax.size(); ++r) {
+        uint32_t q = 0;
+        if (rmax[r] > 0.0f) {
+            q = 1;
+            while (q < 255 && (float)q * unit < rmax[r]) ++q;
+        }
+        out[r] = (uint8_t)q;
+    }
+    *region_cells = rc;
+    *region_unit = unit;
+}
+
 template <typename T>
 rtc8x_status upload_vec(const std::vector<T>& v, T** dst)
 {
@@ -248,8 +291,8 @@ using namespace rtc8x;
 extern "C" rtc8x_status rtc8x_grid_destroy(rtc8x_grid* g)
 try {
     RTC8X_REQUIRE(g != nullptr, "rtc8x_grid_destroy: grid is NULL");
-    const void* ptrs[] = {g->root_tiles, g->upper_tables, g->lower_tables, g->leaf_values,
-                          g->tile_values, g->leaf_origins, g->cell_majorants, g->leaf_map};
+    const void* ptrs[] = {g->root_tiles, g->upper_tables, g->lower_tables, g->leaf_values,     g->tile_values,
+                          g->leaf_origins, g->cell_majorants, g->leaf_map,    g->region_majorants};
     for (const void* p : ptrs)
         if (p) RTC8X_HIP(hipFree(const_cast<void*>(p)));
     std::memset(g, 0, sizeof(*g));
@@ -363,6 +406,11 @@ try {
     std::vector<float> cm;
     int32_t corg[3], cdims[3];
     cell_majorants(origins, leaf_max, bbmin, bbmax, corg, cdims, cm);
+    std::vector<uint8_t> rm;
+    int32_t rdims[3];
+    uint32_t rcells = 0;
+    float runit = 0.0f;
+    region_majorants(cm, cdims, rdims, &rcells, &runit, rm);
 
     rtc8x_grid g;
     std::memset(&g, 0, sizeof(g));
@@ -376,7 +424,10 @@ try {
         g.world_bbox_max[a] = (float)(bbmax[a] + 1) / (float)n;
         g.cell_origin[a] = corg[a];
         g.cell_dims[a] = cdims[a];
+        g.region_dims[a] = rdims[a];
     }
+    g.region_cells = rcells;
+    g.region_unit = runit;
     g.max_value = vmax;
     g.background = 0.0f;
     g.num_root_tiles = 1;
@@ -399,6 +450,9 @@ try {
     g.tile_values = d_tiles;
     RTC8X_TRY(upload_vec(cm, &d_cm));
     g.cell_majorants = d_cm;
+    uint8_t* d_rm;
+    RTC8X_TRY(upload_vec(rm, &d_rm));
+    g.region_majorants = d_rm;
     RTC8X_HIP(hipMalloc(&d_values, ((size_t)num_leaves * 512 + 1) * sizeof(float)));
     RTC8X_HIP(hipMemsetAsync(d_values + (size_t)num_leaves * 512, 0, sizeof(float), s));  // lookup padding
     g.leaf_values = d_values;
@@ -477,6 +531,10 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
     rtc8x_grid g = *h;
     g.max_value = keep_max ? h->max_value : vmax;
     std::vector<float> cm;
+    std::vector<uint8_t> rm;
+    for (int a = 0; a < 3; ++a) g.region_dims[a] = 0;
+    g.region_cells = 0;
+    g.region_unit = 0.0f;
     if (h->num_leaves > 0) {
         int32_t corg[3], cdims[3];
         cell_majorants(origins, leaf_max, h->index_bbox_min, h->index_bbox_max, corg, cdims, cm);
@@ -484,6 +542,7 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
             g.cell_origin[a] = corg[a];
             g.cell_dims[a] = cdims[a];
         }
+        region_majorants(cm, cdims, g.region_dims, &g.region_cells, &g.region_unit, rm);
     }
     std::vector<rtc8x_root_tile> root(h->root_tiles, h->root_tiles + h->num_root_tiles);
     std::vector<uint32_t> upper(h->upper_tables, h->upper_tables + (size_t)h->num_upper * RTC8X_UPPER_TABLE_SIZE);
@@ -503,6 +562,9 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
     RTC8X_TRY(upload_vec(tiles, &d_tiles));
     RTC8X_TRY(upload_vec(origins, &d_origins));
     RTC8X_TRY(upload_vec(cm, &d_cm));
+    uint8_t* d_rm;
+    RTC8X_TRY(upload_vec(rm, &d_rm));
+    g.region_majorants = d_rm;
     g.root_tiles = d_root;
     g.upper_tables = d_upper;
     g.lower_tables = d_lower;
@@ -560,6 +622,12 @@ try {
     if (d->cell_majorants && ncells) {
         t.cells.resize(ncells);
         RTC8X_HIP(hipMemcpy(t.cells.data(), d->cell_majorants, ncells * 4, hipMemcpyDeviceToHost));
+    }
+    const size_t nregions =
+        (size_t)std::max(d->region_dims[0], 0) * std::max(d->region_dims[1], 0) * std::max(d->region_dims[2], 0);
+    if (d->region_majorants && nregions) {
+        t.regions.resize(nregions);
+        RTC8X_HIP(hipMemcpy(t.regions.data(), d->region_majorants, nregions, hipMemcpyDeviceToHost));
     }
     rtc8x_grid g = *d;
     g.leaf_map = nullptr;

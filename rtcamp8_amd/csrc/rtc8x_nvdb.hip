@@ -933,6 +933,12 @@ rtc8x_status adopt_host_grid(OwnedHostGrid&& t, const rtc8x_grid& g0, rtc8x_grid
     g.cell_majorants = o.cells.empty() ? nullptr : o.cells.data();
     if (o.cells.empty())
         for (int a = 0; a < 3; ++a) g.cell_dims[a] = 0;
+    g.region_majorants = o.regions.empty() ? nullptr : o.regions.data();
+    if (o.regions.empty()) {
+        for (int a = 0; a < 3; ++a) g.region_dims[a] = 0;
+        g.region_cells = 0;
+        g.region_unit = 0.0f;
+    }
     g.leaf_map = nullptr;
     g.leaf_masks = o.leaf_masks.empty() ? nullptr : o.leaf_masks.data();
     g.lower_masks = o.lower_masks.empty() ? nullptr : o.lower_masks.data();

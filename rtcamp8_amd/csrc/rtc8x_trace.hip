@@ -59,7 +59,7 @@ struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
         d = rd;
         return true;
     }
-    __device__ __forceinline__ bool advance(const rtc8x_grid&, const Medium& m, Pcg& rng, float& mu)
+    __device__ __forceinline__ bool advance(const rtc8x_grid&, const Medium& m, Pcg& rng, float& mu, const uint8_t*)
     {
         t += -rtc8x_logf(1.0f - rng.uni()) / m.majorant;
         if (t > t1) return false;
@@ -71,7 +71,8 @@ struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
     __device__ __forceinline__ void after_shadow(Pcg&, float sigma, float mu, float& vis) { vis *= 1.0f - sigma / mu; }
 };
 
-// Local majorants over 8^3 cells, the loops of track_local / ratio_local made resumable.
+// Local majorants over the grid's regions, the loops of track_local / ratio_local made resumable; the
+// region bytes come from `table` (a copy staged in LDS by the single-image kernel).
 template <>
 struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
     V3 o, d;
@@ -92,14 +93,15 @@ struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
     }
     // first call after init draws tau (as track_local does right after the DDA set-up)
     __device__ __forceinline__ void first_tau(Pcg& rng) { tau = -rtc8x_logf(1.0f - rng.uni()); }
-    __device__ __forceinline__ bool advance(const rtc8x_grid& g, const Medium& m, Pcg&, float& mu)
+    __device__ __forceinline__ bool advance(const rtc8x_grid& g, const Medium& m, Pcg&, float& mu,
+                                            const uint8_t* __restrict__ table)
     {
         for (;;) {
             if (!in_cell) {
                 axis = s.axis();
                 const float tna = s.tnext(axis);
                 tend = tna < t1 ? tna : t1;
-                mu_cell = cell_majorant(g, m, s);
+                mu_cell = cell_majorant(g, m, s, table);
                 in_cell = 1;
             }
             if (mu_cell > 0.0f) {
@@ -390,7 +392,8 @@ template <int ENTRY, int TRACKING>
 __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t launch_w, uint32_t launch_h,
                                            uint32_t num_samples, uint32_t slot,
                                            const uint32_t* __restrict__ record_offsets,
-                                           uint32_t* __restrict__ record_counts, const TrainStage& stage)
+                                           uint32_t* __restrict__ record_counts, const TrainStage& stage,
+                                           const uint8_t* __restrict__ region_table)
 {
     constexpr bool with_nrc = ENTRY != RTC8X_ENTRY_PATH_TRACE;
     constexpr bool gen_data = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA;
@@ -401,7 +404,10 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     // GENERATE_TRAINING_DATA is launched with a zero size and takes the training size from the device
     uint32_t W = launch_w, H = launch_h;
     if (gen_data && W == 0) train_dims(p, W, H);
-    const uint32_t tiles_x = (W + 7) / 8;
+    // pixel tiles of 64 lanes: 8x8, 16x4, 32x2 or 4x16
+    const uint32_t tile_w = p.tile_shape == 1 ? 16u : (p.tile_shape == 2 ? 32u : (p.tile_shape == 3 ? 4u : 8u));
+    const uint32_t tile_h = 64u / tile_w;
+    const uint32_t tiles_x = (W + tile_w - 1) / tile_w;
     // GENERATE_TRAINING_DATA hands out the unbiased-suffix pixels first (the one pixel of each 4x4 block
     // with ((ly % 4) * 4 + lx % 4 + offset) % 16 == 0): their paths run to full length, so starting
     // them first keeps the launch from waiting on them at the end.
@@ -410,7 +416,9 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     const uint32_t first_nx = W > first_rx ? (W - first_rx + 3u) / 4u : 0u;
     const uint32_t first_ny = H > first_ry ? (H - first_ry + 3u) / 4u : 0u;
     const uint32_t n_first = (gen_data && !p.work_order) ? first_nx * first_ny : 0u;
-    const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + 7) / 8) * 64;
+    const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + tile_h - 1) / tile_h) * 64;
+    const uint32_t ev8 = (p.event_eighths >= 1 && p.event_eighths <= 8) ? p.event_eighths
+                                                                        : (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC ? 4u : 3u);
     const uint32_t lane_id = threadIdx.x & 63;
     // per-sample slices of the render buffers (one launch may trace several samples into the cache)
     const size_t q_stride = ((size_t)W * H + 127u) & ~(size_t)127u, p_stride = (size_t)W * H;
@@ -437,19 +445,19 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
         // many lanes instead of once per lane), while the others keep stepping through lookups.  The
         // plain and training entries then run only the larger of the two vertex events (a real
         // collision: Russian roulette and NEE set-up; an NEE segment's end: phase sampling and the next
-        // segment) together with the path starts and ends, from 3/8 of the active lanes; the other
-        // event's lanes wait for a later pass.  The NRC render entry, whose paths are short, runs every
-        // parked lane from half of the active lanes.
+        // segment) together with the path starts and ends, from event_eighths / 8 of the active lanes
+        // (3/8 by default); the other event's lanes wait for a later pass.  The NRC render entry, whose
+        // paths are short, runs every parked lane from event_eighths / 8 (half by default).
         const uint64_t parked = __ballot(L.mode != kModeTrack && L.mode != kModeShadow && L.mode != kModeIdle);
         const uint64_t stepping = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
         if (parked == 0 && stepping == 0) break;
         const uint32_t n_parked = __popcll(parked), n_active = __popcll(parked | stepping);
         bool run_events;
         if constexpr (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC) {
-            run_events = stepping == 0 || 2 * n_parked >= n_active;
+            run_events = stepping == 0 || 8 * n_parked >= ev8 * n_active;
         } else {
             run_events = stepping == 0;
-            if (!run_events && 8 * n_parked >= 3 * n_active) {
+            if (!run_events && 8 * n_parked >= ev8 * n_active) {
                 const uint64_t at_collision = __ballot(L.mode == kModeScatter), at_nee_end = __ballot(L.mode == kModeNeeDone);
                 const uint64_t others = parked & ~(at_collision | at_nee_end);
                 const uint64_t sel =
@@ -482,8 +490,8 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
                 } else {
                     const uint32_t wt = gen_data ? w - n_first : w;
                     const uint32_t tile = wt >> 6, within = wt & 63u;
-                    L.lx = (tile % tiles_x) * 8 + (within & 7u);
-                    L.ly = (tile / tiles_x) * 8 + (within >> 3);
+                    L.lx = (tile % tiles_x) * tile_w + within % tile_w;
+                    L.ly = (tile / tiles_x) * tile_h + within / tile_w;
                     if (L.lx >= W || L.ly >= H) continue;
                     if (gen_data && (L.lx & 3u) == first_rx && (L.ly & 3u) == first_ry) continue;  // taken first
                 }
@@ -841,7 +849,7 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
             if (L.mode == kModeTrack || L.mode == kModeShadow)
                 speculate_global<kSpeculate>(p.grid, med, sg, L, L.mode == kModeTrack);
         } else if (L.mode == kModeTrack || L.mode == kModeShadow) {
-            if (sg.advance(p.grid, med, L.rng, mu)) {
+            if (sg.advance(p.grid, med, L.rng, mu, region_table)) {
                 const float sigma = sample_density(p.grid, med, sg.o, sg.d, sg.t);
                 if (L.mode == kModeTrack) {
                     if (sg.after_track(L.rng, sigma, mu)) L.mode = kModeScatter;
@@ -888,7 +896,20 @@ __global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params
                                                         const uint32_t* __restrict__ record_offsets,
                                                         uint32_t* __restrict__ record_counts, const TrainStage stage)
 {
-    trace_body<ENTRY, TRACKING>(p, launch_w, launch_h, num_samples, slot, record_offsets, record_counts, stage);
+    if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) {
+        // the region majorants staged in LDS: every DDA step reads them there
+        __shared__ __align__(16) uint8_t regions[RTC8X_MAX_REGIONS];
+        const uint32_t n16 = ((uint32_t)p.grid.region_dims[0] * (uint32_t)p.grid.region_dims[1] *
+                                  (uint32_t)p.grid.region_dims[2] + 15u) / 16u;
+        const uint4* src = reinterpret_cast<const uint4*>(p.grid.region_majorants);
+        for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) reinterpret_cast<uint4*>(regions)[i] = src[i];
+        __syncthreads();
+        trace_body<ENTRY, TRACKING>(p, launch_w, launch_h, num_samples, slot, record_offsets, record_counts, stage,
+                                    regions);
+    } else {
+        trace_body<ENTRY, TRACKING>(p, launch_w, launch_h, num_samples, slot, record_offsets, record_counts, stage,
+                                    nullptr);
+    }
 }
 
 // Several images (instances: their own volume, camera, sky, buffers and, with the NRC entry, cache
@@ -914,7 +935,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         const uint32_t inst = first + k < b.count ? first + k : first + k - b.count;
         const rtc8x_frame_params* __restrict__ pp = b.params + inst;
         trace_body<ENTRY, TRACKING>(*pp, pp->image_width, pp->image_height, b.num_samples[inst], b.slot + inst,
-                                    nullptr, nullptr, TrainStage{});
+                                    nullptr, nullptr, TrainStage{}, pp->grid.region_majorants);
     }
 }
 
@@ -1268,8 +1289,17 @@ rtc8x_status check_launch(const rtc8x_frame_params& p, int32_t entry, uint32_t n
         RTC8X_REQUIRE((double)majorant * diag < 1.0e6, "rtc8x_trace: majorant x bbox diagonal = %g is too large",
                       (double)majorant * diag);
         if (p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT)
-            RTC8X_REQUIRE(g.cell_majorants && g.cell_dims[0] > 0 && g.cell_dims[1] > 0 && g.cell_dims[2] > 0,
-                          "rtc8x_trace: local-majorant tracking needs cell majorants");
+            RTC8X_REQUIRE(g.region_majorants && g.region_dims[0] > 0 && g.region_dims[1] > 0 && g.region_dims[2] > 0 &&
+                              g.region_cells >= 1 && (g.region_cells & (g.region_cells - 1)) == 0 &&
+                              std::isfinite(g.region_unit),
+                          "rtc8x_trace: local-majorant tracking needs region majorants");
+    }
+    if (p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT) {
+        // the single-image kernel stages the whole region table in LDS
+        const int64_t nr = (int64_t)std::max(g.region_dims[0], 0) * std::max(g.region_dims[1], 0) *
+                           std::max(g.region_dims[2], 0);
+        RTC8X_REQUIRE(nr <= RTC8X_MAX_REGIONS && (nr == 0 || g.region_majorants),
+                      "rtc8x_trace: %lld regions (at most %d, with their table)", (long long)nr, RTC8X_MAX_REGIONS);
     }
     if (p.env.kind == RTC8X_ENV_LATLONG) {
         RTC8X_REQUIRE(p.env.texels && p.env.width > 0 && p.env.height > 0, "rtc8x_trace: env texture missing");

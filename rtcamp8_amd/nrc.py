@@ -54,7 +54,7 @@ class NeuralRadianceCache:
     def initialize(self, num_hidden_layers: int = 2, learning_rate: float = 1e-2, precision: int = _abi.NRC_FP16,
                    seed: int = 0x9E3779B97F4A7C15, ema_decay: float = 0.99,
                    encoding: int = _abi.NRC_HASH_GRID) -> None:
-        """initialize(posEnc, numHiddenLayers, learningRate) (network_interface.cu:48-134); Adam epsilon
+        """initialize(posEnc, numHiddenLayers, learningRate) (network_interface.cu:50-134); Adam epsilon
         1e-15 with the hash grid and 1e-8 with triangle waves, as the reference configures them."""
         eps = 1e-15 if encoding == _abi.NRC_HASH_GRID else 1e-8
         d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, eps, 1e-6, ema_decay, precision, seed, encoding, 0)
@@ -208,6 +208,7 @@ class NrcRenderer:
         self.stats = NrcStats()
         self.samples = 0
         self.frame = 0
+        self.knobs = (0, 0)  # (event_eighths, tile_shape) of the trace kernel's schedule; 0 = defaults
         call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(cfg.rng_seed))
         self.accum.fill(0)
 
@@ -227,6 +228,7 @@ class NrcRenderer:
         p.nrc.nrc_buffer_index = self.buf_idx
         p.nrc.radiance_scale = self.radiance_scale
         p.nrc.path_termination_factor = 0.01
+        p.event_eighths, p.tile_shape = self.knobs
         return p
 
     def train_iteration(self, is_new_sequence: bool, want_loss: bool = False):

@@ -30,6 +30,7 @@ class PathTracer:
         self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n)) if schedule else None
         self.have_cost = False
         self.samples = 0
+        self.knobs = (0, 0)  # (event_eighths, tile_shape) of the trace kernel's schedule; 0 = defaults
         self.reset()
 
     def reset(self, seed: int | None = None) -> None:
@@ -48,6 +49,7 @@ class PathTracer:
         p.accum = self.accum.ptr
         p.path_vertices = self.vertices.ptr if self.vertices is not None else None
         p.stats = self.stats.ptr
+        p.event_eighths, p.tile_shape = self.knobs
         return p
 
     def _schedule(self, p: _abi.FrameParams) -> None:

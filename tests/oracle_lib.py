@@ -117,10 +117,12 @@ class HostGrid:
         self.arrays = arrays
 
 
-def oracle_build_grid(fbm) -> HostGrid:
-    """The oracle's own fBm grid build (independent of the library's device build)."""
+def oracle_build_grid(fbm, lib=None) -> HostGrid:
+    """The oracle's own fBm grid build (independent of the library's device build); `lib`: another build of
+    the oracle to use."""
+    lib = lib or oracle()
     g = _abi.Grid()
-    rc = oracle().oracle_grid_build_fbm(C.byref(fbm.desc()), C.byref(g))
+    rc = lib.oracle_grid_build_fbm(C.byref(fbm.desc()), C.byref(g))
     assert rc == 0, f"oracle grid build failed: {rc}"
     arrays = {
         "root": np.ctypeslib.as_array((_abi.RootTile * g.num_root_tiles).from_address(g.root_tiles)).copy(),
@@ -130,10 +132,11 @@ def oracle_build_grid(fbm) -> HostGrid:
         "tiles": np.ctypeslib.as_array((C.c_float * 1).from_address(g.tile_values)).copy(),
         "origins": np.ctypeslib.as_array((C.c_int32 * (3 * g.num_leaves)).from_address(g.leaf_origins)).copy(),
         "cells": np.ctypeslib.as_array((C.c_float * int(np.prod(g.cell_dims[:]))).from_address(g.cell_majorants)).copy(),
+        "regions": np.ctypeslib.as_array((C.c_uint8 * int(np.prod(g.region_dims[:]))).from_address(g.region_majorants)).copy(),
     }
     template = _abi.Grid()
     C.memmove(C.byref(template), C.byref(g), C.sizeof(_abi.Grid))
-    oracle().oracle_grid_free(C.byref(g))
+    lib.oracle_grid_free(C.byref(g))
     return host_grid_from_arrays(template, arrays)
 
 
@@ -147,6 +150,7 @@ def host_grid_from_arrays(template: _abi.Grid, arrays: dict) -> HostGrid:
     g.tile_values = arrays["tiles"].ctypes.data
     g.leaf_origins = arrays["origins"].ctypes.data if arrays.get("origins") is not None else None
     g.cell_majorants = arrays["cells"].ctypes.data if arrays.get("cells") is not None else None
+    g.region_majorants = arrays["regions"].ctypes.data if arrays.get("regions") is not None else None
     return HostGrid(g, arrays)
 
 
@@ -165,6 +169,7 @@ def download_grid(dg: _abi.Grid) -> HostGrid:
         "tiles": download_ptr(dg.tile_values, np.float32, 1),
         "origins": download_ptr(dg.leaf_origins, np.int32, 3 * dg.num_leaves) if dg.leaf_origins else None,
         "cells": download_ptr(dg.cell_majorants, np.float32, int(np.prod(dg.cell_dims[:]))) if dg.cell_majorants else None,
+        "regions": download_ptr(dg.region_majorants, np.uint8, int(np.prod(dg.region_dims[:]))) if dg.region_majorants else None,
     }
     return host_grid_from_arrays(dg, arrays)
 

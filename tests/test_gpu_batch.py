@@ -124,3 +124,41 @@ def test_batch_rejects_bad_input():
         batch._launch([p, bad], [1, 1], _abi.ENTRY_PATH_TRACE, None, None)
     t.close()
     s.close()
+
+
+@pytest.mark.parametrize("entry", ["plain", "nrc"])
+def test_scheduling_knobs_do_not_change_results(entry):
+    """Event threshold and tile shape change the schedule only: accumulators, PCG states, statistics and NRC
+    trace outputs are those of the default schedule, bit for bit."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.nrc import NrcRenderer
+    from rtcamp8_amd.scene import DeviceScene
+    from rtcamp8_amd.tracer import PathTracer
+
+    cfg = _scenes()[1]
+    s = DeviceScene(cfg)
+    n = cfg.width * cfg.height
+    outs = []
+    for knobs in ((0, 0), (2, 1), (5, 3), (8, 2)):
+        if entry == "plain":
+            t = PathTracer(s, schedule=False)
+            t.knobs = knobs
+            t.trace(3)
+            outs.append((t.accum.download(np.float64, 3 * n), t.rng.download(np.uint64, n), t.stats.download(np.uint64, 4)))
+            t.close()
+        else:
+            r = NrcRenderer(s, num_hidden_layers=2, samples_per_launch=2)
+            r.knobs = knobs
+            p = r.params()
+            p.nrc.num_samples = 2
+            call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, 2)
+            b = r.buffers
+            outs.append(tuple(getattr(b, k).download(np.uint32, getattr(b, k).nbytes // 4)
+                              for k in ("inference_queries", "inference_terminals", "per_frame_contribution")) +
+                        (r.rng.download(np.uint64, n),))
+            r.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    s.close()

@@ -7,8 +7,9 @@
   the plain tracer at equal samples against the FP64 oracle, on a reduced image.  The numbers are
   printed and pinned (what the trained cache gains, and the bias it costs), not hidden.
 * Eight uneven instances (64^3 to 1024^3 fBm, sigma_t max 5 to 200) in one persistent launch with
-  autotuned shares, each with its own NRC in BF16 training and E4M3 inference, at reduced images: every
-  instance's trace outputs equal its own launch's, and those single launches equal the FP32 oracle.
+  autotuned scheduling knobs and shares, each with its own NRC in BF16 training and E4M3 inference, at
+  reduced images: every instance's trace outputs equal its own launch's, and those single launches equal
+  the FP32 oracle.
 """
 import ctypes as C
 import time
@@ -145,9 +146,13 @@ def test_eight_instances_fp8_inference_autotuned_against_single_launches_and_the
         def outputs(r):
             return [getattr(r.buffers, k).download(np.uint32, getattr(r.buffers, k).nbytes // 4) for k in keys]
 
-        shares = batch.autotune_shares(
-            lambda i: call("rtc8x_trace", None, C.byref(params(rs[i])), _abi.ENTRY_PATH_TRACE_WITH_NRC, 1), len(rs))
+        shares, knobs = batch.autotune(
+            lambda i, ev, shape: call("rtc8x_trace", None, C.byref(batch.apply_knobs(params(rs[i]), (ev, shape))),
+                                      _abi.ENTRY_PATH_TRACE_WITH_NRC, 1), len(rs), repeats=1)
         assert len(shares) == 8 and all(s > 0 for s in shares)
+        print(f"[configs] autotuned (event eighths, tile shape) per instance: {knobs}", flush=True)
+        for r, k in zip(rs, knobs):
+            r.knobs = k
         rng0 = [r.rng.download(np.uint64, r.width * r.height) for r in rs]
         single, before = [], []
         for r, g in zip(rs, rng0):  # each instance's own launch, on cleared output buffers

@@ -33,9 +33,12 @@ def test_grid_build_matches_oracle(res, sparse):
         ref = oracle_build_grid(fbm)
         for f in ("num_lower", "num_leaves", "max_value"):
             assert getattr(got.grid, f) == getattr(ref.grid, f), f
-        for f in ("world_bbox_min", "world_bbox_max", "index_bbox_min", "index_bbox_max", "cell_origin", "cell_dims"):
+        for f in ("world_bbox_min", "world_bbox_max", "index_bbox_min", "index_bbox_max", "cell_origin", "cell_dims",
+                  "region_dims"):
             assert list(getattr(got.grid, f)) == list(getattr(ref.grid, f)), f
-        for k in ("upper", "lower", "values", "origins", "cells"):
+        for f in ("region_cells", "region_unit"):
+            assert getattr(got.grid, f) == getattr(ref.grid, f), f
+        for k in ("upper", "lower", "values", "origins", "cells", "regions"):
             assert np.array_equal(got.arrays[k], ref.arrays[k]), k
         if sparse:
             frac = got.grid.num_leaves / (res // 8) ** 3

@@ -42,22 +42,52 @@ Z = 4.5  # standard errors allowed for a statistical check (two-sided p ~ 7e-6 e
 
 
 # ------------------------------------------------------------------ media
+def with_majorants(hg):
+    """The host grid with the oracle's own cell and region majorants (local-majorant tracking)."""
+    oracle().oracle_compute_cell_majorants(C.byref(hg.grid))
+    g = hg.grid
+    hg.arrays["cells"] = np.ctypeslib.as_array((C.c_float * int(np.prod(g.cell_dims[:]))).from_address(g.cell_majorants)).copy()
+    hg.arrays["regions"] = np.ctypeslib.as_array((C.c_uint8 * int(np.prod(g.region_dims[:]))).from_address(g.region_majorants)).copy()
+    g.cell_majorants = hg.arrays["cells"].ctypes.data
+    g.region_majorants = hg.arrays["regions"].ctypes.data
+    return hg
+
+
 def homogeneous_medium():
     """Index space = world space (voxel size 1); 3 x 3 x 3 leaves of density 1 cover [0, 24)^3, except the
     voxel (23, 23, 23) = 4, which sets the grid maximum and with it the majorant to 4 x the density.  The
-    world box [4, 20]^3 is the part of the medium where every trilinear tap reads 1."""
+    world box [4, 20]^3 is the part of the medium where every trilinear tap reads 1.  The local majorants
+    are 4 in the 8^3 cells whose 3 x 3 x 3 neighbourhood of leaves holds that voxel (every index >= 1) and
+    1 elsewhere, quantised up to q / 255 of 4."""
     origins = np.array([(x, y, z) for x in (0, 8, 16) for y in (0, 8, 16) for z in (0, 8, 16)], np.int32)
     values = np.ones((27, 512), np.float32)
     values[-1, 511] = 4.0  # leaf (16, 16, 16), voxel (7, 7, 7)
-    return tables_from_leaves(origins, values, 1.0, (4.0, 4.0, 4.0, 20.0, 20.0, 20.0))
+    return with_majorants(tables_from_leaves(origins, values, 1.0, (4.0, 4.0, 4.0, 20.0, 20.0, 20.0)))
 
 
-def frame(grid: _abi.Grid, sigma: float) -> _abi.FrameParams:
+def frame(grid: _abi.Grid, sigma: float, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT) -> _abi.FrameParams:
     p = _abi.FrameParams()
     p.grid = grid
     p.density_coeff = sigma
-    p.tracking = _abi.TRACKING_GLOBAL_MAJORANT
+    p.tracking = tracking
     return p
+
+
+def homogeneous_majorants(o, local: bool):
+    """Per ray (along +x from x = 4 to 20): the majorant on [4, 8), [8, 16), [16, 20] (the three cells)."""
+    if not local:
+        return np.full((o.shape[0], 3), MU)
+    unit = np.float32(4.0) / np.float32(255.0)
+    while np.float32(255.0) * unit < np.float32(4.0):
+        unit = np.nextafter(unit, np.float32(np.inf))
+    q1 = 1
+    while np.float32(q1) * unit < np.float32(1.0):
+        q1 += 1
+    lo, hi = float(np.float32(q1) * unit), float(np.float32(255) * unit)
+    inner = (o[:, 1] >= 8) & (o[:, 2] >= 8)  # cells y, z >= 1; the x cells 1, 2 then see the 4
+    m = np.full((o.shape[0], 3), lo)
+    m[inner, 1:] = hi
+    return SIGMA * m
 
 
 def track(lib, p, transmittance: bool, o, d, f64: bool, seed: int, tmax: float = 1e30):
@@ -84,9 +114,9 @@ def axis_rays(n: int, seed: int):
 SIGMA, CHORD, MU = 0.1, 16.0, 0.4
 
 
-def check_free_flight_homogeneous(lib, f64: bool):
+def check_free_flight_homogeneous(lib, f64: bool, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT):
     hg = homogeneous_medium()
-    p = frame(hg.grid, SIGMA)
+    p = frame(hg.grid, SIGMA, tracking)
     n = 100_000
     o, d = axis_rays(n, 11)
     t, lookups = track(lib, p, False, o, d, f64, seed=12)
@@ -102,21 +132,30 @@ def check_free_flight_homogeneous(lib, f64: bool):
     expect = n * np.diff(1.0 - np.exp(-SIGMA * edges))
     chi2 = float(((counts - expect) ** 2 / expect).sum())
     assert chi2 < 55.0, chi2  # 16 degrees of freedom; P(chi2 > 55) ~ 5e-6
-    # tentative collisions form the majorant's Poisson process: E[lookups] = mu E[min(X, d)], X ~ Exp(sigma)
-    mean_lookups = MU * (1.0 - p_esc) / SIGMA
+    # tentative collisions form the majorant's Poisson process: E[lookups] = int_0^d mu(s) e^(-sigma s) ds
+    # (global: mu E[min(X, d)], X ~ Exp(sigma); local: piecewise over the three cells)
+    mus = homogeneous_majorants(o, tracking == _abi.TRACKING_LOCAL_MAJORANT)
+    seg = np.array([[0.0, 4.0], [4.0, 12.0], [12.0, 16.0]])
+    mean_lookups = float((mus * (np.exp(-SIGMA * seg[:, 0]) - np.exp(-SIGMA * seg[:, 1]))[None] / SIGMA).sum(1).mean())
     assert abs(lookups / n / mean_lookups - 1.0) < 0.01, (lookups / n, mean_lookups)
 
 
-def check_ratio_tracking_homogeneous(lib, f64: bool):
+def check_ratio_tracking_homogeneous(lib, f64: bool, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT):
     hg = homogeneous_medium()
-    p = frame(hg.grid, SIGMA)
+    p = frame(hg.grid, SIGMA, tracking)
     n = 100_000
     o, d = axis_rays(n, 21)
     T, lookups = track(lib, p, True, o, d, f64, seed=22)
-    m1, m2 = math.exp(-SIGMA * CHORD), math.exp(MU * CHORD * ((1 - SIGMA / MU) ** 2 - 1))
+    # T = prod over the Poisson(mu) tentative collisions of (1 - sigma / mu): E[T] = e^(-sigma d) and
+    # E[T^2] = exp(int mu ((1 - sigma / mu)^2 - 1)), per cell for local majorants
+    mus = homogeneous_majorants(o, tracking == _abi.TRACKING_LOCAL_MAJORANT)
+    length = np.array([4.0, 8.0, 4.0])
+    m1 = math.exp(-SIGMA * CHORD)
+    m2 = float(np.exp((mus * ((1 - SIGMA / mus) ** 2 - 1) * length[None]).sum(1)).mean())
     assert abs(T.mean() - m1) < Z * T.std() / math.sqrt(n), (T.mean(), m1)
     assert abs((T * T).mean() - m2) < Z * (T * T).std() / math.sqrt(n), ((T * T).mean(), m2)
-    assert abs(lookups / n / (MU * CHORD) - 1.0) < 0.01  # K ~ Poisson(mu d)
+    want = float((mus * length[None]).sum(1).mean())  # K ~ Poisson(int mu)
+    assert abs(lookups / n / want - 1.0) < 0.01, (lookups / n, want)
 
 
 def brute_force_optical_depth(hg, sigma: float, o, d, t0: float, t1: float, m: int = 20_000) -> float:
@@ -134,7 +173,7 @@ def clip_unit_cube(o, d):
     return max(0.0, float(np.minimum(a, b).max())), float(np.maximum(a, b).min())
 
 
-def check_tracking_against_brute_force_optical_depth(lib, f64: bool):
+def check_tracking_against_brute_force_optical_depth(lib, f64: bool, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT):
     hg = oracle_build_grid(FbmConfig(resolution=16, seed=0x0123))
     sigma = 8.0 / max(hg.grid.max_value, 1e-6)  # majorant 8 per unit length
     r = np.random.default_rng(31)
@@ -150,9 +189,9 @@ def check_tracking_against_brute_force_optical_depth(lib, f64: bool):
         tmid = 0.5 * (t0 + t1)
         tau_mid = brute_force_optical_depth(hg, sigma, o, d, t0, tmid)
         O, D = np.tile(o, (n, 1)), np.tile(d, (n, 1))
-        T, _ = track(lib, frame(hg.grid, sigma), True, O, D, f64, seed=100 + ray)
+        T, _ = track(lib, frame(hg.grid, sigma, tracking), True, O, D, f64, seed=100 + ray)
         assert abs(T.mean() - math.exp(-tau)) < Z * T.std() / math.sqrt(n) + 1e-4, (ray, T.mean(), math.exp(-tau))
-        t, _ = track(lib, frame(hg.grid, sigma), False, O, D, f64, seed=200 + ray)
+        t, _ = track(lib, frame(hg.grid, sigma, tracking), False, O, D, f64, seed=200 + ray)
         for frac, want in (((t < 0).mean(), math.exp(-tau)), (((t >= 0) & (t < tmid)).mean(), 1 - math.exp(-tau_mid))):
             se = math.sqrt(max(want * (1 - want), 1e-12) / n)
             assert abs(frac - want) < Z * se + 1e-4, (ray, frac, want, tau)
@@ -363,20 +402,57 @@ def check_importance_map_definition(lib, w: int, h: int):
     assert abs(integral[0] / mip0.sum() - 1.0) < 1e-5
 
 
+def check_region_majorants(lib, res: int, sparse: bool):
+    """The region table of rtc8x.h's definition: regions of the least power-of-two size that keeps their
+    count <= 32768, each byte the least q with q x unit >= the largest cell majorant of the region."""
+    hg = oracle_build_grid(FbmConfig(resolution=res, seed=0x0456, sparse=sparse), lib)
+    g = hg.grid
+    cd = np.array(g.cell_dims[:])
+    cells = hg.arrays["cells"].reshape(cd)
+    rc = 1
+    while np.prod(-(-cd // rc)) > _abi.MAX_REGIONS:
+        rc *= 2
+    assert g.region_cells == rc and list(g.region_dims) == list(-(-cd // rc))
+    rd = -(-cd // rc)
+    pad = np.zeros(rd * rc, np.float32)
+    pad[:cd[0], :cd[1], :cd[2]] = cells
+    rmax = pad.reshape(rd[0], rc, rd[1], rc, rd[2], rc).max(axis=(1, 3, 5))
+    unit = np.float32(g.region_unit)
+    assert np.float32(255) * unit >= np.float32(cells.max()) and unit <= np.float32(cells.max()) / 255 * (1 + 1e-6)
+    q = hg.arrays["regions"].reshape(rd).astype(np.int64)
+    covered = q.astype(np.float32) * unit >= rmax
+    assert covered.all()
+    assert np.array_equal(q == 0, rmax == 0)
+    below = (q - 1).astype(np.float32) * unit < rmax  # q is the least such value
+    assert below[q > 0].all()
+
+
 # ------------------------------------------------------------------ the tests
+LOCAL = _abi.TRACKING_LOCAL_MAJORANT
+TRACKINGS = [_abi.TRACKING_GLOBAL_MAJORANT, LOCAL]
+
+
+@pytest.mark.parametrize("tracking", TRACKINGS)
 @pytest.mark.parametrize("f64", [False, True])
-def test_free_flight_in_a_homogeneous_medium(f64):
-    check_free_flight_homogeneous(oracle(), f64)
+def test_free_flight_in_a_homogeneous_medium(f64, tracking):
+    check_free_flight_homogeneous(oracle(), f64, tracking)
 
 
+@pytest.mark.parametrize("tracking", TRACKINGS)
 @pytest.mark.parametrize("f64", [False, True])
-def test_ratio_tracking_in_a_homogeneous_medium(f64):
-    check_ratio_tracking_homogeneous(oracle(), f64)
+def test_ratio_tracking_in_a_homogeneous_medium(f64, tracking):
+    check_ratio_tracking_homogeneous(oracle(), f64, tracking)
 
 
+@pytest.mark.parametrize("tracking", TRACKINGS)
 @pytest.mark.parametrize("f64", [False, True])
-def test_tracking_against_the_brute_force_optical_depth(f64):
-    check_tracking_against_brute_force_optical_depth(oracle(), f64)
+def test_tracking_against_the_brute_force_optical_depth(f64, tracking):
+    check_tracking_against_brute_force_optical_depth(oracle(), f64, tracking)
+
+
+@pytest.mark.parametrize("res,sparse", [(64, False), (128, True), (256, True)])
+def test_region_majorants_match_their_definition(res, sparse):
+    check_region_majorants(oracle(), res, sparse)
 
 
 @pytest.mark.parametrize("f64", [False, True])
@@ -414,6 +490,14 @@ MUTATIONS = [
      lambda lib: check_ratio_tracking_homogeneous(lib, False)),
     ("rtc8_oracle_tracer.inc", "R vy = v000 + wz * (v001 - v000);", "R vy = v001 + wz * (v000 - v001);",
      lambda lib: check_tracking_against_brute_force_optical_depth(lib, False)),
+    ("rtc8_oracle_tracer.inc", "tau = tau - mu * (tend - t);", "tau = tau - mu * (tend - t) * (R)0.5;",
+     lambda lib: check_free_flight_homogeneous(lib, False, LOCAL)),
+    ("rtc8_oracle_tracer.inc", "s->tnext[a] = s->step[a] != 0 ? SUF(dda_boundary)(s, a) : (R)INFINITY;",
+     "s->tnext[a] = s->step[a] != 0 ? SUF(dda_boundary)(s, a) + (R)1 : (R)INFINITY;",
+     lambda lib: check_ratio_tracking_homogeneous(lib, False, LOCAL)),
+    ("rtc8_oracle_core.c", "size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + y / (int32_t)rc) * rd[2] + z / (int32_t)rc;",
+     "size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + z / (int32_t)rc) * rd[2] + y / (int32_t)rc;",
+     lambda lib: check_region_majorants(lib, 128, True)),
     ("rtc8_oracle_tracer.inc", "*dir_local = SUF(v3)(cp * sin_t, sp * sin_t, -cos_t);",
      "*dir_local = SUF(v3)(cp * sin_t, sp * sin_t, cos_t);",
      lambda lib: check_phase_sampling(lib, False, _abi.PHASE_HENYEY_GREENSTEIN, 0.85)),
@@ -443,8 +527,8 @@ def build_mutant(tmp: Path, i: int) -> Path:
     shutil.copytree(ROOT / "include", src.parent / "include")
     f = src / fname
     body = f.read_text()
-    assert body.count(text) == 1, (fname, text)
-    f.write_text(body.replace(text, mistake))
+    assert body.count(text) >= 1, (fname, text)
+    f.write_text(body.replace(text, mistake, 1))  # the first occurrence (free flight before ratio tracking)
     subprocess.run(["make", "-s", "-C", str(src)], check=True, capture_output=True)
     return src / "liboracle.so"
 

@@ -171,10 +171,13 @@ typedef struct rtc8x_grid {
      * RTC8X_TRACKING_LOCAL_MAJORANT, which walks regions, not cells): the cell grid is grouped into regions
      * of region_cells^3 cells from cell_origin (region_cells a power of two, the least >= 1 that keeps
      * region_dims[0] * region_dims[1] * region_dims[2] <= RTC8X_MAX_REGIONS, so the tracer can stage the
-     * table in LDS).  One byte q per region, x-major: the region's majorant is (float)q * region_unit,
-     * q the least value >= the region's largest cell majorant (q = 0: every cell of the region is 0).
-     * region_unit is the largest cell majorant / 255, raised to the next float until 255 * region_unit
-     * covers it. */
+     * table in LDS).  One byte per region, x-major: for a region whose largest cell majorant m is > 0, the
+     * least q in 1..127 with (float)q * region_unit >= m, the region's majorant being (float)q *
+     * region_unit; for an empty region, RTC8X_REGION_EMPTY + k, k (<= 127) the number of region layers
+     * around it that are empty too (its Chebyshev distance to the nearest region with density, minus one;
+     * space outside the region grid counts as empty), which lets the tracker skip them at once.
+     * region_unit is the largest cell majorant / 127, raised to the next float until 127 * region_unit
+     * covers it.  The table's allocation is padded with zeros to a multiple of 16 bytes. */
     int32_t region_dims[3];
     uint32_t region_cells;
     float region_unit;
@@ -182,6 +185,7 @@ typedef struct rtc8x_grid {
 } rtc8x_grid;
 
 #define RTC8X_MAX_REGIONS 32768
+#define RTC8X_REGION_EMPTY 128u
 
 /* Procedural fBm volume description (the benchmark's synthetic input; the reference loads a .nvdb). */
 typedef struct rtc8x_fbm_desc {

@@ -481,10 +481,12 @@ void oracle_compute_cell_majorants(rtc8x_grid* g)
     oracle_compute_region_majorants(g);
 }
 
-/* Region majorants over groups of region_cells^3 cells (the grid's definition in rtc8x.h): the least
+/* Region table over groups of region_cells^3 cells (the grid's definition in rtc8x.h): the least
  * power-of-two region_cells >= 1 with at most RTC8X_MAX_REGIONS regions; region_unit = the largest cell
- * majorant / 255, raised float by float until 255 * region_unit covers it; per region the least q with
- * q * region_unit >= the region's largest cell majorant. */
+ * majorant / 127, raised float by float until 127 * region_unit covers it; per region with density the
+ * least q in 1..127 with q * region_unit >= its largest cell majorant; per empty region
+ * RTC8X_REGION_EMPTY + (its Chebyshev distance to the nearest region with density) - 1, at most 127, found
+ * here by a brute-force search over the regions with density. */
 void oracle_compute_region_majorants(rtc8x_grid* g)
 {
     free((void*)g->region_majorants);
@@ -516,17 +518,42 @@ void oracle_compute_region_majorants(rtc8x_grid* g)
                 if (m > rmax[r]) rmax[r] = m;
                 if (m > top) top = m;
             }
-    float unit = top / 255.0f;
-    while (255.0f * unit < top) unit = nextafterf(unit, INFINITY);
-    uint8_t* q = (uint8_t*)malloc(nr);
-    for (size_t r = 0; r < nr; ++r) {
-        uint32_t v = 0;
-        if (rmax[r] > 0.0f) {
-            v = 1;
-            while (v < 255 && (float)v * unit < rmax[r]) ++v;
-        }
-        q[r] = (uint8_t)v;
-    }
+    float unit = top / 127.0f;
+    while (127.0f * unit < top) unit = nextafterf(unit, INFINITY);
+    /* the regions with density, for the distance search */
+    size_t nd = 0;
+    int32_t* dense = (int32_t*)malloc(sizeof(int32_t) * 3 * (nr ? nr : 1));
+    for (int32_t x = 0; x < rd[0]; ++x)
+        for (int32_t y = 0; y < rd[1]; ++y)
+            for (int32_t z = 0; z < rd[2]; ++z)
+                if (rmax[((size_t)x * rd[1] + y) * rd[2] + z] > 0.0f) {
+                    dense[3 * nd] = x;
+                    dense[3 * nd + 1] = y;
+                    dense[3 * nd + 2] = z;
+                    ++nd;
+                }
+    uint8_t* q = (uint8_t*)calloc((nr + 15) & ~(size_t)15, 1);
+    for (int32_t x = 0; x < rd[0]; ++x)
+        for (int32_t y = 0; y < rd[1]; ++y)
+            for (int32_t z = 0; z < rd[2]; ++z) {
+                size_t r = ((size_t)x * rd[1] + y) * rd[2] + z;
+                uint32_t v;
+                if (rmax[r] > 0.0f) {
+                    v = 1;
+                    while (v < 127 && (float)v * unit < rmax[r]) ++v;
+                } else {
+                    int32_t best = 128;  /* no region with density within 127 layers */
+                    for (size_t i = 0; i < nd; ++i) {
+                        int32_t dx = abs(dense[3 * i] - x), dy = abs(dense[3 * i + 1] - y), dz = abs(dense[3 * i + 2] - z);
+                        int32_t c = dx > dy ? dx : dy;
+                        c = c > dz ? c : dz;
+                        if (c < best) best = c;
+                    }
+                    v = RTC8X_REGION_EMPTY + (uint32_t)(best - 1 < 127 ? best - 1 : 127);
+                }
+                q[r] = (uint8_t)v;
+            }
+    free(dense);
     free(rmax);
     for (int a = 0; a < 3; ++a) g->region_dims[a] = rd[a];
     g->region_cells = rc;

@@ -313,38 +313,7 @@ __device__ __forceinline__ bool clip(const rtc8x_grid& g, V3 o, V3 d, float tmax
     return true;
 }
 
-// -------------------------------------------------------------- tracking
-// Global majorant, reference algorithm (optix_pathtracing_kernels.cu:110-125)
-__device__ __forceinline__ bool track_global(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0,
-                                             float t1, float& thit)
-{
-    float t = fmaxf(0.0f, t0);
-    for (;;) {
-        t += -rtc8x_logf(1.0f - rng.uni()) / m.majorant;
-        if (t > t1) return false;
-        float sigma = sample_density(g, m, o, d, t);
-        if (rng.uni() < sigma / m.majorant) {
-            thit = t;
-            return true;
-        }
-    }
-}
-
-// Ratio tracking, reference algorithm (renderer_kernel_common.h:371-382)
-__device__ __forceinline__ float ratio_global(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0,
-                                              float t1)
-{
-    float vis = 1.0f;
-    float t = fmaxf(0.0f, t0);
-    for (;;) {
-        t += -rtc8x_logf(1.0f - rng.uni()) / m.majorant;
-        if (t > t1) break;
-        float sigma = sample_density(g, m, o, d, t);
-        vis *= 1.0f - sigma / m.majorant;
-    }
-    return vis;
-}
-
+// -------------------------------------------------------------- local majorants
 // 3D DDA over the majorant regions (region_cells^3 cells of 8^3 voxels) in index space; boundaries
 // recomputed from the region index.  Per-axis state is held in named scalars and selected by axis (no
 // indexed arrays, so it stays in registers).
@@ -405,6 +374,17 @@ struct Dda {
         if (tn2 < best) a = 2;
         return a;
     }
+    // The ray parameter at which the ray leaves the cube of regions within k of the current one: the
+    // first crossing of its faces (boundaries computed as the steps compute them).
+    __device__ __forceinline__ float skip_exit(const rtc8x_grid& g, int32_t k) const
+    {
+        const int32_t size = region_size(g);
+        float tx = __builtin_huge_valf();
+        if (s0 != 0) tx = fminf(tx, boundary_of(g.cell_origin[0], size, c0 + s0 * k, s0, io0, iv0));
+        if (s1 != 0) tx = fminf(tx, boundary_of(g.cell_origin[1], size, c1 + s1 * k, s1, io1, iv1));
+        if (s2 != 0) tx = fminf(tx, boundary_of(g.cell_origin[2], size, c2 + s2 * k, s2, io2, iv2));
+        return tx;
+    }
     // Steps the cell along axis a; false when it leaves the cell grid.  Updates that axis' boundary.
     // Written with value selects and unconditional stores: per-axis branches that each store a field
     // get merged into one store through a selected field address, which keeps the state in memory.
@@ -425,103 +405,17 @@ struct Dda {
     }
 };
 
-// The region's majorant, densityCoeff x (q x region_unit), with q from `table`: the grid's region bytes,
-// or a copy of them staged in LDS.
-__device__ __forceinline__ float cell_majorant(const rtc8x_grid& g, const Medium& m, const Dda& s,
-                                              const uint8_t* __restrict__ table)
+// Region codes: 1..127 is q for a region with density, majorant densityCoeff x (q x region_unit);
+// RTC8X_REGION_EMPTY + k marks an empty region whose neighbours up to k regions away (Chebyshev distance)
+// are empty too.  `table` is the grid's region bytes or a copy of them staged in LDS.
+__device__ __forceinline__ uint32_t region_code(const rtc8x_grid& g, const Dda& s, const uint8_t* __restrict__ table)
 {
-    const uint32_t idx = ((uint32_t)s.c0 * (uint32_t)g.region_dims[1] + (uint32_t)s.c1) * (uint32_t)g.region_dims[2] +
-                         (uint32_t)s.c2;
-    return m.coeff * ((float)table[idx] * g.region_unit);
+    return table[((uint32_t)s.c0 * (uint32_t)g.region_dims[1] + (uint32_t)s.c1) * (uint32_t)g.region_dims[2] +
+                 (uint32_t)s.c2];
 }
-
-__device__ __forceinline__ bool track_local(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0, float t1,
-                                            float& thit)
+__device__ __forceinline__ float region_majorant(const rtc8x_grid& g, const Medium& m, uint32_t code)
 {
-    float t = fmaxf(0.0f, t0);
-    Dda s;
-    s.init(g, o, d, t);
-    float tau = -rtc8x_logf(1.0f - rng.uni());
-    for (;;) {
-        int a = s.axis();
-        const float tna = s.tnext(a);
-        float tend = tna < t1 ? tna : t1;
-        float mu = cell_majorant(g, m, s, g.region_majorants);
-        if (mu > 0.0f) {
-            for (;;) {
-                float tc = t + tau / mu;
-                if (!(tc < tend)) {
-                    tau = tau - mu * (tend - t);
-                    if (tau < 0.0f) tau = 0.0f;
-                    break;
-                }
-                t = tc;
-                float sigma = sample_density(g, m, o, d, t);
-                if (rng.uni() < sigma / mu) {
-                    thit = t;
-                    return true;
-                }
-                tau = -rtc8x_logf(1.0f - rng.uni());
-            }
-        }
-        if (!(tna < t1)) return false;
-        t = tend;
-        if (!s.step(g, a)) return false;
-    }
-}
-
-__device__ __forceinline__ float ratio_local(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float t0, float t1)
-{
-    float vis = 1.0f;
-    float t = fmaxf(0.0f, t0);
-    Dda s;
-    s.init(g, o, d, t);
-    float tau = -rtc8x_logf(1.0f - rng.uni());
-    for (;;) {
-        int a = s.axis();
-        const float tna = s.tnext(a);
-        float tend = tna < t1 ? tna : t1;
-        float mu = cell_majorant(g, m, s, g.region_majorants);
-        if (mu > 0.0f) {
-            for (;;) {
-                float tc = t + tau / mu;
-                if (!(tc < tend)) {
-                    tau = tau - mu * (tend - t);
-                    if (tau < 0.0f) tau = 0.0f;
-                    break;
-                }
-                t = tc;
-                float sigma = sample_density(g, m, o, d, t);
-                vis *= 1.0f - sigma / mu;
-                tau = -rtc8x_logf(1.0f - rng.uni());
-            }
-        }
-        if (!(tna < t1)) break;
-        t = tend;
-        if (!s.step(g, a)) break;
-    }
-    return vis;
-}
-
-template <int TRACKING>
-__device__ __forceinline__ bool free_flight(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float tmax,
-                                            float& thit)
-{
-    float t0, t1;
-    if (!(m.majorant > 0.0f)) return false;
-    if (!clip(g, o, d, tmax, t0, t1)) return false;
-    if (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) return track_local(g, m, rng, o, d, t0, t1, thit);
-    return track_global(g, m, rng, o, d, t0, t1, thit);
-}
-
-template <int TRACKING>
-__device__ __forceinline__ float transmittance(const rtc8x_grid& g, Medium& m, Pcg& rng, V3 o, V3 d, float tmax)
-{
-    float t0, t1;
-    if (!(m.majorant > 0.0f)) return 1.0f;
-    if (!clip(g, o, d, tmax, t0, t1)) return 1.0f;
-    if (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) return ratio_local(g, m, rng, o, d, t0, t1);
-    return ratio_global(g, m, rng, o, d, t0, t1);
+    return code < RTC8X_REGION_EMPTY ? m.coeff * ((float)code * g.region_unit) : 0.0f;
 }
 
 // RadianceQuery of a medium vertex (optix_pathtracing_kernels.cu:14-29): AABB-local position, the

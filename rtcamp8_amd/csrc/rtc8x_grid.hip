@@ -197,9 +197,11 @@ void cell_majorants(const std::vector<int32_t>& origins, const std::vector<float
             }
 }
 
-// Region majorants from the cell majorants (the layout rtc8x.h defines for rtc8x_grid): the least
-// power-of-two region_cells with at most RTC8X_MAX_REGIONS regions, a unit of (largest cell majorant) / 255
-// raised until 255 units cover it, and per region the least q whose q units cover its largest cell.
+// Region table from the cell majorants (the layout rtc8x.h defines for rtc8x_grid): the least power-of-two
+// region_cells with at most RTC8X_MAX_REGIONS regions; a unit of (largest cell majorant) / 127 raised until
+// 127 units cover it; per region with density the least q whose q units cover its largest cell, per empty
+// region RTC8X_REGION_EMPTY + the number of empty layers around it (a Chebyshev distance transform by
+// repeated 26-neighbour dilation of the regions with density).
 void region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3], int32_t region_dims[3],
                       uint32_t* region_cells, float* region_unit, std::vector<uint8_t>& out)
 {
@@ -213,28 +215,60 @@ void region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3
         if (n <= RTC8X_MAX_REGIONS) break;
         rc *= 2;
     }
-    std::vector<float> rmax((size_t)region_dims[0] * region_dims[1] * region_dims[2], 0.0f);
+    const int32_t rx = region_dims[0], ry = region_dims[1], rz = region_dims[2];
+    const size_t nr = (size_t)rx * ry * rz;
+    std::vector<float> rmax(nr, 0.0f);
     float top = 0.0f;
     for (int32_t x = 0; x < cell_dims[0]; ++x)
         for (int32_t y = 0; y < cell_dims[1]; ++y)
             for (int32_t z = 0; z < cell_dims[2]; ++z) {
                 const float m = cells[((size_t)x * cell_dims[1] + y) * cell_dims[2] + z];
-                float& r = rmax[((size_t)(x / (int32_t)rc) * region_dims[1] + y / (int32_t)rc) * region_dims[2] +
-                                z / (int32_t)rc];
+                float& r = rmax[((size_t)(x / (int32_t)rc) * ry + y / (int32_t)rc) * rz + z / (int32_t)rc];
                 r = std::max(r, m);
                 top = std::max(top, m);
             }
-    float unit = top / 255.0f;
-    while (255.0f * unit < top) unit = std::nextafter(unit, INFINITY);
+    float unit = top / 127.0f;
+    while (127.0f * unit < top) unit = std::nextafter(unit, INFINITY);
+    // distance (in regions) to the nearest region with density; 0 for those regions themselves
+    const uint32_t kFar = 255;
+    std::vector<uint8_t> dist(nr, (uint8_t)kFar);
+    for (size_t r = 0; r < nr; ++r)
+        if (rmax[r] > 0.0f) dist[r] = 0;
+    for (uint32_t k = 1; k <= 128; ++k) {
+        bool grew = false;
+        std::vector<uint8_t> next = dist;
+        for (int32_t x = 0; x < rx; ++x)
+            for (int32_t y = 0; y < ry; ++y)
+                for (int32_t z = 0; z < rz; ++z) {
+                    const size_t r = ((size_t)x * ry + y) * rz + z;
+                    if (dist[r] != kFar) continue;
+                    bool near = false;
+                    for (int dx = -1; dx <= 1 && !near; ++dx)
+                        for (int dy = -1; dy <= 1 && !near; ++dy)
+                            for (int dz = -1; dz <= 1 && !near; ++dz) {
+                                const int32_t a = x + dx, b = y + dy, c = z + dz;
+                                if (a < 0 || b < 0 || c < 0 || a >= rx || b >= ry || c >= rz) continue;
+                                near = dist[((size_t)a * ry + b) * rz + c] == k - 1;
+                            }
+                    if (near) {
+                        next[r] = (uint8_t)k;
+                        grew = true;
+                    }
+                }
+        dist.swap(next);
+        if (!grew) break;
+    }
     // a multiple of 16 bytes: the tracer stages the table in LDS 16 bytes at a time
-    out.assign((rmax.size() + 15) & ~(size_t)15, 0);
-    for (size_t r = 0; r < rmax.size(); ++r) {
-        uint32_t q = 0;
+    out.assign((nr + 15) & ~(size_t)15, 0);
+    for (size_t r = 0; r < nr; ++r) {
         if (rmax[r] > 0.0f) {
-            q = 1;
-            while (q < 255 && (float)q * unit < rmax[r]) ++q;
+            uint32_t q = 1;
+            while (q < 127 && (float)q * unit < rmax[r]) ++q;
+            out[r] = (uint8_t)q;
+        } else {
+            const uint32_t k = dist[r] == kFar ? 127u : std::min<uint32_t>(dist[r] - 1u, 127u);
+            out[r] = (uint8_t)(RTC8X_REGION_EMPTY + k);
         }
-        out[r] = (uint8_t)q;
     }
     *region_cells = rc;
     *region_unit = unit;

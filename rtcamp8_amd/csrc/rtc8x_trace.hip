@@ -98,10 +98,21 @@ struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
     {
         for (;;) {
             if (!in_cell) {
+                const uint32_t code = region_code(g, s, table);
+                if (code > RTC8X_REGION_EMPTY) {
+                    // an empty region with k empty layers around it: leave their cube in one jump
+                    const float tx = s.skip_exit(g, (int32_t)(code - RTC8X_REGION_EMPTY));
+                    if (tx > t) {
+                        if (!(tx < t1)) return false;
+                        t = tx;
+                        s.init(g, o, d, t);
+                        continue;
+                    }
+                }
                 axis = s.axis();
                 const float tna = s.tnext(axis);
                 tend = tna < t1 ? tna : t1;
-                mu_cell = cell_majorant(g, m, s, table);
+                mu_cell = region_majorant(g, m, code);
                 in_cell = 1;
             }
             if (mu_cell > 0.0f) {

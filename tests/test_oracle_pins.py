@@ -173,13 +173,21 @@ def clip_unit_cube(o, d):
     return max(0.0, float(np.minimum(a, b).max())), float(np.maximum(a, b).min())
 
 
-def check_tracking_against_brute_force_optical_depth(lib, f64: bool, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT):
-    hg = oracle_build_grid(FbmConfig(resolution=16, seed=0x0123))
-    sigma = 8.0 / max(hg.grid.max_value, 1e-6)  # majorant 8 per unit length
+def check_tracking_against_brute_force_optical_depth(lib, f64: bool, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT,
+                                                     sparse: bool = False):
+    """sparse: a 128^3 cloud with 5 % of its leaves, whose empty regions the local tracker skips."""
+    hg = oracle_build_grid(FbmConfig(resolution=128, seed=0x0123, sparse=True, active_leaf_fraction=0.05) if sparse else
+                           FbmConfig(resolution=16, seed=0x0123), lib)
+    # majorant 8 per unit length; for the sparse cloud 40, with the rays aimed at its densest leaves so
+    # that they cross empty space and dense regions alike
+    sigma = (40.0 if sparse else 8.0) / max(hg.grid.max_value, 1e-6)
     r = np.random.default_rng(31)
     n = 20_000
+    if sparse:
+        org = hg.arrays["origins"].reshape(-1, 3)
+        dense_leaves = org[np.argsort(hg.arrays["values"].reshape(-1, 512).max(axis=1))[-max(1, len(org) // 4):]]
     for ray in range(6):
-        target = r.uniform(0.3, 0.7, 3)
+        target = (dense_leaves[r.integers(len(dense_leaves))] + 4.0) / 128.0 if sparse else r.uniform(0.3, 0.7, 3)
         v = r.normal(size=3)
         o = target + 2.0 * v / np.linalg.norm(v)
         d = (target - o) / np.linalg.norm(target - o)
@@ -195,6 +203,36 @@ def check_tracking_against_brute_force_optical_depth(lib, f64: bool, tracking: i
         for frac, want in (((t < 0).mean(), math.exp(-tau)), (((t >= 0) & (t < tmid)).mean(), 1 - math.exp(-tau_mid))):
             se = math.sqrt(max(want * (1 - want), 1e-12) / n)
             assert abs(frac - want) < Z * se + 1e-4, (ray, frac, want, tau)
+
+
+def check_local_tracking_across_empty_space(lib, f64: bool):
+    """Local tracking that skips empty regions: a 3 x 3 x 3 block of leaves of density 1 at the centre of a
+    128^3 index box (two single leaves at opposite corners set the box), so that rays aimed at the block
+    cross long stretches of empty regions (skipped in jumps of up to several regions) before they reach it.
+    E[T] = e^(-tau) and P(escape) = e^(-tau) with tau from the brute-force integral."""
+    origins = [(0, 0, 0), (120, 120, 120)] + [(x, y, z) for x in (56, 64, 72) for y in (56, 64, 72) for z in (56, 64, 72)]
+    values = np.ones((len(origins), 512), np.float32)
+    hg = with_majorants(tables_from_leaves(np.array(origins, np.int32), values, 1.0 / 128, (0, 0, 0, 1, 1, 1)))
+    codes = hg.arrays["regions"]
+    assert (codes > 130).any()  # empty regions with at least two empty layers around them
+    sigma = 8.0
+    r = np.random.default_rng(41)
+    n = 20_000
+    for ray in range(6):
+        target = (64.0 + r.uniform(-6, 6, 3)) / 128.0
+        v = r.normal(size=3)
+        o = target + 2.0 * v / np.linalg.norm(v)
+        d = (target - o) / np.linalg.norm(target - o)
+        o, d = o.astype(np.float32).astype(np.float64), d.astype(np.float32).astype(np.float64)
+        t0, t1 = clip_unit_cube(o, d)
+        tau = brute_force_optical_depth(hg, sigma, o, d, t0, t1)
+        assert 0.3 < tau < 4.0, tau
+        O, D = np.tile(o, (n, 1)), np.tile(d, (n, 1))
+        T, _ = track(lib, frame(hg.grid, sigma, LOCAL), True, O, D, f64, seed=300 + ray)
+        assert abs(T.mean() - math.exp(-tau)) < Z * T.std() / math.sqrt(n) + 1e-4, (ray, T.mean(), math.exp(-tau))
+        t, _ = track(lib, frame(hg.grid, sigma, LOCAL), False, O, D, f64, seed=400 + ray)
+        want = math.exp(-tau)
+        assert abs((t < 0).mean() - want) < Z * math.sqrt(want * (1 - want) / n) + 1e-4, (ray, (t < 0).mean(), want)
 
 
 # ------------------------------------------------------------------ phase functions
@@ -404,7 +442,11 @@ def check_importance_map_definition(lib, w: int, h: int):
 
 def check_region_majorants(lib, res: int, sparse: bool):
     """The region table of rtc8x.h's definition: regions of the least power-of-two size that keeps their
-    count <= 32768, each byte the least q with q x unit >= the largest cell majorant of the region."""
+    count <= 32768; for a region with density the least q in 1..127 with q x unit >= its largest cell
+    majorant; for an empty region 128 + (its chessboard distance to the nearest region with density) - 1,
+    the distance from scipy's chamfer transform."""
+    from scipy.ndimage import distance_transform_cdt
+
     hg = oracle_build_grid(FbmConfig(resolution=res, seed=0x0456, sparse=sparse), lib)
     g = hg.grid
     cd = np.array(g.cell_dims[:])
@@ -418,13 +460,17 @@ def check_region_majorants(lib, res: int, sparse: bool):
     pad[:cd[0], :cd[1], :cd[2]] = cells
     rmax = pad.reshape(rd[0], rc, rd[1], rc, rd[2], rc).max(axis=(1, 3, 5))
     unit = np.float32(g.region_unit)
-    assert np.float32(255) * unit >= np.float32(cells.max()) and unit <= np.float32(cells.max()) / 255 * (1 + 1e-6)
-    q = hg.arrays["regions"].reshape(rd).astype(np.int64)
-    covered = q.astype(np.float32) * unit >= rmax
-    assert covered.all()
-    assert np.array_equal(q == 0, rmax == 0)
-    below = (q - 1).astype(np.float32) * unit < rmax  # q is the least such value
-    assert below[q > 0].all()
+    assert np.float32(127) * unit >= np.float32(cells.max()) and unit <= np.float32(cells.max()) / 127 * (1 + 1e-6)
+    code = hg.arrays["regions"].reshape(rd).astype(np.int64)
+    dense = rmax > 0
+    q = np.where(dense, code, 0)
+    assert ((q >= 1) & (q <= 127))[dense].all()
+    assert (q.astype(np.float32) * unit >= rmax)[dense].all()
+    assert ((q - 1).astype(np.float32) * unit < rmax)[dense].all()  # q is the least such value
+    dist = distance_transform_cdt(~dense, metric="chessboard")
+    want = 128 + np.minimum(dist - 1, 127)
+    assert np.array_equal(code[~dense], want[~dense])
+    assert dense.any() and (not sparse or (~dense).any())
 
 
 # ------------------------------------------------------------------ the tests
@@ -444,10 +490,16 @@ def test_ratio_tracking_in_a_homogeneous_medium(f64, tracking):
     check_ratio_tracking_homogeneous(oracle(), f64, tracking)
 
 
+@pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("tracking", TRACKINGS)
 @pytest.mark.parametrize("f64", [False, True])
-def test_tracking_against_the_brute_force_optical_depth(f64, tracking):
-    check_tracking_against_brute_force_optical_depth(oracle(), f64, tracking)
+def test_tracking_against_the_brute_force_optical_depth(f64, tracking, sparse):
+    check_tracking_against_brute_force_optical_depth(oracle(), f64, tracking, sparse)
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_local_tracking_across_empty_space(f64):
+    check_local_tracking_across_empty_space(oracle(), f64)
 
 
 @pytest.mark.parametrize("res,sparse", [(64, False), (128, True), (256, True)])
@@ -495,6 +547,10 @@ MUTATIONS = [
     ("rtc8_oracle_tracer.inc", "s->tnext[a] = s->step[a] != 0 ? SUF(dda_boundary)(s, a) : (R)INFINITY;",
      "s->tnext[a] = s->step[a] != 0 ? SUF(dda_boundary)(s, a) + (R)1 : (R)INFINITY;",
      lambda lib: check_ratio_tracking_homogeneous(lib, False, LOCAL)),
+    # the cube's full width taken for its half-width (one layer too many is absorbed by the 3x3x3 dilation of
+    # the cell majorants, which is there to cover boundary rounding)
+    ("rtc8_oracle_tracer.inc", "int32_t c = s->c[a] + s->step[a] * k;", "int32_t c = s->c[a] + s->step[a] * (2 * k + 1);",
+     lambda lib: check_local_tracking_across_empty_space(lib, False)),
     ("rtc8_oracle_core.c", "size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + y / (int32_t)rc) * rd[2] + z / (int32_t)rc;",
      "size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + z / (int32_t)rc) * rd[2] + y / (int32_t)rc;",
      lambda lib: check_region_majorants(lib, 128, True)),

@@ -138,10 +138,13 @@ typedef struct rtc8x_grid {
     const float* leaf_values;            /* num_leaves * 512 voxel values */
     const float* tile_values;            /* tile_values[0] == background */
     const int32_t* leaf_origins;         /* num_leaves * 3 index-space origins (optional, may be NULL) */
-    /* Local-majorant acceleration (optional, used by RTC8X_TRACKING_LOCAL_MAJORANT):
-     * one float per 8^3 leaf cell over the index bbox: the max voxel value over the 3x3x3 block of
-     * leaves around the cell (the trilinear stencil reaches one leaf further in +x/+y/+z, and a
-     * rounded sample position can sit a rounding error into either neighbour). */
+    /* Local-majorant acceleration (library-built, optional, used by RTC8X_TRACKING_LOCAL_MAJORANT
+     * through the region table below): one float per 8^3 cell over the index bbox, cell c spanning
+     * index [cell_origin + 8c, cell_origin + 8c + 8): the largest voxel value (leaves, tiles and
+     * background as lookups see them, and at least 0) over [cell_origin + 8c - 1, cell_origin + 8c + 9]
+     * per axis.  A point of the cell reads its trilinear taps from [8c, 8c + 8]; the rim of one voxel
+     * more on each side covers a sample position that rounding puts just outside the cell.  NULL (and
+     * no region table) when the index bbox spans more than 2^26 cells. */
     int32_t cell_origin[3];              /* index-space origin of cell (0,0,0), a multiple of 8 */
     int32_t cell_dims[3];
     const float* cell_majorants;         /* cell_dims[0]*cell_dims[1]*cell_dims[2], x-major */
@@ -169,9 +172,10 @@ typedef struct rtc8x_grid {
     double map_vec_d[3];
     /* Region majorants (library-built with the cell majorants, optional; used by
      * RTC8X_TRACKING_LOCAL_MAJORANT, which walks regions, not cells): the cell grid is grouped into regions
-     * of region_cells^3 cells from cell_origin (region_cells a power of two, the least >= 1 that keeps
-     * region_dims[0] * region_dims[1] * region_dims[2] <= RTC8X_MAX_REGIONS, so the tracer can stage the
-     * table in LDS).  One byte per region, x-major: for a region whose largest cell majorant m is > 0, the
+     * of region_cells^3 cells from cell_origin (region_cells a power of two: RTC8X_DEFAULT_REGION_CELLS,
+     * doubled until region_dims[0] * region_dims[1] * region_dims[2] <= RTC8X_MAX_REGIONS so the tracer can
+     * stage the table in LDS, or the size rtc8x_grid_build_regions was given).  One byte per region,
+     * x-major: for a region whose largest cell majorant m is > 0, the
      * least q in 1..127 with (float)q * region_unit >= m, the region's majorant being (float)q *
      * region_unit; for an empty region, RTC8X_REGION_EMPTY + k, k (<= 127) the number of region layers
      * around it that are empty too (its Chebyshev distance to the nearest region with density, minus one;
@@ -186,6 +190,7 @@ typedef struct rtc8x_grid {
 
 #define RTC8X_MAX_REGIONS 32768
 #define RTC8X_REGION_EMPTY 128u
+#define RTC8X_DEFAULT_REGION_CELLS 4u  /* regions of 32^3 voxels */
 
 /* Procedural fBm volume description (the benchmark's synthetic input; the reference loads a .nvdb). */
 typedef struct rtc8x_fbm_desc {
@@ -208,6 +213,13 @@ rtc8x_status rtc8x_grid_build_fbm(const rtc8x_fbm_desc* desc, rtc8x_grid* out, v
  * (renderer/renderer_scene.cpp:491-537). */
 rtc8x_status rtc8x_grid_upload(const rtc8x_grid* host, rtc8x_grid* out);
 rtc8x_status rtc8x_grid_destroy(rtc8x_grid* grid);
+/* Rebuilds a device grid's region table from its cell majorants with regions of region_cells^3 cells
+ * (a power of two; 0: the default choice).  RTC8X_ERR_INVALID_ARGUMENT when the grid has no cell
+ * majorants or the size gives more than RTC8X_MAX_REGIONS regions.  Synchronises the device.  The local
+ * tracker's cost trades region steps (fewer with larger regions) against density lookups (fewer with
+ * tighter majorants); callers tune it per volume.  No counterpart in the reference, which tracks with
+ * the global majorant only (renderer_scene.cpp:499-511). */
+rtc8x_status rtc8x_grid_build_regions(rtc8x_grid* grid, uint32_t region_cells);
 /* Parity probe: for n world-space points, the index coordinate, node offsets and trilinear value
  * exactly as the tracer computes them.  out_ijk[3n], out_offsets[3n] = (upper, lower, leaf) offsets,
  * out_value[n]; all device pointers. */

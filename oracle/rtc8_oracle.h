@@ -36,8 +36,9 @@ float oracle_fbm_value(const rtc8x_fbm_desc* desc, int32_t i, int32_t j, int32_t
 float oracle_grid_get_value(const rtc8x_grid* g, int32_t i, int32_t j, int32_t k, uint32_t offsets[3]);
 void oracle_grid_probe(const rtc8x_grid* g, uint32_t n, const float* world_xyz,
                        int32_t* out_ijk, uint32_t* out_offsets, float* out_value);
-void oracle_compute_cell_majorants(rtc8x_grid* g);  /* fills cell_* and region_* (malloc'd) */
-void oracle_compute_region_majorants(rtc8x_grid* g);  /* region_* from cell_* (malloc'd) */
+/* cell_* and region_* (malloc'd); region_cells 0: the default region size */
+void oracle_compute_cell_majorants(rtc8x_grid* g, uint32_t region_cells);
+void oracle_compute_region_majorants(rtc8x_grid* g, uint32_t region_cells);  /* region_* from cell_* */
 int oracle_ray_clip(const float eye[3], const float dir[3], float tmax, const float bmin[3],
                     const float bmax[3], float* t0, float* t1);
 

@@ -334,7 +334,7 @@ int oracle_grid_build_fbm(const rtc8x_fbm_desc* d, rtc8x_grid* g)
     g->leaf_values = values;
     g->tile_values = tile_values;
     g->leaf_origins = origins;
-    oracle_compute_cell_majorants(g);
+    oracle_compute_cell_majorants(g, 0);
     return 0;
 }
 
@@ -369,6 +369,24 @@ float oracle_grid_get_value(const rtc8x_grid* g, int32_t i, int32_t j, int32_t k
     uint32_t e2 = g->lower_tables[(size_t)e1 * RTC8X_LOWER_TABLE_SIZE + lo];
     if (e2 & RTC8X_TILE_BIT) return g->tile_values[e2 & ~RTC8X_TILE_BIT];
     return g->leaf_values[(size_t)e2 * RTC8X_LEAF_SIZE + vo];
+}
+
+/* The tree walk of getValue stopped at the leaf: the leaf index holding voxel (i, j, k), or UINT32_MAX with
+ * *tile = the tile (or background) value that covers it. */
+static uint32_t leaf_of(const rtc8x_grid* g, int32_t i, int32_t j, int32_t k, float* tile)
+{
+    uint64_t key = root_key(i, j, k);
+    for (uint32_t t = 0; t < g->num_root_tiles; ++t) {
+        if (g->root_tiles[t].key != key) continue;
+        uint32_t e = g->root_tiles[t].entry;
+        if (!(e & RTC8X_TILE_BIT)) e = g->upper_tables[(size_t)e * RTC8X_UPPER_TABLE_SIZE + upper_offset(i, j, k)];
+        if (!(e & RTC8X_TILE_BIT)) e = g->lower_tables[(size_t)e * RTC8X_LOWER_TABLE_SIZE + lower_offset(i, j, k)];
+        if (!(e & RTC8X_TILE_BIT)) return e;
+        *tile = g->tile_values[e & ~RTC8X_TILE_BIT];
+        return UINT32_MAX;
+    }
+    *tile = g->background;
+    return UINT32_MAX;
 }
 
 /* Map::applyInverseMapF (NanoVDB.h:2029-2032) with matMult (NanoVDB.h:1409-1414). */
@@ -430,18 +448,24 @@ void oracle_grid_probe(const rtc8x_grid* g, uint32_t n, const float* world_xyz,
     }
 }
 
-/* Per-cell majorants for local-majorant tracking: cell c covers index [8c, 8c+8)^3.  The trilinear
- * stencil of a point in cell c reads leaves c and c+1; a sample position computed in floating point
- * can land a rounding error outside the cell's t-range, in cell c-1 or c+1.  The bound is therefore
- * the max voxel over the 3x3x3 block of leaves {c-1, c, c+1}^3. */
-void oracle_compute_cell_majorants(rtc8x_grid* g)
+/* Per-cell majorants for local-majorant tracking (the grid's definition in rtc8x.h): cell c covers index
+ * [8c, 8c+8)^3 from cell_origin, over the cells of the index bbox.  A point of the cell reads its
+ * trilinear taps from [8c, 8c+8]; a sample position computed in floating point can land a rounding error
+ * outside the cell's t-range, so the bound is the largest value getValue returns (leaves, tiles,
+ * background; at least 0) over index [8c-1, 8c+9]^3.  Found here block by block: per axis that range is
+ * the last voxel of block c-1, all of block c and the first two of block c+1; a block under a tile
+ * contributes the tile's value, a leaf the voxels of its part of the range.  No tables when the index
+ * bbox spans more than 2^26 cells. */
+void oracle_compute_cell_majorants(rtc8x_grid* g, uint32_t region_cells)
 {
     free((void*)g->cell_majorants);
     g->cell_majorants = NULL;
-    if (g->num_leaves == 0) {
-        for (int a = 0; a < 3; ++a) { g->cell_origin[a] = 0; g->cell_dims[a] = 0; }
-        return;
-    }
+    free((void*)g->region_majorants);
+    g->region_majorants = NULL;
+    g->region_cells = 0;
+    g->region_unit = 0.0f;
+    for (int a = 0; a < 3; ++a) { g->cell_origin[a] = 0; g->cell_dims[a] = 0; g->region_dims[a] = 0; }
+    if (g->num_leaves == 0) return;
     int32_t lo[3], dims[3];
     for (int a = 0; a < 3; ++a) {
         lo[a] = (int32_t)floorf((float)g->index_bbox_min[a] / 8.0f);
@@ -450,44 +474,49 @@ void oracle_compute_cell_majorants(rtc8x_grid* g)
         g->cell_origin[a] = lo[a] * 8;
         g->cell_dims[a] = dims[a];
     }
-    /* leaf maxima on a grid padded by one cell on each side */
-    int32_t px = dims[0] + 2, py = dims[1] + 2, pz = dims[2] + 2;
-    float* lm = (float*)calloc((size_t)px * py * pz, sizeof(float));
-    for (uint32_t l = 0; l < g->num_leaves; ++l) {
-        int32_t cx = g->leaf_origins[3 * l] / 8 - lo[0] + 1, cy = g->leaf_origins[3 * l + 1] / 8 - lo[1] + 1,
-                cz = g->leaf_origins[3 * l + 2] / 8 - lo[2] + 1;
-        float m = 0.0f;
-        for (int v = 0; v < 512; ++v) {
-            float val = g->leaf_values[(size_t)l * 512 + v];
-            if (val > m) m = val;
-        }
-        lm[((size_t)cx * py + cy) * pz + cz] = m;
+    if ((size_t)dims[0] * dims[1] * dims[2] > ((size_t)1 << 26)) {  /* too large an index bbox: no tables */
+        for (int a = 0; a < 3; ++a) { g->cell_origin[a] = 0; g->cell_dims[a] = 0; }
+        return;
     }
     float* cm = (float*)malloc(sizeof(float) * (size_t)dims[0] * dims[1] * dims[2]);
     for (int32_t x = 0; x < dims[0]; ++x)
         for (int32_t y = 0; y < dims[1]; ++y)
             for (int32_t z = 0; z < dims[2]; ++z) {
+                const int32_t c[3] = {g->cell_origin[0] + 8 * x, g->cell_origin[1] + 8 * y, g->cell_origin[2] + 8 * z};
+                /* per axis and block b = -1, 0, +1: the voxel range [first, last] inside the block */
+                static const int32_t first[3] = {7, 0, 0}, last[3] = {7, 7, 1};
                 float m = 0.0f;
-                for (int dx = 0; dx < 3; ++dx)
-                    for (int dy = 0; dy < 3; ++dy)
-                        for (int dz = 0; dz < 3; ++dz) {
-                            float v = lm[((size_t)(x + dx) * py + (y + dy)) * pz + (z + dz)];
-                            if (v > m) m = v;
+                for (int bx = 0; bx < 3; ++bx)
+                    for (int by = 0; by < 3; ++by)
+                        for (int bz = 0; bz < 3; ++bz) {
+                            int32_t o[3] = {c[0] + 8 * (bx - 1), c[1] + 8 * (by - 1), c[2] + 8 * (bz - 1)};
+                            float tile = 0.0f;
+                            uint32_t leaf = leaf_of(g, o[0], o[1], o[2], &tile);
+                            if (leaf == UINT32_MAX) {
+                                if (tile > m) m = tile;
+                                continue;
+                            }
+                            const float* lv = g->leaf_values + (size_t)leaf * RTC8X_LEAF_SIZE;
+                            for (int32_t i = first[bx]; i <= last[bx]; ++i)
+                                for (int32_t j = first[by]; j <= last[by]; ++j)
+                                    for (int32_t k = first[bz]; k <= last[bz]; ++k) {
+                                        float v = lv[(i << 6) | (j << 3) | k];
+                                        if (v > m) m = v;
+                                    }
                         }
                 cm[((size_t)x * dims[1] + y) * dims[2] + z] = m;
             }
-    free(lm);
     g->cell_majorants = cm;
-    oracle_compute_region_majorants(g);
+    oracle_compute_region_majorants(g, region_cells);
 }
 
-/* Region table over groups of region_cells^3 cells (the grid's definition in rtc8x.h): the least
- * power-of-two region_cells >= 1 with at most RTC8X_MAX_REGIONS regions; region_unit = the largest cell
- * majorant / 127, raised float by float until 127 * region_unit covers it; per region with density the
- * least q in 1..127 with q * region_unit >= its largest cell majorant; per empty region
- * RTC8X_REGION_EMPTY + (its Chebyshev distance to the nearest region with density) - 1, at most 127, found
- * here by a brute-force search over the regions with density. */
-void oracle_compute_region_majorants(rtc8x_grid* g)
+/* Region table over groups of region_cells^3 cells (the grid's definition in rtc8x.h): region_cells as
+ * given, or for 0 RTC8X_DEFAULT_REGION_CELLS doubled until there are at most RTC8X_MAX_REGIONS regions;
+ * region_unit = the largest cell majorant / 127, raised float by float until 127 * region_unit covers it;
+ * per region with density the least q in 1..127 with q * region_unit >= its largest cell majorant; per
+ * empty region RTC8X_REGION_EMPTY + (its Chebyshev distance to the nearest region with density) - 1, at
+ * most 127, found here by a brute-force search over the regions with density. */
+void oracle_compute_region_majorants(rtc8x_grid* g, uint32_t region_cells)
 {
     free((void*)g->region_majorants);
     g->region_majorants = NULL;
@@ -496,7 +525,7 @@ void oracle_compute_region_majorants(rtc8x_grid* g)
     for (int a = 0; a < 3; ++a) g->region_dims[a] = 0;
     if (!g->cell_majorants) return;
     const int32_t* cd = g->cell_dims;
-    uint32_t rc = 1;
+    uint32_t rc = region_cells ? region_cells : RTC8X_DEFAULT_REGION_CELLS;
     int32_t rd[3];
     for (;;) {
         size_t n = 1;
@@ -504,7 +533,7 @@ void oracle_compute_region_majorants(rtc8x_grid* g)
             rd[a] = (cd[a] + (int32_t)rc - 1) / (int32_t)rc;
             n *= (size_t)rd[a];
         }
-        if (n <= RTC8X_MAX_REGIONS) break;
+        if (n <= RTC8X_MAX_REGIONS || region_cells) break;
         rc *= 2;
     }
     size_t nr = (size_t)rd[0] * rd[1] * rd[2];

@@ -32,6 +32,7 @@ INVALID_VERTEX_INDEX = 0x003FFFFF
 NUM_TRAINING_DATA_PER_FRAME = 65536
 TRAIN_BUFFER_SIZE = 2 * 65536
 MAX_REGIONS = 32768
+DEFAULT_REGION_CELLS = 4
 
 
 class RootTile(C.Structure):
@@ -375,6 +376,7 @@ PROTOTYPES = {
     "rtc8x_grid_build_fbm": (_S, [C.POINTER(FbmDesc), C.POINTER(Grid), _P]),
     "rtc8x_grid_upload": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
     "rtc8x_grid_destroy": (_S, [C.POINTER(Grid)]),
+    "rtc8x_grid_build_regions": (_S, [C.POINTER(Grid), C.c_uint32]),
     "rtc8x_grid_probe": (_S, [C.POINTER(Grid), C.c_uint32, _P, _P, _P, _P, _P]),
     "rtc8x_nvdb_list": (_S, [_P, C.c_size_t, C.POINTER(NvdbGridInfo), C.c_uint32, C.POINTER(C.c_uint32)]),
     "rtc8x_nanovdb_decode": (_S, [_P, C.c_size_t, C.POINTER(Grid)]),

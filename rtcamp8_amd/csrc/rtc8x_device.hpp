@@ -338,15 +338,27 @@ struct Dda {
         int32_t b = origin + size * (c + (st > 0 ? 1 : 0));
         return ((float)b - io) * iv;
     }
-    static __device__ __forceinline__ void axis_init(float t, float io, float id, int32_t origin, int32_t size,
-                                                     int32_t dims, int32_t& c, int32_t& st, float& iv, float& tn)
+    // the region holding the ray at t (clamped to the grid) and the axis' next boundary
+    static __device__ __forceinline__ void axis_cell(float t, float io, float id, float iv, int32_t st, int32_t origin,
+                                                     int32_t size, int32_t dims, int32_t& c, float& tn)
     {
         float pa = fmaf(t, id, io);
         float cf = floorf((pa - (float)origin) * (1.0f / (float)size));
         c = cf < 0.0f ? 0 : (cf > (float)(dims - 1) ? dims - 1 : (int32_t)cf);
+        tn = st != 0 ? boundary_of(origin, size, c, st, io, iv) : __builtin_huge_valf();
+    }
+    static __device__ __forceinline__ void axis_init(float t, float io, float id, int32_t origin, int32_t size,
+                                                     int32_t dims, int32_t& c, int32_t& st, float& iv, float& tn)
+    {
         st = id > 0.0f ? 1 : (id < 0.0f ? -1 : 0);
         iv = 1.0f / id;
-        tn = st != 0 ? boundary_of(origin, size, c, st, io, iv) : __builtin_huge_valf();
+        axis_cell(t, io, id, iv, st, origin, size, dims, c, tn);
+    }
+    static __device__ __forceinline__ V3 index_dir(const rtc8x_grid& g, V3 d)
+    {
+        const float* mm = g.inv_mat;
+        return V3{fmaf(d.x, mm[0], fmaf(d.y, mm[1], d.z * mm[2])), fmaf(d.x, mm[3], fmaf(d.y, mm[4], d.z * mm[5])),
+                  fmaf(d.x, mm[6], fmaf(d.y, mm[7], d.z * mm[8]))};
     }
     __device__ __forceinline__ void init(const rtc8x_grid& g, V3 o, V3 d, float t)
     {
@@ -355,13 +367,21 @@ struct Dda {
         io0 = fmaf(x, mm[0], fmaf(y, mm[1], z * mm[2]));
         io1 = fmaf(x, mm[3], fmaf(y, mm[4], z * mm[5]));
         io2 = fmaf(x, mm[6], fmaf(y, mm[7], z * mm[8]));
-        const float id0 = fmaf(d.x, mm[0], fmaf(d.y, mm[1], d.z * mm[2]));
-        const float id1 = fmaf(d.x, mm[3], fmaf(d.y, mm[4], d.z * mm[5]));
-        const float id2 = fmaf(d.x, mm[6], fmaf(d.y, mm[7], d.z * mm[8]));
+        const V3 id = index_dir(g, d);
         const int32_t size = region_size(g);
-        axis_init(t, io0, id0, g.cell_origin[0], size, g.region_dims[0], c0, s0, iv0, tn0);
-        axis_init(t, io1, id1, g.cell_origin[1], size, g.region_dims[1], c1, s1, iv1, tn1);
-        axis_init(t, io2, id2, g.cell_origin[2], size, g.region_dims[2], c2, s2, iv2, tn2);
+        axis_init(t, io0, id.x, g.cell_origin[0], size, g.region_dims[0], c0, s0, iv0, tn0);
+        axis_init(t, io1, id.y, g.cell_origin[1], size, g.region_dims[1], c1, s1, iv1, tn1);
+        axis_init(t, io2, id.z, g.cell_origin[2], size, g.region_dims[2], c2, s2, iv2, tn2);
+    }
+    // init(g, o, d, t) for the same ray at a new t, bit for bit, without the divisions: the index-space
+    // origin, steps and reciprocals are those init computed; only the regions and boundaries are redone.
+    __device__ __forceinline__ void jump(const rtc8x_grid& g, V3 d, float t)
+    {
+        const V3 id = index_dir(g, d);
+        const int32_t size = region_size(g);
+        axis_cell(t, io0, id.x, iv0, s0, g.cell_origin[0], size, g.region_dims[0], c0, tn0);
+        axis_cell(t, io1, id.y, iv1, s1, g.cell_origin[1], size, g.region_dims[1], c1, tn1);
+        axis_cell(t, io2, id.z, iv2, s2, g.cell_origin[2], size, g.region_dims[2], c2, tn2);
     }
     __device__ __forceinline__ int axis() const
     {

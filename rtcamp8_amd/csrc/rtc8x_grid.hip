@@ -9,7 +9,8 @@
 //   leaf_values     num_leaves x 512 x f32    (2 KiB per leaf, NanoVDB voxel order x-major), + 1 f32 padding
 //   tile_values     one f32 per tile value, + 1 f32 padding (lookups read voxel pairs z, z+1 unconditionally)
 //   leaf_origins    num_leaves x 3 x i32
-//   cell_majorants  one f32 per 8^3 cell of the index bbox
+//   cell_majorants  one f32 per 8^3 cell of the index bbox (largest voxel over the cell and a one-voxel rim)
+//   region_majorants one byte per region of region_cells^3 cells (quantised majorant or empty-space distance)
 //   leaf_map        one u32 per 8^3 block of the index bbox plus a one-block margin (the tree walk's result)
 #include <hip/hip_runtime.h>
 
@@ -133,6 +134,30 @@ __global__ void leaf_map_kernel(const rtc8x_grid g, uint32_t* __restrict__ map)
                        g.leaf_map_origin[2] + 8 * (int32_t)bz);
 }
 
+// One wavefront per 8^3 cell c (cell_origin + 8c): the largest voxel value, through the tree (leaves,
+// tiles and background alike, never below 0), over index [8c - 1, 8c + 9]^3.  A point of the cell reads
+// its trilinear taps from [8c, 8c + 8]; the rim of one more voxel on every side covers a sample position
+// that rounding puts just outside the cell's span of the ray.
+constexpr int32_t kCellSpan = 11;
+__global__ __launch_bounds__(256) void cell_max_kernel(const rtc8x_grid g, float* __restrict__ out)
+{
+    const uint32_t dy = (uint32_t)g.cell_dims[1], dz = (uint32_t)g.cell_dims[2];
+    const size_t n = (size_t)g.cell_dims[0] * dy * dz;
+    const size_t cell = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (cell >= n) return;  // uniform over the wavefront
+    const uint32_t cz = (uint32_t)(cell % dz), cy = (uint32_t)((cell / dz) % dy), cx = (uint32_t)(cell / ((size_t)dz * dy));
+    const int32_t x0 = g.cell_origin[0] + 8 * (int32_t)cx - 1, y0 = g.cell_origin[1] + 8 * (int32_t)cy - 1,
+                  z0 = g.cell_origin[2] + 8 * (int32_t)cz - 1;
+    float m = 0.0f;
+    for (int32_t v = (int32_t)lane; v < kCellSpan * kCellSpan * kCellSpan; v += 64) {
+        const int32_t i = x0 + v / (kCellSpan * kCellSpan), j = y0 + (v / kCellSpan) % kCellSpan, k = z0 + v % kCellSpan;
+        m = fmaxf(m, *voxel_ptr(g, block_entry(g, i, j, k), i, j, k));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0) out[cell] = m;
+}
+
 __global__ void probe_kernel(const rtc8x_grid g, uint32_t n, const float* __restrict__ xyz, int32_t* __restrict__ out_ijk,
                              uint32_t* __restrict__ out_off, float* __restrict__ out_val)
 {
@@ -163,49 +188,19 @@ struct HostTables {
     uint32_t num_lower = 0, num_leaves = 0;
 };
 
-// 3x3x3-dilated per-cell majorants from per-leaf maxima (leaf origins in index space).
-void cell_majorants(const std::vector<int32_t>& origins, const std::vector<float>& leaf_max, const int32_t bbmin[3],
-                    const int32_t bbmax[3], int32_t cell_origin[3], int32_t cell_dims[3], std::vector<float>& out)
+// Region table from the cell majorants (the layout rtc8x.h defines for rtc8x_grid): region_cells as
+// requested (0: RTC8X_DEFAULT_REGION_CELLS, doubled until there are at most RTC8X_MAX_REGIONS regions);
+// a unit of (largest cell majorant) / 127 raised until 127 units cover it; per region with density the
+// least q whose q units cover its largest cell, per empty region RTC8X_REGION_EMPTY + the number of empty
+// layers around it (a Chebyshev distance transform by repeated 26-neighbour dilation of the regions with
+// density).
+rtc8x_status region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3], uint32_t requested,
+                              int32_t region_dims[3], uint32_t* region_cells, float* region_unit,
+                              std::vector<uint8_t>& out)
 {
-    int32_t lo[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = (int32_t)std::floor((float)bbmin[a] / 8.0f);
-        int32_t hi = (int32_t)std::floor((float)bbmax[a] / 8.0f);
-        cell_dims[a] = hi - lo[a] + 1;
-        cell_origin[a] = lo[a] * 8;
-    }
-    const int32_t px = cell_dims[0] + 2, py = cell_dims[1] + 2, pz = cell_dims[2] + 2;
-    std::vector<float> lm((size_t)px * py * pz, 0.0f);
-    const size_t nleaves = leaf_max.size();
-    for (size_t l = 0; l < nleaves; ++l) {
-        int32_t cx = origins[3 * l] / 8 - lo[0] + 1, cy = origins[3 * l + 1] / 8 - lo[1] + 1,
-                cz = origins[3 * l + 2] / 8 - lo[2] + 1;
-        if (cx < 0 || cy < 0 || cz < 0 || cx >= px || cy >= py || cz >= pz) continue;
-        float m = std::max(0.0f, leaf_max[l]);
-        float& slot = lm[((size_t)cx * py + cy) * pz + cz];
-        slot = std::max(slot, m);
-    }
-    out.assign((size_t)cell_dims[0] * cell_dims[1] * cell_dims[2], 0.0f);
-    for (int32_t x = 0; x < cell_dims[0]; ++x)
-        for (int32_t y = 0; y < cell_dims[1]; ++y)
-            for (int32_t z = 0; z < cell_dims[2]; ++z) {
-                float m = 0.0f;
-                for (int dx = 0; dx < 3; ++dx)
-                    for (int dy = 0; dy < 3; ++dy)
-                        for (int dz = 0; dz < 3; ++dz) m = std::max(m, lm[((size_t)(x + dx) * py + (y + dy)) * pz + (z + dz)]);
-                out[((size_t)x * cell_dims[1] + y) * cell_dims[2] + z] = m;
-            }
-}
-
-// Region table from the cell majorants (the layout rtc8x.h defines for rtc8x_grid): the least power-of-two
-// region_cells with at most RTC8X_MAX_REGIONS regions; a unit of (largest cell majorant) / 127 raised until
-// 127 units cover it; per region with density the least q whose q units cover its largest cell, per empty
-// region RTC8X_REGION_EMPTY + the number of empty layers around it (a Chebyshev distance transform by
-// repeated 26-neighbour dilation of the regions with density).
-void region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3], int32_t region_dims[3],
-                      uint32_t* region_cells, float* region_unit, std::vector<uint8_t>& out)
-{
-    uint32_t rc = 1;
+    RTC8X_REQUIRE(requested <= 4096 && (requested & (requested - 1)) == 0,
+                  "region_cells %u must be 0 or a power of two up to 4096", requested);
+    uint32_t rc = requested ? requested : RTC8X_DEFAULT_REGION_CELLS;
     for (;;) {
         size_t n = 1;
         for (int a = 0; a < 3; ++a) {
@@ -213,6 +208,8 @@ void region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3
             n *= (size_t)region_dims[a];
         }
         if (n <= RTC8X_MAX_REGIONS) break;
+        RTC8X_REQUIRE(requested == 0, "region_cells %u gives %zu regions, more than %u", requested, n,
+                      (unsigned)RTC8X_MAX_REGIONS);
         rc *= 2;
     }
     const int32_t rx = region_dims[0], ry = region_dims[1], rz = region_dims[2];
@@ -272,6 +269,7 @@ void region_majorants(const std::vector<float>& cells, const int32_t cell_dims[3
     }
     *region_cells = rc;
     *region_unit = unit;
+    return RTC8X_OK;
 }
 
 template <typename T>
@@ -317,10 +315,88 @@ rtc8x_status build_leaf_map(rtc8x_grid& g, hipStream_t s)
     return RTC8X_OK;
 }
 
+// The cell majorants of a device grid (cell grid over the index bbox, from cell_max_kernel) and its region
+// table (region_cells as region_majorants takes it).  Grids without leaves, or whose index bbox spans more
+// than 2^26 cells, get neither (local-majorant tracking then refuses them).
+rtc8x_status build_majorant_tables(rtc8x_grid& g, uint32_t region_cells, hipStream_t s)
+{
+    g.cell_majorants = nullptr;
+    g.region_majorants = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        g.cell_origin[a] = g.cell_dims[a] = 0;
+        g.region_dims[a] = 0;
+    }
+    g.region_cells = 0;
+    g.region_unit = 0.0f;
+    if (g.num_leaves == 0) return RTC8X_OK;
+    size_t n = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int32_t lo = (int32_t)std::floor((float)g.index_bbox_min[a] / 8.0f);
+        const int32_t hi = (int32_t)std::floor((float)g.index_bbox_max[a] / 8.0f);
+        g.cell_origin[a] = lo * 8;
+        g.cell_dims[a] = hi - lo + 1;
+        n *= (size_t)g.cell_dims[a];
+    }
+    if (n > ((size_t)1 << 26)) {
+        for (int a = 0; a < 3; ++a) g.cell_origin[a] = g.cell_dims[a] = 0;
+        return RTC8X_OK;
+    }
+    float* d_cm = nullptr;
+    RTC8X_HIP(hipMalloc((void**)&d_cm, n * sizeof(float)));
+    g.cell_majorants = d_cm;  // owned by g from here on
+    hipLaunchKernelGGL(dev::cell_max_kernel, dim3((unsigned)((n * 64 + 255) / 256)), dim3(256), 0, s, g, d_cm);
+    RTC8X_HIP(hipGetLastError());
+    std::vector<float> cm(n);
+    RTC8X_HIP(hipMemcpyAsync(cm.data(), d_cm, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    RTC8X_HIP(hipStreamSynchronize(s));
+    std::vector<uint8_t> rm;
+    int32_t rdims[3];
+    uint32_t rc = 0;
+    float unit = 0.0f;
+    RTC8X_TRY(region_majorants(cm, g.cell_dims, region_cells, rdims, &rc, &unit, rm));
+    uint8_t* d_rm = nullptr;
+    RTC8X_TRY(upload_vec(rm, &d_rm));
+    g.region_majorants = d_rm;
+    for (int a = 0; a < 3; ++a) g.region_dims[a] = rdims[a];
+    g.region_cells = rc;
+    g.region_unit = unit;
+    return RTC8X_OK;
+}
+
 }  // namespace
 }  // namespace rtc8x
 
 using namespace rtc8x;
+
+extern "C" rtc8x_status rtc8x_grid_build_regions(rtc8x_grid* g, uint32_t region_cells)
+try {
+    RTC8X_REQUIRE(g != nullptr, "rtc8x_grid_build_regions: grid is NULL");
+    const size_t n = (size_t)std::max(g->cell_dims[0], 0) * std::max(g->cell_dims[1], 0) * std::max(g->cell_dims[2], 0);
+    RTC8X_REQUIRE(g->cell_majorants && n > 0, "rtc8x_grid_build_regions: the grid has no cell majorants");
+    std::vector<float> cm(n);
+    RTC8X_HIP(hipDeviceSynchronize());
+    RTC8X_HIP(hipMemcpy(cm.data(), g->cell_majorants, n * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> rm;
+    int32_t rdims[3];
+    uint32_t rc = 0;
+    float unit = 0.0f;
+    RTC8X_TRY(region_majorants(cm, g->cell_dims, region_cells, rdims, &rc, &unit, rm));
+    uint8_t* d_rm = nullptr;
+    RTC8X_TRY(upload_vec(rm, &d_rm));
+    if (g->region_majorants) {
+        const hipError_t e = hipFree(const_cast<uint8_t*>(g->region_majorants));
+        if (e != hipSuccess) {
+            (void)hipFree(d_rm);
+            RTC8X_HIP(e);
+        }
+    }
+    g->region_majorants = d_rm;
+    for (int a = 0; a < 3; ++a) g->region_dims[a] = rdims[a];
+    g->region_cells = rc;
+    g->region_unit = unit;
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_grid_build_regions")
 
 extern "C" rtc8x_status rtc8x_grid_destroy(rtc8x_grid* g)
 try {
@@ -437,14 +513,6 @@ try {
     tile.state = 0;
     std::vector<rtc8x_root_tile> root(1, tile);
     std::vector<float> tile_values(2, 0.0f);  // the background tile and the lookup's padding float
-    std::vector<float> cm;
-    int32_t corg[3], cdims[3];
-    cell_majorants(origins, leaf_max, bbmin, bbmax, corg, cdims, cm);
-    std::vector<uint8_t> rm;
-    int32_t rdims[3];
-    uint32_t rcells = 0;
-    float runit = 0.0f;
-    region_majorants(cm, cdims, rdims, &rcells, &runit, rm);
 
     rtc8x_grid g;
     std::memset(&g, 0, sizeof(g));
@@ -456,12 +524,7 @@ try {
         g.index_bbox_max[a] = bbmax[a];
         g.world_bbox_min[a] = (float)bbmin[a] / (float)n;
         g.world_bbox_max[a] = (float)(bbmax[a] + 1) / (float)n;
-        g.cell_origin[a] = corg[a];
-        g.cell_dims[a] = cdims[a];
-        g.region_dims[a] = rdims[a];
     }
-    g.region_cells = rcells;
-    g.region_unit = runit;
     g.max_value = vmax;
     g.background = 0.0f;
     g.num_root_tiles = 1;
@@ -471,7 +534,7 @@ try {
     rtc8x_root_tile* d_root;
     uint32_t *d_upper, *d_lower;
     int32_t* d_origins;
-    float *d_tiles, *d_cm, *d_values;
+    float *d_tiles, *d_values;
     RTC8X_TRY(upload_vec(root, &d_root));
     g.root_tiles = d_root;
     RTC8X_TRY(upload_vec(upper, &d_upper));
@@ -482,11 +545,6 @@ try {
     g.leaf_origins = d_origins;
     RTC8X_TRY(upload_vec(tile_values, &d_tiles));
     g.tile_values = d_tiles;
-    RTC8X_TRY(upload_vec(cm, &d_cm));
-    g.cell_majorants = d_cm;
-    uint8_t* d_rm;
-    RTC8X_TRY(upload_vec(rm, &d_rm));
-    g.region_majorants = d_rm;
     RTC8X_HIP(hipMalloc(&d_values, ((size_t)num_leaves * 512 + 1) * sizeof(float)));
     RTC8X_HIP(hipMemsetAsync(d_values + (size_t)num_leaves * 512, 0, sizeof(float), s));  // lookup padding
     g.leaf_values = d_values;
@@ -494,6 +552,7 @@ try {
     RTC8X_HIP(hipGetLastError());
     RTC8X_HIP(hipStreamSynchronize(s));
     RTC8X_TRY(build_leaf_map(g, s));
+    RTC8X_TRY(build_majorant_tables(g, 0, s));
     *out = g;
     return RTC8X_OK;
 }
@@ -550,34 +609,14 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
             }
         }
     }
-    std::vector<float> leaf_max(h->num_leaves, 0.0f);
     float vmax = -FLT_MAX;
-    for (uint32_t l = 0; l < h->num_leaves; ++l) {
-        float m = -FLT_MAX;
-        for (uint32_t v = 0; v < 512; ++v) m = std::max(m, h->leaf_values[(size_t)l * 512 + v]);
-        leaf_max[l] = m;
-        vmax = std::max(vmax, m);
-    }
+    for (size_t v = 0; v < (size_t)h->num_leaves * 512; ++v) vmax = std::max(vmax, h->leaf_values[v]);
     for (uint32_t t = 0; t < num_tiles_values; ++t) vmax = std::max(vmax, h->tile_values[t]);
     RTC8X_REQUIRE(std::isfinite(vmax), "rtc8x_grid_upload: non-finite voxel values");
     RTC8X_REQUIRE(!keep_max || std::isfinite(h->max_value), "rtc8x_grid_upload: non-finite majorant");
 
     rtc8x_grid g = *h;
     g.max_value = keep_max ? h->max_value : vmax;
-    std::vector<float> cm;
-    std::vector<uint8_t> rm;
-    for (int a = 0; a < 3; ++a) g.region_dims[a] = 0;
-    g.region_cells = 0;
-    g.region_unit = 0.0f;
-    if (h->num_leaves > 0) {
-        int32_t corg[3], cdims[3];
-        cell_majorants(origins, leaf_max, h->index_bbox_min, h->index_bbox_max, corg, cdims, cm);
-        for (int a = 0; a < 3; ++a) {
-            g.cell_origin[a] = corg[a];
-            g.cell_dims[a] = cdims[a];
-        }
-        region_majorants(cm, cdims, g.region_dims, &g.region_cells, &g.region_unit, rm);
-    }
     std::vector<rtc8x_root_tile> root(h->root_tiles, h->root_tiles + h->num_root_tiles);
     std::vector<uint32_t> upper(h->upper_tables, h->upper_tables + (size_t)h->num_upper * RTC8X_UPPER_TABLE_SIZE);
     std::vector<uint32_t> lower(h->lower_tables, h->lower_tables + (size_t)h->num_lower * RTC8X_LOWER_TABLE_SIZE);
@@ -588,27 +627,25 @@ rtc8x_status upload_host_grid(const rtc8x_grid* h, rtc8x_grid* out, bool keep_ma
     rtc8x_root_tile* d_root;
     uint32_t *d_upper, *d_lower;
     int32_t* d_origins;
-    float *d_values, *d_tiles, *d_cm;
+    float *d_values, *d_tiles;
     RTC8X_TRY(upload_vec(root, &d_root));
     RTC8X_TRY(upload_vec(upper, &d_upper));
     RTC8X_TRY(upload_vec(lower, &d_lower));
     RTC8X_TRY(upload_vec(values, &d_values));
     RTC8X_TRY(upload_vec(tiles, &d_tiles));
     RTC8X_TRY(upload_vec(origins, &d_origins));
-    RTC8X_TRY(upload_vec(cm, &d_cm));
-    uint8_t* d_rm;
-    RTC8X_TRY(upload_vec(rm, &d_rm));
-    g.region_majorants = d_rm;
     g.root_tiles = d_root;
     g.upper_tables = d_upper;
     g.lower_tables = d_lower;
     g.leaf_values = d_values;
     g.tile_values = d_tiles;
     g.leaf_origins = d_origins;
-    g.cell_majorants = d_cm;
+    g.cell_majorants = nullptr;
+    g.region_majorants = nullptr;
     g.leaf_map = nullptr;
     g.leaf_masks = g.lower_masks = g.upper_masks = nullptr;  // host-side activity is not uploaded
     if (h->num_root_tiles > 0 && h->num_leaves > 0) RTC8X_TRY(build_leaf_map(g, nullptr));
+    RTC8X_TRY(build_majorant_tables(g, 0, nullptr));
     *out = g;
     return RTC8X_OK;
 }

@@ -33,6 +33,9 @@ constexpr uint32_t kHardMaxPathLength = 4096u;
 constexpr uint32_t kSpeculate = 2;
 // A wavefront with at least this many idle lanes evaluates its stepping lanes' segments together.
 constexpr uint32_t kCoopIdle = 48;
+// Regions a local-majorant lane walks per stepping pass before it yields to the wavefront's lookups.
+constexpr uint32_t kRegionBudget = 8;
+enum : int { kAdvanceEnd = 0, kAdvanceCollision = 1, kAdvanceBudget = 2 };
 constexpr uint32_t kCounterSlots = 1024;
 constexpr uint32_t kMaxBatch = 16;  // instances of one batched launch
 __device__ uint32_t g_work_counter[kCounterSlots];
@@ -76,8 +79,7 @@ struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
 template <>
 struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
     V3 o, d;
-    float t, t1, tau, mu_cell, tend;
-    int32_t axis, in_cell;
+    float t, t1, tau;
     Dda s;
     __device__ __forceinline__ bool init(const rtc8x_grid& g, const Medium& m, V3 ro, V3 rd, float tmax)
     {
@@ -88,48 +90,65 @@ struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
         o = ro;
         d = rd;
         s.init(g, ro, rd, t);
-        in_cell = 0;
         return true;
     }
     // first call after init draws tau (as track_local does right after the DDA set-up)
     __device__ __forceinline__ void first_tau(Pcg& rng) { tau = -rtc8x_logf(1.0f - rng.uni()); }
-    __device__ __forceinline__ bool advance(const rtc8x_grid& g, const Medium& m, Pcg&, float& mu,
-                                            const uint8_t* __restrict__ table)
+    // Walks the regions up to the next tentative collision: kAdvanceCollision with t and mu set,
+    // kAdvanceEnd when the segment ends first, kAdvanceBudget after kRegionBudget region visits without
+    // either (the walk resumes where it stopped on the next call, so every lane of a wavefront returns to
+    // the density lookups after a bounded number of region steps).  A visit derives everything it needs
+    // (the region's code and majorant, the axis and end of its span) from the walk's position, so a call
+    // that resumes inside a region after a null collision recomputes them instead of keeping them.
+    __device__ __forceinline__ int advance(const rtc8x_grid& g, const Medium& m, Pcg&, float& mu,
+                                           const uint8_t* __restrict__ table)
     {
-        for (;;) {
-            if (!in_cell) {
-                const uint32_t code = region_code(g, s, table);
-                if (code > RTC8X_REGION_EMPTY) {
-                    // an empty region with k empty layers around it: leave their cube in one jump
-                    const float tx = s.skip_exit(g, (int32_t)(code - RTC8X_REGION_EMPTY));
-                    if (tx > t) {
-                        if (!(tx < t1)) return false;
-                        t = tx;
-                        s.init(g, o, d, t);
-                        continue;
+        int res = kAdvanceBudget;
+        float mu_r = 0.0f;
+#pragma unroll 1
+        for (uint32_t visit = 0; visit < kRegionBudget; ++visit) {
+            const uint32_t code = region_code(g, s, table);
+            if (code > RTC8X_REGION_EMPTY) {
+                // an empty region with k empty layers around it: leave their cube in one jump
+                const float tx = s.skip_exit(g, (int32_t)(code - RTC8X_REGION_EMPTY));
+                if (tx > t) {
+                    if (!(tx < t1)) {
+                        res = kAdvanceEnd;
+                        break;
                     }
+                    t = tx;
+                    s.jump(g, d, t);
+                    continue;
                 }
-                axis = s.axis();
-                const float tna = s.tnext(axis);
-                tend = tna < t1 ? tna : t1;
-                mu_cell = region_majorant(g, m, code);
-                in_cell = 1;
             }
-            if (mu_cell > 0.0f) {
-                float tc = t + tau / mu_cell;
-                if (tc < tend) {
-                    t = tc;
-                    mu = mu_cell;
-                    return true;
+            const int axis = s.axis();
+            const float tna = s.tnext(axis);
+            const float tend = tna < t1 ? tna : t1;
+            mu_r = region_majorant(g, m, code);
+            if (mu_r > 0.0f) {
+                // the optical depth left in the region decides; the division only for a collision
+                const float need = mu_r * (tend - t);
+                if (tau < need) {
+                    res = kAdvanceCollision;
+                    break;
                 }
-                tau = tau - mu_cell * (tend - t);
-                if (tau < 0.0f) tau = 0.0f;
+                tau = tau - need;
             }
-            if (!(s.tnext(axis) < t1)) return false;
+            if (!(tna < t1)) {
+                res = kAdvanceEnd;
+                break;
+            }
             t = tend;
-            if (!s.step(g, axis)) return false;
-            in_cell = 0;
+            if (!s.step(g, axis)) {
+                res = kAdvanceEnd;
+                break;
+            }
         }
+        if (res == kAdvanceCollision) {
+            t = t + tau / mu_r;
+            mu = mu_r;
+        }
+        return res;
     }
     __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu)
     {
@@ -860,14 +879,15 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
             if (L.mode == kModeTrack || L.mode == kModeShadow)
                 speculate_global<kSpeculate>(p.grid, med, sg, L, L.mode == kModeTrack);
         } else if (L.mode == kModeTrack || L.mode == kModeShadow) {
-            if (sg.advance(p.grid, med, L.rng, mu, region_table)) {
+            const int r = sg.advance(p.grid, med, L.rng, mu, region_table);
+            if (r == kAdvanceCollision) {
                 const float sigma = sample_density(p.grid, med, sg.o, sg.d, sg.t);
                 if (L.mode == kModeTrack) {
                     if (sg.after_track(L.rng, sigma, mu)) L.mode = kModeScatter;
                 } else {
                     sg.after_shadow(L.rng, sigma, mu, L.vis);
                 }
-            } else {
+            } else if (r == kAdvanceEnd) {
                 L.mode = L.mode == kModeTrack ? kModeEscape : kModeNeeDone;
             }
         }
@@ -901,8 +921,13 @@ inline uint32_t take_counter_slots()
 }
 
 // One launch of one image.
+// Local tracking keeps a region walk per lane on top of the lane state: the render entries are held to
+// three wavefronts per SIMD (168 VGPRs, a few spilled); the training entry, whose lane state is larger,
+// would spill too much and keeps its registers.
 template <int ENTRY, int TRACKING>
-__global__ __launch_bounds__(256) void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT && ENTRY != RTC8X_ENTRY_GENERATE_TRAINING_DATA ? 3 : 1)))
+void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
                                                         uint32_t launch_h, uint32_t num_samples, uint32_t slot,
                                                         const uint32_t* __restrict__ record_offsets,
                                                         uint32_t* __restrict__ record_counts, const TrainStage stage)

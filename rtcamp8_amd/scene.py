@@ -75,6 +75,7 @@ class SceneConfig:
     camera_target: tuple = (0.5, 0.5, 0.5)
     fov_y_deg: float = 34.0
     tracking: int = _abi.TRACKING_GLOBAL_MAJORANT
+    region_cells: int = 0               # local-majorant region size in 8^3 cells (0: the library's default)
     rng_seed: int = REFERENCE_RNG_SEED
     volume_file: str = ""               # a .nvdb file to render instead of the procedural volume
     volume_grid: int = 0                # which grid of the file (nanovdb::io::readGrid's n)
@@ -199,6 +200,8 @@ class DeviceScene:
         else:
             self.grid = _abi.Grid()
             call("rtc8x_grid_build_fbm", C.byref(cfg.fbm.desc()), C.byref(self.grid), C.c_void_p(stream))
+        if cfg.region_cells:
+            call("rtc8x_grid_build_regions", C.byref(self.grid), cfg.region_cells)
         self.env = _abi.EnvLight()
         self._env_buffers = []
         self.env.power_coeff = 1.0

@@ -47,7 +47,8 @@ def load_oracle(path) -> C.CDLL:
         "oracle_fbm_value": (C.c_float, [C.POINTER(_abi.FbmDesc), C.c_int32, C.c_int32, C.c_int32]),
         "oracle_grid_get_value": (C.c_float, [C.POINTER(_abi.Grid), C.c_int32, C.c_int32, C.c_int32, _P]),
         "oracle_grid_probe": (None, [C.POINTER(_abi.Grid), C.c_uint32, _P, _P, _P, _P]),
-        "oracle_compute_cell_majorants": (None, [C.POINTER(_abi.Grid)]),
+        "oracle_compute_cell_majorants": (None, [C.POINTER(_abi.Grid), C.c_uint32]),
+        "oracle_compute_region_majorants": (None, [C.POINTER(_abi.Grid), C.c_uint32]),
         "oracle_ray_clip": (C.c_int, [_P, _P, C.c_float, _P, _P, _P, _P]),
         "oracle_logf": (C.c_float, [C.c_float]),
         "oracle_sincosf": (None, [C.c_float, _P, _P]),

@@ -42,9 +42,10 @@ Z = 4.5  # standard errors allowed for a statistical check (two-sided p ~ 7e-6 e
 
 
 # ------------------------------------------------------------------ media
-def with_majorants(hg):
-    """The host grid with the oracle's own cell and region majorants (local-majorant tracking)."""
-    oracle().oracle_compute_cell_majorants(C.byref(hg.grid))
+def with_majorants(hg, region_cells: int = 0):
+    """The host grid with the oracle's own cell and region majorants (local-majorant tracking); region_cells
+    0: the default region size."""
+    oracle().oracle_compute_cell_majorants(C.byref(hg.grid), region_cells)
     g = hg.grid
     hg.arrays["cells"] = np.ctypeslib.as_array((C.c_float * int(np.prod(g.cell_dims[:]))).from_address(g.cell_majorants)).copy()
     hg.arrays["regions"] = np.ctypeslib.as_array((C.c_uint8 * int(np.prod(g.region_dims[:]))).from_address(g.region_majorants)).copy()
@@ -56,13 +57,13 @@ def with_majorants(hg):
 def homogeneous_medium():
     """Index space = world space (voxel size 1); 3 x 3 x 3 leaves of density 1 cover [0, 24)^3, except the
     voxel (23, 23, 23) = 4, which sets the grid maximum and with it the majorant to 4 x the density.  The
-    world box [4, 20]^3 is the part of the medium where every trilinear tap reads 1.  The local majorants
-    are 4 in the 8^3 cells whose 3 x 3 x 3 neighbourhood of leaves holds that voxel (every index >= 1) and
-    1 elsewhere, quantised up to q / 255 of 4."""
+    world box [4, 20]^3 is the part of the medium where every trilinear tap reads 1.  With one cell per
+    region, the local majorants are 4 in the 8^3 cell (2, 2, 2), whose rim-extended span [15, 25]^3 holds
+    that voxel, and 1 elsewhere (cell 1 spans [7, 17] per axis), quantised up to q / 127 of 4."""
     origins = np.array([(x, y, z) for x in (0, 8, 16) for y in (0, 8, 16) for z in (0, 8, 16)], np.int32)
     values = np.ones((27, 512), np.float32)
     values[-1, 511] = 4.0  # leaf (16, 16, 16), voxel (7, 7, 7)
-    return with_majorants(tables_from_leaves(origins, values, 1.0, (4.0, 4.0, 4.0, 20.0, 20.0, 20.0)))
+    return with_majorants(tables_from_leaves(origins, values, 1.0, (4.0, 4.0, 4.0, 20.0, 20.0, 20.0)), 1)
 
 
 def frame(grid: _abi.Grid, sigma: float, tracking: int = _abi.TRACKING_GLOBAL_MAJORANT) -> _abi.FrameParams:
@@ -77,16 +78,16 @@ def homogeneous_majorants(o, local: bool):
     """Per ray (along +x from x = 4 to 20): the majorant on [4, 8), [8, 16), [16, 20] (the three cells)."""
     if not local:
         return np.full((o.shape[0], 3), MU)
-    unit = np.float32(4.0) / np.float32(255.0)
-    while np.float32(255.0) * unit < np.float32(4.0):
+    unit = np.float32(4.0) / np.float32(127.0)
+    while np.float32(127.0) * unit < np.float32(4.0):
         unit = np.nextafter(unit, np.float32(np.inf))
     q1 = 1
     while np.float32(q1) * unit < np.float32(1.0):
         q1 += 1
-    lo, hi = float(np.float32(q1) * unit), float(np.float32(255) * unit)
-    inner = (o[:, 1] >= 8) & (o[:, 2] >= 8)  # cells y, z >= 1; the x cells 1, 2 then see the 4
+    lo, hi = float(np.float32(q1) * unit), float(np.float32(127) * unit)
+    inner = (o[:, 1] >= 16) & (o[:, 2] >= 16)  # cells y = z = 2; the x cell 2 then sees the 4
     m = np.full((o.shape[0], 3), lo)
-    m[inner, 1:] = hi
+    m[inner, 2] = hi
     return SIGMA * m
 
 
@@ -207,12 +208,13 @@ def check_tracking_against_brute_force_optical_depth(lib, f64: bool, tracking: i
 
 def check_local_tracking_across_empty_space(lib, f64: bool):
     """Local tracking that skips empty regions: a 3 x 3 x 3 block of leaves of density 1 at the centre of a
-    128^3 index box (two single leaves at opposite corners set the box), so that rays aimed at the block
+    128^3 index box (two single leaves at opposite corners set the box), regions of one 8^3 cell, so that
+    rays aimed at the block
     cross long stretches of empty regions (skipped in jumps of up to several regions) before they reach it.
     E[T] = e^(-tau) and P(escape) = e^(-tau) with tau from the brute-force integral."""
     origins = [(0, 0, 0), (120, 120, 120)] + [(x, y, z) for x in (56, 64, 72) for y in (56, 64, 72) for z in (56, 64, 72)]
     values = np.ones((len(origins), 512), np.float32)
-    hg = with_majorants(tables_from_leaves(np.array(origins, np.int32), values, 1.0 / 128, (0, 0, 0, 1, 1, 1)))
+    hg = with_majorants(tables_from_leaves(np.array(origins, np.int32), values, 1.0 / 128, (0, 0, 0, 1, 1, 1)), 1)
     codes = hg.arrays["regions"]
     assert (codes > 130).any()  # empty regions with at least two empty layers around them
     sigma = 8.0
@@ -440,19 +442,47 @@ def check_importance_map_definition(lib, w: int, h: int):
     assert abs(integral[0] / mip0.sum() - 1.0) < 1e-5
 
 
-def check_region_majorants(lib, res: int, sparse: bool):
-    """The region table of rtc8x.h's definition: regions of the least power-of-two size that keeps their
-    count <= 32768; for a region with density the least q in 1..127 with q x unit >= its largest cell
-    majorant; for an empty region 128 + (its chessboard distance to the nearest region with density) - 1,
-    the distance from scipy's chamfer transform."""
+def check_cell_majorants(hg):
+    """Cell majorants of rtc8x.h's definition, from a dense copy of the voxels: cell c's bound is the largest
+    voxel (at least 0; background 0) over index [8c - 1, 8c + 9] per axis."""
+    g = hg.grid
+    org = hg.arrays["origins"].reshape(-1, 3)
+    vals = hg.arrays["values"].reshape(-1, 8, 8, 8)
+    co, cd = np.array(g.cell_origin[:]), np.array(g.cell_dims[:])
+    lo = co - 8  # a block of background on every side
+    dense = np.zeros(tuple(8 * cd + 16), np.float32)
+    for n, (x, y, z) in enumerate(org - lo):
+        dense[x:x + 8, y:y + 8, z:z + 8] = vals[n]
+    cells = hg.arrays["cells"].reshape(cd)
+    for x in range(cd[0]):
+        for y in range(cd[1]):
+            for z in range(cd[2]):
+                b = dense[8 * x + 7:8 * x + 18, 8 * y + 7:8 * y + 18, 8 * z + 7:8 * z + 18]
+                assert cells[x, y, z] == max(np.float32(0), b.max()), (x, y, z, cells[x, y, z], b.max())
+
+
+def check_region_majorants(lib, res: int, sparse: bool, region_cells: int = 0):
+    """The region table of rtc8x.h's definition: regions of region_cells^3 cells (0: 4, doubled until there
+    are at most 32768 of them); for a region with density the least q in 1..127 with q x unit >= its largest
+    cell majorant; for an empty region 128 + (its chessboard distance to the nearest region with density) - 1,
+    the distance from scipy's chamfer transform.  The cell majorants follow their own definition."""
     from scipy.ndimage import distance_transform_cdt
 
     hg = oracle_build_grid(FbmConfig(resolution=res, seed=0x0456, sparse=sparse), lib)
+    if region_cells:
+        hg.grid.cell_majorants = hg.grid.region_majorants = None  # numpy's memory: the oracle must not free it
+        lib.oracle_compute_cell_majorants(C.byref(hg.grid), region_cells)
+        g = hg.grid
+        hg.arrays["cells"] = np.ctypeslib.as_array((C.c_float * int(np.prod(g.cell_dims[:]))).from_address(g.cell_majorants)).copy()
+        hg.arrays["regions"] = np.ctypeslib.as_array((C.c_uint8 * int(np.prod(g.region_dims[:]))).from_address(g.region_majorants)).copy()
+        g.cell_majorants = hg.arrays["cells"].ctypes.data
+        g.region_majorants = hg.arrays["regions"].ctypes.data
+    check_cell_majorants(hg)
     g = hg.grid
     cd = np.array(g.cell_dims[:])
     cells = hg.arrays["cells"].reshape(cd)
-    rc = 1
-    while np.prod(-(-cd // rc)) > _abi.MAX_REGIONS:
+    rc = region_cells or 4
+    while not region_cells and np.prod(-(-cd // rc)) > _abi.MAX_REGIONS:
         rc *= 2
     assert g.region_cells == rc and list(g.region_dims) == list(-(-cd // rc))
     rd = -(-cd // rc)
@@ -502,9 +532,9 @@ def test_local_tracking_across_empty_space(f64):
     check_local_tracking_across_empty_space(oracle(), f64)
 
 
-@pytest.mark.parametrize("res,sparse", [(64, False), (128, True), (256, True)])
-def test_region_majorants_match_their_definition(res, sparse):
-    check_region_majorants(oracle(), res, sparse)
+@pytest.mark.parametrize("res,sparse,region_cells", [(64, False, 0), (128, True, 0), (128, True, 1), (256, True, 2)])
+def test_region_majorants_match_their_definition(res, sparse, region_cells):
+    check_region_majorants(oracle(), res, sparse, region_cells)
 
 
 @pytest.mark.parametrize("f64", [False, True])
@@ -542,18 +572,21 @@ MUTATIONS = [
      lambda lib: check_ratio_tracking_homogeneous(lib, False)),
     ("rtc8_oracle_tracer.inc", "R vy = v000 + wz * (v001 - v000);", "R vy = v001 + wz * (v000 - v001);",
      lambda lib: check_tracking_against_brute_force_optical_depth(lib, False)),
-    ("rtc8_oracle_tracer.inc", "tau = tau - mu * (tend - t);", "tau = tau - mu * (tend - t) * (R)0.5;",
+    ("rtc8_oracle_tracer.inc", "R need = mu * (tend - t);", "R need = mu * (tend - t) * (R)0.5;",
      lambda lib: check_free_flight_homogeneous(lib, False, LOCAL)),
     ("rtc8_oracle_tracer.inc", "s->tnext[a] = s->step[a] != 0 ? SUF(dda_boundary)(s, a) : (R)INFINITY;",
      "s->tnext[a] = s->step[a] != 0 ? SUF(dda_boundary)(s, a) + (R)1 : (R)INFINITY;",
-     lambda lib: check_ratio_tracking_homogeneous(lib, False, LOCAL)),
-    # the cube's full width taken for its half-width (one layer too many is absorbed by the 3x3x3 dilation of
-    # the cell majorants, which is there to cover boundary rounding)
+     lambda lib: check_local_tracking_across_empty_space(lib, False)),
+    # the cube's full width taken for its half-width (one layer too many would jump into the block itself)
     ("rtc8_oracle_tracer.inc", "int32_t c = s->c[a] + s->step[a] * k;", "int32_t c = s->c[a] + s->step[a] * (2 * k + 1);",
      lambda lib: check_local_tracking_across_empty_space(lib, False)),
     ("rtc8_oracle_core.c", "size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + y / (int32_t)rc) * rd[2] + z / (int32_t)rc;",
      "size_t r = ((size_t)(x / (int32_t)rc) * rd[1] + z / (int32_t)rc) * rd[2] + y / (int32_t)rc;",
      lambda lib: check_region_majorants(lib, 128, True)),
+    # the rim forgotten on the far side: the taps at 8c + 8 and the rounding margin beyond
+    ("rtc8_oracle_core.c", "static const int32_t first[3] = {7, 0, 0}, last[3] = {7, 7, 1};",
+     "static const int32_t first[3] = {7, 0, 0}, last[3] = {7, 7, 0};",
+     lambda lib: check_region_majorants(lib, 128, True, 1)),
     ("rtc8_oracle_tracer.inc", "*dir_local = SUF(v3)(cp * sin_t, sp * sin_t, -cos_t);",
      "*dir_local = SUF(v3)(cp * sin_t, sp * sin_t, cos_t);",
      lambda lib: check_phase_sampling(lib, False, _abi.PHASE_HENYEY_GREENSTEIN, 0.85)),

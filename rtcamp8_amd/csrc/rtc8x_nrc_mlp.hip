@@ -604,11 +604,73 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
     }
 }
 
+// The hash tables' gradient without global atomics.  A level's entries are split in kScatterParts
+// ranges and the samples in up to kScatterSlices slices (of at least kScatterMinSlice samples, so that a
+// small batch still spreads over 256 blocks); block (level, range, slice) walks the slice's samples,
+// adds the trilinear contributions that land in its range to a copy of the range in LDS, and writes
+// the copy to the slice's partial table.  Every table float of every slice is written, so
+// grid_reduce_kernel sums the partials into the gradient without clearing it first.  (Global float
+// atomics, one per corner, feature and sample, ran at about 2e10 per second: 16.8 M of them made up
+// four fifths of a 65,536-sample step.)
+constexpr uint32_t kScatterParts = 4;
+constexpr uint32_t kScatterSlices = 4;
+constexpr uint32_t kScatterMinSlice = 4096;
+constexpr uint32_t kScatterRange = kHashT / kScatterParts;  // entries per range (at most)
+__device__ __forceinline__ uint32_t level_entries(const HashGeom& gm, uint32_t l, uint32_t p_grid)
+{
+    return ((l + 1 < kHashLevels ? gm.offset[l + 1] : p_grid) - gm.offset[l]) / RTC8X_NRC_HASH_FEATURES;
+}
+
+__global__ __launch_bounds__(256) void grid_scatter_kernel(const rtc8x_radiance_query* __restrict__ in, uint32_t n,
+                                                           const float* __restrict__ dE, const HashGeom geom,
+                                                           uint32_t p_grid, uint32_t slice_len, float* __restrict__ part)
+{
+    __shared__ float acc[2 * kScatterRange];
+    const uint32_t lv = blockIdx.x / kScatterParts, r = blockIdx.x % kScatterParts, slice = blockIdx.y;
+    const uint32_t entries = level_entries(geom, lv, p_grid);
+    const uint32_t per = (entries + kScatterParts - 1) / kScatterParts;
+    const uint32_t e0 = min(entries, r * per), e1 = min(entries, e0 + per);
+    for (uint32_t i = threadIdx.x; i < 2 * (e1 - e0); i += 256) acc[i] = 0.0f;
+    __syncthreads();
+    const uint32_t s1 = min(n, (slice + 1) * slice_len);
+    for (uint32_t sm = slice * slice_len + threadIdx.x; sm < s1; sm += 256) {
+        float p[3];
+        query_position01(in[sm], p);
+        uint32_t gi[3];
+        float f[3];
+        hash_cell(geom, lv, p, gi, f);
+        const float g0 = dE[(size_t)sm * 32 + 2 * lv], g1 = dE[(size_t)sm * 32 + 2 * lv + 1];
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            uint32_t idx;
+            float w;
+            hash_corner(geom, lv, gi, f, c, idx, w);
+            const uint32_t e = (idx - geom.offset[lv]) / 2u;
+            if (e >= e0 && e < e1) {
+                atomicAdd(&acc[2 * (e - e0)], w * g0);
+                atomicAdd(&acc[2 * (e - e0) + 1], w * g1);
+            }
+        }
+    }
+    __syncthreads();
+    float* out = part + (size_t)slice * p_grid + geom.offset[lv] + 2 * e0;
+    for (uint32_t i = threadIdx.x; i < 2 * (e1 - e0); i += 256) out[i] = acc[i];
+}
+
+__global__ __launch_bounds__(256) void grid_reduce_kernel(const float* __restrict__ part, uint32_t slices,
+                                                          uint32_t p_grid, float* __restrict__ ggrid)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p_grid) return;
+    float sum = part[i];
+    for (uint32_t h = 1; h < slices; ++h) sum += part[(size_t)h * p_grid + i];
+    ggrid[i] = sum;
+}
+
 template <typename T, int ENC>
 __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt, uint32_t L, uint32_t n,
                                                        const T* __restrict__ hbuf, const T* __restrict__ dout,
-                                                       float* __restrict__ gpart, const rtc8x_radiance_query* __restrict__ in,
-                                                       const HashGeom geom, float* __restrict__ ggrid)
+                                                       float* __restrict__ gpart, float* __restrict__ dEout)
 {
     using V = typename Vec8<T>::type;
     // FP32 keeps dZ only feature-major, so that the three tiles fit the LDS
@@ -691,24 +753,12 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
                     for (int i = 0; i < 4; ++i)
                         dE[(wave * 64 + mt * 16 + 4 * (lane >> 4) + i) * 32 + nt * 16 + r16] = acc[mt][nt][i];
             __syncthreads();
-            // scatter into the tables' gradient with the trilinear weights
-            if (b0 + tid < n) {
-                float p[3];
-                query_position01(in[b0 + tid], p);
-                for (uint32_t lv = 0; lv < kHashLevels; ++lv) {
-                    uint32_t gi[3];
-                    float f[3];
-                    hash_cell(geom, lv, p, gi, f);
-                    const float g0 = dE[tid * 32 + 2 * lv], g1 = dE[tid * 32 + 2 * lv + 1];
-#pragma unroll
-                    for (uint32_t c = 0; c < 8; ++c) {
-                        uint32_t idx;
-                        float w;
-                        hash_corner(geom, lv, gi, f, c, idx, w);
-                        atomicAdd(ggrid + idx, w * g0);
-                        atomicAdd(ggrid + idx + 1, w * g1);
-                    }
-                }
+            // the rows of dE go out for grid_scatter_kernel, 16 bytes per lane and store
+            for (uint32_t i = tid; i < 256 * 8; i += 256) {
+                const uint32_t sm = i >> 3;
+                if (b0 + sm < n)
+                    *reinterpret_cast<f4*>(dEout + (size_t)(b0 + sm) * 32 + 4 * (i & 7u)) =
+                        *reinterpret_cast<const f4*>(dE + sm * 32 + 4 * (i & 7u));
             }
             break;
         }
@@ -738,8 +788,14 @@ __global__ void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nbl
     if (i >= P_total) return;
     float g;
     if (i < P) {
-        g = 0.0f;
-        for (uint32_t b = 0; b < nblocks; ++b) g += gpart[(size_t)b * P + i];
+        // eight independent partial sums keep eight loads in flight; they are added in a fixed order
+        float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t b = 0; b < nblocks; b += 8) {
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k)
+                if (b + k < nblocks) acc[k] += gpart[(size_t)(b + k) * P + i];
+        }
+        g = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         grad[i] = g;
     } else {
         g = grad[i];
@@ -835,6 +891,8 @@ struct rtc8x_nrc {
     float *master = nullptr, *m = nullptr, *v = nullptr, *ema = nullptr, *grad = nullptr, *gpart = nullptr;
     float* loss_part = nullptr;
     float* grid_inf = nullptr;  // EMA (debiased) hash tables for inference
+    float* grid_de = nullptr;   // hash grid: the encoded input's gradient, 32 floats per training sample
+    float* grid_part = nullptr; // hash grid: the tables' gradient per slice of the batch (kScatterSlices)
     uint32_t* bad = nullptr;
     void *w = nullptr, *wt = nullptr, *winf = nullptr, *hbuf = nullptr, *dout = nullptr;
     uint8_t* w8 = nullptr;      // E4M3 inference weights (RTC8X_NRC_BF16_FP8)
@@ -875,14 +933,20 @@ template <typename T>
 hipError_t launch_backward(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n)
 {
     const uint32_t blocks = (n + 255) / 256;
-    if (h->enc == RTC8X_NRC_HASH_GRID)
-        hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, (const T*)h->wt,
-                           h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, (const rtc8x_radiance_query*)in, h->geom,
-                           h->grad + h->P_mlp);
-    else
+    if (h->enc != RTC8X_NRC_HASH_GRID) {
         hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_TRIANGLE_WAVE>), dim3(blocks), dim3(256), 0, s,
-                           (const T*)h->wt, h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart,
-                           (const rtc8x_radiance_query*)in, h->geom, (float*)nullptr);
+                           (const T*)h->wt, h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, (float*)nullptr);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, (const T*)h->wt,
+                       h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, h->grid_de);
+    const uint32_t slices = std::min(nrc::kScatterSlices, (n + nrc::kScatterMinSlice - 1) / nrc::kScatterMinSlice);
+    const uint32_t slice_len = (n + slices - 1) / slices;
+    hipLaunchKernelGGL(nrc::grid_scatter_kernel, dim3(nrc::kHashLevels * nrc::kScatterParts, slices), dim3(256), 0, s,
+                       (const rtc8x_radiance_query*)in, n, (const float*)h->grid_de, h->geom, h->P_grid, slice_len,
+                       h->grid_part);
+    hipLaunchKernelGGL(nrc::grid_reduce_kernel, dim3((h->P_grid + 255) / 256), dim3(256), 0, s,
+                       (const float*)h->grid_part, slices, h->P_grid, h->grad + h->P_mlp);
     return hipGetLastError();
 }
 
@@ -947,7 +1011,6 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
 rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const float* target, uint32_t n)
 {
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, true, s, in, n, nullptr, target)));
-    if (h->P_grid) RTC8X_HIP(hipMemsetAsync(h->grad + h->P_mlp, 0, sizeof(float) * h->P_grid, s));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_backward<T>(h, s, in, n)));
     RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->gpart, (n + 255) / 256,
@@ -962,7 +1025,7 @@ extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
 try {
     if (!h) return RTC8X_OK;
     void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout,
-                    h->grid_inf, h->w8, h->w8_inv};
+                    h->grid_inf, h->grid_de, h->grid_part, h->w8, h->w8_inv};
     for (void* p : ptrs)
         if (p) RTC8X_HIP(hipFree(p));
     delete h;
@@ -1001,6 +1064,8 @@ try {
               alloc((void**)&h->gpart, 4 * P_mlp * max_blocks) && alloc((void**)&h->loss_part, 4 * 4 * max_blocks) &&
               alloc((void**)&h->bad, 4) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
               alloc(&h->winf, h->tsize * P_mlp) && (h->P_grid == 0 || alloc((void**)&h->grid_inf, 4 * (size_t)h->P_grid)) &&
+              (h->P_grid == 0 || alloc((void**)&h->grid_de, 4 * 32 * (size_t)nrc::kMaxTrain)) &&
+              (h->P_grid == 0 || alloc((void**)&h->grid_part, 4 * (size_t)h->P_grid * nrc::kScatterSlices)) &&
               alloc(&h->hbuf, h->tsize * 64 * (size_t)(h->L + 1) * nrc::kMaxTrain) &&
               alloc(&h->dout, h->tsize * nrc::kOutPad * (size_t)nrc::kMaxTrain) &&
               (d->precision != RTC8X_NRC_BF16_FP8 ||

@@ -6,9 +6,10 @@ with alias tables, 1920x1080, up to 256 bounces, Russian roulette after bounce 8
 of the workload: 16 samples per pixel over the full image in one launch.
 
 Prints one JSON line: value = path vertices per second of the plain tracer (rays traced through the
-medium per second, all inputs resident in HBM), with the trace kernel's roofline (HIP-event kernel time,
-32 algorithmic bytes per trilinear density lookup) and the CPU oracle timed on the host's cores as a
-baseline.  Two NRC legs ride along: the standalone network step (2^16 training samples + 2^21 inference
+medium per second, all inputs resident in HBM) with local-majorant tracking (a walk over the grid's
+32^3-voxel regions, their majorants staged in LDS), with the trace kernel's roofline (HIP-event kernel
+time, 32 algorithmic bytes per trilinear density lookup) and the CPU oracle timed on the host's cores as
+a baseline.  The reference's own global-majorant tracking of the same frame is a leg beside it.  Two NRC legs ride along: the standalone network step (2^16 training samples + 2^21 inference
 queries, 5 hidden layers, FP16) giving nrc_queries_per_s, and NRC-integrated rendering of the same scene.
 """
 from __future__ import annotations
@@ -46,7 +47,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="sparse512")
-    ap.add_argument("--tracking", choices=["local", "global"], default="global")
+    ap.add_argument("--tracking", choices=["local", "global"], default="local")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nrc-steps", type=int, default=8, help="steps of each NRC leg (0 skips them)")
@@ -69,7 +70,8 @@ def workload_text(cfg, spp: int) -> str:
 
 
 def saturated_leg(args, stream, tracking: int) -> dict:
-    """The plain tracer on the same cloud with contrast 4 (a saturated interior at the majorant)."""
+    """The plain tracer on the same cloud with contrast 4 (a saturated interior at the majorant), with the
+    given tracking mode."""
     from rtcamp8_amd import scene as sc
     from rtcamp8_amd.device import EventTimer
     from rtcamp8_amd.tracer import PathTracer
@@ -100,13 +102,15 @@ def saturated_leg(args, stream, tracking: int) -> dict:
            "roofline_frac": c["lookups"] / steps * BYTES_PER_LOOKUP / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     pt.close()
     scene.close()
-    print(f"[bench] saturated cloud: {out['path_vertices_per_s']:.3g} vertices/s, {ms:.2f} ms per frame",
-          file=sys.stderr, flush=True)
+    out["tracking"] = "local" if tracking else "global"
+    print(f"[bench] saturated cloud ({out['tracking']} majorant): {out['path_vertices_per_s']:.3g} vertices/s, "
+          f"{ms:.2f} ms per frame", file=sys.stderr, flush=True)
     return out
 
 
 def traffic_from_profiles(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if there is one."""
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary (profiles/<kernel>_<workload>_hbm.json),
+    if there is one."""
     f = ROOT / "profiles" / f"{kernel}_{workload}_hbm.json"
     if not f.exists():
         return None
@@ -377,10 +381,12 @@ def latency_roofline(scene, params, lookups_per_launch: float, lookups_per_path:
     light = (C.c_float * 3)(*[float(v) for v in sc.sun_direction(scene.cfg)])
     stats = DeviceBuffer.zeros(16)
     cus = device_cus()
+    walk = ("the local tracker's region walk over the LDS-staged region table, one lookup per tentative collision"
+            if params.tracking else "delta tracking at the global majorant's spacing, 2 tentative collisions per lookup round")
     res = {"probe": "march probe: persistent lanes walk the frame's camera paths in 8x8 pixel tiles, "
-                    f"{scene.cfg.spp} per pixel (delta tracking at the majorant's spacing, 2 tentative collisions "
-                    "per lookup round, shadow segment towards the sun and a phase sample per collision, no vertex "
-                    "shading), a fixed lookup budget per lane totalling one frame's lookups"}
+                    f"{scene.cfg.spp} per pixel ({walk}, shadow segment towards the sun and a phase sample per "
+                    "collision, no vertex shading), a fixed lookup budget per lane totalling one frame's lookups",
+           "tracking": "local" if params.tracking else "global"}
     for w in waves:
         budget = max(1, int(lookups_per_launch / (cus * w * 256)))
         call("rtc8x_march_probe", None, C.byref(params), light, scene.cfg.spp, budget, w, C.c_void_p(stats.ptr))
@@ -534,6 +540,46 @@ def dense128_equal_rmse_leg(ref_spp: int, stream, spp: int = 16, crop_ref_spp: i
     return out
 
 
+def global_majorant_leg(scene, pt, spp: int, steps: int, stream) -> dict:
+    """The same frame traced with the reference's global-majorant tracking (optix_pathtracing_kernels.cu:98-126,
+    renderer_kernel_common.h:360-383), bit-exact with the oracle's restatement of it, and its own latency
+    roofline."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd.device import EventTimer
+
+    params = pt.params()
+    params.tracking = _abi.TRACKING_GLOBAL_MAJORANT
+    pt.trace(spp, params)
+    stream.synchronize()
+    pt.stats.fill(0)
+    timers = [EventTimer() for _ in range(steps)]
+    t0 = time.perf_counter()
+    for t in timers:
+        t.start(stream.handle)
+        pt.trace(spp, params)
+        t.stop(stream.handle)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [t.elapsed_ms() for t in timers]
+    for t in timers:
+        t.destroy()
+    c = pt.counters()
+    lookups_per_launch = c["lookups"] / steps
+    ms = float(np.mean(kernel_ms))
+    out = {"steps": steps, "ms_per_step": wall * 1e3 / steps, "kernel_ms_avg": ms,
+           "path_vertices_per_s": c["vertices"] / wall, "paths_per_s": c["paths"] / wall,
+           "vertices_per_path": c["vertices"] / max(c["paths"], 1), "lookups_per_path": c["lookups"] / max(c["paths"], 1),
+           "hbm_roofline_frac": lookups_per_launch * BYTES_PER_LOOKUP / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "kernel": "wavefront_kernel<PATH_TRACE, GLOBAL_MAJORANT>"}
+    out["latency_roofline"] = latency_roofline(scene, params, lookups_per_launch, out["lookups_per_path"],
+                                               c["lookups"] / sum(kernel_ms) * 1e3, out["paths_per_s"])
+    pt.stats.fill(0)
+    print(f"[bench] global-majorant tracking: {ms:.2f} ms per frame, {out['path_vertices_per_s']:.3g} vertices/s, "
+          f"{out['lookups_per_path']:.1f} lookups/path, {out['latency_roofline']['paths_frac']:.2f} of its latency "
+          "roofline", file=sys.stderr, flush=True)
+    return out
+
+
 def calibrate_copy(mib: int, steps: int) -> int:
     """HBM counter calibration: `steps` launches of the plain copy kernel over `mib` MiB."""
     from rtcamp8_amd._lib import call
@@ -637,22 +683,31 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("wavefront_kernel", cfg.name),
-            "kernel": "wavefront_kernel<PATH_TRACE, GLOBAL_MAJORANT>",
+            "traffic": traffic_from_profiles(f"wavefront_kernel_{args.tracking}", cfg.name),
+            "kernel": f"wavefront_kernel<PATH_TRACE, {args.tracking.upper()}_MAJORANT>",
             "kernel_ms_avg": float(np.mean(kernel_ms)),
             "algorithmic_bytes_per_launch": lookups_per_launch * BYTES_PER_LOOKUP,
         },
         "cpu_baseline": None,
     }
+    result["lookups_per_path"] = c["lookups"] / max(c["paths"], 1)
+    waves = (3, 4) if tracking == _abi.TRACKING_LOCAL_MAJORANT else (3, 8)
     lat = latency_roofline(scene, params, lookups_per_launch, c["lookups"] / max(c["paths"], 1),
-                           c["lookups"] / sum(kernel_ms) * 1e3, c["paths"] / wall)
+                           c["lookups"] / sum(kernel_ms) * 1e3, c["paths"] / wall, waves=waves)
     result["latency_roofline"] = lat
-    print(f"[bench] latency roofline: march probe {lat['lookups_per_s_3waves']:.3g} lookups/s at 3 waves/SIMD "
-          f"({lat['lookups_per_s_8waves']:.3g} at 8); trace at {lat['frac']:.2f} of it, paths at "
-          f"{lat['paths_frac']:.2f}", file=sys.stderr, flush=True)
+    print(f"[bench] latency roofline ({lat['tracking']} tracking): march probe {lat['lookups_per_s_3waves']:.3g} "
+          f"lookups/s at 3 waves/SIMD ({lat[f'lookups_per_s_{waves[1]}waves']:.3g} at {waves[1]}); trace at "
+          f"{lat['frac']:.2f} of it, paths at {lat['paths_frac']:.2f}", file=sys.stderr, flush=True)
+    if tracking == _abi.TRACKING_LOCAL_MAJORANT:
+        print("[bench] global-majorant leg ...", file=sys.stderr, flush=True)
+        glob = global_majorant_leg(scene, pt, spp, max(2, steps // 2), stream)
+        glob["local_speedup"] = glob["kernel_ms_avg"] / float(np.mean(kernel_ms))
+        result["global_majorant"] = glob
     if args.config == "sparse512":
-        print("[bench] saturated-cloud leg ...", file=sys.stderr, flush=True)
+        print("[bench] saturated-cloud legs ...", file=sys.stderr, flush=True)
         result["saturated_cloud"] = saturated_leg(args, stream, tracking)
+        if tracking == _abi.TRACKING_LOCAL_MAJORANT:
+            result["saturated_cloud_global_majorant"] = saturated_leg(args, stream, _abi.TRACKING_GLOBAL_MAJORANT)
     if args.nrc_steps > 0:
         print("[bench] NRC network leg ...", file=sys.stderr, flush=True)
         # the network's inputs are drawn over the active leaves of the dense 128^3 cloud

@@ -76,16 +76,33 @@ __global__ void gather_probe_kernel(const rtc8x_grid g, uint32_t steps, uint32_t
 __device__ uint32_t g_probe_counter;
 __global__ void reset_probe_kernel() { g_probe_counter = 0u; }
 
+template <int TRACKING>
 __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, uint32_t spp, uint32_t budget,
                                    unsigned long long* __restrict__ stats)
 {
     extern __shared__ float occupancy_limiter[];
+    const uint8_t* table = nullptr;
+    if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) {
+        // the region table in LDS, as the tracer stages it
+        __shared__ __align__(16) uint8_t regions[RTC8X_MAX_REGIONS];
+        const uint32_t n16 = ((uint32_t)p.grid.region_dims[0] * (uint32_t)p.grid.region_dims[1] *
+                                  (uint32_t)p.grid.region_dims[2] + 15u) / 16u;
+        const uint4* src = reinterpret_cast<const uint4*>(p.grid.region_majorants);
+        for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) reinterpret_cast<uint4*>(regions)[i] = src[i];
+        __syncthreads();
+        table = regions;
+    }
     const uint32_t W = p.image_width, H = p.image_height;
     const uint32_t tiles_x = (W + 7) / 8, tiles = tiles_x * ((H + 7) / 8);
     const uint32_t lane_id = threadIdx.x & 63;
     const float maj = p.density_coeff * p.grid.max_value;
     const float coeff = p.density_coeff;
     const float trace_len = 2.0f * p.world_radius;
+    dev::Medium med;
+    med.coeff = coeff;
+    med.majorant = maj;
+    med.lookups = 0;
+    dev::Seg<RTC8X_TRACKING_LOCAL_MAJORANT> sg;
     dev::Pcg rng;
     rng.state = 0;
     dev::V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
@@ -93,6 +110,21 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
     uint32_t mode = 3;  // 0: start a path, 1: tracking, 2: shadow, 3: needs a pixel
     uint32_t looked = 0, paths = 0, lx = 0, ly = 0, sample = 0;
     dev::V3 pos{0.0f, 0.0f, 0.0f}, dir{0.0f, 0.0f, 1.0f};
+    // a segment from so along sd: the global walk keeps (o, d, t, t1), the local one its region walk
+    auto start = [&](dev::V3 so, dev::V3 sd, float tmax) -> bool {
+        if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) {
+            if (!sg.init(p.grid, med, so, sd, tmax)) return false;
+            sg.first_tau(rng);
+            return true;
+        } else {
+            float t0;
+            if (!dev::clip(p.grid, so, sd, tmax, t0, t1)) return false;
+            o = so;
+            d = sd;
+            t = fmaxf(0.0f, t0);
+            return true;
+        }
+    };
     for (uint32_t iter = 0; iter < 8u * budget + 256u && looked < budget; ++iter) {
         if (mode == 3) {
             // the next pixel, one atomic per wavefront
@@ -118,11 +150,7 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
             dir = dev::normalize(dev::V3{dev::dot(dev::V3{M[0], M[1], M[2]}, ldir), dev::dot(dev::V3{M[3], M[4], M[5]}, ldir),
                                          dev::dot(dev::V3{M[6], M[7], M[8]}, ldir)});
             pos = dev::V3{p.camera.position[0], p.camera.position[1], p.camera.position[2]};
-            float t0;
-            if (dev::clip(p.grid, pos, dir, 1e10f, t0, t1)) {
-                o = pos;
-                d = dir;
-                t = fmaxf(0.0f, t0);
+            if (start(pos, dir, 1e10f)) {
                 mode = 1;
             } else {  // the camera ray misses the volume: the path ends at once
                 ++paths;
@@ -130,45 +158,58 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
             }
             continue;
         }
-        // two tentative collisions (a collision past the segment's end is looked up at the current
-        // position, as the tracer does); a tracking segment stops at the first real one
-        float tk[2], tl[2], u[2];
-        tk[0] = t - rtc8x_logf(1.0f - rng.uni()) / maj;
-        tk[1] = tk[0] - rtc8x_logf(1.0f - rng.uni()) / maj;
-        u[0] = rng.uni();
-        u[1] = rng.uni();
-        tl[0] = tk[0] > t1 ? t : tk[0];
-        tl[1] = tk[1] > t1 ? t : tk[1];
-        float sig[2];
-        dev::density_batch<2>(p.grid, coeff, o, d, tl, sig);
         uint32_t event = 0;  // 1: real collision at t, 2: segment end
-        for (int k = 0; k < 2 && event == 0; ++k) {
-            if (tk[k] > t1) {
+        if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) {
+            // the tracer's region walk: a bounded number of region visits, then the lookup of a
+            // tentative collision if one was reached
+            float mu = 0.0f;
+            const int r = sg.advance(p.grid, med, rng, mu, table);
+            if (r == dev::kAdvanceCollision) {
+                const float sig = dev::sample_density(p.grid, med, sg.o, sg.d, sg.t);
+                ++looked;
+                if (mode == 1) {
+                    if (sg.after_track(rng, sig, mu)) event = 1;
+                } else {
+                    sg.after_shadow(rng, sig, mu, vis);
+                }
+            } else if (r == dev::kAdvanceEnd) {
                 event = 2;
-                break;
             }
-            ++looked;
-            t = tk[k];
-            if (mode == 1) {
-                if (u[k] < sig[k] / maj) event = 1;
-            } else {
-                vis *= 1.0f - sig[k] / maj;
+            o = sg.o;
+            d = sg.d;
+            t = sg.t;
+        } else {
+            // two tentative collisions (a collision past the segment's end is looked up at the current
+            // position, as the tracer does); a tracking segment stops at the first real one
+            float tk[2], tl[2], u[2];
+            tk[0] = t - rtc8x_logf(1.0f - rng.uni()) / maj;
+            tk[1] = tk[0] - rtc8x_logf(1.0f - rng.uni()) / maj;
+            u[0] = rng.uni();
+            u[1] = rng.uni();
+            tl[0] = tk[0] > t1 ? t : tk[0];
+            tl[1] = tk[1] > t1 ? t : tk[1];
+            float sig[2];
+            dev::density_batch<2>(p.grid, coeff, o, d, tl, sig);
+            for (int k = 0; k < 2 && event == 0; ++k) {
+                if (tk[k] > t1) {
+                    event = 2;
+                    break;
+                }
+                ++looked;
+                t = tk[k];
+                if (mode == 1) {
+                    if (u[k] < sig[k] / maj) event = 1;
+                } else {
+                    vis *= 1.0f - sig[k] / maj;
+                }
             }
         }
         if (event == 1) {  // real collision: the shadow segment, ending at the collision
             pos = dev::V3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
             const dev::V3 lp{pos.x + trace_len * light.x, pos.y + trace_len * light.y, pos.z + trace_len * light.z};
-            const dev::V3 sd{-light.x, -light.y, -light.z};
-            float t0;
             vis = 1.0f;
             mode = 2;
-            if (dev::clip(p.grid, lp, sd, trace_len, t0, t1)) {
-                o = lp;
-                d = sd;
-                t = fmaxf(0.0f, t0);
-            } else {
-                event = 2;
-            }
+            if (!start(lp, dev::V3{-light.x, -light.y, -light.z}, trace_len)) event = 2;
         }
         if (event == 2) {
             bool ended = mode == 1;  // escaped
@@ -178,11 +219,7 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
                 dev::phase_sample(p.phase, p.scattering_forwardness, us0, us1, dl);
                 const dev::Frame fr = dev::make_frame(dev::V3{-dir.x, -dir.y, -dir.z});
                 dir = dev::from_local(fr, dl);
-                float t0;
-                if (dev::clip(p.grid, pos, dir, 1e10f, t0, t1)) {
-                    o = pos;
-                    d = dir;
-                    t = fmaxf(0.0f, t0);
+                if (start(pos, dir, 1e10f)) {
                     mode = 1;
                 } else {
                     ended = true;
@@ -208,18 +245,19 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
 
 // The dynamic LDS per 256-lane workgroup that admits `waves_per_simd` workgroups per CU and not one more
 // (a 256-lane workgroup puts one wave on each of the CU's 4 SIMDs).
-static rtc8x_status occupancy_lds(uint32_t waves_per_simd, size_t* lds_out, const char* who)
+static rtc8x_status occupancy_lds(uint32_t waves_per_simd, size_t* lds_out, const char* who, size_t static_lds = 0)
 {
     int dev = 0, lds_cu = 0, lds_block = 0;
     RTC8X_HIP(hipGetDevice(&dev));
     RTC8X_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     RTC8X_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    size_t lds = (size_t)lds_cu / waves_per_simd;
-    lds = lds > 1024 ? lds - 1024 : lds;
-    if (!(lds <= (size_t)lds_block && (size_t)lds_cu / lds == waves_per_simd))
+    // the workgroup's whole LDS (the kernel's static arrays and the dynamic part) sets the occupancy
+    size_t total = (size_t)lds_cu / waves_per_simd;
+    total = total > 1024 ? total - 1024 : total;
+    if (!(total > static_lds && total <= (size_t)lds_block && (size_t)lds_cu / total == waves_per_simd))
         return fail(RTC8X_ERR_INVALID_ARGUMENT, "%s: %u waves per SIMD cannot be set with %d bytes of LDS per CU", who,
                     waves_per_simd, lds_cu);
-    *lds_out = lds;
+    *lds_out = total - static_lds;
     return RTC8X_OK;
 }
 
@@ -280,9 +318,18 @@ try {
                   "rtc8x_march_probe: the grid must be on the device with a positive majorant");
     RTC8X_REQUIRE(spp > 0 && budget > 0 && budget <= (1u << 24), "rtc8x_march_probe: spp must be > 0, budget in [1, 2^24]");
     RTC8X_REQUIRE(waves_per_simd >= 2 && waves_per_simd <= 8, "rtc8x_march_probe: waves_per_simd must be in [2, 8]");
+    RTC8X_REQUIRE(p.tracking == RTC8X_TRACKING_GLOBAL_MAJORANT || p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT,
+                  "rtc8x_march_probe: unknown tracking mode %d", (int)p.tracking);
+    const bool local = p.tracking == RTC8X_TRACKING_LOCAL_MAJORANT;
+    if (local) {
+        const size_t nr = (size_t)std::max(p.grid.region_dims[0], 0) * std::max(p.grid.region_dims[1], 0) *
+                          std::max(p.grid.region_dims[2], 0);
+        RTC8X_REQUIRE(p.grid.region_majorants && nr > 0 && nr <= RTC8X_MAX_REGIONS && p.grid.region_cells > 0,
+                      "rtc8x_march_probe: local tracking needs the grid's region table");
+    }
     size_t lds = 0;
     {
-        const rtc8x_status st = occupancy_lds(waves_per_simd, &lds, "rtc8x_march_probe");
+        const rtc8x_status st = occupancy_lds(waves_per_simd, &lds, "rtc8x_march_probe", local ? RTC8X_MAX_REGIONS : 0);
         if (st != RTC8X_OK) return st;
     }
     const float n = std::sqrt(light_dir[0] * light_dir[0] + light_dir[1] * light_dir[1] + light_dir[2] * light_dir[2]);
@@ -295,8 +342,12 @@ try {
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(reset_probe_kernel, dim3(1), dim3(1), 0, s);
     RTC8X_HIP(hipGetLastError());
-    hipLaunchKernelGGL(march_probe_kernel, dim3((uint32_t)cus * waves_per_simd), dim3(256), lds, s, p, l, spp, budget,
-                       (unsigned long long*)stats);
+    if (local)
+        hipLaunchKernelGGL(march_probe_kernel<RTC8X_TRACKING_LOCAL_MAJORANT>, dim3((uint32_t)cus * waves_per_simd), dim3(256),
+                           lds, s, p, l, spp, budget, (unsigned long long*)stats);
+    else
+        hipLaunchKernelGGL(march_probe_kernel<RTC8X_TRACKING_GLOBAL_MAJORANT>, dim3((uint32_t)cus * waves_per_simd),
+                           dim3(256), lds, s, p, l, spp, budget, (unsigned long long*)stats);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }

@@ -33,18 +33,12 @@ constexpr uint32_t kHardMaxPathLength = 4096u;
 constexpr uint32_t kSpeculate = 2;
 // A wavefront with at least this many idle lanes evaluates its stepping lanes' segments together.
 constexpr uint32_t kCoopIdle = 48;
-// Regions a local-majorant lane walks per stepping pass before it yields to the wavefront's lookups.
-constexpr uint32_t kRegionBudget = 8;
-enum : int { kAdvanceEnd = 0, kAdvanceCollision = 1, kAdvanceBudget = 2 };
 constexpr uint32_t kCounterSlots = 1024;
 constexpr uint32_t kMaxBatch = 16;  // instances of one batched launch
 __device__ uint32_t g_work_counter[kCounterSlots];
 
 __global__ void reset_counter_kernel(uint32_t slot) { g_work_counter[slot] = 0u; }
 __global__ void reset_value_kernel(uint32_t* v) { *v = 0u; }
-
-template <int TRACKING>
-struct Seg;
 
 // Global majorant, the reference's loops (optix_pathtracing_kernels.cu:110-125,
 // renderer_kernel_common.h:371-382).
@@ -72,95 +66,6 @@ struct Seg<RTC8X_TRACKING_GLOBAL_MAJORANT> {
     // returns true when the tentative collision is real
     __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu) { return rng.uni() < sigma / mu; }
     __device__ __forceinline__ void after_shadow(Pcg&, float sigma, float mu, float& vis) { vis *= 1.0f - sigma / mu; }
-};
-
-// Local majorants over the grid's regions, the loops of track_local / ratio_local made resumable; the
-// region bytes come from `table` (a copy staged in LDS by the single-image kernel).
-template <>
-struct Seg<RTC8X_TRACKING_LOCAL_MAJORANT> {
-    V3 o, d;
-    float t, t1, tau;
-    Dda s;
-    __device__ __forceinline__ bool init(const rtc8x_grid& g, const Medium& m, V3 ro, V3 rd, float tmax)
-    {
-        float t0;
-        if (!(m.majorant > 0.0f)) return false;
-        if (!clip(g, ro, rd, tmax, t0, t1)) return false;
-        t = fmaxf(0.0f, t0);
-        o = ro;
-        d = rd;
-        s.init(g, ro, rd, t);
-        return true;
-    }
-    // first call after init draws tau (as track_local does right after the DDA set-up)
-    __device__ __forceinline__ void first_tau(Pcg& rng) { tau = -rtc8x_logf(1.0f - rng.uni()); }
-    // Walks the regions up to the next tentative collision: kAdvanceCollision with t and mu set,
-    // kAdvanceEnd when the segment ends first, kAdvanceBudget after kRegionBudget region visits without
-    // either (the walk resumes where it stopped on the next call, so every lane of a wavefront returns to
-    // the density lookups after a bounded number of region steps).  A visit derives everything it needs
-    // (the region's code and majorant, the axis and end of its span) from the walk's position, so a call
-    // that resumes inside a region after a null collision recomputes them instead of keeping them.
-    __device__ __forceinline__ int advance(const rtc8x_grid& g, const Medium& m, Pcg&, float& mu,
-                                           const uint8_t* __restrict__ table)
-    {
-        int res = kAdvanceBudget;
-        float mu_r = 0.0f;
-#pragma unroll 1
-        for (uint32_t visit = 0; visit < kRegionBudget; ++visit) {
-            const uint32_t code = region_code(g, s, table);
-            if (code > RTC8X_REGION_EMPTY) {
-                // an empty region with k empty layers around it: leave their cube in one jump
-                const float tx = s.skip_exit(g, (int32_t)(code - RTC8X_REGION_EMPTY));
-                if (tx > t) {
-                    if (!(tx < t1)) {
-                        res = kAdvanceEnd;
-                        break;
-                    }
-                    t = tx;
-                    s.jump(g, d, t);
-                    continue;
-                }
-            }
-            const int axis = s.axis();
-            const float tna = s.tnext(axis);
-            const float tend = tna < t1 ? tna : t1;
-            mu_r = region_majorant(g, m, code);
-            if (mu_r > 0.0f) {
-                // the optical depth left in the region decides; the division only for a collision
-                const float need = mu_r * (tend - t);
-                if (tau < need) {
-                    res = kAdvanceCollision;
-                    break;
-                }
-                tau = tau - need;
-            }
-            if (!(tna < t1)) {
-                res = kAdvanceEnd;
-                break;
-            }
-            t = tend;
-            if (!s.step(g, axis)) {
-                res = kAdvanceEnd;
-                break;
-            }
-        }
-        if (res == kAdvanceCollision) {
-            t = t + tau / mu_r;
-            mu = mu_r;
-        }
-        return res;
-    }
-    __device__ __forceinline__ bool after_track(Pcg& rng, float sigma, float mu)
-    {
-        if (rng.uni() < sigma / mu) return true;
-        tau = -rtc8x_logf(1.0f - rng.uni());
-        return false;
-    }
-    __device__ __forceinline__ void after_shadow(Pcg& rng, float sigma, float mu, float& vis)
-    {
-        vis *= 1.0f - sigma / mu;
-        tau = -rtc8x_logf(1.0f - rng.uni());
-    }
 };
 
 enum : uint32_t {

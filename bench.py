@@ -55,6 +55,9 @@ def parse():
                     help="spp of each half of the equal-RMSE reference (0 skips the equal-RMSE leg)")
     ap.add_argument("--crop-ref-spp", type=int, default=256,
                     help="spp of each half of the FP64 oracle's 64 x 64 crop judging the equal-RMSE leg (0 skips it)")
+    ap.add_argument("--legs", choices=["all", "none", "network"], default="all",
+                    help="legs beside the headline frame: all of them, none (e.g. for counter passes), or only the NRC "
+                         "network steps")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -691,6 +694,8 @@ def main() -> int:
         "cpu_baseline": None,
     }
     result["lookups_per_path"] = c["lookups"] / max(c["paths"], 1)
+    if args.legs != "all":
+        return finish(args, result, timers, pt, scene, stream, network_only=args.legs == "network")
     waves = (3, 4) if tracking == _abi.TRACKING_LOCAL_MAJORANT else (3, 8)
     lat = latency_roofline(scene, params, lookups_per_launch, c["lookups"] / max(c["paths"], 1),
                            c["lookups"] / sum(kernel_ms) * 1e3, c["paths"] / wall, waves=waves)
@@ -742,6 +747,21 @@ def main() -> int:
             for k in ("fp16", "bf16", "bf16_fp8")), file=sys.stderr, flush=True)
         print("[bench] equal-RMSE leg on the 128^3 dense cloud ...", file=sys.stderr, flush=True)
         result["equal_rmse_dense128"] = dense128_equal_rmse_leg(args.rmse_ref_spp, stream, crop_ref_spp=args.crop_ref_spp)
+    return finish(args, result, timers, pt, scene, stream)
+
+
+def finish(args, result, timers, pt, scene, stream, network_only: bool = False) -> int:
+    """The NRC network legs alone (network_only), the CPU baseline, the JSON line, and the clean-up."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd import scene as sc
+
+    if network_only and args.nrc_steps > 0:
+        net_scene = sc.DeviceScene(sc.CONFIGS["dense128"])
+        result["nrc"] = {"network": nrc_mlp_leg(net_scene, args.nrc_steps),
+                         "network_hash_grid": nrc_mlp_leg(net_scene, args.nrc_steps, hidden_layers=2, encoding=1),
+                         "network_fp8": nrc_mlp_leg(net_scene, args.nrc_steps, precision=_abi.NRC_BF16_FP8)}
+        result["nrc_queries_per_s"] = result["nrc"]["network"]["queries_per_s"]
+        net_scene.close()
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)

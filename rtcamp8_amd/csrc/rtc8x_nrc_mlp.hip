@@ -781,26 +781,43 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
 
 // Sum of the per-workgroup weight gradients in workgroup order; flags non-finite sums.
 // (Hash-grid table gradients, entries P .. P_total, are accumulated directly and only checked here.)
-__global__ void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P, uint32_t P_total,
-                                   float* __restrict__ grad, uint32_t* __restrict__ bad)
+// The MLP gradient from the per-block partials, and the finiteness check of the whole gradient.
+// Workgroups below mlp_groups reduce 64 MLP parameters each: lane (i, c) sums the partials of blocks
+// c, c + 8, c + 16, ... (four loads in flight), and the eight slices are added in a fixed order, so the
+// result does not depend on scheduling.  The other workgroups check 512 hash-table gradients each.
+constexpr uint32_t kReduceSlices = 8;
+__global__ __launch_bounds__(512) void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P,
+                                                          uint32_t P_total, float* __restrict__ grad,
+                                                          uint32_t* __restrict__ bad)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P_total) return;
-    float g;
-    if (i < P) {
-        // eight independent partial sums keep eight loads in flight; they are added in a fixed order
-        float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (uint32_t b = 0; b < nblocks; b += 8) {
-#pragma unroll
-            for (uint32_t k = 0; k < 8; ++k)
-                if (b + k < nblocks) acc[k] += gpart[(size_t)(b + k) * P + i];
+    const uint32_t mlp_groups = (P + 63) / 64;
+    bool nonfinite = false;
+    if (blockIdx.x < mlp_groups) {
+        __shared__ float part[kReduceSlices][64];
+        const uint32_t il = threadIdx.x & 63u, c = threadIdx.x >> 6, i = blockIdx.x * 64 + il;
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+        if (i < P) {
+            for (uint32_t b = c; b < nblocks; b += 4 * kReduceSlices) {
+                a0 += gpart[(size_t)b * P + i];
+                if (b + kReduceSlices < nblocks) a1 += gpart[(size_t)(b + kReduceSlices) * P + i];
+                if (b + 2 * kReduceSlices < nblocks) a2 += gpart[(size_t)(b + 2 * kReduceSlices) * P + i];
+                if (b + 3 * kReduceSlices < nblocks) a3 += gpart[(size_t)(b + 3 * kReduceSlices) * P + i];
+            }
         }
-        g = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-        grad[i] = g;
+        part[c][il] = (a0 + a1) + (a2 + a3);
+        __syncthreads();
+        if (c == 0 && i < P) {
+            float g = part[0][il];
+#pragma unroll
+            for (uint32_t k = 1; k < kReduceSlices; ++k) g += part[k][il];
+            grad[i] = g;
+            nonfinite = !isfinite(g);
+        }
     } else {
-        g = grad[i];
+        const size_t i = (size_t)P + (size_t)(blockIdx.x - mlp_groups) * 512 + threadIdx.x;
+        if (i < P_total) nonfinite = !isfinite(grad[i]);
     }
-    if (!isfinite(g)) atomicOr(bad, 1u);
+    if (__ballot(nonfinite) != 0 && (threadIdx.x & 63u) == 0) atomicOr(bad, 1u);
 }
 
 struct AdamArgs {
@@ -1013,8 +1030,9 @@ rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const f
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, true, s, in, n, nullptr, target)));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_backward<T>(h, s, in, n)));
     RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
-    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->gpart, (n + 255) / 256,
-                       h->P_mlp, h->P, h->grad, h->bad);
+    const uint32_t groups = (h->P_mlp + 63) / 64 + (h->P - h->P_mlp + 511) / 512;
+    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3(groups), dim3(512), 0, s, h->gpart, (n + 255) / 256, h->P_mlp, h->P,
+                       h->grad, h->bad);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }

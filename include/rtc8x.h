@@ -634,8 +634,9 @@ size_t rtc8x_nrc_scratch_bytes(uint32_t width, uint32_t height);
  * sum_c (p_c - t_c)^2 / (lum(p)^2 + 0.01) with the normaliser held constant.  Optimiser: Adam with L2
  * regularisation, wrapped by an exponential moving average of the weights that inference uses.
  * Weights and activations are FP16 or BF16 with FP32 accumulation (MFMA); master weights, moments and
- * the EMA are FP32.  The per-sample loss gradient is used unscaled in the backward pass (a loss scale
- * of 3n) and divided out in the optimiser step.
+ * the EMA are FP32.  The per-sample loss gradient enters the backward pass multiplied by the loss scale
+ * (rtc8x_nrc_desc.loss_scale; the mean over the batch's 3n terms is taken in the optimiser step, which
+ * divides both out).
  * ---------------------------------------------------------------------------------------------- */
 typedef struct rtc8x_nrc rtc8x_nrc;
 
@@ -667,7 +668,12 @@ typedef struct rtc8x_nrc_desc {
     int32_t precision;           /* rtc8x_nrc_precision */
     uint64_t seed;               /* weight initialisation (Xavier-uniform from PCG32) */
     int32_t encoding;            /* rtc8x_nrc_encoding */
-    uint32_t reserved;
+    /* The initial loss scale of training: 0 for the default (128 with RTC8X_NRC_FP16, 1 otherwise), or a
+     * power of two in [1, 32768].  The per-sample loss gradient is multiplied by it before it is rounded
+     * to the network's working type and divided out in the optimiser step.  With FP16 the scale is
+     * dynamic (Micikevicius et al. 2018): a step with a non-finite gradient is skipped and the scale
+     * halves (not below 1); after 128 finite steps in a row it doubles (up to 32768). */
+    float loss_scale;
 } rtc8x_nrc_desc;
 
 /* HashGrid position encoding (multiresolution hash encoding, Mueller et al. 2022, with the reference's
@@ -709,6 +715,8 @@ rtc8x_status rtc8x_nrc_train(rtc8x_nrc* nrc, void* stream, const float* in, cons
 /* Number of weights (layer matrices [out][in], the output layer padded to 16 rows), and host copies of
  * them: which = 0 master weights, 1 the (bias-corrected) EMA used for inference. */
 uint32_t rtc8x_nrc_num_params(const rtc8x_nrc* nrc);
+/* The current loss scale (waits for the device).  No counterpart in the reference. */
+rtc8x_status rtc8x_nrc_get_loss_scale(const rtc8x_nrc* nrc, float* scale);
 rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* nrc, int32_t which, float* host, size_t count);
 /* Replaces the master weights (and resets optimiser state and EMA) from a host array. */
 rtc8x_status rtc8x_nrc_set_weights(rtc8x_nrc* nrc, const float* host, size_t count);

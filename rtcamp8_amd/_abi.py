@@ -305,7 +305,7 @@ class NrcDesc(C.Structure):
         ("precision", C.c_int32),
         ("seed", C.c_uint64),
         ("encoding", C.c_int32),
-        ("reserved", C.c_uint32),
+        ("loss_scale", C.c_float),
     ]
 
 
@@ -419,6 +419,7 @@ PROTOTYPES = {
     "rtc8x_nrc_infer": (_S, [_P, _P, _P, C.c_uint32, _P]),
     "rtc8x_nrc_infer_training": (_S, [_P, _P, C.POINTER(FrameParams)]),
     "rtc8x_nrc_train": (_S, [_P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
+    "rtc8x_nrc_get_loss_scale": (_S, [C.c_void_p, C.POINTER(C.c_float)]),
     "rtc8x_nrc_num_params": (C.c_uint32, [_P]),
     "rtc8x_nrc_get_weights": (_S, [_P, C.c_int32, _P, C.c_size_t]),
     "rtc8x_nrc_set_weights": (_S, [_P, _P, C.c_size_t]),

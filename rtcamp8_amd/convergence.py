@@ -66,7 +66,7 @@ def nrc_image(scene: DeviceScene, spp: int, seed: int, stream=None, num_hidden_l
     frames = (spp + spp_per_frame - 1) // spp_per_frame
     for _ in range(frames):
         r.render_frame(spp=spp_per_frame, train_iterations=None if train else 0)
-    img = r.image()  # synchronises
+    img = r.image()  # waits for the renderer's stream
     secs = time.perf_counter() - t0
     r.close()
     return NrcRun(img, secs, frames * spp_per_frame)

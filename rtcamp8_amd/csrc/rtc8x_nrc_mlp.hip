@@ -304,7 +304,8 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
                                                       float* __restrict__ out, const float* __restrict__ target,
                                                       T* __restrict__ hbuf, T* __restrict__ dout,
                                                       float* __restrict__ loss_part, const float* __restrict__ tables,
-                                                      const HashGeom geom, const int32_t* __restrict__ count_dims)
+                                                      const HashGeom geom, const int32_t* __restrict__ count_dims,
+                                                      const float* __restrict__ loss_scale)
 {
     using V = typename Vec8<T>::type;
     n = device_count(n, count_dims);
@@ -398,8 +399,10 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
         }
         return;
     }
-    // relative L2 over luminance; per-sample gradient 2 (p - t) / (lum(p)^2 + 0.01)
+    // relative L2 over luminance; per-sample gradient 2 (p - t) / (lum(p)^2 + 0.01), times the loss
+    // scale (a power of two) before it is rounded to the working type
     float loss = 0.0f;
+    const float ls = *loss_scale;
     if (s < n) {
         const float p0 = pred[wave][lane][0], p1 = pred[wave][lane][1], p2 = pred[wave][lane][2];
         const float lum = (float)0.2126729 * p0 + (float)0.7151522 * p1 + (float)0.0721750 * p2;
@@ -407,9 +410,9 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
         const float d0 = p0 - target[3 * (size_t)s], d1 = p1 - target[3 * (size_t)s + 1], d2 = p2 - target[3 * (size_t)s + 2];
         loss = (d0 * d0 + d1 * d1 + d2 * d2) / denom;
         T* g = dout + (size_t)s * kOutPad;
-        g[0] = (T)(2.0f * d0 / denom);
-        g[1] = (T)(2.0f * d1 / denom);
-        g[2] = (T)(2.0f * d2 / denom);
+        g[0] = (T)(2.0f * d0 / denom * ls);
+        g[1] = (T)(2.0f * d1 / denom * ls);
+        g[2] = (T)(2.0f * d2 / denom * ls);
 #pragma unroll
         for (int c = 3; c < (int)kOutPad; ++c) g[c] = (T)0.0f;
     }
@@ -832,13 +835,14 @@ struct AdamArgs {
 template <typename T>
 __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
                             float* __restrict__ ema, const float* __restrict__ grad, uint32_t* __restrict__ bad,
-                            T* __restrict__ w, T* __restrict__ wt, T* __restrict__ winf, float* __restrict__ grid_inf)
+                            T* __restrict__ w, T* __restrict__ wt, T* __restrict__ winf, float* __restrict__ grid_inf,
+                            const float* __restrict__ loss_scale)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.P) return;
     if (*bad) return;  // a non-finite gradient skips the whole step
     const float w0 = master[i];
-    const float g = grad[i] * a.inv_scale + a.l2 * w0;
+    const float g = grad[i] * (a.inv_scale / *loss_scale) + a.l2 * w0;
     const float mi = a.beta1 * m[i] + (1.0f - a.beta1) * g;
     const float vi = a.beta2 * v[i] + (1.0f - a.beta2) * g * g;
     m[i] = mi;
@@ -858,6 +862,33 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
         wt[(size_t)l * kLayer + in * kWidth + out] = (T)w1;
     else
         wt[(size_t)a.L * kLayer + in * kOutT + out] = (T)w1;
+}
+
+// Dynamic loss scaling (Micikevicius et al. 2018, "Mixed Precision Training"): state[0] is the scale,
+// a power of two; state[1] counts the finite steps since it last changed.  A step whose gradient is not
+// finite was skipped by the optimiser: the scale halves (not below 1).  After kLossScaleGrowth finite
+// steps in a row it doubles (not above kLossScaleMax).  One lane; after the optimiser step.
+constexpr float kLossScaleMax = 32768.0f;
+constexpr float kLossScaleGrowth = 128.0f;
+__global__ void loss_scale_kernel(const uint32_t* __restrict__ bad, float* __restrict__ state)
+{
+    if (*bad) {
+        state[0] = fmaxf(1.0f, state[0] * 0.5f);
+        state[1] = 0.0f;
+    } else if (state[1] + 1.0f >= kLossScaleGrowth) {
+        state[0] = fminf(kLossScaleMax, state[0] * 2.0f);
+        state[1] = 0.0f;
+    } else {
+        state[1] += 1.0f;
+    }
+}
+
+// g / loss scale, for the gradient handed out by rtc8x_nrc_gradient
+__global__ void unscale_kernel(const float* __restrict__ g, uint32_t P, const float* __restrict__ state,
+                               float* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P) out[i] = g[i] / state[0];
 }
 
 template <typename T>
@@ -911,6 +942,7 @@ struct rtc8x_nrc {
     float* grid_de = nullptr;   // hash grid: the encoded input's gradient, 32 floats per training sample
     float* grid_part = nullptr; // hash grid: the tables' gradient per slice of the batch (kScatterSlices)
     uint32_t* bad = nullptr;
+    float* loss_scale = nullptr;  // [scale, finite steps since it changed] (dynamic for FP16)
     void *w = nullptr, *wt = nullptr, *winf = nullptr, *hbuf = nullptr, *dout = nullptr;
     uint8_t* w8 = nullptr;      // E4M3 inference weights (RTC8X_NRC_BF16_FP8)
     float* w8_inv = nullptr;    // per layer: 1 / the layer's weight scale
@@ -926,11 +958,11 @@ hipError_t launch_forward_enc(rtc8x_nrc* h, bool train, const void* w, const flo
     if (train)
         hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
                            (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part, tables,
-                           h->geom, (const int32_t*)nullptr);
+                           h->geom, (const int32_t*)nullptr, (const float*)h->loss_scale);
     else
         hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
                            (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr, (float*)nullptr,
-                           tables, h->geom, count_dims);
+                           tables, h->geom, count_dims, (const float*)h->loss_scale);
     return hipGetLastError();
 }
 
@@ -971,7 +1003,7 @@ template <typename T>
 hipError_t launch_adam(rtc8x_nrc* h, hipStream_t s, const nrc::AdamArgs& a)
 {
     hipLaunchKernelGGL((nrc::adam_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, a, h->master, h->m, h->v, h->ema,
-                       h->grad, h->bad, (T*)h->w, (T*)h->wt, (T*)h->winf, h->grid_inf);
+                       h->grad, h->bad, (T*)h->w, (T*)h->wt, (T*)h->winf, h->grid_inf, (const float*)h->loss_scale);
     return hipGetLastError();
 }
 
@@ -1019,6 +1051,9 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
     RTC8X_HIP(hipMemset(h->ema, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->wt, 0, h->tsize * h->PT));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_convert<T>(h, nullptr)));
+    const float ls[2] = {h->desc.loss_scale > 0.0f ? h->desc.loss_scale : (h->desc.precision == RTC8X_NRC_FP16 ? 128.0f : 1.0f),
+                         0.0f};
+    RTC8X_HIP(hipMemcpy(h->loss_scale, ls, sizeof(ls), hipMemcpyHostToDevice));
     RTC8X_HIP(hipDeviceSynchronize());
     h->step = 0;
     return RTC8X_OK;
@@ -1043,7 +1078,7 @@ extern "C" rtc8x_status rtc8x_nrc_destroy(rtc8x_nrc* h)
 try {
     if (!h) return RTC8X_OK;
     void* ptrs[] = {h->master, h->m, h->v, h->ema, h->grad, h->gpart, h->loss_part, h->bad, h->w, h->wt, h->winf, h->hbuf, h->dout,
-                    h->grid_inf, h->grid_de, h->grid_part, h->w8, h->w8_inv};
+                    h->grid_inf, h->grid_de, h->grid_part, h->w8, h->w8_inv, h->loss_scale};
     for (void* p : ptrs)
         if (p) RTC8X_HIP(hipFree(p));
     delete h;
@@ -1062,6 +1097,9 @@ try {
     RTC8X_REQUIRE(d->learning_rate > 0.0f && d->beta1 >= 0.0f && d->beta1 < 1.0f && d->beta2 >= 0.0f && d->beta2 < 1.0f &&
                       d->epsilon > 0.0f && d->ema_decay >= 0.0f && d->ema_decay < 1.0f && d->l2_reg >= 0.0f,
                   "rtc8x_nrc_create: bad optimiser parameters");
+    RTC8X_REQUIRE(d->loss_scale == 0.0f || (d->loss_scale >= 1.0f && d->loss_scale <= 32768.0f &&
+                                             std::exp2(std::floor(std::log2(d->loss_scale))) == d->loss_scale),
+                  "rtc8x_nrc_create: loss_scale must be 0 or a power of two in [1, 32768]");
     RTC8X_REQUIRE(d->encoding == RTC8X_NRC_TRIANGLE_WAVE || d->encoding == RTC8X_NRC_HASH_GRID,
                   "rtc8x_nrc_create: unknown encoding %d", d->encoding);
     *out = nullptr;
@@ -1080,7 +1118,7 @@ try {
     bool ok = alloc((void**)&h->master, 4 * P) && alloc((void**)&h->m, 4 * P) && alloc((void**)&h->v, 4 * P) &&
               alloc((void**)&h->ema, 4 * P) && alloc((void**)&h->grad, 4 * P) &&
               alloc((void**)&h->gpart, 4 * P_mlp * max_blocks) && alloc((void**)&h->loss_part, 4 * 4 * max_blocks) &&
-              alloc((void**)&h->bad, 4) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
+              alloc((void**)&h->bad, 4) && alloc((void**)&h->loss_scale, 8) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
               alloc(&h->winf, h->tsize * P_mlp) && (h->P_grid == 0 || alloc((void**)&h->grid_inf, 4 * (size_t)h->P_grid)) &&
               (h->P_grid == 0 || alloc((void**)&h->grid_de, 4 * 32 * (size_t)nrc::kMaxTrain)) &&
               (h->P_grid == 0 || alloc((void**)&h->grid_part, 4 * (size_t)h->P_grid * nrc::kScatterSlices)) &&
@@ -1127,6 +1165,15 @@ try {
 RTC8X_CATCH_ALL("rtc8x_nrc_create")
 
 extern "C" uint32_t rtc8x_nrc_num_params(const rtc8x_nrc* h) { return h ? h->P : 0; }
+
+extern "C" rtc8x_status rtc8x_nrc_get_loss_scale(const rtc8x_nrc* h, float* scale)
+try {
+    RTC8X_REQUIRE(h && scale, "rtc8x_nrc_get_loss_scale: NULL argument");
+    RTC8X_HIP(hipDeviceSynchronize());
+    RTC8X_HIP(hipMemcpy(scale, h->loss_scale, sizeof(float), hipMemcpyDeviceToHost));
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_nrc_get_loss_scale")
 
 extern "C" rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* h, int32_t which, float* host, size_t count)
 try {
@@ -1192,7 +1239,9 @@ try {
     hipStream_t s = (hipStream_t)stream;
     rtc8x_status st = gradient_pass(h, s, in, target, n);
     if (st != RTC8X_OK) return st;
-    RTC8X_HIP(hipMemcpyAsync(grad, h->grad, 4 * h->P, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(nrc::unscale_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, (const float*)h->grad, h->P,
+                       (const float*)h->loss_scale, grad);
+    RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_nrc_gradient")
@@ -1222,6 +1271,10 @@ try {
     a.P_mlp = h->P_mlp;
     a.L = h->L;
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_adam<T>(h, s, a)));
+    if (h->desc.precision == RTC8X_NRC_FP16) {
+        hipLaunchKernelGGL(nrc::loss_scale_kernel, dim3(1), dim3(1), 0, s, (const uint32_t*)h->bad, h->loss_scale);
+        RTC8X_HIP(hipGetLastError());
+    }
     if (loss_on_cpu) {
         const uint32_t nw = 4 * ((n + 255) / 256);
         std::vector<float> parts(nw);

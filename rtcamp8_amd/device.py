@@ -35,6 +35,14 @@ def synchronize() -> None:
     call("rtc8x_device_synchronize")
 
 
+def wait_stream(stream) -> None:
+    """Waits for the work queued on `stream` (None: the null stream, which the blocking copies of
+    DeviceBuffer already wait for).  Streams from Stream() do not block on the null stream, so a copy
+    on it does not wait for them: read their results after this."""
+    if stream:
+        call("rtc8x_stream_synchronize", C.c_void_p(stream))
+
+
 class DeviceBuffer:
     """An owned device allocation of `nbytes` bytes."""
 

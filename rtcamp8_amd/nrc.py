@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _abi
 from ._lib import call, lib
-from .device import DeviceBuffer
+from .device import DeviceBuffer, wait_stream
 from .scene import DeviceScene
 
 TRAIN_BATCH = _abi.NUM_TRAINING_DATA_PER_FRAME // 4  # renderer_main.cpp:1595: 4 steps of 16384 per iteration
@@ -53,11 +53,13 @@ class NeuralRadianceCache:
 
     def initialize(self, num_hidden_layers: int = 2, learning_rate: float = 1e-2, precision: int = _abi.NRC_FP16,
                    seed: int = 0x9E3779B97F4A7C15, ema_decay: float = 0.99,
-                   encoding: int = _abi.NRC_HASH_GRID) -> None:
+                   encoding: int = _abi.NRC_HASH_GRID, loss_scale: float = 0.0) -> None:
         """initialize(posEnc, numHiddenLayers, learningRate) (network_interface.cu:50-134); Adam epsilon
-        1e-15 with the hash grid and 1e-8 with triangle waves, as the reference configures them."""
+        1e-15 with the hash grid and 1e-8 with triangle waves, as the reference configures them.
+        loss_scale: the initial loss scale (0: 128 with FP16, 1 otherwise; dynamic with FP16)."""
         eps = 1e-15 if encoding == _abi.NRC_HASH_GRID else 1e-8
-        d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, eps, 1e-6, ema_decay, precision, seed, encoding, 0)
+        d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, eps, 1e-6, ema_decay, precision, seed, encoding,
+                         float(loss_scale))
         h = C.c_void_p()
         call("rtc8x_nrc_create", C.byref(d), C.byref(h))
         self.handle = h.value
@@ -83,6 +85,11 @@ class NeuralRadianceCache:
              C.c_void_p(target_ptr), int(num), C.byref(loss) if want_loss else None)
         return float(loss.value) if want_loss else None
 
+    def loss_scale(self) -> float:
+        v = C.c_float(0.0)
+        call("rtc8x_nrc_get_loss_scale", C.c_void_p(self.handle), C.byref(v))
+        return float(v.value)
+
     def num_params(self) -> int:
         return int(lib().rtc8x_nrc_num_params(C.c_void_p(self.handle)))
 
@@ -99,6 +106,7 @@ class NeuralRadianceCache:
         g = DeviceBuffer(4 * self.num_params())
         call("rtc8x_nrc_gradient", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr),
              C.c_void_p(target_ptr), int(num), C.c_void_p(g.ptr))
+        wait_stream(stream)
         out = g.download(np.float32, self.num_params())
         g.free()
         return out
@@ -288,10 +296,12 @@ class NrcRenderer:
         self.frame += 1
 
     def image(self) -> np.ndarray:
+        wait_stream(self.stream)
         a = self.accum.download(np.float64, 3 * self.width * self.height).reshape(self.height, self.width, 3)
         return a / max(self.samples, 1)
 
     def counters(self) -> dict:
+        wait_stream(self.stream)
         s = self.stats_buf.download(np.uint64, _abi.STAT_COUNT)
         return {"paths": int(s[_abi.STAT_PATHS]), "vertices": int(s[_abi.STAT_VERTICES]),
                 "lookups": int(s[_abi.STAT_LOOKUPS])}

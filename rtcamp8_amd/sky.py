@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _abi
 from ._lib import call
-from .device import DeviceBuffer
+from .device import DeviceBuffer, wait_stream
 
 MODEL = Path(__file__).resolve().parent / "data" / "hosek_model.bin"
 
@@ -76,6 +76,7 @@ class HosekSky:
         try:
             call("rtc8x_sky_eval", C.c_void_p(stream), C.byref(self.struct), sd, n, C.c_void_p(dp.ptr),
                  C.c_void_p(dt.ptr), C.c_void_p(out.ptr))
+            wait_stream(stream)
             return out.download(np.float32, 3 * n).reshape(n, 3)
         finally:
             for b in (dp, dt, out):

@@ -9,7 +9,7 @@ import numpy as np
 
 from . import _abi
 from ._lib import call, lib
-from .device import DeviceBuffer
+from .device import DeviceBuffer, wait_stream
 from .scene import DeviceScene
 
 
@@ -73,10 +73,12 @@ class PathTracer:
 
     def image(self) -> np.ndarray:
         cfg = self.scene.cfg
+        wait_stream(self.stream)
         a = self.accum.download(np.float64, 3 * cfg.width * cfg.height).reshape(cfg.height, cfg.width, 3)
         return a / max(self.samples, 1)
 
     def counters(self) -> dict:
+        wait_stream(self.stream)
         s = self.stats.download(np.uint64, _abi.STAT_COUNT)
         return {"paths": int(s[_abi.STAT_PATHS]), "vertices": int(s[_abi.STAT_VERTICES]),
                 "lookups": int(s[_abi.STAT_LOOKUPS]), "scatters": int(s[_abi.STAT_SCATTER])}

@@ -81,14 +81,98 @@ def test_inference_matches_oracle(L, precision, mode, tol, encoding):
     c.finalize()
 
 
+def test_loss_scale_keeps_small_fp16_gradients():
+    """The loss scale against FP16's small-number range.  With the output layer's weights scaled by 1e-2
+    the outputs p are ~1e-2, and targets t = p + d put the residual at d = 1e-7 or d x 2^12.  The small
+    residual's per-sample loss gradients (~2e-5) and the hidden layers' input gradients behind them
+    (~2e-7) are FP16 subnormals with few significant bits; 2^12 times larger they are normal FP16 numbers.
+    The gradient is linear in the residual, so g(d) x 2^12 should equal g(d x 2^12).  Unscaled (loss scale
+    1) the hidden layers' weight gradients for the small residual lose most of their bits; with the
+    default scale (128) they are computed in FP16's normal range.  Measured: relative error 0.856 unscaled,
+    0.0080 with the scale."""
+    from rtcamp8_amd.device import DeviceBuffer
+
+    L, n, k = 3, 512, 2.0 ** 12
+    q = random_queries(n, seed=21)
+    w = random_weights(L, seed=22, scale=1.2, encoding=0, table_scale=1.0)
+    w[L * 4096:L * 4096 + 16 * 64] *= np.float32(1e-2)
+    dq = DeviceBuffer.from_array(q)
+    sign = np.where(np.random.default_rng(23).random((n, 3)) < 0.5, -1.0, 1.0).astype(np.float32)
+    res = {}
+    for scale in (1.0, 0.0):
+        c = _cache_scaled(L, 0, 0, scale)
+        c.set_weights(w)
+        dp = DeviceBuffer(12 * n)
+        c.infer(None, dq.ptr, n, dp.ptr)
+        p = dp.download(np.float32, 3 * n).reshape(n, 3)
+        grads = []
+        for d in (1e-7, 1e-7 * k):
+            t = (p + np.float32(d) * sign).astype(np.float32)
+            dt = DeviceBuffer.from_array(t)
+            grads.append(c.gradient(None, dq.ptr, dt.ptr, n).astype(np.float64))
+            dt.free()
+        g_small, g_big = grads
+        hidden = slice(0, L * 4096)
+        res[scale] = float(np.linalg.norm(g_small[hidden] * k - g_big[hidden]) / np.linalg.norm(g_big[hidden]))
+        print(f"[loss scale] initial {c.loss_scale():g}: relative error of the hidden layers' small-residual "
+              f"gradient {res[scale]:.4f} (output |p| mean {np.abs(p).mean():.3g})", flush=True)
+        dp.free()
+        c.finalize()
+    dq.free()
+    assert res[0.0] < LOSS_SCALE_ERR_SCALED, res
+    assert res[1.0] > 4.0 * res[0.0], res
+
+
+LOSS_SCALE_ERR_SCALED = 0.011
+
+
+def _cache_scaled(L, precision, encoding, loss_scale):
+    from rtcamp8_amd.nrc import NeuralRadianceCache
+
+    c = NeuralRadianceCache()
+    c.initialize(num_hidden_layers=L, learning_rate=1e-2, precision=precision, encoding=encoding, loss_scale=loss_scale)
+    return c
+
+
+def test_loss_scale_backs_off_after_overflow():
+    """The dynamic loss scale: started at 2^15 with targets far from the outputs (per-sample loss
+    gradients ~1e3), the scaled FP16 loss gradient overflows; those steps are skipped and the scale halves
+    until it fits (at most 2^5), after which training goes on and the loss falls."""
+    from rtcamp8_amd.device import DeviceBuffer
+
+    n = 4096
+    q = random_queries(n, seed=31)
+    t = (10.0 * np.random.default_rng(32).random((n, 3))).astype(np.float32)
+    dq, dt = DeviceBuffer.from_array(q), DeviceBuffer.from_array(t)
+    c = _cache_scaled(2, 0, 0, 32768.0)
+    first = c.train(None, dq.ptr, dt.ptr, n, want_loss=True)
+    w0 = c.get_weights()
+    for _ in range(30):
+        last = c.train(None, dq.ptr, dt.ptr, n, want_loss=True)
+    s = c.loss_scale()
+    print(f"[loss scale] after 31 steps: scale {s:g}, loss {first:.4g} -> {last:.4g}", flush=True)
+    assert 1.0 <= s <= 64.0, s
+    assert np.isfinite(c.get_weights()).all()
+    assert np.any(c.get_weights() != w0)
+    assert last < first, (first, last)
+    c.finalize()
+    dq.free()
+    dt.free()
+
+
+FP8_REL_RMS = {2: 0.085, 5: 0.155}
+FP8_MAX_OF_RMS = {2: 0.33, 5: 0.85}
+
+
 @pytest.mark.parametrize("encoding", ENCODINGS)
 @pytest.mark.parametrize("L", [2, 5])
 def test_fp8_inference_matches_oracle(L, encoding):
     """E4M3 inference (BF16 training) against the exact FP64 forward of the same weights.  E4M3 keeps 3
     mantissa bits: each operand carries up to 2^-4 relative rounding (RMS about 3.6 %), and a 64-term
-    dot product of random-sign terms keeps about 5 % of it per layer.  Measured: relative RMS error
-    0.057-0.065 at L = 2 and 0.096-0.119 at L = 5, max 0.20-0.60 of the RMS.  Tolerance: relative RMS
-    error 0.04 per layer (L + 1 layers), max error 0.8 of the RMS."""
+    dot product of random-sign terms keeps about 5 % of it per layer.  Measured with activations kept in
+    E4M3 between layers (per-tile scales from the FP32 accumulators): relative RMS error 0.064 / 0.058 at
+    L = 2 and 0.094 / 0.119 at L = 5 (triangle waves / hash grid), max 0.25 / 0.20 and 0.42 / 0.65 of the
+    RMS.  Tolerance: about 1.3x the larger measured value at each depth."""
     from rtcamp8_amd.device import DeviceBuffer
 
     n = 1000
@@ -105,8 +189,8 @@ def test_fp8_inference_matches_oracle(L, encoding):
     err = np.sqrt(np.mean((got - ref) ** 2)) / rms
     print(f"[fp8] L={L} encoding={encoding}: relative RMS error {err:.4f}, max {np.max(np.abs(got - ref)) / rms:.4f}")
     assert rms > 1e-3 and np.all(np.isfinite(got))
-    assert err < 0.04 * (L + 1), err
-    assert np.max(np.abs(got - ref)) < 0.8 * rms
+    assert err < FP8_REL_RMS[L], err
+    assert np.max(np.abs(got - ref)) < FP8_MAX_OF_RMS[L] * rms
     c.finalize()
 
 
@@ -215,6 +299,41 @@ def _nrc_case(width=96, height=64, tracking=0, density=None):
     ds = sc.DeviceScene(cfg)
     r = NrcRenderer(ds, num_hidden_layers=2)
     return cfg, ds, r
+
+
+def test_fp8_render_matches_bf16_render():
+    """An NRC render with E4M3 inference against the same render with BF16 inference: one cache trained
+    online in BF16 on the dense cloud (96 x 64, frame 0's 8 iterations), its inference weights copied into
+    a BF16 and a BF16_FP8 cache, each rendering 4 spp from the same seeds without further training.  The
+    paths are identical, so the images differ only by the cache's E4M3 rounding at the path ends.
+    Measured: relative RMS difference 0.012, mean shift 0.0048; bounds about 1.3x those."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd.nrc import NrcRenderer
+
+    cfg, ds, r = _nrc_case()
+    r.render_frame(spp=1)
+    w = r.cache.get_weights(1)  # the EMA weights inference uses
+    r.close()
+    imgs = {}
+    for prec in (_abi.NRC_BF16, _abi.NRC_BF16_FP8):
+        rr = NrcRenderer(ds, num_hidden_layers=2, precision=prec)
+        rr.cache.set_weights(w)
+        rr.reset(0xF8F8)
+        rr.render_frame(spp=4, train_iterations=0)
+        imgs[prec] = rr.image()
+        rr.close()
+    a, b = imgs[_abi.NRC_BF16], imgs[_abi.NRC_BF16_FP8]
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    rel = float(np.sqrt(np.mean((b - a) ** 2)) / np.sqrt(np.mean(a ** 2)))
+    mean_shift = float(abs(b.mean() - a.mean()) / a.mean())
+    print(f"[fp8 render] relative RMS difference {rel:.5f}, mean shift {mean_shift:.5f}", flush=True)
+    assert 0.0 < rel < FP8_RENDER_REL_RMS, rel
+    assert mean_shift < FP8_RENDER_MEAN_SHIFT, mean_shift
+    ds.close()
+
+
+FP8_RENDER_REL_RMS = 0.016
+FP8_RENDER_MEAN_SHIFT = 0.0065
 
 
 def _host_params(cfg, ds, r, host_nrc, rng, accum):

@@ -333,10 +333,11 @@ def multi_instance_leg(steps: int, stream):
 def nrc_render_leg(scene, steps: int, stream):
     """NRC-integrated rendering of the same scene: per step one training iteration and one sample per
     pixel traced into the cache (2 hidden layers, FP16), after the first frame's 8 training iterations."""
+    from rtcamp8_amd import _abi
     from rtcamp8_amd.device import EventTimer
     from rtcamp8_amd.nrc import NrcRenderer
 
-    r = NrcRenderer(scene, num_hidden_layers=2, stream=stream.handle)
+    r = NrcRenderer(scene, num_hidden_layers=2, stream=stream.handle, train_tracking=_abi.TRACKING_GLOBAL_MAJORANT)
     r.render_frame(spp=1)
     stream.synchronize()
     r.stats_buf.fill(0)
@@ -484,7 +485,9 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream, precisions=None, encod
         def crop_rel(img):
             return sub_rel(img[crop[2]:crop[3], crop[0]:crop[1]])
     res = {"judge": f"plain GPU tracer, 2 x {ref_spp} spp with independent seeds (cross-product MSE)",
-           "nrc_schedule": "frame 0: 8 training iterations, later frames 1; 4 spp traced into the cache per frame",
+           "nrc_schedule": "frame 0: 8 training iterations, later frames 1; 4 spp traced into the cache per frame; "
+                           f"the render launches track with the {'local' if cfg.tracking else 'global'} majorant, "
+                           "the training launches with the global one",
            "nrc_settings": {"encoding": "hash grid" if encoding == _abi.NRC_HASH_GRID else "triangle waves",
                             "hidden_layers": 2, "radiance_scale": NRC_RADIANCE_SCALE,
                             "ema_decay": NRC_EMA_DECAY},
@@ -496,7 +499,8 @@ def equal_rmse_leg(scene, ref_spp: int, spp: int, stream, precisions=None, encod
                             "gpu_judge_rel_rmse": crop_rel((a + b) / 2)}
     for name, prec in precisions:
         run = nrc_image(scene, spp, seed=0xD00D, stream=stream.handle, precision=prec, spp_per_frame=4,
-                        encoding=encoding, radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY)
+                        encoding=encoding, radiance_scale=NRC_RADIANCE_SCALE, ema_decay=NRC_EMA_DECAY,
+                        train_tracking=_abi.TRACKING_GLOBAL_MAJORANT)
         mse = cross_mse(run.image, a, b)
         n_star = spp * mse_plain / mse if mse > 0 else float("inf")
         entry = {"nrc_seconds": run.seconds, "nrc_rel_rmse": float(np.sqrt(max(mse, 0.0)) / mean),

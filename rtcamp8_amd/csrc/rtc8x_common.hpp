@@ -56,8 +56,9 @@ struct StageSnapshot {
 // final slot) and the staging counter.
 struct NrcScratchLayout {
     size_t counts, offsets, shuffle, rng_save, path_length, lookups, scatters, lost;
-    size_t queries, targets, info, tag, snap, fmap, counter, total;
+    size_t queries, targets, info, tag, snap, fmap, counter, tiles, total;
 };
+constexpr uint32_t kPrefixTile = 4096;  // record counts per workgroup of the prefix sum
 inline NrcScratchLayout nrc_scratch_layout(uint32_t width, uint32_t height)
 {
     constexpr size_t kShuffleFloats = 9 * (RTC8X_NUM_TRAINING_DATA_PER_FRAME / 256);
@@ -79,7 +80,8 @@ inline NrcScratchLayout nrc_scratch_layout(uint32_t width, uint32_t height)
     l.snap = l.tag + 8 * cap;
     l.fmap = l.snap + sizeof(StageSnapshot) * cap;
     l.counter = up16(l.fmap + 4 * cap);
-    l.total = l.counter + 16;
+    l.tiles = l.counter + 16;
+    l.total = l.tiles + 8 * ((px + kPrefixTile - 1) / kPrefixTile);
     return l;
 }
 

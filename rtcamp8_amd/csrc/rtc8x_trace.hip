@@ -885,36 +885,82 @@ __global__ void reset_counters_kernel(uint32_t slot, uint32_t count)
     if (threadIdx.x < count) g_work_counter[slot + threadIdx.x] = 0u;
 }
 
-// Exclusive prefix sum of the per-path record counts (one workgroup, a few hundred thousand entries).
-__global__ __launch_bounds__(1024) void record_prefix_kernel(const rtc8x_frame_params p, const uint32_t* __restrict__ counts,
-                                                             uint32_t* __restrict__ offsets, uint32_t* __restrict__ total_out)
+// Exclusive prefix sum of the per-path record counts, in tiles of kPrefixTile pixels: record_tile_sum_kernel
+// sums each tile, record_tile_scan_kernel adds the sums of the tiles before its own and scans the tile
+// (16 counts per lane).  Sums are 64-bit; offsets and the total saturate at 2^32 - 1.
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* lds4)
 {
-    __shared__ uint64_t part[1024];
-    uint32_t tw, th;
-    train_dims(p, tw, th);
-    const uint32_t n = tw * th;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t chunk = (n + 1023) / 1024;
-    const uint32_t b = tid * chunk, e = min(n, b + chunk);
-    uint64_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += counts[i];
-    part[tid] = s;
+    for (int o = 32; o > 0; o >>= 1) v += (uint64_t)__shfl_xor((unsigned long long)v, o, 64);
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint64_t v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    uint64_t run = part[tid] - s;
-    for (uint32_t i = b; i < e; ++i) {
-        offsets[i] = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
-        run += counts[i];
-    }
-    if (tid == 1023) *total_out = part[1023] > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)part[1023];
+    if ((threadIdx.x & 63u) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return lds4[0] + lds4[1] + lds4[2] + lds4[3];
 }
 
+__global__ __launch_bounds__(256) void record_tile_sum_kernel(const rtc8x_frame_params p,
+                                                              const uint32_t* __restrict__ counts,
+                                                              uint64_t* __restrict__ tile_sums)
+{
+    __shared__ uint64_t lds4[4];
+    uint32_t tw, th;
+    train_dims(p, tw, th);
+    const uint32_t n = tw * th, b0 = blockIdx.x * kPrefixTile;
+    if (b0 >= n) return;  // uniform over the workgroup
+    const uint32_t e = min(n, b0 + kPrefixTile);
+    uint64_t s = 0;
+    for (uint32_t i = b0 + threadIdx.x; i < e; i += 256) s += counts[i];
+    s = block_sum_u64(s, lds4);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
+}
 
+__global__ __launch_bounds__(256) void record_tile_scan_kernel(const rtc8x_frame_params p,
+                                                               const uint32_t* __restrict__ counts,
+                                                               const uint64_t* __restrict__ tile_sums,
+                                                               uint32_t* __restrict__ offsets,
+                                                               uint32_t* __restrict__ total_out)
+{
+    __shared__ uint64_t lds4[4];
+    __shared__ uint64_t wave_tot[4];
+    uint32_t tw, th;
+    train_dims(p, tw, th);
+    const uint32_t n = tw * th, b0 = blockIdx.x * kPrefixTile;
+    if (b0 >= n) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *total_out = 0u;  // no pixels
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t before = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 256) before += tile_sums[j];
+    const uint64_t base = block_sum_u64(before, lds4);
+    // lane t: counts [b0 + 16 t, b0 + 16 t + 16)
+    uint32_t c[16];
+    uint64_t mine = 0;
+    const uint32_t i0 = b0 + 16u * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        c[k] = i0 + k < n ? counts[i0 + k] : 0u;
+        mine += c[k];
+    }
+    // exclusive scan of the lanes' sums: within the wavefront, then over the four wavefronts
+    uint64_t incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t v = (uint64_t)__shfl_up((unsigned long long)incl, o, 64);
+        if (lane >= (uint32_t)o) incl += v;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    uint64_t run = base + (incl - mine);
+    for (uint32_t w = 0; w < wave; ++w) run += wave_tot[w];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (i0 + k < n) offsets[i0 + k] = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
+        run += c[k];
+    }
+    if (b0 + kPrefixTile >= n && threadIdx.x == 255) {
+        const uint64_t tot = base + wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        *total_out = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
+    }
+}
 
 // Final slot of every staged record: its pixel's offset plus its index on the path.
 __global__ __launch_bounds__(256) void stage_map_kernel(const uint32_t* __restrict__ total, const uint32_t* __restrict__ tag,
@@ -1322,8 +1368,16 @@ try {
     hipLaunchKernelGGL(dev::reset_value_kernel, dim3(1), dim3(1), 0, s, stage.counter);
     RTC8X_HIP(hipGetLastError());
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, 0, 0, 1, nullptr, counts, s, stage)));
-    hipLaunchKernelGGL(dev::record_prefix_kernel, dim3(1), dim3(1024), 0, s, p, counts, offsets, total);
-    RTC8X_HIP(hipGetLastError());
+    {
+        // the training image is at most the frame's image: tiles over that bound, the extra ones exit
+        const uint32_t tiles = (uint32_t)(((size_t)p.image_width * p.image_height + kPrefixTile - 1) / kPrefixTile);
+        uint64_t* tile_sums = reinterpret_cast<uint64_t*>(sb + lay.tiles);
+        hipLaunchKernelGGL(dev::record_tile_sum_kernel, dim3(tiles), dim3(256), 0, s, p, counts, tile_sums);
+        RTC8X_HIP(hipGetLastError());
+        hipLaunchKernelGGL(dev::record_tile_scan_kernel, dim3(tiles), dim3(256), 0, s, p, counts,
+                           (const uint64_t*)tile_sums, offsets, total);
+        RTC8X_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(dev::stage_map_kernel, dim3(kCompactBlocks), dim3(256), 0, s, total, stage.tag, offsets, fmap);
     RTC8X_HIP(hipGetLastError());
     hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, p, total, stage, fmap, offsets,

@@ -189,9 +189,12 @@ class NrcRenderer:
     def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
                  precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121,
                  encoding: int = _abi.NRC_HASH_GRID, ema_decay: float = 0.99, first_frame_iterations: int = 8,
-                 samples_per_launch: int = 4):
+                 samples_per_launch: int = 4, train_tracking: int | None = None):
         """samples_per_launch: samples traced into the cache by one launch (and inferred and accumulated
-        together); the sums equal those of one-sample launches."""
+        together); the sums equal those of one-sample launches.  train_tracking: the tracker of the
+        training-data launches (None: the scene's); both trackers give training data of the same
+        distribution, and the global one runs the training entry faster (its local variant holds more
+        registers and runs two waves per SIMD)."""
         cfg = scene.cfg
         self.scene = scene
         self.stream = stream
@@ -217,6 +220,7 @@ class NrcRenderer:
         self.samples = 0
         self.frame = 0
         self.knobs = (0, 0)  # (event_eighths, tile_shape) of the trace kernel's schedule; 0 = defaults
+        self.train_tracking = train_tracking
         call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(cfg.rng_seed))
         self.accum.fill(0)
 
@@ -243,6 +247,8 @@ class NrcRenderer:
         s = C.c_void_p(self.stream)
         self.buf_idx = (self.buf_idx + 1) % 2
         p = self.params()
+        if self.train_tracking is not None:
+            p.tracking = self.train_tracking
         p.nrc.offset_to_select_unbiased_path = self.frame_rng()
         call("rtc8x_nrc_prepare", s, C.byref(p), 1 if is_new_sequence else 0)
         call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_GENERATE_TRAINING_DATA, 1)

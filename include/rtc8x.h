@@ -668,11 +668,12 @@ typedef struct rtc8x_nrc_desc {
     int32_t precision;           /* rtc8x_nrc_precision */
     uint64_t seed;               /* weight initialisation (Xavier-uniform from PCG32) */
     int32_t encoding;            /* rtc8x_nrc_encoding */
-    /* The initial loss scale of training: 0 for the default (128 with RTC8X_NRC_FP16, 1 otherwise), or a
-     * power of two in [1, 32768].  The per-sample loss gradient is multiplied by it before it is rounded
-     * to the network's working type and divided out in the optimiser step.  With FP16 the scale is
-     * dynamic (Micikevicius et al. 2018): a step with a non-finite gradient is skipped and the scale
-     * halves (not below 1); after 128 finite steps in a row it doubles (up to 32768). */
+    /* The initial loss scale of training: 0 for the default (128), or a power of two in [1, 32768].  The
+     * per-sample loss gradient is multiplied by it before it is rounded to the network's working type and
+     * divided out in the optimiser step.  The scale is dynamic (Micikevicius et al. 2018), with the same
+     * rule in every precision: a step whose gradient is not finite, or whose scaled per-sample loss
+     * gradient leaves FP16's finite range (|x| > 65504), is skipped and the scale halves (not below 1);
+     * after 128 finite steps in a row it doubles (up to 32768). */
     float loss_scale;
 } rtc8x_nrc_desc;
 

@@ -295,6 +295,8 @@ __device__ __forceinline__ uint32_t device_count(uint32_t n, const int32_t* coun
     return m < n ? (uint32_t)m : n;
 }
 
+constexpr float kHalfMax = 65504.0f;  // the largest finite FP16 value
+
 template <typename T, bool TRAIN, int ENC>
 // (256, 3): the register budget of three workgroups per CU, under which the FP16/BF16 accumulators stay
 // in VGPRs and the kernel runs four waves per SIMD (0.19 ms per 2^21 queries at L = 5, against 0.21 ms
@@ -400,7 +402,9 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
         return;
     }
     // relative L2 over luminance; per-sample gradient 2 (p - t) / (lum(p)^2 + 0.01), times the loss
-    // scale (a power of two) before it is rounded to the working type
+    // scale (a power of two) before it is rounded to the working type.  A scaled gradient outside FP16's
+    // finite range is made infinite in every precision (as the FP16 rounding makes it), so the step is
+    // skipped and the loss scale backs off: all precisions take the same steps.
     float loss = 0.0f;
     const float ls = *loss_scale;
     if (s < n) {
@@ -410,9 +414,11 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
         const float d0 = p0 - target[3 * (size_t)s], d1 = p1 - target[3 * (size_t)s + 1], d2 = p2 - target[3 * (size_t)s + 2];
         loss = (d0 * d0 + d1 * d1 + d2 * d2) / denom;
         T* g = dout + (size_t)s * kOutPad;
-        g[0] = (T)(2.0f * d0 / denom * ls);
-        g[1] = (T)(2.0f * d1 / denom * ls);
-        g[2] = (T)(2.0f * d2 / denom * ls);
+        const float x0 = 2.0f * d0 / denom * ls, x1 = 2.0f * d1 / denom * ls, x2 = 2.0f * d2 / denom * ls;
+        const bool out = !(fabsf(x0) <= kHalfMax && fabsf(x1) <= kHalfMax && fabsf(x2) <= kHalfMax);
+        g[0] = (T)(out ? INFINITY : x0);
+        g[1] = (T)x1;
+        g[2] = (T)x2;
 #pragma unroll
         for (int c = 3; c < (int)kOutPad; ++c) g[c] = (T)0.0f;
     }
@@ -864,10 +870,11 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
         wt[(size_t)a.L * kLayer + in * kOutT + out] = (T)w1;
 }
 
-// Dynamic loss scaling (Micikevicius et al. 2018, "Mixed Precision Training"): state[0] is the scale,
-// a power of two; state[1] counts the finite steps since it last changed.  A step whose gradient is not
-// finite was skipped by the optimiser: the scale halves (not below 1).  After kLossScaleGrowth finite
-// steps in a row it doubles (not above kLossScaleMax).  One lane; after the optimiser step.
+// Dynamic loss scaling (Micikevicius et al. 2018, "Mixed Precision Training"), in every precision:
+// state[0] is the scale, a power of two; state[1] counts the finite steps since it last changed.  A step
+// whose gradient is not finite (or whose scaled loss gradient left FP16's range) was skipped by the
+// optimiser: the scale halves (not below 1).  After kLossScaleGrowth finite steps in a row it doubles
+// (not above kLossScaleMax).  One lane; after the optimiser step.
 constexpr float kLossScaleMax = 32768.0f;
 constexpr float kLossScaleGrowth = 128.0f;
 __global__ void loss_scale_kernel(const uint32_t* __restrict__ bad, float* __restrict__ state)
@@ -1051,8 +1058,7 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
     RTC8X_HIP(hipMemset(h->ema, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->wt, 0, h->tsize * h->PT));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_convert<T>(h, nullptr)));
-    const float ls[2] = {h->desc.loss_scale > 0.0f ? h->desc.loss_scale : (h->desc.precision == RTC8X_NRC_FP16 ? 128.0f : 1.0f),
-                         0.0f};
+    const float ls[2] = {h->desc.loss_scale > 0.0f ? h->desc.loss_scale : 128.0f, 0.0f};
     RTC8X_HIP(hipMemcpy(h->loss_scale, ls, sizeof(ls), hipMemcpyHostToDevice));
     RTC8X_HIP(hipDeviceSynchronize());
     h->step = 0;
@@ -1271,10 +1277,8 @@ try {
     a.P_mlp = h->P_mlp;
     a.L = h->L;
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_adam<T>(h, s, a)));
-    if (h->desc.precision == RTC8X_NRC_FP16) {
-        hipLaunchKernelGGL(nrc::loss_scale_kernel, dim3(1), dim3(1), 0, s, (const uint32_t*)h->bad, h->loss_scale);
-        RTC8X_HIP(hipGetLastError());
-    }
+    hipLaunchKernelGGL(nrc::loss_scale_kernel, dim3(1), dim3(1), 0, s, (const uint32_t*)h->bad, h->loss_scale);
+    RTC8X_HIP(hipGetLastError());
     if (loss_on_cpu) {
         const uint32_t nw = 4 * ((n + 255) / 256);
         std::vector<float> parts(nw);

@@ -56,7 +56,8 @@ class NeuralRadianceCache:
                    encoding: int = _abi.NRC_HASH_GRID, loss_scale: float = 0.0) -> None:
         """initialize(posEnc, numHiddenLayers, learningRate) (network_interface.cu:50-134); Adam epsilon
         1e-15 with the hash grid and 1e-8 with triangle waves, as the reference configures them.
-        loss_scale: the initial loss scale (0: 128 with FP16, 1 otherwise; dynamic with FP16)."""
+        loss_scale: the initial loss scale (0: 128; dynamic, with FP16's range as the overflow rule in
+        every precision)."""
         eps = 1e-15 if encoding == _abi.NRC_HASH_GRID else 1e-8
         d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, eps, 1e-6, ema_decay, precision, seed, encoding,
                          float(loss_scale))

@@ -59,14 +59,15 @@ def test_dense128_nrc_against_plain_and_the_fp64_oracle():
     mean((X - A)(X - B)) is X's MSE against the converged image with the oracle's noise removed.
 
     Pinned (measured on MI355X at 96 x 96, triangle-wave encoding: plain 0.1015 relative RMSE, all of it
-    noise (0.1023); NRC BF16 0.146 with noise 0.069, FP32 0.150 with noise 0.070; untrained cache 0.454):
+    noise (0.1023); NRC BF16 0.0570 with noise 0.0469, FP32 0.0574 with noise 0.0504; untrained cache 0.454):
     * the plain tracer is unbiased: its error is its noise, about 0.1 relative RMSE at 16 spp here;
-    * the online-trained cache takes a third of the noise out (two NRC renders with different seeds
+    * the online-trained cache takes more than half of the noise out (two NRC renders with different seeds
       differ less than two plain renders do: the variance it removes from every path suffix);
-    * what it costs is bias: the NRC's relative RMSE against the oracle is its noise plus the cache's
-      error, above the plain tracer's at 16 spp on this small scene, so the NRC loses here; BF16 and
-      FP32 networks land within 15 % of each other;
-    * training is what gains: the untrained cache's error is three times the trained one's."""
+    * what it costs is bias, the NRC's relative RMSE against the oracle being its noise plus the cache's
+      error; with the dynamic loss scale skipping the first, overflowing steps the bias stays small and
+      the NRC's error is about half of the plain tracer's at equal samples; BF16 and FP32 networks land
+      within 15 % of each other;
+    * training is what gains: the untrained cache's error is several times the trained one's."""
     from rtcamp8_amd import _abi
     from rtcamp8_amd import scene as sc
     from rtcamp8_amd.convergence import cross_mse, nrc_image, plain_image
@@ -104,8 +105,9 @@ def test_dense128_nrc_against_plain_and_the_fp64_oracle():
     assert 0.08 < plain_rel < 0.13 and abs(plain_noise / plain_rel - 1) < 0.15
     for name in ("bf16", "fp32"):
         rel, noise, _ = res[name]
-        assert np.isfinite(rel) and 0.1 < rel < 0.2, (name, rel)  # bias included: above the plain tracer's
-        assert noise < 0.8 * plain_noise, (name, noise, plain_noise)
+        assert np.isfinite(rel) and 0.03 < rel < 0.08, (name, rel)  # bias included, below the plain tracer's
+        assert rel < 0.8 * plain_rel, (name, rel, plain_rel)
+        assert noise < 0.6 * plain_noise, (name, noise, plain_noise)
     assert abs(res["bf16"][0] / res["fp32"][0] - 1) < 0.15
     assert untrained_rel > 2.0 * max(res["bf16"][0], res["fp32"][0])
 

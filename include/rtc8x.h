@@ -631,12 +631,12 @@ size_t rtc8x_nrc_scratch_bytes(uint32_t width, uint32_t height);
  * polar angles scaled by 1/pi, azimuths by 1/(2 pi)), identity on the 6 reflectance dims, then a
  * constant 1 (the first layer's bias) and one zero pad.  Network: num_hidden_layers ReLU layers of
  * width 64 and a linear layer to RGB, no biases (FullyFusedMLP).  Loss: relative L2 over luminance,
- * sum_c (p_c - t_c)^2 / (lum(p)^2 + 0.01) with the normaliser held constant.  Optimiser: Adam with L2
+ * the mean over the batch's 3n outputs of (p_c - t_c)^2 / (lum(p)^2 + 0.01) with the normaliser held
+ * constant (tiny-cuda-nn's RelativeL2Luminance).  Optimiser: Adam with L2
  * regularisation, wrapped by an exponential moving average of the weights that inference uses.
  * Weights and activations are FP16 or BF16 with FP32 accumulation (MFMA); master weights, moments and
- * the EMA are FP32.  The per-sample loss gradient enters the backward pass multiplied by the loss scale
- * (rtc8x_nrc_desc.loss_scale; the mean over the batch's 3n terms is taken in the optimiser step, which
- * divides both out).
+ * the EMA are FP32.  The per-sample gradient of the mean loss enters the backward pass multiplied by
+ * the loss scale (rtc8x_nrc_desc.loss_scale), which the optimiser step divides out.
  * ---------------------------------------------------------------------------------------------- */
 typedef struct rtc8x_nrc rtc8x_nrc;
 
@@ -668,12 +668,15 @@ typedef struct rtc8x_nrc_desc {
     int32_t precision;           /* rtc8x_nrc_precision */
     uint64_t seed;               /* weight initialisation (Xavier-uniform from PCG32) */
     int32_t encoding;            /* rtc8x_nrc_encoding */
-    /* The initial loss scale of training: 0 for the default (128), or a power of two in [1, 32768].  The
-     * per-sample loss gradient is multiplied by it before it is rounded to the network's working type and
-     * divided out in the optimiser step.  The scale is dynamic (Micikevicius et al. 2018), with the same
-     * rule in every precision: a step whose gradient is not finite, or whose scaled per-sample loss
-     * gradient leaves FP16's finite range (|x| > 65504), is skipped and the scale halves (not below 1);
-     * after 128 finite steps in a row it doubles (up to 32768). */
+    /* The initial loss scale of training: 0 for the default (128, tiny-cuda-nn's for FP16), or a power of
+     * two in [1, 2^24].  The per-sample gradient of the mean loss is multiplied by it before it is rounded
+     * to the network's working type and divided out in the optimiser step.  The scale is dynamic
+     * (Micikevicius et al. 2018), with the same rule in every precision: a step whose gradient is not
+     * finite, or whose scaled per-sample loss gradient leaves FP16's finite range (|x| > 65504), is
+     * skipped and the scale halves (not below 1); after 128 finite steps in a row it doubles (up to 2^24).
+     * With the default the loss gradients of this cache stay far inside FP16's range and every step is
+     * applied, as in the reference; a large initial scale skips a fresh network's first steps, those
+     * whose gradients hold outliers, until the scale has backed off. */
     float loss_scale;
 } rtc8x_nrc_desc;
 
@@ -721,8 +724,8 @@ rtc8x_status rtc8x_nrc_get_loss_scale(const rtc8x_nrc* nrc, float* scale);
 rtc8x_status rtc8x_nrc_get_weights(const rtc8x_nrc* nrc, int32_t which, float* host, size_t count);
 /* Replaces the master weights (and resets optimiser state and EMA) from a host array. */
 rtc8x_status rtc8x_nrc_set_weights(rtc8x_nrc* nrc, const float* host, size_t count);
-/* Parity probes: the 64-dim encoding of n queries (out[64n]), and the summed per-sample loss gradient
- * with respect to the weights for one batch without an update (grad[num_params], device). */
+/* Parity probes: the 64-dim encoding of n queries (out[64n]), and the gradient of the batch's mean loss
+ * with respect to the weights, without an update (grad[num_params], device). */
 rtc8x_status rtc8x_nrc_encode(void* stream, const float* in, uint32_t n, float* out);
 /* The network's own input encoding (either kind), with its training (which = 0) or inference (1)
  * hash tables; out[64n]. */

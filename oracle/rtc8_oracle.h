@@ -82,7 +82,9 @@ void oracle_nrc_propagate(const rtc8x_frame_params* p);
 void oracle_nrc_shuffle(const rtc8x_frame_params* p);
 void oracle_nrc_accumulate(const rtc8x_frame_params* p);
 
-/* NRC network (FP64 forward / gradient, FP32 Adam). round_mode: 0 none, 1 FP16, 2 BF16. */
+/* NRC network (FP64 forward / gradient, FP32 Adam). round_mode: 0 none, 1 FP16, 2 BF16.  The loss and
+ * gradient functions return sums over the n samples (the mean loss is the sum over 3n); Adam takes the
+ * gradient of the mean loss. */
 uint32_t oracle_nrc_num_params(uint32_t L);
 void oracle_nrc_encode(const rtc8x_radiance_query* q, uint32_t n, double* out);
 void oracle_nrc_forward(const float* weights, uint32_t L, const rtc8x_radiance_query* q, uint32_t n, double* out3,
@@ -100,8 +102,8 @@ void oracle_nrc_forward_ex(const float* weights, uint32_t L, int32_t encoding, c
                            double* out3, int round_mode);
 double oracle_nrc_gradient_ex(const float* weights, uint32_t L, int32_t encoding, const rtc8x_radiance_query* q,
                               const float* target, uint32_t n, double* grad, int round_mode);
-void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_sum, uint32_t P, uint32_t n,
-                     float lr, float beta1, float beta2, float eps, float l2, float decay, uint64_t step);
+void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_mean, uint32_t P, float lr,
+                     float beta1, float beta2, float eps, float l2, float decay, uint64_t step);
 float oracle_round_fp16(float x);
 float oracle_round_bf16(float x);
 

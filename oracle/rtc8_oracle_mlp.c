@@ -331,16 +331,15 @@ double oracle_nrc_gradient_ex(const float* weights, uint32_t L, int32_t encoding
     return run(weights, L, encoding, q, target, n, NULL, grad, round_mode);
 }
 
-/* One Adam + EMA step in FP32, in the library's expression order, from a summed gradient. */
-void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_sum, uint32_t P, uint32_t n,
-                     float lr, float beta1, float beta2, float eps, float l2, float decay, uint64_t step)
+/* One Adam + EMA step in FP32, in the library's expression order, from the gradient of the mean loss. */
+void oracle_nrc_adam(float* master, float* m, float* v, float* ema, const float* grad_mean, uint32_t P, float lr,
+                     float beta1, float beta2, float eps, float l2, float decay, uint64_t step)
 {
-    float inv_scale = (float)(1.0 / (3.0 * (double)n));
     float bc1 = (float)(1.0 - pow((double)beta1, (double)step));
     float bc2 = (float)(1.0 - pow((double)beta2, (double)step));
     for (uint32_t i = 0; i < P; ++i) {
         float w0 = master[i];
-        float g = grad_sum[i] * inv_scale + l2 * w0;
+        float g = grad_mean[i] + l2 * w0;
         float mi = beta1 * m[i] + (1.0f - beta1) * g;
         float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
         m[i] = mi;

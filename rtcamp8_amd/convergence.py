@@ -56,14 +56,15 @@ class NrcRun:
 def nrc_image(scene: DeviceScene, spp: int, seed: int, stream=None, num_hidden_layers: int = 2,
               precision: int = 0, spp_per_frame: int = 1, encoding: int = 1, radiance_scale: float = 0.0,
               ema_decay: float = 0.99, first_frame_iterations: int = 8, train: bool = True,
-              train_tracking: int | None = None) -> NrcRun:
+              train_tracking: int | None = None, loss_scale: float = 0.0) -> NrcRun:
     """NRC-accelerated rendering in the reference's batch schedule (renderer_main.cpp:1554-1644): frame 0
     runs 8 training iterations, later frames one, then `spp_per_frame` samples traced into the cache.
     Wall time includes all training.  train=False renders with the untrained (initial) cache.
-    train_tracking: the training launches' tracker (None: the scene's)."""
+    train_tracking: the training launches' tracker (None: the scene's).  loss_scale: the cache's initial
+    loss scale (0: 128)."""
     r = NrcRenderer(scene, num_hidden_layers=num_hidden_layers, precision=precision, stream=stream, encoding=encoding,
                     radiance_scale=radiance_scale, ema_decay=ema_decay, first_frame_iterations=first_frame_iterations,
-                    train_tracking=train_tracking)
+                    train_tracking=train_tracking, loss_scale=loss_scale)
     r.reset(seed)
     t0 = time.perf_counter()
     frames = (spp + spp_per_frame - 1) // spp_per_frame

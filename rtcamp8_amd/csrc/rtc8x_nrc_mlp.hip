@@ -401,12 +401,13 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
         }
         return;
     }
-    // relative L2 over luminance; per-sample gradient 2 (p - t) / (lum(p)^2 + 0.01), times the loss
-    // scale (a power of two) before it is rounded to the working type.  A scaled gradient outside FP16's
-    // finite range is made infinite in every precision (as the FP16 rounding makes it), so the step is
-    // skipped and the loss scale backs off: all precisions take the same steps.
+    // relative L2 over luminance, the mean over the batch's 3n outputs (the loss tiny-cuda-nn's
+    // RelativeL2Luminance computes); per-sample gradient 2 (p - t) / (lum(p)^2 + 0.01) / (3n), times the
+    // loss scale before it is rounded to the working type.  A scaled gradient outside FP16's finite range
+    // is made infinite in every precision (as the FP16 rounding makes it), so the step is skipped and the
+    // loss scale backs off: all precisions take the same steps.
     float loss = 0.0f;
-    const float ls = *loss_scale;
+    const float ls = *loss_scale / (3.0f * (float)n);
     if (s < n) {
         const float p0 = pred[wave][lane][0], p1 = pred[wave][lane][1], p2 = pred[wave][lane][2];
         const float lum = (float)0.2126729 * p0 + (float)0.7151522 * p1 + (float)0.0721750 * p2;
@@ -831,7 +832,6 @@ __global__ __launch_bounds__(512) void grad_reduce_kernel(const float* __restric
 
 struct AdamArgs {
     float lr, beta1, beta2, eps, l2, decay;
-    float inv_scale;      // 1 / (3 n): the loss scale of the backward pass
     float bc1, bc2;       // 1 - beta^t
     float ema_bc;         // 1 - decay^t
     uint32_t P, L;        // P = all parameters (MLP, then hash tables)
@@ -848,7 +848,7 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
     if (i >= a.P) return;
     if (*bad) return;  // a non-finite gradient skips the whole step
     const float w0 = master[i];
-    const float g = grad[i] * (a.inv_scale / *loss_scale) + a.l2 * w0;
+    const float g = grad[i] / *loss_scale + a.l2 * w0;  // exact: the scale is a power of two
     const float mi = a.beta1 * m[i] + (1.0f - a.beta1) * g;
     const float vi = a.beta2 * v[i] + (1.0f - a.beta2) * g * g;
     m[i] = mi;
@@ -875,7 +875,7 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
 // whose gradient is not finite (or whose scaled loss gradient left FP16's range) was skipped by the
 // optimiser: the scale halves (not below 1).  After kLossScaleGrowth finite steps in a row it doubles
 // (not above kLossScaleMax).  One lane; after the optimiser step.
-constexpr float kLossScaleMax = 32768.0f;
+constexpr float kLossScaleMax = 16777216.0f;  // 2^24
 constexpr float kLossScaleGrowth = 128.0f;
 __global__ void loss_scale_kernel(const uint32_t* __restrict__ bad, float* __restrict__ state)
 {
@@ -1103,9 +1103,9 @@ try {
     RTC8X_REQUIRE(d->learning_rate > 0.0f && d->beta1 >= 0.0f && d->beta1 < 1.0f && d->beta2 >= 0.0f && d->beta2 < 1.0f &&
                       d->epsilon > 0.0f && d->ema_decay >= 0.0f && d->ema_decay < 1.0f && d->l2_reg >= 0.0f,
                   "rtc8x_nrc_create: bad optimiser parameters");
-    RTC8X_REQUIRE(d->loss_scale == 0.0f || (d->loss_scale >= 1.0f && d->loss_scale <= 32768.0f &&
+    RTC8X_REQUIRE(d->loss_scale == 0.0f || (d->loss_scale >= 1.0f && d->loss_scale <= nrc::kLossScaleMax &&
                                              std::exp2(std::floor(std::log2(d->loss_scale))) == d->loss_scale),
-                  "rtc8x_nrc_create: loss_scale must be 0 or a power of two in [1, 32768]");
+                  "rtc8x_nrc_create: loss_scale must be 0 or a power of two in [1, 2^24]");
     RTC8X_REQUIRE(d->encoding == RTC8X_NRC_TRIANGLE_WAVE || d->encoding == RTC8X_NRC_HASH_GRID,
                   "rtc8x_nrc_create: unknown encoding %d", d->encoding);
     *out = nullptr;
@@ -1269,7 +1269,6 @@ try {
     a.eps = d.epsilon;
     a.l2 = d.l2_reg;
     a.decay = d.ema_decay;
-    a.inv_scale = (float)(1.0 / (3.0 * (double)n));
     a.bc1 = (float)(1.0 - std::pow((double)d.beta1, (double)h->step));
     a.bc2 = (float)(1.0 - std::pow((double)d.beta2, (double)h->step));
     a.ema_bc = (float)(1.0 - std::pow((double)d.ema_decay, (double)h->step));

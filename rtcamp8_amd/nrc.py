@@ -56,8 +56,8 @@ class NeuralRadianceCache:
                    encoding: int = _abi.NRC_HASH_GRID, loss_scale: float = 0.0) -> None:
         """initialize(posEnc, numHiddenLayers, learningRate) (network_interface.cu:50-134); Adam epsilon
         1e-15 with the hash grid and 1e-8 with triangle waves, as the reference configures them.
-        loss_scale: the initial loss scale (0: 128; dynamic, with FP16's range as the overflow rule in
-        every precision)."""
+        loss_scale: the initial loss scale (0: 128, tiny-cuda-nn's; dynamic, with FP16's range as the
+        overflow rule in every precision; a large one skips a fresh network's outlying first steps)."""
         eps = 1e-15 if encoding == _abi.NRC_HASH_GRID else 1e-8
         d = _abi.NrcDesc(num_hidden_layers, learning_rate, 0.9, 0.99, eps, 1e-6, ema_decay, precision, seed, encoding,
                          float(loss_scale))
@@ -190,12 +190,12 @@ class NrcRenderer:
     def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
                  precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121,
                  encoding: int = _abi.NRC_HASH_GRID, ema_decay: float = 0.99, first_frame_iterations: int = 8,
-                 samples_per_launch: int = 4, train_tracking: int | None = None):
+                 samples_per_launch: int = 4, train_tracking: int | None = None, loss_scale: float = 0.0):
         """samples_per_launch: samples traced into the cache by one launch (and inferred and accumulated
         together); the sums equal those of one-sample launches.  train_tracking: the tracker of the
         training-data launches (None: the scene's); both trackers give training data of the same
         distribution, and the global one runs the training entry faster (its local variant holds more
-        registers and runs two waves per SIMD)."""
+        registers and runs two waves per SIMD).  loss_scale: the cache's initial loss scale (0: 128)."""
         cfg = scene.cfg
         self.scene = scene
         self.stream = stream
@@ -212,7 +212,8 @@ class NrcRenderer:
         self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n))
         self.have_cost = False
         self.cache = NeuralRadianceCache()
-        self.cache.initialize(num_hidden_layers, learning_rate, precision, encoding=encoding, ema_decay=ema_decay)
+        self.cache.initialize(num_hidden_layers, learning_rate, precision, encoding=encoding, ema_decay=ema_decay,
+                              loss_scale=loss_scale)
         self.first_frame_iterations = first_frame_iterations
         self.frame_rng = Mt19937(frame_rng_seed)
         self.buf_idx = 1  # the first iteration toggles to 0 (renderer_main.cpp:1558-1559)

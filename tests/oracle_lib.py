@@ -75,8 +75,8 @@ def load_oracle(path) -> C.CDLL:
         "oracle_nrc_encode_ex": (None, [_P, C.c_uint32, C.c_int32, _P, C.c_uint32, _P]),
         "oracle_nrc_forward_ex": (None, [_P, C.c_uint32, C.c_int32, _P, C.c_uint32, _P, C.c_int]),
         "oracle_nrc_gradient_ex": (C.c_double, [_P, C.c_uint32, C.c_int32, _P, _P, C.c_uint32, _P, C.c_int]),
-        "oracle_nrc_adam": (None, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float,
-                                   C.c_float, C.c_float, C.c_float, C.c_uint64]),
+        "oracle_nrc_adam": (None, [_P, _P, _P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, C.c_uint64]),
         "oracle_round_fp16": (C.c_float, [C.c_float]),
         "oracle_expf": (C.c_float, [C.c_float]),
         "oracle_powf": (C.c_float, [C.c_float, C.c_float]),

@@ -83,14 +83,14 @@ def test_fp16_and_bf16_rounding_emulation():
 
 def test_adam_first_step_closed_form():
     """With zero moments, step 1 moves each weight by lr * g / (|g| + eps') (bias-corrected Adam)."""
-    P, n = 1000, 16
+    P = 1000
     rng = np.random.default_rng(6)
     master = rng.standard_normal(P).astype(np.float32)
     m, v, ema = np.zeros(P, np.float32), np.zeros(P, np.float32), np.zeros(P, np.float32)
-    g = rng.standard_normal(P).astype(np.float32) * 3 * n
+    g = rng.standard_normal(P).astype(np.float32)
     w0 = master.copy()
-    oracle().oracle_nrc_adam(ptr(master), ptr(m), ptr(v), ptr(ema), ptr(g), P, n, 1e-2, 0.9, 0.99, 1e-8, 0.0, 0.99, 1)
-    gt = g.astype(np.float64) / (3 * n)
+    oracle().oracle_nrc_adam(ptr(master), ptr(m), ptr(v), ptr(ema), ptr(g), P, 1e-2, 0.9, 0.99, 1e-8, 0.0, 0.99, 1)
+    gt = g.astype(np.float64)
     expect = w0 - 1e-2 * gt / (np.abs(gt) + 1e-8)
     assert np.allclose(master, expect, rtol=0, atol=2e-6)
     assert np.allclose(ema, 0.01 * master, rtol=1e-6)

@@ -67,6 +67,11 @@ rtc8x_status rtc8x_event_create(void** event);
 rtc8x_status rtc8x_event_destroy(void* event);
 rtc8x_status rtc8x_event_record(void* event, void* stream);
 rtc8x_status rtc8x_event_elapsed_ms(float* ms, void* start, void* end);
+/* Work queued on `stream` after this call waits for the work before the event's last record
+ * (hipStreamWaitEvent; cudaStreamWaitEvent in CUDA).  The NRC renderer's host code uses it to trace the
+ * next launch on a second stream while the network trains; no counterpart in the reference, whose
+ * renderer runs one stream. */
+rtc8x_status rtc8x_stream_wait_event(void* stream, void* event);
 
 /* Measurement kernels (no counterpart in the reference).
  * rtc8x_copy_async: device-to-device copy by a plain 16-byte-per-lane kernel on `stream` (dst, src and

@@ -373,6 +373,7 @@ PROTOTYPES = {
     "rtc8x_event_destroy": (_S, [_P]),
     "rtc8x_event_record": (_S, [_P, _P]),
     "rtc8x_event_elapsed_ms": (_S, [C.POINTER(C.c_float), _P, _P]),
+    "rtc8x_stream_wait_event": (_S, [_P, _P]),
     "rtc8x_grid_build_fbm": (_S, [C.POINTER(FbmDesc), C.POINTER(Grid), _P]),
     "rtc8x_grid_upload": (_S, [C.POINTER(Grid), C.POINTER(Grid)]),
     "rtc8x_grid_destroy": (_S, [C.POINTER(Grid)]),

@@ -177,6 +177,14 @@ try {
 }
 RTC8X_CATCH_ALL("rtc8x_event_record")
 
+rtc8x_status rtc8x_stream_wait_event(void* stream, void* event)
+try {
+    RTC8X_REQUIRE(event != nullptr, "rtc8x_stream_wait_event: event is NULL");
+    RTC8X_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_stream_wait_event")
+
 rtc8x_status rtc8x_event_elapsed_ms(float* ms, void* start, void* end)
 try {
     RTC8X_REQUIRE(ms && start && end, "rtc8x_event_elapsed_ms: NULL argument");

@@ -119,6 +119,27 @@ class Stream:
             self.handle = None
 
 
+class Event:
+    """A HIP event for ordering streams: record() marks the work queued so far on a stream, and
+    make_wait() makes later work on another stream wait for it."""
+
+    def __init__(self):
+        e = C.c_void_p()
+        call("rtc8x_event_create", C.byref(e))
+        self.handle = e.value
+
+    def record(self, stream) -> None:
+        call("rtc8x_event_record", C.c_void_p(self.handle), C.c_void_p(stream))
+
+    def make_wait(self, stream) -> None:
+        call("rtc8x_stream_wait_event", C.c_void_p(stream), C.c_void_p(self.handle))
+
+    def destroy(self) -> None:
+        if self.handle:
+            call("rtc8x_event_destroy", C.c_void_p(self.handle))
+            self.handle = None
+
+
 class EventTimer:
     """A pair of HIP events recorded on a stream (cudau::Timer, common/utils/cuda_util.h:257-300)."""
 

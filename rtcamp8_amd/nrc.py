@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _abi
 from ._lib import call, lib
-from .device import DeviceBuffer, wait_stream
+from .device import DeviceBuffer, Event, Stream, wait_stream
 from .scene import DeviceScene
 
 TRAIN_BATCH = _abi.NUM_TRAINING_DATA_PER_FRAME // 4  # renderer_main.cpp:1595: 4 steps of 16384 per iteration
@@ -185,17 +185,26 @@ class NrcRenderer:
     """Frame loop of the NRC-accelerated tracer (renderer_main.cpp:1554-1638): per frame, training
     iterations (8 on the first frame, then 1) of prepare -> trace training paths -> infer terminal
     queries -> propagate -> shuffle -> 4 x train(16384); then per sample trace with NRC -> infer ->
-    accumulate.  The plain tracer (rtcamp8_amd.tracer) stays the unbiased reference."""
+    accumulate.  The plain tracer (rtcamp8_amd.tracer) stays the unbiased reference.
+
+    The traces run on a second stream of the renderer's own.  A trace that directly follows a training
+    iteration starts after that iteration's shuffle and runs beside its four network steps: the trace
+    neither reads the network nor touches the shuffled training buffers the steps read, and the
+    network is read again only by the inference that waits for the trace.  Every other trace waits for
+    all work queued on the renderer's stream.  Results equal those of one stream."""
 
     def __init__(self, scene: DeviceScene, num_hidden_layers: int = 2, learning_rate: float = 1e-2,
                  precision: int = _abi.NRC_FP16, stream=None, radiance_scale: float = 0.0, frame_rng_seed: int = 72139121,
                  encoding: int = _abi.NRC_HASH_GRID, ema_decay: float = 0.99, first_frame_iterations: int = 8,
-                 samples_per_launch: int = 4, train_tracking: int | None = None, loss_scale: float = 0.0):
+                 samples_per_launch: int = 4, train_tracking: int | None = None, loss_scale: float = 0.0,
+                 overlap: bool = True):
         """samples_per_launch: samples traced into the cache by one launch (and inferred and accumulated
         together); the sums equal those of one-sample launches.  train_tracking: the tracker of the
         training-data launches (None: the scene's); both trackers give training data of the same
         distribution, and the global one runs the training entry faster (its local variant holds more
-        registers and runs two waves per SIMD).  loss_scale: the cache's initial loss scale (0: 128)."""
+        registers and runs two waves per SIMD).  loss_scale: the cache's initial loss scale (0: 128).
+        overlap: traces that follow a training iteration run beside its network steps (False: every
+        trace waits for all earlier work)."""
         cfg = scene.cfg
         self.scene = scene
         self.stream = stream
@@ -223,17 +232,28 @@ class NrcRenderer:
         self.frame = 0
         self.knobs = (0, 0)  # (event_eighths, tile_shape) of the trace kernel's schedule; 0 = defaults
         self.train_tracking = train_tracking
+        self.side = Stream()
+        self.gate = Event()    # recorded after a training iteration's shuffle, before its network steps
+        self.fence = Event()
+        self.traced = Event()
+        self.gate_open = False  # no launch on the renderer's buffers since the gate was recorded
+        self.overlap = overlap
         call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(cfg.rng_seed))
         self.accum.fill(0)
 
     def reset(self, seed: int) -> None:
         """Restarts the accumulation from new per-pixel seeds (the network keeps its training)."""
+        wait_stream(self.stream)
+        self.gate_open = False
         cfg = self.scene.cfg
         call("rtc8x_seed_rng", C.c_void_p(self.rng.ptr), cfg.width, cfg.height, C.c_uint64(seed))
         self.accum.fill(0)
         self.samples = 0
 
     def params(self) -> _abi.FrameParams:
+        """The frame parameters of a launch on the renderer's buffers; closes the gate, so that the next
+        trace waits for everything queued before it."""
+        self.gate_open = False
         p = self.scene.params()
         p.rng_states = self.rng.ptr
         p.accum = self.accum.ptr
@@ -245,19 +265,38 @@ class NrcRenderer:
         p.event_eighths, p.tile_shape = self.knobs
         return p
 
+    def _trace(self, after_gate: bool, issue) -> None:
+        """Queues issue(side stream) after the gate (after_gate) or after all work on the renderer's
+        stream; the renderer's stream then waits for it."""
+        side = self.side.handle
+        if after_gate:
+            self.gate.make_wait(side)
+        else:
+            self.fence.record(self.stream)
+            self.fence.make_wait(side)
+        issue(C.c_void_p(side))
+        self.traced.record(side)
+        self.traced.make_wait(self.stream)
+
     def train_iteration(self, is_new_sequence: bool, want_loss: bool = False):
         s = C.c_void_p(self.stream)
+        after_gate = self.gate_open and self.overlap
         self.buf_idx = (self.buf_idx + 1) % 2
         p = self.params()
         if self.train_tracking is not None:
             p.tracking = self.train_tracking
         p.nrc.offset_to_select_unbiased_path = self.frame_rng()
-        call("rtc8x_nrc_prepare", s, C.byref(p), 1 if is_new_sequence else 0)
-        call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_GENERATE_TRAINING_DATA, 1)
+
+        def trace(side):
+            call("rtc8x_nrc_prepare", side, C.byref(p), 1 if is_new_sequence else 0)
+            call("rtc8x_trace", side, C.byref(p), _abi.ENTRY_GENERATE_TRAINING_DATA, 1)
+
+        self._trace(after_gate, trace)
         # the training size stays on the device: no host synchronisation in the iteration
         self.cache.infer_training(self.stream, p)
         call("rtc8x_nrc_propagate", s, C.byref(p))
         call("rtc8x_nrc_shuffle", s, C.byref(p))
+        self.gate.record(self.stream)
         loss = None
         for step in range(4):
             off = step * TRAIN_BATCH
@@ -267,6 +306,7 @@ class NrcRenderer:
         self.stats.train_iterations += 1
         self.stats.train_steps += 4
         self.stats.last_loss = loss
+        self.gate_open = True
         return loss
 
     def render_samples(self, k: int = 1) -> None:
@@ -274,15 +314,22 @@ class NrcRenderer:
         network call and accumulated in sample order."""
         assert 1 <= k <= self.samples_per_launch
         s = C.c_void_p(self.stream)
+        after_gate = self.gate_open and self.overlap
         p = self.params()
         p.nrc.num_samples = k
-        if self.have_cost:
-            call("rtc8x_order_by_cost", s, C.c_void_p(self.cost.ptr), self.width * self.height, C.c_void_p(self.order.ptr),
-                 C.c_void_p(self.order_scratch.ptr))
+        ordered = self.have_cost
+        if ordered:
             p.work_order = self.order.ptr
         p.pixel_cost = self.cost.ptr
         self.have_cost = True
-        call("rtc8x_trace", s, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, k)
+
+        def trace(side):
+            if ordered:
+                call("rtc8x_order_by_cost", side, C.c_void_p(self.cost.ptr), self.width * self.height,
+                     C.c_void_p(self.order.ptr), C.c_void_p(self.order_scratch.ptr))
+            call("rtc8x_trace", side, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, k)
+
+        self._trace(after_gate, trace)
         nq = round_up(self.width * self.height, 128) * k
         self.cache.infer(self.stream, self.buffers.inference_queries.ptr, nq, self.buffers.inferred_radiance.ptr)
         call("rtc8x_nrc_accumulate", s, C.byref(p))
@@ -315,6 +362,11 @@ class NrcRenderer:
                 "lookups": int(s[_abi.STAT_LOOKUPS])}
 
     def close(self) -> None:
+        wait_stream(self.stream)
+        self.side.synchronize()
+        for e in (self.gate, self.fence, self.traced):
+            e.destroy()
+        self.side.destroy()
         self.cache.finalize()
         self.buffers.free()
         for b in (self.rng, self.accum, self.stats_buf, self.cost, self.order, self.order_scratch):

@@ -337,6 +337,40 @@ def test_fp8_render_matches_bf16_render():
     ds.close()
 
 
+def test_frame_loop_overlap_matches_one_stream():
+    """The frame loop with each trace beside the preceding training iteration's network steps (the
+    default) against the loop with every trace after all earlier work: images, weights and counters equal
+    bit for bit over three frames (frame 0's 8 iterations, then 1 per frame, 2 spp each), on the caller's
+    stream and on the null stream.  Triangle-wave encoding, whose training is deterministic."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd import scene as sc
+    from rtcamp8_amd.device import Stream
+    from rtcamp8_amd.nrc import NrcRenderer
+
+    cfg = sc.CONFIGS["dense128"].with_(width=96, height=64, fbm=sc.FbmConfig(resolution=64, seed=0x5EED0001))
+    ds = sc.DeviceScene(cfg)
+    st = Stream()
+    for stream in (st.handle, None):
+        out = {}
+        for overlap in (True, False):
+            print(f"[frame loop] overlap {overlap}, stream {'own' if stream else 'null'}", flush=True)
+            r = NrcRenderer(ds, num_hidden_layers=2, encoding=_abi.NRC_TRIANGLE_WAVE, stream=stream, overlap=overlap,
+                            samples_per_launch=2, radiance_scale=10.0)
+            r.reset(0x0F0F)
+            for _ in range(3):
+                r.render_frame(spp=2)
+            out[overlap] = (r.image(), r.cache.get_weights(0), r.cache.get_weights(1), r.counters())
+            r.close()
+        (ia, wa, ea, ca), (ib, wb, eb, cb) = out[True], out[False]
+        assert np.isfinite(ia).all() and ia.mean() > 0
+        assert np.array_equal(ia.view(np.uint64), ib.view(np.uint64))
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+        assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
+        assert ca == cb and ca["paths"] > 0
+    st.destroy()
+    ds.close()
+
+
 FP8_RENDER_REL_RMS = 0.016
 FP8_RENDER_MEAN_SHIFT = 0.0065
 

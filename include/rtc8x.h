@@ -558,7 +558,7 @@ typedef struct rtc8x_frame_params {
     uint64_t* stats;                         /* optional RTC8X_STAT_COUNT counters */
     /* Optional scheduling (results do not depend on it): pixel_cost[i] receives the density lookups of
      * launch pixel i (row-major over the launch) in this launch; work_order, a permutation of the
-     * launch's pixels, is the order in which lanes take them (see rtc8x_order_by_cost).  Without
+     * launch's pixels, is the order in which lanes take them (see rtc8x_order_by_tile_cost).  Without
      * work_order lanes take pixel tiles (tile_shape below) in order (GENERATE_TRAINING_DATA: the
      * unbiased-suffix pixels first). */
     uint32_t* pixel_cost;
@@ -586,6 +586,16 @@ rtc8x_status rtc8x_trace(void* stream, const rtc8x_frame_params* params, int32_t
  * scratch: rtc8x_order_scratch_bytes(n) bytes. */
 rtc8x_status rtc8x_order_by_cost(void* stream, const uint32_t* cost, uint32_t n, uint32_t* order, void* scratch);
 size_t rtc8x_order_scratch_bytes(uint32_t n);
+/* The same for a width x height image by 8 x 8 tiles: tiles sorted by their summed cost S (saturating
+ * at 2^32 - 1) in quarter-octave classes below the costliest tile's, k = min(31, q(max S) - q(S)) with
+ * q(c) = 4 floor(log2 c) + the two bits below c's leading one (k = 31 for S = 0), stable in row-major
+ * tile order; each tile's pixels row-major and contiguous in `order` (edge tiles hold their in-image
+ * pixels only).  The lanes of a wavefront then take neighbouring pixels, whose
+ * paths share cache lines, while the costliest tiles still start first.  No counterpart in the
+ * reference.  scratch: rtc8x_order_tiles_scratch_bytes(width, height) bytes. */
+rtc8x_status rtc8x_order_by_tile_cost(void* stream, const uint32_t* cost, uint32_t width, uint32_t height,
+                                      uint32_t* order, void* scratch);
+size_t rtc8x_order_tiles_scratch_bytes(uint32_t width, uint32_t height);
 
 /* Several instances -- each its own volume, camera, sky, image buffers and, for the NRC entry, cache
  * buffers -- in one persistent launch of a full-image entry (PATH_TRACE or PATH_TRACE_WITH_NRC);

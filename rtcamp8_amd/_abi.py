@@ -407,6 +407,8 @@ PROTOTYPES = {
     "rtc8x_trace_batch_static": (_S, [_P, _P, _P, _P, C.c_uint32, C.c_int32]),
     "rtc8x_order_by_cost": (_S, [_P, _P, C.c_uint32, _P, _P]),
     "rtc8x_order_scratch_bytes": (C.c_size_t, [C.c_uint32]),
+    "rtc8x_order_by_tile_cost": (_S, [_P, _P, C.c_uint32, C.c_uint32, _P, _P]),
+    "rtc8x_order_tiles_scratch_bytes": (C.c_size_t, [C.c_uint32, C.c_uint32]),
     "rtc8x_seed_rng": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint64]),
     "rtc8x_sky_bake": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, _P]),
     "rtc8x_sky_eval": (_S, [_P, C.POINTER(HosekSky), C.POINTER(C.c_float), C.c_uint32, _P, _P, _P]),

@@ -47,7 +47,7 @@ def render_nrc(renderers: Sequence[NrcRenderer], samples: Sequence[int], shares=
         p = r.params()
         p.nrc.num_samples = int(k)
         if r.have_cost:
-            call("rtc8x_order_by_cost", C.c_void_p(stream), C.c_void_p(r.cost.ptr), r.width * r.height,
+            call("rtc8x_order_by_tile_cost", C.c_void_p(stream), C.c_void_p(r.cost.ptr), r.width, r.height,
                  C.c_void_p(r.order.ptr), C.c_void_p(r.order_scratch.ptr))
             p.work_order = r.order.ptr
         p.pixel_cost = r.cost.ptr

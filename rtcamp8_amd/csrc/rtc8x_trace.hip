@@ -1137,6 +1137,93 @@ __global__ __launch_bounds__(256) void order_scatter_kernel(const uint32_t* __re
     }
 }
 
+// Tile-grouped work order: the pixels' costs summed per kOrderTile^2 tile (saturating), the tiles
+// sorted by quarter-octave class below the costliest tile (stable, with the pixels' counting sort
+// above), then each tile's pixels written row-major at its offset
+// (an exclusive scan of the tiles' pixel counts in sorted order; edge tiles hold fewer pixels).  A
+// wavefront's lanes then take the pixels of one tile, whose paths cross nearby voxels.
+constexpr uint32_t kOrderTile = 8;
+
+__device__ __forceinline__ void tile_rect(uint32_t t, uint32_t width, uint32_t height, uint32_t& x0, uint32_t& y0,
+                                          uint32_t& ew, uint32_t& eh)
+{
+    const uint32_t tx = (width + kOrderTile - 1) / kOrderTile;
+    x0 = (t % tx) * kOrderTile;
+    y0 = (t / tx) * kOrderTile;
+    ew = min(kOrderTile, width - x0);
+    eh = min(kOrderTile, height - y0);
+}
+
+// one wavefront per tile
+__global__ __launch_bounds__(256) void tile_cost_kernel(const uint32_t* __restrict__ cost, uint32_t width,
+                                                        uint32_t height, uint32_t n_tiles,
+                                                        uint32_t* __restrict__ tile_cost)
+{
+    const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (t >= n_tiles) return;
+    uint32_t x0, y0, ew, eh;
+    tile_rect(t, width, height, x0, y0, ew, eh);
+    const uint32_t dx = lane % kOrderTile, dy = lane / kOrderTile;
+    uint64_t c = (dx < ew && dy < eh) ? cost[(size_t)(y0 + dy) * width + x0 + dx] : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) tile_cost[t] = c < 0xFFFFFFFFull ? (uint32_t)c : 0xFFFFFFFFu;
+}
+
+// Quarter-octave classes for the tiles, counted down from the costliest tile: class k holds the tiles
+// whose cost lies k quarter octaves below the maximum (k <= 30), class 31 the cheaper ones and the
+// tiles without lookups.  Each tile's cost is replaced by 2^(31 - k) - 1, whose cost_class is k, so
+// that the pixels' counting sort orders the tiles.  One workgroup.
+__device__ __forceinline__ uint32_t quarter_octave(uint32_t c)  // c >= 1; monotone in c
+{
+    const uint32_t e = 31u - (uint32_t)__clz(c);
+    const uint32_t m = e >= 2 ? (c >> (e - 2)) & 3u : (c << (2 - e)) & 3u;
+    return 4u * e + m;
+}
+
+__global__ __launch_bounds__(1024) void tile_class_kernel(uint32_t* __restrict__ tile_cost, uint32_t n_tiles)
+{
+    __shared__ uint32_t part[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    uint32_t m = 0;
+    for (uint32_t i = tid; i < n_tiles; i += 1024) m = max(m, tile_cost[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    if (lane == 0) part[tid >> 6] = m;
+    __syncthreads();
+    m = 0;
+    for (uint32_t w = 0; w < 16; ++w) m = max(m, part[w]);
+    if (m == 0) return;  // no lookups anywhere: every tile keeps cost 0 (class 31)
+    const uint32_t top = quarter_octave(m);
+    for (uint32_t i = tid; i < n_tiles; i += 1024) {
+        const uint32_t c = tile_cost[i];
+        const uint32_t k = c == 0 ? 31u : min(31u, top - quarter_octave(c));
+        tile_cost[i] = (1u << (31u - k)) - 1u;
+    }
+}
+
+__global__ __launch_bounds__(256) void tile_count_kernel(const uint32_t* __restrict__ tile_rank, uint32_t width,
+                                                         uint32_t height, uint32_t n_tiles,
+                                                         uint32_t* __restrict__ counts)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_tiles) return;
+    uint32_t x0, y0, ew, eh;
+    tile_rect(tile_rank[r], width, height, x0, y0, ew, eh);
+    counts[r] = ew * eh;
+}
+
+// one wavefront per tile rank; offsets[r]: exclusive scan of the counts in rank order
+__global__ __launch_bounds__(256) void tile_expand_kernel(const uint32_t* __restrict__ tile_rank,
+                                                          const uint32_t* __restrict__ offsets, uint32_t width,
+                                                          uint32_t height, uint32_t n_tiles,
+                                                          uint32_t* __restrict__ order)
+{
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (r >= n_tiles) return;
+    uint32_t x0, y0, ew, eh;
+    tile_rect(tile_rank[r], width, height, x0, y0, ew, eh);
+    if (lane < ew * eh) order[offsets[r] + lane] = (y0 + lane / ew) * width + x0 + lane % ew;
+}
+
 // One persistent launch: as many 256-lane workgroups as fit on the device at once, a fresh work
 // counter slot (cleared by a one-lane kernel on the same stream; launches on different streams take
 // different slots).
@@ -1233,6 +1320,64 @@ try {
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_order_by_cost")
+
+namespace {
+struct TileOrderLayout {
+    uint32_t n_tiles;
+    size_t tile_cost, tile_rank, counts, hist, bytes;  // byte offsets into the scratch buffer
+};
+TileOrderLayout tile_order_layout(uint32_t width, uint32_t height)
+{
+    TileOrderLayout l{};
+    const uint64_t tiles = (uint64_t)((width + dev::kOrderTile - 1) / dev::kOrderTile) *
+                           ((height + dev::kOrderTile - 1) / dev::kOrderTile);
+    l.n_tiles = (uint32_t)std::min<uint64_t>(tiles, 0xFFFFFFFFull);
+    const size_t a = ((size_t)l.n_tiles * 4 + 255) & ~(size_t)255;
+    l.tile_cost = 0;
+    l.tile_rank = a;
+    l.counts = 2 * a;
+    l.hist = 3 * a;
+    l.bytes = l.hist + std::max<size_t>(rtc8x_order_scratch_bytes(l.n_tiles), 4);
+    return l;
+}
+}  // namespace
+
+extern "C" size_t rtc8x_order_tiles_scratch_bytes(uint32_t width, uint32_t height)
+{
+    return tile_order_layout(width, height).bytes;
+}
+
+extern "C" rtc8x_status rtc8x_order_by_tile_cost(void* stream, const uint32_t* cost, uint32_t width, uint32_t height,
+                                                 uint32_t* order, void* scratch)
+try {
+    RTC8X_REQUIRE(cost && order && scratch, "rtc8x_order_by_tile_cost: NULL argument");
+    RTC8X_REQUIRE((uint64_t)width * height <= 0xFFFFFFFFull, "rtc8x_order_by_tile_cost: image too large");
+    if (width == 0 || height == 0) return RTC8X_OK;
+    const TileOrderLayout l = tile_order_layout(width, height);
+    char* sb = (char*)scratch;
+    uint32_t* tile_cost = (uint32_t*)(sb + l.tile_cost);
+    uint32_t* tile_rank = (uint32_t*)(sb + l.tile_rank);
+    uint32_t* counts = (uint32_t*)(sb + l.counts);
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t wave_blocks = (l.n_tiles + 3) / 4;
+    hipLaunchKernelGGL(dev::tile_cost_kernel, dim3(wave_blocks), dim3(256), 0, s, cost, width, height, l.n_tiles,
+                       tile_cost);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::tile_class_kernel, dim3(1), dim3(1024), 0, s, tile_cost, l.n_tiles);
+    RTC8X_HIP(hipGetLastError());
+    rtc8x_status st = rtc8x_order_by_cost(stream, tile_cost, l.n_tiles, tile_rank, sb + l.hist);
+    if (st != RTC8X_OK) return st;
+    hipLaunchKernelGGL(dev::tile_count_kernel, dim3((l.n_tiles + 255) / 256), dim3(256), 0, s, tile_rank, width,
+                       height, l.n_tiles, counts);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::order_scan_kernel, dim3(1), dim3(1024), 0, s, counts, l.n_tiles);
+    RTC8X_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::tile_expand_kernel, dim3(wave_blocks), dim3(256), 0, s, tile_rank, counts, width, height,
+                       l.n_tiles, order);
+    RTC8X_HIP(hipGetLastError());
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_order_by_tile_cost")
 
 static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 

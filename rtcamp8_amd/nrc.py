@@ -218,7 +218,7 @@ class NrcRenderer:
         # cost-ordered scheduling of the render launches (the full-image entry)
         self.cost = DeviceBuffer.zeros(4 * n)
         self.order = DeviceBuffer(4 * n)
-        self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n))
+        self.order_scratch = DeviceBuffer(lib().rtc8x_order_tiles_scratch_bytes(cfg.width, cfg.height))
         self.have_cost = False
         self.cache = NeuralRadianceCache()
         self.cache.initialize(num_hidden_layers, learning_rate, precision, encoding=encoding, ema_decay=ema_decay,
@@ -325,7 +325,7 @@ class NrcRenderer:
 
         def trace(side):
             if ordered:
-                call("rtc8x_order_by_cost", side, C.c_void_p(self.cost.ptr), self.width * self.height,
+                call("rtc8x_order_by_tile_cost", side, C.c_void_p(self.cost.ptr), self.width, self.height,
                      C.c_void_p(self.order.ptr), C.c_void_p(self.order_scratch.ptr))
             call("rtc8x_trace", side, C.byref(p), _abi.ENTRY_PATH_TRACE_WITH_NRC, k)
 

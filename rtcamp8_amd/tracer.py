@@ -23,11 +23,13 @@ class PathTracer:
         self.accum = DeviceBuffer(24 * n)
         self.vertices = DeviceBuffer(4 * n) if track_vertices else None
         self.stats = DeviceBuffer(8 * _abi.STAT_COUNT)
-        # cost-ordered scheduling: each launch records per-pixel lookups, the next starts the costliest first
+        # cost-ordered scheduling: each launch records per-pixel lookups, the next starts the costliest
+        # 8 x 8 tiles first
         self.schedule = schedule
         self.cost = DeviceBuffer.zeros(4 * n) if schedule else None
         self.order = DeviceBuffer(4 * n) if schedule else None
-        self.order_scratch = DeviceBuffer(lib().rtc8x_order_scratch_bytes(n)) if schedule else None
+        self.order_scratch = (DeviceBuffer(lib().rtc8x_order_tiles_scratch_bytes(cfg.width, cfg.height))
+                              if schedule else None)
         self.have_cost = False
         self.samples = 0
         self.knobs = (0, 0)  # (event_eighths, tile_shape) of the trace kernel's schedule; 0 = defaults
@@ -57,7 +59,7 @@ class PathTracer:
             return
         cfg = self.scene.cfg
         if self.have_cost:
-            call("rtc8x_order_by_cost", C.c_void_p(self.stream), C.c_void_p(self.cost.ptr), cfg.width * cfg.height,
+            call("rtc8x_order_by_tile_cost", C.c_void_p(self.stream), C.c_void_p(self.cost.ptr), cfg.width, cfg.height,
                  C.c_void_p(self.order.ptr), C.c_void_p(self.order_scratch.ptr))
             p.work_order = self.order.ptr
         else:

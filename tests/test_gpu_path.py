@@ -146,6 +146,47 @@ def test_uploaded_grid_matches_nanovdb_fixture():
         call("rtc8x_grid_destroy", C.byref(dg))
 
 
+def test_order_by_tile_cost_groups_tiles_costliest_first():
+    """The tile-grouped work order equals its restatement: 8 x 8 tiles (ragged at the right and bottom
+    edges) sorted stably by quarter-octave class of their summed cost (saturating at 2^32 - 1) below the
+    costliest tile's, each tile's in-image pixels row-major and contiguous."""
+    from rtcamp8_amd._lib import call, lib
+    from rtcamp8_amd.device import DeviceBuffer
+
+    rng = np.random.default_rng(6)
+    for w, h in ((1, 1), (9, 17), (100, 37), (64, 64), (1920, 1080)):
+        n = w * h
+        print(f"[order] tiles of a {w}x{h} image", flush=True)
+        cost = rng.integers(0, 1 << rng.integers(1, 31, n), dtype=np.int64).astype(np.uint32)
+        if n >= 64:
+            cost[:8] = 0xFFFFFFFF  # the first tile's sum saturates
+        dc, do = DeviceBuffer.from_array(cost), DeviceBuffer(4 * n)
+        ds = DeviceBuffer(int(lib().rtc8x_order_tiles_scratch_bytes(w, h)))
+        call("rtc8x_order_by_tile_cost", None, C.c_void_p(dc.ptr), w, h, C.c_void_p(do.ptr), C.c_void_p(ds.ptr))
+        got = do.download(np.uint32, n).astype(np.int64)
+        tx, ty = (w + 7) // 8, (h + 7) // 8
+        c2 = cost.reshape(h, w).astype(np.uint64)
+        def q(c):  # 4 floor(log2 c) + the two bits below the leading one
+            e = c.bit_length() - 1
+            return 4 * e + ((c >> (e - 2)) & 3 if e >= 2 else (c << (2 - e)) & 3)
+
+        sums = []
+        for t in range(tx * ty):
+            x0, y0 = (t % tx) * 8, (t // tx) * 8
+            sums.append(min(int(c2[y0:y0 + 8, x0:x0 + 8].sum()), 0xFFFFFFFF))
+        top = max(sums)
+        tiles = []
+        for t, s in enumerate(sums):
+            x0, y0 = (t % tx) * 8, (t // tx) * 8
+            cls = 31 if s == 0 else min(31, q(top) - q(s))
+            ys, xs = np.mgrid[y0:min(y0 + 8, h), x0:min(x0 + 8, w)]
+            tiles.append((cls, t, (ys * w + xs).ravel()))
+        want = np.concatenate([px for _, _, px in sorted(tiles, key=lambda z: (z[0], z[1]))])
+        assert np.array_equal(got, want), (w, h)
+        for b_ in (dc, do, ds):
+            b_.free()
+
+
 def test_order_by_cost_is_a_stable_costliest_first_permutation():
     from rtcamp8_amd._lib import call, lib
     from rtcamp8_amd.device import DeviceBuffer

@@ -69,7 +69,7 @@ __global__ void gather_probe_kernel(const rtc8x_grid g, uint32_t steps, uint32_t
 // collision a ratio-tracked shadow segment towards `light` (the length the tracer gives it, 2 x the
 // world radius) and a phase-function sample of the next direction, until the path escapes.  Tentative
 // collisions are taken two at a time and their lookups issued together (density_batch<2>), as the tracer
-// does, and one copy of the lookup code serves both segment kinds.  Every lane stops after `budget`
+// does with either walk, and one copy of the lookup code serves both segment kinds.  Every lane stops after `budget`
 // lookups, so the launch has no tail; what it leaves out is the tracer's vertex work: NEE light sampling
 // and MIS, the environment, Russian roulette, NRC records and accumulation.
 // stats[0] += lookups, stats[1] += paths that ended.
@@ -160,17 +160,41 @@ __global__ void march_probe_kernel(const rtc8x_frame_params p, dev::V3 light, ui
         }
         uint32_t event = 0;  // 1: real collision at t, 2: segment end
         if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) {
-            // the tracer's region walk: a bounded number of region visits, then the lookup of a
-            // tentative collision if one was reached
+            // the tracer's region walk: a bounded number of region visits to a tentative collision, a
+            // speculative second one past it, and their two lookups issued together
             float mu = 0.0f;
             const int r = sg.advance(p.grid, med, rng, mu, table);
             if (r == dev::kAdvanceCollision) {
-                const float sig = dev::sample_density(p.grid, med, sg.o, sg.d, sg.t);
-                ++looked;
+                const float t_first = sg.t, mu_first = mu;
+                float u_first = 0.0f;
+                uint64_t st_first = 0;
                 if (mode == 1) {
-                    if (sg.after_track(rng, sig, mu)) event = 1;
+                    u_first = rng.uni();
+                    st_first = rng.state;
+                }
+                sg.tau = -rtc8x_logf(1.0f - rng.uni());
+                float mu_second = 0.0f;
+                const int r2 = sg.advance(p.grid, med, rng, mu_second, table);
+                const float tk[2] = {t_first, r2 == dev::kAdvanceCollision ? sg.t : t_first};
+                float sig[2];
+                dev::density_batch<2>(p.grid, coeff, sg.o, sg.d, tk, sig);
+                ++looked;
+                if (mode == 1 && u_first < sig[0] / mu_first) {
+                    sg.t = t_first;
+                    rng.state = st_first;
+                    event = 1;
                 } else {
-                    sg.after_shadow(rng, sig, mu, vis);
+                    if (mode != 1) vis *= 1.0f - sig[0] / mu_first;
+                    if (r2 == dev::kAdvanceCollision) {
+                        ++looked;
+                        if (mode == 1) {
+                            if (sg.after_track(rng, sig[1], mu_second)) event = 1;
+                        } else {
+                            sg.after_shadow(rng, sig[1], mu_second, vis);
+                        }
+                    } else if (r2 == dev::kAdvanceEnd) {
+                        event = 2;
+                    }
                 }
             } else if (r == dev::kAdvanceEnd) {
                 event = 2;

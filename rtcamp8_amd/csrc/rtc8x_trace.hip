@@ -784,13 +784,44 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
             if (L.mode == kModeTrack || L.mode == kModeShadow)
                 speculate_global<kSpeculate>(p.grid, med, sg, L, L.mode == kModeTrack);
         } else if (L.mode == kModeTrack || L.mode == kModeShadow) {
+            const bool track = L.mode == kModeTrack;
             const int r = sg.advance(p.grid, med, L.rng, mu, region_table);
             if (r == kAdvanceCollision) {
-                const float sigma = sample_density(p.grid, med, sg.o, sg.d, sg.t);
-                if (L.mode == kModeTrack) {
-                    if (sg.after_track(L.rng, sigma, mu)) L.mode = kModeScatter;
+                // A speculative pair: the numbers the sequential loop draws after a null collision at
+                // the first tentative collision (its acceptance number, then the next optical depth) are
+                // drawn now, the walk goes on to the second, and both lookups are issued together.  The
+                // results are consumed in order: an acceptance at the first rewinds the PCG state to
+                // just after its acceptance number and drops the second (not counted as a lookup).
+                const float t_first = sg.t, mu_first = mu;
+                float u_first = 0.0f;
+                uint64_t st_first = 0;
+                if (track) {
+                    u_first = L.rng.uni();
+                    st_first = L.rng.state;
+                }
+                sg.tau = -rtc8x_logf(1.0f - L.rng.uni());
+                float mu_second = 0.0f;
+                const int r2 = sg.advance(p.grid, med, L.rng, mu_second, region_table);
+                const float tk[2] = {t_first, r2 == kAdvanceCollision ? sg.t : t_first};
+                float sig[2];
+                density_batch<2>(p.grid, med.coeff, sg.o, sg.d, tk, sig);
+                med.lookups++;
+                if (track && u_first < sig[0] / mu_first) {
+                    sg.t = t_first;
+                    L.rng.state = st_first;
+                    L.mode = kModeScatter;
                 } else {
-                    sg.after_shadow(L.rng, sigma, mu, L.vis);
+                    if (!track) L.vis *= 1.0f - sig[0] / mu_first;
+                    if (r2 == kAdvanceCollision) {
+                        med.lookups++;
+                        if (track) {
+                            if (sg.after_track(L.rng, sig[1], mu_second)) L.mode = kModeScatter;
+                        } else {
+                            sg.after_shadow(L.rng, sig[1], mu_second, L.vis);
+                        }
+                    } else if (r2 == kAdvanceEnd) {
+                        L.mode = track ? kModeEscape : kModeNeeDone;
+                    }
                 }
             } else if (r == kAdvanceEnd) {
                 L.mode = L.mode == kModeTrack ? kModeEscape : kModeNeeDone;

@@ -32,7 +32,7 @@ def _probe(tracking: int, budget: int = 48, waves: int = 3):
     stats.free()
     pt.close()
     scene.close()
-    # a lane stops once it has made `budget` lookups; the global walk's pairs can take it one past
+    # a lane stops once it has made `budget` lookups; a speculative pair can take it one past
     return looked, paths, c["lookups"] / max(c["paths"], 1), device_cus() * waves * 256 * (budget + 1)
 
 

@@ -1,7 +1,8 @@
 """The batch renderer end to end on the GPU: its plain-tracer frames are the pixels the library gives for
 the same frame rebuilt through the Python bindings (volume from the .nvdb, sky re-derived for the frame's
 sun, camera at the frame's key interpolation, RNG carried from frame to frame, tone map and SDR
-conversion), and its NRC frames come out as images of the same scene."""
+conversion), whose radiance sums equal the FP32 oracle's on the same inputs, and its NRC frames come out
+as images of the same scene."""
 import ctypes as C
 import json
 import subprocess
@@ -59,8 +60,11 @@ def render(scene_dir, out, *args):
     return r
 
 
-def library_frames(scene_dir, frames, spp):
-    """The same plain-tracer frames through the Python bindings."""
+def library_frames(scene_dir, frames, spp, check_oracle=False):
+    """The same plain-tracer frames through the Python bindings; with check_oracle, every frame's
+    accumulation and carried PCG states also equal the FP32 oracle's on host copies of the same inputs
+    (the .nvdb volume, the baked sky with its hierarchical importance map, the keyframed camera)."""
+    from oracle_lib import download_env, download_grid, nthreads, oracle, ptr
     from rtcamp8_amd import _abi, image, nvdb
     from rtcamp8_amd._lib import call
     from rtcamp8_amd.device import DeviceBuffer
@@ -89,6 +93,7 @@ def library_frames(scene_dir, frames, spp):
     call("rtc8x_seed_rng", C.c_void_p(rng.ptr), w, h, C.c_uint64(591842031321323413))
     p.rng_states, p.accum = rng.ptr, accum.ptr
     out = []
+    hg = download_grid(grid) if check_oracle else None
     try:
         for f in frames:
             pos = (C.c_float * 3)(*f["position"])
@@ -104,7 +109,18 @@ def library_frames(scene_dir, frames, spp):
             env.power_coeff, env.rotation = f["env_coeff"], f["env_rotation"]
             p.env = env
             accum.fill(0)
+            rng0 = rng.download(np.uint64, w * h) if check_oracle else None
             call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE, spp)
+            if check_oracle:
+                he, keep = download_env(env)
+                hp = _abi.FrameParams()
+                C.memmove(C.byref(hp), C.byref(p), C.sizeof(hp))
+                hp.grid, hp.env = hg.grid, he
+                hrng, hacc = rng0.copy(), np.zeros(3 * w * h)
+                hp.rng_states, hp.accum, hp.path_vertices, hp.stats = ptr(hrng), ptr(hacc), None, None
+                assert oracle().oracle_trace_f32(C.byref(hp), _abi.ENTRY_PATH_TRACE, spp, 0, h, nthreads()) == 0
+                assert np.array_equal(accum.download(np.float64, 3 * w * h).view(np.uint64), hacc.view(np.uint64))
+                assert np.array_equal(rng.download(np.uint64, w * h), hrng)
             image.tone_map(accum.ptr, w, h, spp, None, px.ptr, image.sdr_config())
             out.append(px.download(np.uint32, w * h).reshape(h, w))
             call("rtc8x_env_release", C.byref(env))
@@ -122,7 +138,7 @@ def test_plain_frames_match_the_library(scene_dir, tmp_path):
     render(scene_dir, tmp_path, "-no-nrc", "-spp", 4, "-max-frames", 2)
     d = json.loads(subprocess.run([str(EXE), "-scene", str(scene_dir / "scene.txt"), "-dump", "-max-frames", "2"],
                                   capture_output=True, text=True, timeout=60, check=True).stdout)
-    want = library_frames(scene_dir, d["frames"], 4)
+    want = library_frames(scene_dir, d["frames"], 4, check_oracle=True)
     for k in range(2):
         got = decode_png((tmp_path / f"{k:03d}.png").read_bytes())
         assert np.array_equal(got, want[k]), k

@@ -337,7 +337,7 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
 
     // per-pixel state
     DevBuf rng(8 * npx), accum(24 * npx), stats(8 * RTC8X_STAT_COUNT), rgba8(4 * npx), cost(4 * npx),
-        order(4 * npx), order_scratch(std::max<size_t>(rtc8x_order_scratch_bytes((uint32_t)npx), 4));
+        order(4 * npx), order_scratch(std::max<size_t>(rtc8x_order_tiles_scratch_bytes(W, H), 4));
     check(rtc8x_seed_rng(rng.as<uint64_t>(), W, H, 591842031321323413ull), "seed");
     p.rng_states = rng.as<uint64_t>();
     p.accum = accum.as<double>();
@@ -465,8 +465,8 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
                 const uint32_t k = std::min(samples_per_launch, o.spp - done);
                 p.nrc.num_samples = k;
                 if (have_cost) {
-                    check(rtc8x_order_by_cost(stream, cost.as<uint32_t>(), (uint32_t)npx, order.as<uint32_t>(),
-                                              order_scratch.ptr), "order");
+                    check(rtc8x_order_by_tile_cost(stream, cost.as<uint32_t>(), W, H, order.as<uint32_t>(),
+                                                   order_scratch.ptr), "order");
                     p.work_order = order.as<uint32_t>();
                 } else {
                     p.work_order = nullptr;
@@ -481,8 +481,8 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
             }
         } else {
             if (have_cost) {
-                check(rtc8x_order_by_cost(stream, cost.as<uint32_t>(), (uint32_t)npx, order.as<uint32_t>(),
-                                          order_scratch.ptr), "order");
+                check(rtc8x_order_by_tile_cost(stream, cost.as<uint32_t>(), W, H, order.as<uint32_t>(),
+                                               order_scratch.ptr), "order");
                 p.work_order = order.as<uint32_t>();
             }
             p.pixel_cost = cost.as<uint32_t>();

@@ -60,20 +60,15 @@ def render(scene_dir, out, *args):
     return r
 
 
-def library_frames(scene_dir, frames, spp, check_oracle=False):
-    """The same plain-tracer frames through the Python bindings; with check_oracle, every frame's
-    accumulation and carried PCG states also equal the FP32 oracle's on host copies of the same inputs
-    (the .nvdb volume, the baked sky with its hierarchical importance map, the keyframed camera)."""
-    from oracle_lib import download_env, download_grid, nthreads, oracle, ptr
-    from rtcamp8_amd import _abi, image, nvdb
-    from rtcamp8_amd._lib import call
-    from rtcamp8_amd.device import DeviceBuffer
-    from rtcamp8_amd.sky import HosekSky
+W, H = 96, 64
 
-    w, h = 96, 64
-    grid = nvdb.load_grid(scene_dir / "cloud.nvdb")
+
+def _frame_params(grid):
+    """The driver's frame parameters for the scene's volume (rtc8x_render.cpp's set-up)."""
+    from rtcamp8_amd import _abi
+
     p = _abi.FrameParams()
-    p.image_width, p.image_height = w, h
+    p.image_width, p.image_height = W, H
     p.grid = grid
     p.density_coeff, p.scattering_albedo, p.scattering_forwardness = 0.3, 0.99, 0.0
     p.phase = _abi.PHASE_SCHLICK
@@ -88,6 +83,44 @@ def library_frames(scene_dir, frames, spp, check_oracle=False):
     p.world_radius = float(np.sqrt(r2))
     p.rr_after_path_length, p.max_path_length = 0, 0
     p.tracking = _abi.TRACKING_GLOBAL_MAJORANT
+    p.nrc.path_termination_factor = 0.01
+    return p
+
+
+def _set_frame(p, f, tex):
+    """Camera and sky of dumped frame f into p (the sky baked into tex); returns the sky to close and
+    the environment to release after the frame."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.sky import HosekSky
+
+    pos = (C.c_float * 3)(*f["position"])
+    q = (C.c_float * 4)(*f["orientation"])
+    call("rtc8x_camera_setup", C.byref(p.camera), pos, q, C.c_float(f["fov_y"]), C.c_float(np.float32(W) / np.float32(H)))
+    sky = HosekSky(min(max(f["solar_elevation"], 0.0), 1.5707963), f["turbidity"])
+    sd = (C.c_float * 3)(*f["sun"])
+    call("rtc8x_sky_bake", None, C.byref(sky.struct), sd, 3200, 1600, C.c_void_p(tex.ptr))
+    env = _abi.EnvLight()
+    env.kind, env.sampler = _abi.ENV_LATLONG, _abi.ENV_SAMPLER_HIERARCHICAL
+    env.width, env.height, env.texels = 3200, 1600, tex.ptr
+    call("rtc8x_env_build_sampler", C.byref(env), None)
+    env.power_coeff, env.rotation = f["env_coeff"], f["env_rotation"]
+    p.env = env
+    return sky, env
+
+
+def library_frames(scene_dir, frames, spp, check_oracle=False):
+    """The same plain-tracer frames through the Python bindings; with check_oracle, every frame's
+    accumulation and carried PCG states also equal the FP32 oracle's on host copies of the same inputs
+    (the .nvdb volume, the baked sky with its hierarchical importance map, the keyframed camera)."""
+    from oracle_lib import download_env, download_grid, nthreads, oracle, ptr
+    from rtcamp8_amd import _abi, image, nvdb
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
+
+    w, h = W, H
+    grid = nvdb.load_grid(scene_dir / "cloud.nvdb")
+    p = _frame_params(grid)
     rng, accum, px = DeviceBuffer(8 * w * h), DeviceBuffer(24 * w * h), DeviceBuffer(4 * w * h)
     tex = DeviceBuffer(16 * 3200 * 1600)
     call("rtc8x_seed_rng", C.c_void_p(rng.ptr), w, h, C.c_uint64(591842031321323413))
@@ -96,18 +129,7 @@ def library_frames(scene_dir, frames, spp, check_oracle=False):
     hg = download_grid(grid) if check_oracle else None
     try:
         for f in frames:
-            pos = (C.c_float * 3)(*f["position"])
-            q = (C.c_float * 4)(*f["orientation"])
-            call("rtc8x_camera_setup", C.byref(p.camera), pos, q, C.c_float(f["fov_y"]), C.c_float(np.float32(w) / np.float32(h)))
-            sky = HosekSky(min(max(f["solar_elevation"], 0.0), 1.5707963), f["turbidity"])
-            sd = (C.c_float * 3)(*f["sun"])
-            call("rtc8x_sky_bake", None, C.byref(sky.struct), sd, 3200, 1600, C.c_void_p(tex.ptr))
-            env = _abi.EnvLight()
-            env.kind, env.sampler = _abi.ENV_LATLONG, _abi.ENV_SAMPLER_HIERARCHICAL
-            env.width, env.height, env.texels = 3200, 1600, tex.ptr
-            call("rtc8x_env_build_sampler", C.byref(env), None)
-            env.power_coeff, env.rotation = f["env_coeff"], f["env_rotation"]
-            p.env = env
+            sky, env = _set_frame(p, f, tex)
             accum.fill(0)
             rng0 = rng.download(np.uint64, w * h) if check_oracle else None
             call("rtc8x_trace", None, C.byref(p), _abi.ENTRY_PATH_TRACE, spp)
@@ -132,6 +154,61 @@ def library_frames(scene_dir, frames, spp, check_oracle=False):
     return out
 
 
+class _DumpedScene:
+    """What NrcRenderer reads of a scene: the image size, the PCG seed and the frame parameters (here
+    the driver's, set per frame)."""
+
+    def __init__(self, p):
+        import types
+
+        self.cfg = types.SimpleNamespace(width=W, height=H, rng_seed=591842031321323413)
+        self.p = p
+
+    def params(self):
+        from rtcamp8_amd import _abi
+
+        q = _abi.FrameParams()
+        C.memmove(C.byref(q), C.byref(self.p), C.sizeof(q))
+        return q
+
+
+def library_nrc_frames(scene_dir, frames, spp):
+    """The driver's NRC frames through NrcRenderer: the same network settings (two hidden layers, FP16,
+    triangle waves, the default seed, loss scale and EMA), the same frame schedule (8 training iterations
+    on frame 0, then 1; spp samples traced into the cache in one launch) and the accumulator cleared per
+    frame while the PCG states carry over."""
+    from rtcamp8_amd import _abi, image, nvdb
+    from rtcamp8_amd._lib import call
+    from rtcamp8_amd.device import DeviceBuffer
+    from rtcamp8_amd.nrc import NrcRenderer
+
+    grid = nvdb.load_grid(scene_dir / "cloud.nvdb")
+    p = _frame_params(grid)
+    px, tex = DeviceBuffer(4 * W * H), DeviceBuffer(16 * 3200 * 1600)
+    scene = _DumpedScene(p)
+    r = NrcRenderer(scene, num_hidden_layers=2, precision=_abi.NRC_FP16, encoding=_abi.NRC_TRIANGLE_WAVE,
+                    samples_per_launch=spp)
+    out = []
+    try:
+        for f in frames:
+            sky, env = _set_frame(p, f, tex)
+            r.image()  # the renderer's earlier work is done before its accumulator is cleared
+            r.accum.fill(0)
+            r.samples = 0
+            r.render_frame(spp=spp)
+            r.image()
+            image.tone_map(r.accum.ptr, W, H, spp, None, px.ptr, image.sdr_config())
+            out.append(px.download(np.uint32, W * H).reshape(H, W))
+            call("rtc8x_env_release", C.byref(env))
+            sky.close()
+    finally:
+        r.close()
+        for b in (px, tex):
+            b.free()
+        call("rtc8x_grid_destroy", C.byref(grid))
+    return out
+
+
 def test_plain_frames_match_the_library(scene_dir, tmp_path):
     from test_image import decode_png
 
@@ -144,6 +221,23 @@ def test_plain_frames_match_the_library(scene_dir, tmp_path):
         assert np.array_equal(got, want[k]), k
         lum = (got & 255).astype(np.float64)
         assert 5 < lum.mean() < 250 and lum.std() > 1  # a lit cloud, not a flat image
+
+
+def test_nrc_frames_match_the_library(scene_dir, tmp_path):
+    """NRC frames with triangle-wave encoding (deterministic training): the driver's pixels equal those
+    of NrcRenderer driven with the same frame parameters, frame after frame, the online-trained cache
+    carried along."""
+    from test_image import decode_png
+
+    render(scene_dir, tmp_path, "-spp", 2, "-max-frames", 3, "-encoding", "triangle")
+    d = json.loads(subprocess.run([str(EXE), "-scene", str(scene_dir / "scene.txt"), "-dump", "-max-frames", "3"],
+                                  capture_output=True, text=True, timeout=60, check=True).stdout)
+    want = library_nrc_frames(scene_dir, d["frames"], 2)
+    for k in range(3):
+        got = decode_png((tmp_path / f"{k:03d}.png").read_bytes())
+        assert np.array_equal(got, want[k]), k
+        lum = (got & 255).astype(np.float64)
+        assert 5 < lum.mean() < 250 and lum.std() > 1
 
 
 def test_nrc_frames(scene_dir, tmp_path):

@@ -372,15 +372,15 @@ def nrc_render_leg(scene, steps: int, stream):
 
 
 def latency_roofline(scene, params, lookups_per_launch: float, lookups_per_path: float, trace_lookups_per_s: float,
-                     paths_per_s: float, waves=(3, 8)):
+                     paths_per_s: float, waves=(3, 4)):
     """Memory-latency roofline of the tracer's density lookups: the march probe (rtc8x_march_probe) walks
     the tracer's own stepping pattern on this frame -- persistent lanes taking the frame's pixels in 8x8
     tiles, the same camera rays and PCG states, delta tracking at the majorant's spacing with two tentative
     collisions looked up together, a ratio-tracked shadow segment towards the sun and a phase sample at
     every real collision -- with none of the tracer's vertex shading, and every lane stops at the same
     lookup budget (as many lookups in all as one traced frame) so the launch has no tail.  The reference
-    point is the tracer's occupancy (3 waves per SIMD); the gather probe (random positions of random
-    leaves, no coherence) is reported beside it."""
+    point is the tracer's occupancy (3 waves per SIMD, registers allotted as for the tracer), 4 waves
+    beside it; the gather probe (random positions of random leaves, no coherence) is reported too."""
     from rtcamp8_amd import scene as sc
     from rtcamp8_amd._lib import call
     from rtcamp8_amd.device import DeviceBuffer, EventTimer, device_cus
@@ -389,7 +389,7 @@ def latency_roofline(scene, params, lookups_per_launch: float, lookups_per_path:
     light = (C.c_float * 3)(*[float(v) for v in sc.sun_direction(scene.cfg)])
     stats = DeviceBuffer.zeros(16)
     cus = device_cus()
-    walk = ("the local tracker's region walk over the LDS-staged region table, one lookup per tentative collision"
+    walk = ("the local tracker's region walk over the LDS-staged region table, one tentative collision per lookup round"
             if params.tracking else "delta tracking at the global majorant's spacing, 2 tentative collisions per lookup round")
     res = {"probe": "march probe: persistent lanes walk the frame's camera paths in 8x8 pixel tiles, "
                     f"{scene.cfg.spp} per pixel ({walk}, shadow segment towards the sun and a phase sample per "
@@ -705,7 +705,7 @@ def main() -> int:
     result["lookups_per_path"] = c["lookups"] / max(c["paths"], 1)
     if args.legs != "all":
         return finish(args, result, timers, pt, scene, stream, network_only=args.legs == "network")
-    waves = (3, 4) if tracking == _abi.TRACKING_LOCAL_MAJORANT else (3, 8)
+    waves = (3, 4)
     lat = latency_roofline(scene, params, lookups_per_launch, c["lookups"] / max(c["paths"], 1),
                            c["lookups"] / sum(kernel_ms) * 1e3, c["paths"] / wall, waves=waves)
     result["latency_roofline"] = lat

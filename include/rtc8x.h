@@ -90,8 +90,9 @@ rtc8x_status rtc8x_gather_probe(void* stream, const struct rtc8x_grid* grid, uin
 struct rtc8x_frame_params;
 /*
  * rtc8x_march_probe: the memory-latency roofline of the tracer's own lookups (no counterpart in the
- * reference).  A persistent launch, `waves_per_simd` (2..8, set with dynamic LDS) 256-lane workgroups per
- * CU, whose lanes take the frame's pixels in 8x8 tiles from a work counter as the tracer's do (the
+ * reference).  A persistent launch, `waves_per_simd` (2..4, set with dynamic LDS) 256-lane workgroups per
+ * CU (registers allotted for 3 waves per SIMD, as the tracer's render entries have, up to 3, and for 4
+ * at 4), whose lanes take the frame's pixels in 8x8 tiles from a work counter as the tracer's do (the
  * tiles wrapping around the image; a pixel's PCG state is read, never written) and trace `spp` paths per
  * pixel: jittered camera rays, delta tracking as params->tracking selects it (the global majorant's
  * spacing with two tentative collisions looked up together, or the local tracker's region walk over the
@@ -99,7 +100,7 @@ struct rtc8x_frame_params;
  * `light_dir` and a sample of the frame's phase function, with none of the tracer's vertex shading.
  * Every lane stops after `budget` lookups (<= 2^24), so the launch has no tail.  stats[0] += lookups
  * made, stats[1] += paths ended; stats is a device array of two uint64.  With local tracking the 32 KiB
- * region table counts towards the LDS that sets the occupancy, so waves_per_simd is at most 4.
+ * region table counts towards the LDS that sets the occupancy.
  */
 rtc8x_status rtc8x_march_probe(void* stream, const struct rtc8x_frame_params* params, const float light_dir[3],
                                uint32_t spp, uint32_t budget, uint32_t waves_per_simd, uint64_t* stats);

@@ -54,8 +54,10 @@ def test_march_probe_walks_both_trackers():
         assert 0.25 * trace < probe < 4.0 * trace, res
 
 
-def test_march_probe_rejects_local_occupancy_beyond_the_region_table():
+def test_march_probe_rejects_occupancy_beyond_its_register_budget():
+    """At most 4 waves per SIMD: the probe's registers are allotted for 3 (the tracer's occupancy) or 4."""
     from rtcamp8_amd._lib import Rtc8xError
 
-    with pytest.raises(Rtc8xError):
-        _probe(1, waves=8)
+    for tracking in (0, 1):
+        with pytest.raises(Rtc8xError):
+            _probe(tracking, waves=8)

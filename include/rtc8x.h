@@ -567,8 +567,8 @@ typedef struct rtc8x_frame_params {
     rtc8x_nrc_buffers nrc;
     /* Scheduling knobs (no counterpart in the reference; results do not depend on them), 0 = default:
      * event_eighths, 1..8: a wavefront runs its parked lanes' vertex events once that many eighths of its
-     * active lanes are parked (default 3 for PATH_TRACE and GENERATE_TRAINING_DATA, 4 for the NRC render
-     * entry); tile_shape: the pixel tiles lanes take without a work order, 0 = 8x8 (default), 1 = 16x4,
+     * active lanes are parked (default 5 for PATH_TRACE and PATH_TRACE_WITH_NRC with local-majorant
+     * tracking, 4 with the global majorant, 3 for GENERATE_TRAINING_DATA); tile_shape: the pixel tiles lanes take without a work order, 0 = 8x8 (default), 1 = 16x4,
      * 2 = 32x2, 3 = 4x16. */
     uint32_t event_eighths;
     uint32_t tile_shape;

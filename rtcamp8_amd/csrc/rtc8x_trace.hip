@@ -352,8 +352,13 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     const uint32_t first_ny = H > first_ry ? (H - first_ry + 3u) / 4u : 0u;
     const uint32_t n_first = (gen_data && !p.work_order) ? first_nx * first_ny : 0u;
     const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + tile_h - 1) / tile_h) * 64;
-    const uint32_t ev8 = (p.event_eighths >= 1 && p.event_eighths <= 8) ? p.event_eighths
-                                                                        : (ENTRY == RTC8X_ENTRY_PATH_TRACE_WITH_NRC ? 4u : 3u);
+    // default event thresholds, measured on the 512^3 cloud at 1080p: the render entries 5/8 with the
+    // local walk (4/8 gave 36.4 against 35.7 ms per plain 16-spp frame) and 4/8 with the global one
+    // (3/8: 44.0 against 43.2 ms); the training entry 3/8 (no difference measured)
+    constexpr uint32_t kEv8Default = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA     ? 3u
+                                     : TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT ? 5u
+                                                                                 : 4u;
+    const uint32_t ev8 = (p.event_eighths >= 1 && p.event_eighths <= 8) ? p.event_eighths : kEv8Default;
     const uint32_t lane_id = threadIdx.x & 63;
     // per-sample slices of the render buffers (one launch may trace several samples into the cache)
     const size_t q_stride = ((size_t)W * H + 127u) & ~(size_t)127u, p_stride = (size_t)W * H;
@@ -380,9 +385,9 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
         // many lanes instead of once per lane), while the others keep stepping through lookups.  The
         // plain and training entries then run only the larger of the two vertex events (a real
         // collision: Russian roulette and NEE set-up; an NEE segment's end: phase sampling and the next
-        // segment) together with the path starts and ends, from event_eighths / 8 of the active lanes
-        // (3/8 by default); the other event's lanes wait for a later pass.  The NRC render entry, whose
-        // paths are short, runs every parked lane from event_eighths / 8 (half by default).
+        // segment) together with the path starts and ends, from event_eighths / 8 of the active lanes;
+        // the other event's lanes wait for a later pass.  The NRC render entry, whose paths are short,
+        // runs every parked lane from event_eighths / 8.
         const uint64_t parked = __ballot(L.mode != kModeTrack && L.mode != kModeShadow && L.mode != kModeIdle);
         const uint64_t stepping = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
         if (parked == 0 && stepping == 0) break;

@@ -444,7 +444,7 @@ template <int TRACKING>
 struct Seg;
 
 // Regions a local-majorant lane walks per stepping pass before it yields to the wavefront's lookups.
-constexpr uint32_t kRegionBudget = 4;
+constexpr uint32_t kRegionBudget = 6;
 enum : int { kAdvanceEnd = 0, kAdvanceCollision = 1, kAdvanceBudget = 2 };
 
 // Local majorants over the grid's regions, the loops of track_local / ratio_local made resumable; the

@@ -353,8 +353,8 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     const uint32_t n_first = (gen_data && !p.work_order) ? first_nx * first_ny : 0u;
     const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + tile_h - 1) / tile_h) * 64;
     // default event thresholds, measured on the 512^3 cloud at 1080p: the render entries 5/8 with the
-    // local walk (4/8 gave 36.4 against 35.7 ms per plain 16-spp frame) and 4/8 with the global one
-    // (3/8: 44.0 against 43.2 ms); the training entry 3/8 (no difference measured)
+    // local walk (4/8 gave 35.1 against 34.7 ms per plain 16-spp frame, 6/8 34.7) and 4/8 with the global
+    // one (3/8: 44.0 against 43.2 ms); the training entry 3/8 (no difference measured)
     constexpr uint32_t kEv8Default = ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA     ? 3u
                                      : TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT ? 5u
                                                                                  : 4u;

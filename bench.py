@@ -307,13 +307,26 @@ def multi_instance_leg(steps: int, stream):
     pixel_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, None,
                                            stream.handle))
     t_tune = time.perf_counter()
-    shares, knobs = batch.autotune(one_with, len(rs), stream=stream.handle)
+    solo_shares, solo_knobs = batch.autotune(one_with, len(rs), stream=stream.handle)
+    default_knobs = [(0, 0)] * len(rs)
+
+    def launch_batch(sh, kn):
+        for r, k in zip(rs, kn):
+            r.knobs = k
+        batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, sh, stream.handle)
+
+    settings = [(solo_shares, solo_knobs), (None, solo_knobs), (solo_shares, default_knobs), (None, default_knobs)]
+    shares, knobs, _ = batch.pick_batch_setting(launch_batch, settings, stream=stream.handle)
     tune_s = time.perf_counter() - t_tune
-    for r, k in zip(rs, knobs):
+    for r, k in zip(rs, solo_knobs):
         r.knobs = k
     seq_tuned_ms = timed(lambda: [one(i) for i in range(len(rs))])
+    for r, k in zip(rs, knobs):
+        r.knobs = k
     tuned_ms = timed(lambda: batch._launch([params(r) for r in rs], ones, _abi.ENTRY_PATH_TRACE_WITH_NRC, shares,
                                            stream.handle))
+    choice = {"shares": "pixels" if shares is None else "solo launch times",
+              "knobs": "defaults" if knobs is default_knobs else "tuned per instance"}
     batch.render_nrc(rs, ones, shares, stream.handle)
     stream.synchronize()
     finite = all(bool(np.all(np.isfinite(r.image()))) for r in rs)
@@ -323,13 +336,14 @@ def multi_instance_leg(steps: int, stream):
         s_.close()
     return {"instances": [{"fbm_resolution": res, "sparse": sparse, "sigma_t_max": sigma, "image": [w, h],
                            "own_launch_ms": t}
-                          for (res, sparse, sigma, w, h), t in zip(MULTI_INSTANCES, shares)],
+                          for (res, sparse, sigma, w, h), t in zip(MULTI_INSTANCES, solo_shares)],
             "nrc": "hash grid, 2 hidden layers, bf16 training, e4m3 inference; first frame (8 training iterations) done",
             "entry": "PATH_TRACE_WITH_NRC, 1 spp per instance",
             "one_launch_per_instance_ms": seq_ms, "one_launch_per_instance_tuned_ms": seq_tuned_ms,
             "batched_autotuned_ms": tuned_ms, "batched_by_pixels_ms": pixel_ms,
             "autotuned": [{"event_eighths": k[0], "tile": batch.TILE_SHAPES[k[1]], "share_ms": sh}
-                          for k, sh in zip(knobs, shares)],
+                          for k, sh in zip(solo_knobs, solo_shares)],
+            "batch_choice": choice,
             "autotune_seconds": tune_s,
             "speedup_autotuned": seq_ms / tuned_ms, "finite": finite}
 
@@ -744,8 +758,9 @@ def main() -> int:
         multi = multi_instance_leg(args.nrc_steps, stream)
         result["multi_instance"] = multi
         print(f"[bench] multi-instance: 8 NRC render launches {multi['one_launch_per_instance_ms']:.2f} ms, one "
-              f"batched launch {multi['batched_autotuned_ms']:.2f} ms (autotuned), {multi['batched_by_pixels_ms']:.2f} ms "
-              f"(by pixels)", file=sys.stderr, flush=True)
+              f"batched launch {multi['batched_autotuned_ms']:.2f} ms (autotuned: shares by "
+              f"{multi['batch_choice']['shares']}, knobs {multi['batch_choice']['knobs']}), "
+              f"{multi['batched_by_pixels_ms']:.2f} ms (by pixels)", file=sys.stderr, flush=True)
     if args.rmse_ref_spp > 0:
         print("[bench] equal-RMSE leg ...", file=sys.stderr, flush=True)
         eq = equal_rmse_leg(scene, args.rmse_ref_spp, cfg.spp, stream, crop_ref_spp=args.crop_ref_spp)

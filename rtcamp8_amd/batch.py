@@ -5,7 +5,8 @@ scene, image and buffers.  `autotune` times every instance's own launch under ea
 the tracer's scheduling knobs (the event threshold: how many eighths of a wavefront's active lanes park
 before their vertex events run; and the shape of the 64-pixel tiles lanes take), keeps each instance's
 fastest setting, and returns the fastest times as the workgroup shares of the batched launch, so that
-the instances finish together.  The knobs change the schedule, never the results."""
+the instances finish together; `pick_batch_setting` then checks those choices against the plain ones on
+the batched launch.  The knobs change the schedule, never the results."""
 from __future__ import annotations
 
 import ctypes as C
@@ -96,6 +97,20 @@ def autotune(launch_one, count: int, candidates=KNOB_CANDIDATES, repeats: int = 
 def apply_knobs(params: _abi.FrameParams, knobs) -> _abi.FrameParams:
     params.event_eighths, params.tile_shape = int(knobs[0]), int(knobs[1])
     return params
+
+
+def pick_batch_setting(launch_batch, settings, repeats: int = 3, stream=None):
+    """The autotuner's last step, on the batched launch itself: times launch_batch(shares, knobs) for
+    every candidate (shares, knobs) in `settings` (the best of `repeats`, after one untimed launch) and
+    returns (shares, knobs, ms) of the fastest.  Solo launch times make shares only approximately: in the
+    batch the instances share the device and wavefronts move on to unfinished instances, so the solo
+    choices are checked against the plain ones (shares by pixels, default knobs) there."""
+    best = None
+    for shares, knobs in settings:
+        ms = _time(lambda: launch_batch(shares, knobs), repeats, stream)
+        if best is None or ms < best[2]:
+            best = (shares, knobs, ms)
+    return best
 
 
 def autotune_shares(launch_one, count: int, repeats: int = 2, stream=None) -> list[float]:

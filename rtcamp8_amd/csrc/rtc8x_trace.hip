@@ -33,6 +33,7 @@ constexpr uint32_t kHardMaxPathLength = 4096u;
 constexpr uint32_t kSpeculate = 2;
 // A wavefront with at least this many idle lanes evaluates its stepping lanes' segments together.
 constexpr uint32_t kCoopIdle = 48;
+constexpr uint32_t kCoopLocalPaths = 2;  // at most this many stepping lanes for the local walk's cooperative tail
 constexpr uint32_t kCounterSlots = 1024;
 constexpr uint32_t kMaxBatch = 16;  // instances of one batched launch
 __device__ uint32_t g_work_counter[kCounterSlots];
@@ -302,6 +303,113 @@ __device__ __forceinline__ void coop_segments(const rtc8x_grid& g, Medium& med, 
         }
         if (lane == src) {
             if (mode == kModeScatter) sg.t = t;
+            L.rng.state = st;
+            L.vis = vis;
+            L.mode = mode;
+            med.lookups += looked;
+        }
+    }
+}
+
+// The cooperative tail of the local walk.  Its tentative collisions lie where the optical depth drawn for
+// them runs out along the region walk, so lane j replays the sequential walk from the segment's state
+// through the segment's j-th next collision (walk steps only, with the random numbers drawn in the
+// sequential order: a collision's acceptance number when tracking, then the next optical depth), which
+// gives the sequential positions bit for bit.  The 64 lookups then go out in one round and the results
+// are consumed in order, as in coop_segments.  When all 64 collisions are null the wavefront goes on
+// from the last one's walk state with its next optical depth drawn.
+__device__ __forceinline__ void coop_segments_local(const rtc8x_grid& g, Medium& med,
+                                                   Seg<RTC8X_TRACKING_LOCAL_MAJORANT>& sg, Lane& L, uint64_t todo,
+                                                   uint32_t lane, const uint8_t* __restrict__ table)
+{
+    using SegL = Seg<RTC8X_TRACKING_LOCAL_MAJORANT>;
+    auto take = [](const SegL& a, uint32_t from) {
+        SegL b;
+        b.o = V3{lane_f(a.o.x, from), lane_f(a.o.y, from), lane_f(a.o.z, from)};
+        b.d = V3{lane_f(a.d.x, from), lane_f(a.d.y, from), lane_f(a.d.z, from)};
+        b.t = lane_f(a.t, from);
+        b.t1 = lane_f(a.t1, from);
+        b.tau = lane_f(a.tau, from);
+        b.s.c0 = __builtin_amdgcn_readlane(a.s.c0, (int)from);
+        b.s.c1 = __builtin_amdgcn_readlane(a.s.c1, (int)from);
+        b.s.c2 = __builtin_amdgcn_readlane(a.s.c2, (int)from);
+        b.s.s0 = __builtin_amdgcn_readlane(a.s.s0, (int)from);
+        b.s.s1 = __builtin_amdgcn_readlane(a.s.s1, (int)from);
+        b.s.s2 = __builtin_amdgcn_readlane(a.s.s2, (int)from);
+        b.s.io0 = lane_f(a.s.io0, from);
+        b.s.io1 = lane_f(a.s.io1, from);
+        b.s.io2 = lane_f(a.s.io2, from);
+        b.s.iv0 = lane_f(a.s.iv0, from);
+        b.s.iv1 = lane_f(a.s.iv1, from);
+        b.s.iv2 = lane_f(a.s.iv2, from);
+        b.s.tn0 = lane_f(a.s.tn0, from);
+        b.s.tn1 = lane_f(a.s.tn1, from);
+        b.s.tn2 = lane_f(a.s.tn2, from);
+        return b;
+    };
+    while (todo) {
+        const uint32_t src = (uint32_t)__ffsll((long long)todo) - 1u;
+        todo &= todo - 1;
+        const bool track = (uint32_t)__builtin_amdgcn_readlane((int)L.mode, (int)src) == kModeTrack;
+        SegL w = take(sg, src);
+        float vis = lane_f(L.vis, src);
+        uint64_t st = lane_u64(L.rng.state, src);
+        uint32_t looked = 0, mode = 0;
+        float t_hit = 0.0f;
+        for (;;) {
+            SegL c = w;
+            Pcg r;
+            r.state = st;
+            bool valid = true;
+            float mu = 0.0f, ua = 0.0f;
+            uint64_t st_acc = 0;  // after the collision's acceptance number
+#pragma unroll 1
+            for (uint32_t i = 0; i <= lane; ++i) {
+                if (i > 0) c.tau = -rtc8x_logf(1.0f - r.uni());
+                int res;
+                do {
+                    res = c.advance(g, med, r, mu, table);
+                } while (res == kAdvanceBudget);
+                if (res != kAdvanceCollision) {
+                    valid = false;
+                    break;
+                }
+                if (track) {
+                    ua = r.uni();
+                    st_acc = r.state;
+                }
+            }
+            const float sig = valid ? density_at(g, med.coeff, c.o, c.d, c.t) : 0.0f;
+            const uint32_t e = (uint32_t)__popcll(__ballot(valid));  // the valid lanes are 0 .. e-1
+            if (track) {
+                const uint64_t acc = __ballot(valid && ua < sig / mu);
+                if (acc) {
+                    const uint32_t a = (uint32_t)__ffsll((long long)acc) - 1u;
+                    looked += a + 1;
+                    t_hit = lane_f(c.t, a);
+                    st = lane_u64(st_acc, a);
+                    mode = kModeScatter;
+                    break;
+                }
+            } else {
+                const float f = valid ? 1.0f - sig / mu : 1.0f;
+                for (uint32_t i = 0; i < e; ++i) vis *= lane_f(f, i);
+            }
+            looked += e;
+            if (e < 64) {  // the segment ended: the state where its walk ran out
+                st = lane_u64(r.state, e);
+                mode = track ? kModeEscape : kModeNeeDone;
+                break;
+            }
+            // all 64 null: on from the last collision, its next optical depth drawn
+            w = take(c, 63);
+            Pcg rn;
+            rn.state = lane_u64(track ? st_acc : r.state, 63);
+            w.tau = -rtc8x_logf(1.0f - rn.uni());
+            st = rn.state;
+        }
+        if (lane == src) {
+            if (mode == kModeScatter) sg.t = t_hit;
             L.rng.state = st;
             L.vis = vis;
             L.mode = mode;
@@ -781,6 +889,13 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
             const uint64_t steppers = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
             coop = steppers != 0 && __popcll(__ballot(L.mode == kModeIdle)) >= kCoopIdle;
             if (coop) coop_segments(p.grid, med, sg, L, steppers, lane_id);
+        } else if constexpr (ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA) {
+            // a local path's 64 collisions cost one walk over their regions (tens of microseconds), while
+            // the stepping lanes walk side by side: only the last few paths are taken this way
+            const uint64_t steppers = __ballot(L.mode == kModeTrack || L.mode == kModeShadow);
+            coop = steppers != 0 && __popcll(steppers) <= kCoopLocalPaths &&
+                   __popcll(__ballot(L.mode == kModeIdle)) >= kCoopIdle;
+            if (coop) coop_segments_local(p.grid, med, sg, L, steppers, lane_id, region_table);
         }
         if (coop) {
         } else if constexpr (TRACKING == RTC8X_TRACKING_GLOBAL_MAJORANT) {

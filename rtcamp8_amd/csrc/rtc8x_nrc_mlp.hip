@@ -2,15 +2,15 @@
 // NetworkWithInputEncoding + FullyFusedMLP + RelativeL2Luminance + EMA(Adam) behind
 // renderer/network_interface.cu).
 //
-// Forward (inference and training): a 256-lane workgroup = 4 wavefronts x 64 samples.  Each lane
-// encodes one query into 64 features and writes them to its wavefront's LDS tile; every layer is then
+// Forward (inference and training): a workgroup of 4 wavefronts x 64 samples (training batches of up
+// to 32,768 samples: one wavefront, so that they spread over all CUs).  Each lane encodes one query into 64 features and writes them to its wavefront's LDS tile; every layer is then
 // 64x64x64 per wavefront = 4 (samples) x 4 (outputs) x 2 (K) v_mfma_f32_16x16x32 {f16,bf16}
 // instructions with FP32 accumulation (FP32 weights: 8 v_mfma_f32_16x16x4_f32 per 16x16x32 step).  A operands (activations) come from LDS, B operands (weights,
 // [out][in] rows) straight from global memory, where the ~45 KB of weights stay L1/L2-resident for
 // the whole launch.  ReLU and the down-conversion are fused into the write-back to LDS.
 //
-// Backward: a 256-lane workgroup owns 256 samples.  Per layer, from the top: dW = dZ^T H (K = 256
-// samples, operands feature-major in LDS) and dH = dZ W (K = layer width, transposed weights from
+// Backward: a workgroup owns 64 or 256 samples, as the forward pass.  Per layer, from the top:
+// dW = dZ^T H (K = the workgroup's samples, operands feature-major in LDS) and dH = dZ W (K = layer width, transposed weights from
 // global), with the ReLU mask applied on the way down.  Each workgroup writes its dW to its own slot;
 // the slots are summed in a fixed order, so a training step is deterministic.
 //
@@ -84,7 +84,6 @@ constexpr uint32_t kLayer = kWidth * kWidth;             // 4096 weights per hid
 constexpr uint32_t kOutPad = RTC8X_NRC_OUTPUT_PAD;       // output rows padded to one 16-row tile
 constexpr uint32_t kOutT = 32;                           // transposed output layer: [64 in][32]
 constexpr uint32_t kActStride = 72;                      // LDS row stride of a 64-feature row (halves)
-constexpr uint32_t kColStride = 264;                     // LDS row stride of a 256-sample column
 constexpr uint32_t kMaxTrain = 65536;
 constexpr uint32_t kMaxLayers = 8;
 
@@ -247,8 +246,11 @@ __device__ __forceinline__ void encode_hash(const rtc8x_radiance_query& q, const
 }
 
 // The hash encoding of query q (NULL: zeros) straight into an activation row: the grid features go
-// to the row level by level, so no per-lane array is indexed at run time.
-template <typename T>
+// to the row level by level, so no per-lane array is indexed at run time.  With BATCH, all levels'
+// corner gathers are issued before the row is written, so that they are in flight together: at full
+// occupancy other waves hide the gather latency and the level loop is as fast, while a training batch
+// of one wave per CU would wait on each level's gathers in turn.
+template <typename T, bool BATCH>
 __device__ __forceinline__ void encode_hash_row(const rtc8x_radiance_query* q, const float* __restrict__ tables,
                                                 const HashGeom& gm, T* row)
 {
@@ -257,10 +259,21 @@ __device__ __forceinline__ void encode_hash_row(const rtc8x_radiance_query* q, c
     if (q) {
         float p[3];
         query_position01(*q, p);
-        for (uint32_t l = 0; l < kHashLevels; ++l) {
-            const float2 a = hash_level(p, tables, gm, l);
-            row[2 * l] = (T)a.x;
-            row[2 * l + 1] = (T)a.y;
+        if constexpr (BATCH) {
+            float2 a[kHashLevels];
+#pragma unroll
+            for (uint32_t l = 0; l < kHashLevels; ++l) a[l] = hash_level(p, tables, gm, l);
+#pragma unroll
+            for (uint32_t l = 0; l < kHashLevels; ++l) {
+                row[2 * l] = (T)a[l].x;
+                row[2 * l + 1] = (T)a[l].y;
+            }
+        } else {
+            for (uint32_t l = 0; l < kHashLevels; ++l) {
+                const float2 a = hash_level(p, tables, gm, l);
+                row[2 * l] = (T)a.x;
+                row[2 * l + 1] = (T)a.y;
+            }
         }
         encode_hash_tail(*q, t);
     } else {
@@ -297,11 +310,39 @@ __device__ __forceinline__ uint32_t device_count(uint32_t n, const int32_t* coun
 
 constexpr float kHalfMax = 65504.0f;  // the largest finite FP16 value
 
-template <typename T, bool TRAIN, int ENC>
-// (256, 3): the register budget of three workgroups per CU, under which the FP16/BF16 accumulators stay
-// in VGPRs and the kernel runs four waves per SIMD (0.19 ms per 2^21 queries at L = 5, against 0.21 ms
-// at three; FP32 stays at two, bound by its LDS tiles)
-__global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w, uint32_t L,
+// Training batches up to this size run the training kernels with one wave (64 samples) per workgroup,
+// so that a 16,384-sample step spreads over 256 workgroups instead of 64; larger ones with four.
+constexpr uint32_t kSmallBatch = 32768;
+
+// The activations a training forward pass saves for the backward pass: per layer, per 64-sample tile,
+// feature-major (64 features x 64 samples), so that the backward pass reads its column tiles with
+// 16-byte loads.  Samples past the batch end of the last tile hold zeros.
+__host__ __device__ __forceinline__ size_t saved_tile(uint32_t l, uint32_t n, uint32_t tile)
+{
+    return ((size_t)l * ((n + 63) / 64) + tile) * (64 * 64);
+}
+
+// The wave's 64 activation rows (LDS, sample-major) into its saved tile: lane c gathers feature c.
+template <typename T>
+__device__ __forceinline__ void save_tile(const T* A, uint32_t lane, T* __restrict__ tile)
+{
+    using V = typename Vec8<T>::type;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        V v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = A[(8 * k + j) * kActStride + lane];
+        *reinterpret_cast<V*>(&tile[lane * 64 + 8 * k]) = v;
+    }
+}
+
+template <typename T, bool TRAIN, int ENC, int WPB>
+// (256, 3): the register budget of three 4-wave workgroups per CU, under which the FP16/BF16
+// accumulators stay in VGPRs and the kernel runs four waves per SIMD (0.19 ms per 2^21 queries at
+// L = 5, against 0.21 ms at three; FP32 stays at two, bound by its LDS tiles).  One-wave workgroups
+// (training batches) take the registers that keep all hash levels' gathers in flight.
+__global__ __launch_bounds__(64 * WPB, WPB == 1 ? 1 : 3) void forward_kernel(const T* __restrict__ w, uint32_t L,
                                                       const rtc8x_radiance_query* __restrict__ in, uint32_t n,
                                                       float* __restrict__ out, const float* __restrict__ target,
                                                       T* __restrict__ hbuf, T* __restrict__ dout,
@@ -311,20 +352,16 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
 {
     using V = typename Vec8<T>::type;
     n = device_count(n, count_dims);
-    if (blockIdx.x * 256u >= n) return;
-    __shared__ __align__(16) T act[4][64 * kActStride];
-    __shared__ float pred[4][64][3];
+    if (blockIdx.x * (64u * WPB) >= n) return;
+    __shared__ __align__(16) T act[WPB][64 * kActStride];
+    __shared__ float pred[WPB][64][3];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t s0 = (blockIdx.x * 4 + wave) * 64;
+    const uint32_t s0 = (blockIdx.x * WPB + wave) * 64;
     const uint32_t s = s0 + lane;
     T* A = act[wave];
     if constexpr (ENC == kEncHash) {
-        encode_hash_row<T>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
-        if (TRAIN && s < n) {
-#pragma unroll
-            for (int j = 0; j < 64; j += 8)
-                *reinterpret_cast<V*>(&hbuf[(size_t)s * 64 + j]) = *reinterpret_cast<const V*>(&A[lane * kActStride + j]);
-        }
+        encode_hash_row<T, WPB == 1>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
+        if (TRAIN && s0 < n) save_tile(A, lane, hbuf + saved_tile(0, n, s0 / 64));
     } else {
         float e[64];
         if (s < n) {
@@ -339,8 +376,8 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
 #pragma unroll
             for (int k = 0; k < 8; ++k) v[k] = (T)e[j + k];
             *reinterpret_cast<V*>(&A[lane * kActStride + j]) = v;
-            if (TRAIN && s < n) *reinterpret_cast<V*>(&hbuf[(size_t)s * 64 + j]) = v;
         }
+        if (TRAIN && s0 < n) save_tile(A, lane, hbuf + saved_tile(0, n, s0 / 64));
     }
     const uint32_t r16 = lane & 15, kq = 8 * (lane >> 4);
     for (uint32_t l = 0; l <= L; ++l) {
@@ -380,12 +417,7 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
                     for (int i = 0; i < 4; ++i) v[i] = (T)fmaxf(acc[mt][nt][i], 0.0f);
                     *reinterpret_cast<typename Vec4<T>::type*>(&A[(mt * 16 + r16) * kActStride + nt * 16 + 4 * (lane >> 4)]) = v;
                 }
-            if (TRAIN && s < n) {
-                T* dst = hbuf + ((size_t)(l + 1) * n + s) * 64;
-#pragma unroll
-                for (int j = 0; j < 64; j += 8)
-                    *reinterpret_cast<V*>(&dst[j]) = *reinterpret_cast<const V*>(&A[lane * kActStride + j]);
-            }
+            if (TRAIN && s0 < n) save_tile(A, lane, hbuf + saved_tile(l + 1, n, s0 / 64));
         } else if ((lane >> 4) == 0) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
@@ -424,7 +456,7 @@ __global__ __launch_bounds__(256, 3) void forward_kernel(const T* __restrict__ w
         for (int c = 3; c < (int)kOutPad; ++c) g[c] = (T)0.0f;
     }
     for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o, 64);
-    if (lane == 0) loss_part[blockIdx.x * 4 + wave] = loss;
+    if (lane == 0) loss_part[blockIdx.x * WPB + wave] = loss;
 }
 
 // ---------------------------------------------------------------- FP8 inference
@@ -499,7 +531,7 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
     const uint32_t s = (blockIdx.x * 4 + wave) * 64 + lane;
     T* A = act[wave];
     if constexpr (ENC == kEncHash) {
-        encode_hash_row<T>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
+        encode_hash_row<T, false>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
     } else {
         float e[64];
         if (s < n) {
@@ -616,107 +648,135 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
 
 // The hash tables' gradient without global atomics.  A level's entries are split in kScatterParts
 // ranges and the samples in up to kScatterSlices slices (of at least kScatterMinSlice samples, so that a
-// small batch still spreads over 256 blocks); block (level, range, slice) walks the slice's samples,
+// small batch still spreads over 256 workgroups); workgroup (level, range, slice) walks the slice's samples,
 // adds the trilinear contributions that land in its range to a copy of the range in LDS, and writes
 // the copy to the slice's partial table.  Every table float of every slice is written, so
-// grid_reduce_kernel sums the partials into the gradient without clearing it first.  (Global float
+// grad_reduce_kernel sums the partials into the gradient without clearing it first.  (Global float
 // atomics, one per corner, feature and sample, ran at about 2e10 per second: 16.8 M of them made up
 // four fifths of a 65,536-sample step.)
 constexpr uint32_t kScatterParts = 4;
 constexpr uint32_t kScatterSlices = 4;
 constexpr uint32_t kScatterMinSlice = 4096;
 constexpr uint32_t kScatterRange = kHashT / kScatterParts;  // entries per range (at most)
+// The kernel is bound by its LDS float atomics (a third of its wave cycles wait on LDS instructions):
+// 16 wavefronts per workgroup instead of 4, or two ranges of a level per workgroup with eight slices
+// (half the atomic instructions per workgroup), both left a 16,384-sample step at 40 us.
+constexpr uint32_t kScatterThreads = 256;
 __device__ __forceinline__ uint32_t level_entries(const HashGeom& gm, uint32_t l, uint32_t p_grid)
 {
     return ((l + 1 < kHashLevels ? gm.offset[l + 1] : p_grid) - gm.offset[l]) / RTC8X_NRC_HASH_FEATURES;
 }
 
-__global__ __launch_bounds__(256) void grid_scatter_kernel(const rtc8x_radiance_query* __restrict__ in, uint32_t n,
+__global__ __launch_bounds__(kScatterThreads) void grid_scatter_kernel(const rtc8x_radiance_query* __restrict__ in, uint32_t n,
                                                            const float* __restrict__ dE, const HashGeom geom,
                                                            uint32_t p_grid, uint32_t slice_len, float* __restrict__ part)
 {
-    __shared__ float acc[2 * kScatterRange];
+    __shared__ __align__(16) float acc[2 * kScatterRange];
     const uint32_t lv = blockIdx.x / kScatterParts, r = blockIdx.x % kScatterParts, slice = blockIdx.y;
     const uint32_t entries = level_entries(geom, lv, p_grid);
     const uint32_t per = (entries + kScatterParts - 1) / kScatterParts;
     const uint32_t e0 = min(entries, r * per), e1 = min(entries, e0 + per);
-    for (uint32_t i = threadIdx.x; i < 2 * (e1 - e0); i += 256) acc[i] = 0.0f;
+    // level sizes are multiples of 8 entries, so a range is a whole number of 16-byte pairs of entries
+    const uint32_t quads = (e1 - e0) / 2;
+    for (uint32_t i = threadIdx.x; i < quads; i += kScatterThreads) reinterpret_cast<f4*>(acc)[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     __syncthreads();
     const uint32_t s1 = min(n, (slice + 1) * slice_len);
-    for (uint32_t sm = slice * slice_len + threadIdx.x; sm < s1; sm += 256) {
-        float p[3];
-        query_position01(in[sm], p);
-        uint32_t gi[3];
-        float f[3];
-        hash_cell(geom, lv, p, gi, f);
-        const float g0 = dE[(size_t)sm * 32 + 2 * lv], g1 = dE[(size_t)sm * 32 + 2 * lv + 1];
+    constexpr uint32_t kBatch = 4;  // samples per lane whose loads are issued together
+    for (uint32_t sb = slice * slice_len + threadIdx.x; sb < s1; sb += kBatch * kScatterThreads) {
+        float p[kBatch][3];
+        float2 g[kBatch];
 #pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            uint32_t idx;
-            float w;
-            hash_corner(geom, lv, gi, f, c, idx, w);
-            const uint32_t e = (idx - geom.offset[lv]) / 2u;
-            if (e >= e0 && e < e1) {
-                atomicAdd(&acc[2 * (e - e0)], w * g0);
-                atomicAdd(&acc[2 * (e - e0) + 1], w * g1);
+        for (uint32_t k = 0; k < kBatch; ++k) {
+            const uint32_t sm = min(sb + k * kScatterThreads, s1 - 1);
+            query_position01(in[sm], p[k]);
+            __builtin_memcpy(&g[k], dE + (size_t)sm * 32 + 2 * lv, sizeof(float2));
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; ++k) {
+            if (sb + k * kScatterThreads < s1) {
+                uint32_t gi[3];
+                float f[3];
+                hash_cell(geom, lv, p[k], gi, f);
+#pragma unroll
+                for (uint32_t c = 0; c < 8; ++c) {
+                    uint32_t idx;
+                    float w;
+                    hash_corner(geom, lv, gi, f, c, idx, w);
+                    const uint32_t e = (idx - geom.offset[lv]) / 2u;
+                    if (e >= e0 && e < e1) {
+                        atomicAdd(&acc[2 * (e - e0)], w * g[k].x);
+                        atomicAdd(&acc[2 * (e - e0) + 1], w * g[k].y);
+                    }
+                }
             }
         }
     }
     __syncthreads();
     float* out = part + (size_t)slice * p_grid + geom.offset[lv] + 2 * e0;
-    for (uint32_t i = threadIdx.x; i < 2 * (e1 - e0); i += 256) out[i] = acc[i];
+    for (uint32_t i = threadIdx.x; i < quads; i += kScatterThreads) reinterpret_cast<f4*>(out)[i] = reinterpret_cast<const f4*>(acc)[i];
 }
 
-__global__ __launch_bounds__(256) void grid_reduce_kernel(const float* __restrict__ part, uint32_t slices,
-                                                          uint32_t p_grid, float* __restrict__ ggrid)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p_grid) return;
-    float sum = part[i];
-    for (uint32_t h = 1; h < slices; ++h) sum += part[(size_t)h * p_grid + i];
-    ggrid[i] = sum;
-}
-
-template <typename T, int ENC>
-__global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt, uint32_t L, uint32_t n,
-                                                       const T* __restrict__ hbuf, const T* __restrict__ dout,
-                                                       float* __restrict__ gpart, float* __restrict__ dEout)
+template <typename T, int ENC, int WPB>
+__global__ __launch_bounds__(64 * WPB) void backward_kernel(const T* __restrict__ wt, uint32_t L, uint32_t n,
+                                                            const T* __restrict__ hbuf, const T* __restrict__ dout,
+                                                            float* __restrict__ gpart, float* __restrict__ dEout)
 {
     using V = typename Vec8<T>::type;
-    // FP32 keeps dZ only feature-major, so that the three tiles fit the LDS
-    constexpr bool kSampleMajor = sizeof(T) == 2;
-    __shared__ __align__(16) T dzs[kSampleMajor ? 256 * kActStride : 8];   // dZ, sample-major [sample][out]
-    __shared__ __align__(16) T dzf[64 * kColStride];                       // dZ, feature-major [out][sample]
-    __shared__ __align__(16) T hf[64 * kColStride];     // layer input H, feature-major [in][sample]
+    constexpr uint32_t SB = 64 * WPB;      // samples per workgroup
+    constexpr uint32_t CS = SB + 8;        // LDS row stride of a feature-major tile (elements)
+    // FP32 with four waves keeps dZ only feature-major, so that the three tiles fit the LDS
+    constexpr bool kSampleMajor = sizeof(T) == 2 || WPB == 1;
+    __shared__ __align__(16) T dzs[kSampleMajor ? SB * kActStride : 8];   // dZ, sample-major [sample][out]
+    __shared__ __align__(16) T dzf[64 * CS];                              // dZ, feature-major [out][sample]
+    __shared__ __align__(16) T hf[64 * CS];     // layer input H, feature-major [in][sample]
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t r16 = lane & 15, kq = 8 * (lane >> 4);
-    const uint32_t b0 = blockIdx.x * 256;
+    const uint32_t b0 = blockIdx.x * SB;
+    const uint32_t tiles = (n + 63) / 64;
     const uint32_t P = num_params(L);
     float* gblk = gpart + (size_t)blockIdx.x * P;
-    for (uint32_t idx = tid; idx < 256 * 64; idx += 256) {
-        const uint32_t s = idx >> 6, c = idx & 63;
-        const T v = (c < kOutPad && b0 + s < n) ? dout[(size_t)(b0 + s) * kOutPad + c] : (T)0.0f;
-        if constexpr (kSampleMajor) dzs[s * kActStride + c] = v;
-        dzf[c * kColStride + s] = v;
+    {
+        // the output layer's dZ: thread tid holds sample b0 + tid; columns kOutPad..31 are the zero K
+        // padding of the output layer's dH product (columns 32..63 and rows 32..63 are written before
+        // they are read)
+        T g[kOutPad];
+#pragma unroll
+        for (uint32_t c = 0; c < kOutPad; ++c) g[c] = (T)0.0f;
+        if (b0 + tid < n) __builtin_memcpy(g, dout + (size_t)(b0 + tid) * kOutPad, sizeof(g));
+        if constexpr (kSampleMajor) {
+#pragma unroll
+            for (uint32_t c = 0; c < 32; ++c) dzs[tid * kActStride + c] = c < kOutPad ? g[c] : (T)0.0f;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < 32; ++c) dzf[c * CS + tid] = c < kOutPad ? g[c] : (T)0.0f;
     }
     __syncthreads();
     for (int l = (int)L; l >= 0; --l) {
         const bool top = (uint32_t)l == L;
-        for (uint32_t idx = tid; idx < 256 * 64; idx += 256) {
-            const uint32_t s = idx >> 6, c = idx & 63;
-            hf[c * kColStride + s] = (b0 + s < n) ? hbuf[((size_t)l * n + b0 + s) * 64 + c] : (T)0.0f;
+        // H_l of the workgroup's tiles, 16 bytes per copy (zeros for tiles past the batch)
+        for (uint32_t i = tid; i < SB * 8; i += SB) {
+            const uint32_t t = i >> 9, c = (i >> 3) & 63u, j8 = 8 * (i & 7u);
+            const uint32_t tile = b0 / 64 + t;
+            V v;
+            if (tile < tiles) {
+                v = *reinterpret_cast<const V*>(hbuf + saved_tile((uint32_t)l, n, tile) + c * 64 + j8);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = (T)0.0f;
+            }
+            *reinterpret_cast<V*>(&hf[c * CS + t * 64 + j8]) = v;
         }
         __syncthreads();
         // dW_l[out][in] = sum over the block's samples of dZ[s][out] H[s][in]
         const int otiles = top ? 1 : 4;
         const uint32_t goff = (uint32_t)l * kLayer;
-        for (int t = (int)wave; t < otiles * 4; t += 4) {
+        for (int t = (int)wave; t < otiles * 4; t += WPB) {
             const int mt = t >> 2, nt = t & 3;
             f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int ks = 0; ks < 8; ++ks) {
-                const V a = *reinterpret_cast<const V*>(&dzf[(mt * 16 + r16) * kColStride + ks * 32 + kq]);
-                const V b = *reinterpret_cast<const V*>(&hf[(nt * 16 + r16) * kColStride + ks * 32 + kq]);
+            for (int ks = 0; ks < (int)(SB / 32); ++ks) {
+                const V a = *reinterpret_cast<const V*>(&dzf[(mt * 16 + r16) * CS + ks * 32 + kq]);
+                const V b = *reinterpret_cast<const V*>(&hf[(nt * 16 + r16) * CS + ks * 32 + kq]);
                 acc = mfma(a, b, acc);
             }
 #pragma unroll
@@ -741,7 +801,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
                     a[mt] = *reinterpret_cast<const V*>(&dzs[(wave * 64 + mt * 16 + r16) * kActStride + ks * 32 + kq]);
                 } else {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) a[mt][j] = dzf[(ks * 32 + kq + j) * kColStride + wave * 64 + mt * 16 + r16];
+                    for (int j = 0; j < 8; ++j) a[mt][j] = dzf[(ks * 32 + kq + j) * CS + wave * 64 + mt * 16 + r16];
                 }
             }
 #pragma unroll
@@ -764,7 +824,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
                         dE[(wave * 64 + mt * 16 + 4 * (lane >> 4) + i) * 32 + nt * 16 + r16] = acc[mt][nt][i];
             __syncthreads();
             // the rows of dE go out for grid_scatter_kernel, 16 bytes per lane and store
-            for (uint32_t i = tid; i < 256 * 8; i += 256) {
+            for (uint32_t i = tid; i < SB * 8; i += SB) {
                 const uint32_t sm = i >> 3;
                 if (b0 + sm < n)
                     *reinterpret_cast<f4*>(dEout + (size_t)(b0 + sm) * 32 + 4 * (i & 7u)) =
@@ -780,24 +840,25 @@ __global__ __launch_bounds__(256) void backward_kernel(const T* __restrict__ wt,
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const uint32_t s = wave * 64 + mt * 16 + 4 * (lane >> 4) + i, c = nt * 16 + r16;
-                    const float hv = (float)hf[c * kColStride + s];
+                    const float hv = (float)hf[c * CS + s];
                     const T v = (T)(hv > 0.0f ? acc[mt][nt][i] : 0.0f);
                     if constexpr (kSampleMajor) dzs[s * kActStride + c] = v;
-                    dzf[c * kColStride + s] = v;
+                    dzf[c * CS + s] = v;
                 }
         __syncthreads();
     }
 }
 
-// Sum of the per-workgroup weight gradients in workgroup order; flags non-finite sums.
-// (Hash-grid table gradients, entries P .. P_total, are accumulated directly and only checked here.)
-// The MLP gradient from the per-block partials, and the finiteness check of the whole gradient.
-// Workgroups below mlp_groups reduce 64 MLP parameters each: lane (i, c) sums the partials of blocks
-// c, c + 8, c + 16, ... (four loads in flight), and the eight slices are added in a fixed order, so the
-// result does not depend on scheduling.  The other workgroups check 512 hash-table gradients each.
-constexpr uint32_t kReduceSlices = 8;
-__global__ __launch_bounds__(512) void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P,
-                                                          uint32_t P_total, float* __restrict__ grad,
+// The gradient from its partials, and the finiteness check of the whole gradient.  Workgroups below
+// mlp_groups reduce 64 MLP parameters each: lane (i, c) sums the per-workgroup partials of blocks
+// c, c + 16, c + 32, ... (four loads in flight), and the 16 slices are added in a fixed order, so the
+// result does not depend on scheduling.  The other workgroups sum 1,024 hash-table gradients each over
+// the scatter's slices (entries P .. P_total; none without the hash grid).  Any non-finite sum sets *bad.
+constexpr uint32_t kReduceSlices = 16;
+constexpr uint32_t kReduceThreads = 64 * kReduceSlices;
+__global__ __launch_bounds__(kReduceThreads) void grad_reduce_kernel(const float* __restrict__ gpart, uint32_t nblocks, uint32_t P,
+                                                          uint32_t P_total, const float* __restrict__ grid_part,
+                                                          uint32_t slices, float* __restrict__ grad,
                                                           uint32_t* __restrict__ bad)
 {
     const uint32_t mlp_groups = (P + 63) / 64;
@@ -824,8 +885,13 @@ __global__ __launch_bounds__(512) void grad_reduce_kernel(const float* __restric
             nonfinite = !isfinite(g);
         }
     } else {
-        const size_t i = (size_t)P + (size_t)(blockIdx.x - mlp_groups) * 512 + threadIdx.x;
-        if (i < P_total) nonfinite = !isfinite(grad[i]);
+        const size_t j = (size_t)(blockIdx.x - mlp_groups) * kReduceThreads + threadIdx.x, p_grid = P_total - P;
+        if (j < p_grid) {
+            float sum = grid_part[j];
+            for (uint32_t h = 1; h < slices; ++h) sum += grid_part[h * p_grid + j];
+            grad[P + j] = sum;
+            nonfinite = !isfinite(sum);
+        }
     }
     if (__ballot(nonfinite) != 0 && (threadIdx.x & 63u) == 0) atomicOr(bad, 1u);
 }
@@ -877,9 +943,12 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
 // (not above kLossScaleMax).  One lane; after the optimiser step.
 constexpr float kLossScaleMax = 16777216.0f;  // 2^24
 constexpr float kLossScaleGrowth = 128.0f;
-__global__ void loss_scale_kernel(const uint32_t* __restrict__ bad, float* __restrict__ state)
+// The step's last reader of *bad: it clears the flag for the next step.
+__global__ void loss_scale_kernel(uint32_t* __restrict__ bad, float* __restrict__ state)
 {
-    if (*bad) {
+    const bool skipped = *bad != 0u;
+    *bad = 0u;
+    if (skipped) {
         state[0] = fmaxf(1.0f, state[0] * 0.5f);
         state[1] = 0.0f;
     } else if (state[1] + 1.0f >= kLossScaleGrowth) {
@@ -957,19 +1026,32 @@ struct rtc8x_nrc {
 
 namespace {
 
+// Waves per workgroup of the training kernels for a batch of n samples (the loss parts and the
+// per-workgroup weight gradients follow it: one loss part per wave, one gradient slot per workgroup).
+uint32_t train_waves_per_block(uint32_t n) { return n <= nrc::kSmallBatch ? 1u : 4u; }
+uint32_t train_blocks(uint32_t n) { return (n + 64 * train_waves_per_block(n) - 1) / (64 * train_waves_per_block(n)); }
+// Sample slices of the hash tables' gradient scatter for a batch of n samples.
+uint32_t scatter_slices(uint32_t n)
+{
+    return std::min(nrc::kScatterSlices, (n + nrc::kScatterMinSlice - 1) / nrc::kScatterMinSlice);
+}
+
 template <typename T, int ENC>
 hipError_t launch_forward_enc(rtc8x_nrc* h, bool train, const void* w, const float* tables, hipStream_t s, const float* in,
                               uint32_t n, float* out, const float* target, const int32_t* count_dims)
 {
-    const uint32_t blocks = (n + 255) / 256;
-    if (train)
-        hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
-                           (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part, tables,
-                           h->geom, (const int32_t*)nullptr, (const float*)h->loss_scale);
+    if (train && train_waves_per_block(n) == 1)
+        hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC, 1>), dim3(train_blocks(n)), dim3(64), 0, s, (const T*)w,
+                           h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part,
+                           tables, h->geom, (const int32_t*)nullptr, (const float*)h->loss_scale);
+    else if (train)
+        hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC, 4>), dim3(train_blocks(n)), dim3(256), 0, s, (const T*)w,
+                           h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part,
+                           tables, h->geom, (const int32_t*)nullptr, (const float*)h->loss_scale);
     else
-        hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC>), dim3(blocks), dim3(256), 0, s, (const T*)w, h->L,
-                           (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr, (float*)nullptr,
-                           tables, h->geom, count_dims, (const float*)h->loss_scale);
+        hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC, 4>), dim3((n + 255) / 256), dim3(256), 0, s, (const T*)w,
+                           h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr,
+                           (float*)nullptr, tables, h->geom, count_dims, (const float*)h->loss_scale);
     return hipGetLastError();
 }
 
@@ -988,21 +1070,29 @@ hipError_t launch_forward(rtc8x_nrc* h, bool train, hipStream_t s, const float* 
 template <typename T>
 hipError_t launch_backward(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n)
 {
-    const uint32_t blocks = (n + 255) / 256;
+    const dim3 blocks(train_blocks(n));
+    const bool one = train_waves_per_block(n) == 1;
     if (h->enc != RTC8X_NRC_HASH_GRID) {
-        hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_TRIANGLE_WAVE>), dim3(blocks), dim3(256), 0, s,
-                           (const T*)h->wt, h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, (float*)nullptr);
+        if (one)
+            hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_TRIANGLE_WAVE, 1>), blocks, dim3(64), 0, s,
+                               (const T*)h->wt, h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, (float*)nullptr);
+        else
+            hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_TRIANGLE_WAVE, 4>), blocks, dim3(256), 0, s,
+                               (const T*)h->wt, h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, (float*)nullptr);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, (const T*)h->wt,
-                       h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, h->grid_de);
-    const uint32_t slices = std::min(nrc::kScatterSlices, (n + nrc::kScatterMinSlice - 1) / nrc::kScatterMinSlice);
+    if (one)
+        hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_HASH_GRID, 1>), blocks, dim3(64), 0, s, (const T*)h->wt,
+                           h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, h->grid_de);
+    else
+        hipLaunchKernelGGL((nrc::backward_kernel<T, RTC8X_NRC_HASH_GRID, 4>), blocks, dim3(256), 0, s, (const T*)h->wt,
+                           h->L, n, (const T*)h->hbuf, (const T*)h->dout, h->gpart, h->grid_de);
+    const uint32_t slices = scatter_slices(n);
     const uint32_t slice_len = (n + slices - 1) / slices;
-    hipLaunchKernelGGL(nrc::grid_scatter_kernel, dim3(nrc::kHashLevels * nrc::kScatterParts, slices), dim3(256), 0, s,
+    hipLaunchKernelGGL(nrc::grid_scatter_kernel, dim3(nrc::kHashLevels * nrc::kScatterParts, slices),
+                       dim3(nrc::kScatterThreads), 0, s,
                        (const rtc8x_radiance_query*)in, n, (const float*)h->grid_de, h->geom, h->P_grid, slice_len,
                        h->grid_part);
-    hipLaunchKernelGGL(nrc::grid_reduce_kernel, dim3((h->P_grid + 255) / 256), dim3(256), 0, s,
-                       (const float*)h->grid_part, slices, h->P_grid, h->grad + h->P_mlp);
     return hipGetLastError();
 }
 
@@ -1056,6 +1146,7 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
     RTC8X_HIP(hipMemset(h->m, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->v, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->ema, 0, sizeof(float) * h->P));
+    RTC8X_HIP(hipMemset(h->bad, 0, sizeof(uint32_t)));
     RTC8X_HIP(hipMemset(h->wt, 0, h->tsize * h->PT));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_convert<T>(h, nullptr)));
     const float ls[2] = {h->desc.loss_scale > 0.0f ? h->desc.loss_scale : 128.0f, 0.0f};
@@ -1070,10 +1161,11 @@ rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const f
 {
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, true, s, in, n, nullptr, target)));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_backward<T>(h, s, in, n)));
-    RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
-    const uint32_t groups = (h->P_mlp + 63) / 64 + (h->P - h->P_mlp + 511) / 512;
-    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3(groups), dim3(512), 0, s, h->gpart, (n + 255) / 256, h->P_mlp, h->P,
-                       h->grad, h->bad);
+    // *bad is zero here: set so at creation and cleared by each step's loss_scale_kernel (and after a
+    // gradient handed out by rtc8x_nrc_gradient)
+    const uint32_t groups = (h->P_mlp + 63) / 64 + (h->P - h->P_mlp + nrc::kReduceThreads - 1) / nrc::kReduceThreads;
+    hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3(groups), dim3(nrc::kReduceThreads), 0, s, h->gpart, train_blocks(n), h->P_mlp, h->P,
+                       (const float*)h->grid_part, scatter_slices(n), h->grad, h->bad);
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
@@ -1119,11 +1211,12 @@ try {
     h->P = h->P_mlp + h->P_grid;
     h->PT = nrc::num_params_t(h->L);
     const size_t P = h->P, P_mlp = h->P_mlp;
-    const size_t max_blocks = nrc::kMaxTrain / 256;
+    const size_t max_blocks = std::max(nrc::kSmallBatch / 64, nrc::kMaxTrain / 256);  // gradient slots
+    const size_t max_waves = nrc::kMaxTrain / 64;                                    // loss parts
     auto alloc = [&](void** p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
     bool ok = alloc((void**)&h->master, 4 * P) && alloc((void**)&h->m, 4 * P) && alloc((void**)&h->v, 4 * P) &&
               alloc((void**)&h->ema, 4 * P) && alloc((void**)&h->grad, 4 * P) &&
-              alloc((void**)&h->gpart, 4 * P_mlp * max_blocks) && alloc((void**)&h->loss_part, 4 * 4 * max_blocks) &&
+              alloc((void**)&h->gpart, 4 * P_mlp * max_blocks) && alloc((void**)&h->loss_part, 4 * max_waves) &&
               alloc((void**)&h->bad, 4) && alloc((void**)&h->loss_scale, 8) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
               alloc(&h->winf, h->tsize * P_mlp) && (h->P_grid == 0 || alloc((void**)&h->grid_inf, 4 * (size_t)h->P_grid)) &&
               (h->P_grid == 0 || alloc((void**)&h->grid_de, 4 * 32 * (size_t)nrc::kMaxTrain)) &&
@@ -1248,6 +1341,7 @@ try {
     hipLaunchKernelGGL(nrc::unscale_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, (const float*)h->grad, h->P,
                        (const float*)h->loss_scale, grad);
     RTC8X_HIP(hipGetLastError());
+    RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_nrc_gradient")
@@ -1276,10 +1370,10 @@ try {
     a.P_mlp = h->P_mlp;
     a.L = h->L;
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_adam<T>(h, s, a)));
-    hipLaunchKernelGGL(nrc::loss_scale_kernel, dim3(1), dim3(1), 0, s, (const uint32_t*)h->bad, h->loss_scale);
+    hipLaunchKernelGGL(nrc::loss_scale_kernel, dim3(1), dim3(1), 0, s, h->bad, h->loss_scale);
     RTC8X_HIP(hipGetLastError());
     if (loss_on_cpu) {
-        const uint32_t nw = 4 * ((n + 255) / 256);
+        const uint32_t nw = train_waves_per_block(n) * train_blocks(n);
         std::vector<float> parts(nw);
         RTC8X_HIP(hipMemcpyAsync(parts.data(), h->loss_part, 4 * nw, hipMemcpyDeviceToHost, s));
         RTC8X_HIP(hipStreamSynchronize(s));

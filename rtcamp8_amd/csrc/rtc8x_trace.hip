@@ -33,6 +33,7 @@ constexpr uint32_t kHardMaxPathLength = 4096u;
 constexpr uint32_t kSpeculate = 2;
 // A wavefront with at least this many idle lanes evaluates its stepping lanes' segments together.
 constexpr uint32_t kCoopIdle = 48;
+constexpr uint32_t kTrainSpreadLanes = 64 - kCoopIdle;  // paths per wavefront of a spread training launch
 constexpr uint32_t kCoopLocalPaths = 2;  // at most this many stepping lanes for the local walk's cooperative tail
 constexpr uint32_t kCounterSlots = 1024;
 constexpr uint32_t kMaxBatch = 16;  // instances of one batched launch
@@ -196,6 +197,8 @@ struct TrainStage {
     // redo launch (redo_lost != NULL): the pixels with a lost record, from their saved states
     const uint32_t* redo_lost;
     const uint64_t* rng_save;            // launch pixel order
+    const uint32_t* staged;              // the staging launch's counter: no record was lost when it is at
+                                         //   most kStageCapacity, and the redo launch then exits at once
 };
 
 // PCG32 jump-ahead: the state j draws later is mul[j] * state + add[j] (the LCG composed j times).
@@ -460,6 +463,16 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
     const uint32_t first_ny = H > first_ry ? (H - first_ry + 3u) / 4u : 0u;
     const uint32_t n_first = (gen_data && !p.work_order) ? first_nx * first_ny : 0u;
     const uint32_t total = p.work_order ? W * H : n_first + tiles_x * ((H + tile_h - 1) / tile_h) * 64;
+    // A training launch with few paths for the persistent grid (frame 0's quarter-size training image,
+    // a small frame) is bound by its longest paths' chains of lookups, not by throughput.  Its
+    // wavefronts then take at most `lanes` paths each, so that the paths spread over the whole grid and
+    // every wavefront has the idle lanes that take its paths' segments cooperatively from the start.
+    uint32_t lanes = 64;
+    if constexpr (gen_data) {
+        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+        const uint32_t per = (total + waves - 1) / waves;
+        if (per <= kTrainSpreadLanes) lanes = per > 0 ? per : 1u;
+    }
     // default event thresholds, measured on the 512^3 cloud at 1080p: the render entries 5/8 with the
     // local walk (4/8 gave 35.1 against 34.7 ms per plain 16-spp frame, 6/8 34.7) and 4/8 with the global
     // one (3/8: 44.0 against 43.2 ms); the training entry 3/8 (no difference measured)
@@ -516,6 +529,10 @@ __device__ __forceinline__ void trace_body(const rtc8x_frame_params& p, uint32_t
         // ---- events until this lane has a tracking or shadow segment under way, or no work left
         while (run_events && L.mode != kModeIdle && L.mode != kModeTrack && L.mode != kModeShadow) {
             if (L.mode == kModeWork) {
+                if (lane_id >= lanes) {
+                    L.mode = kModeIdle;
+                    continue;
+                }
                 // the next pixel from the work counter, one atomic per wavefront
                 const uint64_t ask = __ballot(1);
                 const uint32_t leader = (uint32_t)__ffsll((long long)ask) - 1u;
@@ -988,6 +1005,11 @@ void wavefront_kernel(const rtc8x_frame_params p, uint32_t launch_w,
                                                         const uint32_t* __restrict__ record_offsets,
                                                         uint32_t* __restrict__ record_counts, const TrainStage stage)
 {
+    if constexpr (ENTRY == RTC8X_ENTRY_GENERATE_TRAINING_DATA) {
+        // every record found a staging slot: no pixel for the redo launch (a walk over the training image
+        // that skips every pixel took about 0.1 ms)
+        if (stage.redo_lost != nullptr && *stage.staged <= kStageCapacity) return;
+    }
     if constexpr (TRACKING == RTC8X_TRACKING_LOCAL_MAJORANT) {
         // the region majorants staged in LDS: every DDA step reads them there
         __shared__ __align__(16) uint8_t regions[RTC8X_MAX_REGIONS];
@@ -1682,6 +1704,7 @@ try {
     dev::TrainStage redo{};
     redo.redo_lost = stage.lost;
     redo.rng_save = rng_save;
+    redo.staged = stage.counter;
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, 0, 0, 1, offsets, nullptr, s, redo)));
     return RTC8X_OK;
 }

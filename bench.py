@@ -59,9 +59,10 @@ def parse():
                     help="spp of each half of the equal-RMSE reference (0 skips the equal-RMSE leg)")
     ap.add_argument("--crop-ref-spp", type=int, default=256,
                     help="spp of each half of the FP64 oracle's 64 x 64 crop judging the equal-RMSE leg (0 skips it)")
-    ap.add_argument("--legs", choices=["all", "none", "network"], default="all",
-                    help="legs beside the headline frame: all of them, none (e.g. for counter passes), or only the NRC "
-                         "network steps")
+    ap.add_argument("--legs", choices=["all", "none", "network", "dense128"], default="all",
+                    help="legs beside the headline frame: all of them, none (e.g. for counter passes), only the NRC "
+                         "network steps, or only the equal-RMSE leg on the dense 128^3 cloud (e.g. to profile the NRC "
+                         "schedule)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -786,6 +787,9 @@ def finish(args, result, timers, pt, scene, stream, network_only: bool = False) 
                          "network_fp8": nrc_mlp_leg(net_scene, args.nrc_steps, precision=_abi.NRC_BF16_FP8)}
         result["nrc_queries_per_s"] = result["nrc"]["network"]["queries_per_s"]
         net_scene.close()
+    if args.legs == "dense128" and args.rmse_ref_spp > 0:
+        print("[bench] equal-RMSE leg on the 128^3 dense cloud ...", file=sys.stderr, flush=True)
+        result["equal_rmse_dense128"] = dense128_equal_rmse_leg(args.rmse_ref_spp, stream, crop_ref_spp=args.crop_ref_spp)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)

@@ -904,13 +904,42 @@ struct AdamArgs {
     uint32_t P_mlp;
 };
 
+// Dynamic loss scaling (Micikevicius et al. 2018, "Mixed Precision Training"), in every precision:
+// a state is [scale, finite steps since it last changed], the scale a power of two.  A step whose
+// gradient is not finite (or whose scaled loss gradient left FP16's range) was skipped by the
+// optimiser: the scale halves (not below 1).  After kLossScaleGrowth finite steps in a row it doubles
+// (not above kLossScaleMax).  The state and the non-finite flag are double-buffered by step parity:
+// step k reads state[k % 2] and flag[k % 2], and lane 0 of its optimiser kernel writes
+// state[(k + 1) % 2] and clears flag[(k + 1) % 2], which no kernel of step k reads (a one-lane kernel
+// after the optimiser took 5 us per step).
+constexpr float kLossScaleMax = 16777216.0f;  // 2^24
+constexpr float kLossScaleGrowth = 128.0f;
+__device__ __forceinline__ void next_loss_scale(bool skipped, const float* __restrict__ cur, float* __restrict__ next)
+{
+    if (skipped) {
+        next[0] = fmaxf(1.0f, cur[0] * 0.5f);
+        next[1] = 0.0f;
+    } else if (cur[1] + 1.0f >= kLossScaleGrowth) {
+        next[0] = fminf(kLossScaleMax, cur[0] * 2.0f);
+        next[1] = 0.0f;
+    } else {
+        next[0] = cur[0];
+        next[1] = cur[1] + 1.0f;
+    }
+}
+
 template <typename T>
 __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
                             float* __restrict__ ema, const float* __restrict__ grad, uint32_t* __restrict__ bad,
                             T* __restrict__ w, T* __restrict__ wt, T* __restrict__ winf, float* __restrict__ grid_inf,
-                            const float* __restrict__ loss_scale)
+                            const float* __restrict__ loss_scale, float* __restrict__ next_scale,
+                            uint32_t* __restrict__ next_bad)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        next_loss_scale(*bad != 0u, loss_scale, next_scale);
+        *next_bad = 0u;
+    }
     if (i >= a.P) return;
     if (*bad) return;  // a non-finite gradient skips the whole step
     const float w0 = master[i];
@@ -934,29 +963,6 @@ __global__ void adam_kernel(const AdamArgs a, float* __restrict__ master, float*
         wt[(size_t)l * kLayer + in * kWidth + out] = (T)w1;
     else
         wt[(size_t)a.L * kLayer + in * kOutT + out] = (T)w1;
-}
-
-// Dynamic loss scaling (Micikevicius et al. 2018, "Mixed Precision Training"), in every precision:
-// state[0] is the scale, a power of two; state[1] counts the finite steps since it last changed.  A step
-// whose gradient is not finite (or whose scaled loss gradient left FP16's range) was skipped by the
-// optimiser: the scale halves (not below 1).  After kLossScaleGrowth finite steps in a row it doubles
-// (not above kLossScaleMax).  One lane; after the optimiser step.
-constexpr float kLossScaleMax = 16777216.0f;  // 2^24
-constexpr float kLossScaleGrowth = 128.0f;
-// The step's last reader of *bad: it clears the flag for the next step.
-__global__ void loss_scale_kernel(uint32_t* __restrict__ bad, float* __restrict__ state)
-{
-    const bool skipped = *bad != 0u;
-    *bad = 0u;
-    if (skipped) {
-        state[0] = fmaxf(1.0f, state[0] * 0.5f);
-        state[1] = 0.0f;
-    } else if (state[1] + 1.0f >= kLossScaleGrowth) {
-        state[0] = fminf(kLossScaleMax, state[0] * 2.0f);
-        state[1] = 0.0f;
-    } else {
-        state[1] += 1.0f;
-    }
 }
 
 // g / loss scale, for the gradient handed out by rtc8x_nrc_gradient
@@ -1018,13 +1024,16 @@ struct rtc8x_nrc {
     float* grid_de = nullptr;   // hash grid: the encoded input's gradient, 32 floats per training sample
     float* grid_part = nullptr; // hash grid: the tables' gradient per slice of the batch (kScatterSlices)
     uint32_t* bad = nullptr;
-    float* loss_scale = nullptr;  // [scale, finite steps since it changed] (dynamic for FP16)
+    float* loss_scale = nullptr;  // two [scale, finite steps since it changed] states, by step parity
     void *w = nullptr, *wt = nullptr, *winf = nullptr, *hbuf = nullptr, *dout = nullptr;
     uint8_t* w8 = nullptr;      // E4M3 inference weights (RTC8X_NRC_BF16_FP8)
     float* w8_inv = nullptr;    // per layer: 1 / the layer's weight scale
 };
 
 namespace {
+
+// The loss-scale state the next training step reads (by step parity)
+const float* scale_state(const rtc8x_nrc* h) { return h->loss_scale + 2 * (h->step & 1u); }
 
 // Waves per workgroup of the training kernels for a batch of n samples (the loss parts and the
 // per-workgroup weight gradients follow it: one loss part per wave, one gradient slot per workgroup).
@@ -1043,15 +1052,15 @@ hipError_t launch_forward_enc(rtc8x_nrc* h, bool train, const void* w, const flo
     if (train && train_waves_per_block(n) == 1)
         hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC, 1>), dim3(train_blocks(n)), dim3(64), 0, s, (const T*)w,
                            h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part,
-                           tables, h->geom, (const int32_t*)nullptr, (const float*)h->loss_scale);
+                           tables, h->geom, (const int32_t*)nullptr, scale_state(h));
     else if (train)
         hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC, 4>), dim3(train_blocks(n)), dim3(256), 0, s, (const T*)w,
                            h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part,
-                           tables, h->geom, (const int32_t*)nullptr, (const float*)h->loss_scale);
+                           tables, h->geom, (const int32_t*)nullptr, scale_state(h));
     else
         hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC, 4>), dim3((n + 255) / 256), dim3(256), 0, s, (const T*)w,
                            h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr,
-                           (float*)nullptr, tables, h->geom, count_dims, (const float*)h->loss_scale);
+                           (float*)nullptr, tables, h->geom, count_dims, scale_state(h));
     return hipGetLastError();
 }
 
@@ -1096,11 +1105,13 @@ hipError_t launch_backward(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_
     return hipGetLastError();
 }
 
+// step parity par: reads state and flag par, writes state and clears flag par ^ 1
 template <typename T>
-hipError_t launch_adam(rtc8x_nrc* h, hipStream_t s, const nrc::AdamArgs& a)
+hipError_t launch_adam(rtc8x_nrc* h, hipStream_t s, const nrc::AdamArgs& a, uint32_t par)
 {
     hipLaunchKernelGGL((nrc::adam_kernel<T>), dim3((h->P + 255) / 256), dim3(256), 0, s, a, h->master, h->m, h->v, h->ema,
-                       h->grad, h->bad, (T*)h->w, (T*)h->wt, (T*)h->winf, h->grid_inf, (const float*)h->loss_scale);
+                       h->grad, h->bad + par, (T*)h->w, (T*)h->wt, (T*)h->winf, h->grid_inf,
+                       (const float*)(h->loss_scale + 2 * par), h->loss_scale + 2 * (par ^ 1u), h->bad + (par ^ 1u));
     return hipGetLastError();
 }
 
@@ -1146,10 +1157,11 @@ rtc8x_status reset_from_master(rtc8x_nrc* h)
     RTC8X_HIP(hipMemset(h->m, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->v, 0, sizeof(float) * h->P));
     RTC8X_HIP(hipMemset(h->ema, 0, sizeof(float) * h->P));
-    RTC8X_HIP(hipMemset(h->bad, 0, sizeof(uint32_t)));
+    RTC8X_HIP(hipMemset(h->bad, 0, 2 * sizeof(uint32_t)));
     RTC8X_HIP(hipMemset(h->wt, 0, h->tsize * h->PT));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_convert<T>(h, nullptr)));
-    const float ls[2] = {h->desc.loss_scale > 0.0f ? h->desc.loss_scale : 128.0f, 0.0f};
+    const float s0 = h->desc.loss_scale > 0.0f ? h->desc.loss_scale : 128.0f;
+    const float ls[4] = {s0, 0.0f, s0, 0.0f};
     RTC8X_HIP(hipMemcpy(h->loss_scale, ls, sizeof(ls), hipMemcpyHostToDevice));
     RTC8X_HIP(hipDeviceSynchronize());
     h->step = 0;
@@ -1161,11 +1173,11 @@ rtc8x_status gradient_pass(rtc8x_nrc* h, hipStream_t s, const float* in, const f
 {
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, true, s, in, n, nullptr, target)));
     RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_backward<T>(h, s, in, n)));
-    // *bad is zero here: set so at creation and cleared by each step's loss_scale_kernel (and after a
-    // gradient handed out by rtc8x_nrc_gradient)
+    // the step's flag is zero here: set so at creation, cleared by the previous step's optimiser kernel
+    // (and after a gradient handed out by rtc8x_nrc_gradient)
     const uint32_t groups = (h->P_mlp + 63) / 64 + (h->P - h->P_mlp + nrc::kReduceThreads - 1) / nrc::kReduceThreads;
     hipLaunchKernelGGL(nrc::grad_reduce_kernel, dim3(groups), dim3(nrc::kReduceThreads), 0, s, h->gpart, train_blocks(n), h->P_mlp, h->P,
-                       (const float*)h->grid_part, scatter_slices(n), h->grad, h->bad);
+                       (const float*)h->grid_part, scatter_slices(n), h->grad, h->bad + (h->step & 1u));
     RTC8X_HIP(hipGetLastError());
     return RTC8X_OK;
 }
@@ -1217,7 +1229,7 @@ try {
     bool ok = alloc((void**)&h->master, 4 * P) && alloc((void**)&h->m, 4 * P) && alloc((void**)&h->v, 4 * P) &&
               alloc((void**)&h->ema, 4 * P) && alloc((void**)&h->grad, 4 * P) &&
               alloc((void**)&h->gpart, 4 * P_mlp * max_blocks) && alloc((void**)&h->loss_part, 4 * max_waves) &&
-              alloc((void**)&h->bad, 4) && alloc((void**)&h->loss_scale, 8) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
+              alloc((void**)&h->bad, 8) && alloc((void**)&h->loss_scale, 16) && alloc(&h->w, h->tsize * P_mlp) && alloc(&h->wt, h->tsize * h->PT) &&
               alloc(&h->winf, h->tsize * P_mlp) && (h->P_grid == 0 || alloc((void**)&h->grid_inf, 4 * (size_t)h->P_grid)) &&
               (h->P_grid == 0 || alloc((void**)&h->grid_de, 4 * 32 * (size_t)nrc::kMaxTrain)) &&
               (h->P_grid == 0 || alloc((void**)&h->grid_part, 4 * (size_t)h->P_grid * nrc::kScatterSlices)) &&
@@ -1269,7 +1281,7 @@ extern "C" rtc8x_status rtc8x_nrc_get_loss_scale(const rtc8x_nrc* h, float* scal
 try {
     RTC8X_REQUIRE(h && scale, "rtc8x_nrc_get_loss_scale: NULL argument");
     RTC8X_HIP(hipDeviceSynchronize());
-    RTC8X_HIP(hipMemcpy(scale, h->loss_scale, sizeof(float), hipMemcpyDeviceToHost));
+    RTC8X_HIP(hipMemcpy(scale, scale_state(h), sizeof(float), hipMemcpyDeviceToHost));
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_nrc_get_loss_scale")
@@ -1339,9 +1351,9 @@ try {
     rtc8x_status st = gradient_pass(h, s, in, target, n);
     if (st != RTC8X_OK) return st;
     hipLaunchKernelGGL(nrc::unscale_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, (const float*)h->grad, h->P,
-                       (const float*)h->loss_scale, grad);
+                       scale_state(h), grad);
     RTC8X_HIP(hipGetLastError());
-    RTC8X_HIP(hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s));
+    RTC8X_HIP(hipMemsetAsync(h->bad + (h->step & 1u), 0, sizeof(uint32_t), s));
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_nrc_gradient")
@@ -1352,6 +1364,7 @@ try {
     RTC8X_REQUIRE(h && in && target, "rtc8x_nrc_train: NULL argument");
     RTC8X_REQUIRE(n > 0 && n <= nrc::kMaxTrain, "rtc8x_nrc_train: n must be in [1, %u]", nrc::kMaxTrain);
     hipStream_t s = (hipStream_t)stream;
+    const uint32_t par = (uint32_t)(h->step & 1u);
     rtc8x_status st = gradient_pass(h, s, in, target, n);
     if (st != RTC8X_OK) return st;
     h->step += 1;
@@ -1369,9 +1382,7 @@ try {
     a.P = h->P;
     a.P_mlp = h->P_mlp;
     a.L = h->L;
-    RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_adam<T>(h, s, a)));
-    hipLaunchKernelGGL(nrc::loss_scale_kernel, dim3(1), dim3(1), 0, s, h->bad, h->loss_scale);
-    RTC8X_HIP(hipGetLastError());
+    RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_adam<T>(h, s, a, par)));
     if (loss_on_cpu) {
         const uint32_t nw = train_waves_per_block(n) * train_blocks(n);
         std::vector<float> parts(nw);

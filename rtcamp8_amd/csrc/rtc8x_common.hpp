@@ -56,7 +56,7 @@ struct StageSnapshot {
 // final slot) and the staging counter.
 struct NrcScratchLayout {
     size_t counts, offsets, shuffle, rng_save, path_length, lookups, scatters, lost;
-    size_t queries, targets, info, tag, snap, fmap, counter, tiles, total;
+    size_t queries, targets, info, tag, snap, counter, tiles, total;
 };
 constexpr uint32_t kPrefixTile = 4096;  // record counts per workgroup of the prefix sum
 inline NrcScratchLayout nrc_scratch_layout(uint32_t width, uint32_t height)
@@ -78,8 +78,7 @@ inline NrcScratchLayout nrc_scratch_layout(uint32_t width, uint32_t height)
     l.info = up16(l.targets + 12 * cap);
     l.tag = l.info + sizeof(rtc8x_training_vertex_info) * cap;
     l.snap = l.tag + 8 * cap;
-    l.fmap = l.snap + sizeof(StageSnapshot) * cap;
-    l.counter = up16(l.fmap + 4 * cap);
+    l.counter = up16(l.snap + sizeof(StageSnapshot) * cap);
     l.tiles = l.counter + 16;
     l.total = l.tiles + 8 * ((px + kPrefixTile - 1) / kPrefixTile);
     return l;

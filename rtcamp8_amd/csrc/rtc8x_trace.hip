@@ -177,7 +177,7 @@ __device__ __forceinline__ void speculate_global(const rtc8x_grid& g, Medium& me
 // GENERATE_TRAINING_DATA in one pass.  Each record takes the next slot of a staging area (query,
 // target, vertex info, tag = launch pixel and the record's index on its path, and a snapshot of the path
 // at the record) in whatever order lanes reach it, and each pixel leaves its record count; a prefix sum
-// of the counts then gives every record its slot in pixel order, and stage_map/stage_copy move the
+// of the counts then gives every record its slot in pixel order, and stage_copy_kernel moves the
 // records there.  A path whose records run past the buffer ends, in pixel-order indexing, at its first
 // record beyond it: the copy rebuilds that end from the record and its snapshot.  When more records come
 // than the staging area holds, the last ones find no slot; their pixels are traced again from their
@@ -1135,13 +1135,10 @@ __global__ __launch_bounds__(256) void record_tile_scan_kernel(const rtc8x_frame
     }
 }
 
-// Final slot of every staged record: its pixel's offset plus its index on the path.
-__global__ __launch_bounds__(256) void stage_map_kernel(const uint32_t* __restrict__ total, const uint32_t* __restrict__ tag,
-                                                        const uint32_t* __restrict__ offsets, uint32_t* __restrict__ fmap)
+// Final slot of staged record j: its pixel's offset plus its index on the path.
+__device__ __forceinline__ uint32_t staged_slot(const TrainStage& st, const uint32_t* __restrict__ offsets, uint32_t j)
 {
-    const uint32_t n = (uint32_t)min((size_t)*total, kStageCapacity);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        fmap[i] = offsets[tag[2 * (size_t)i]] + tag[2 * (size_t)i + 1];
+    return offsets[st.tag[2 * (size_t)j]] + st.tag[2 * (size_t)j + 1];
 }
 
 // Moves the staged records to their final slots (vertex links renamed), and finishes every pixel whose
@@ -1149,10 +1146,10 @@ __global__ __launch_bounds__(256) void stage_map_kernel(const uint32_t* __restri
 // link renamed, its statistics added); a pixel whose records run past the buffer ends at its first
 // record past it (pixel-order slot max(offset, capacity)), as in recordGeneration's full-buffer branch:
 // that record's query becomes the suffix query, and the PCG state, path length and statistics are the
-// record's snapshot.  Pixels with a lost record are left to the redo launch.
+// record's snapshot.  Pixels with a lost record are left to the redo launch.  Slots are computed where
+// they are needed (a kernel writing a slot map first took 5 us per training launch).
 __global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_frame_params p, const uint32_t* __restrict__ total,
-                                                         const TrainStage st, const uint32_t* __restrict__ fmap,
-                                                         const uint32_t* __restrict__ offsets,
+                                                         const TrainStage st, const uint32_t* __restrict__ offsets,
                                                          const uint32_t* __restrict__ counts)
 {
     const rtc8x_nrc_buffers& nb = p.nrc;
@@ -1163,11 +1160,12 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_frame_param
     const uint32_t stride = gridDim.x * blockDim.x;
     uint64_t sum[4] = {0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t f = fmap[i];
+        const uint32_t f = staged_slot(st, offsets, i);
         if (f < RTC8X_TRAIN_BUFFER_SIZE) {
             nb.train_queries[0][f] = st.queries[i];
             rtc8x_training_vertex_info vi = st.info[i];
-            if (vi.prev_vertex_index != RTC8X_INVALID_VERTEX_INDEX) vi.prev_vertex_index = fmap[vi.prev_vertex_index];
+            if (vi.prev_vertex_index != RTC8X_INVALID_VERTEX_INDEX)
+                vi.prev_vertex_index = staged_slot(st, offsets, vi.prev_vertex_index);
             nb.train_vertex_info[f] = vi;
 #pragma unroll
             for (int c = 0; c < 3; ++c) nb.train_targets[0][3 * (size_t)f + c] = st.targets[3 * (size_t)i + c];
@@ -1195,7 +1193,7 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(const rtc8x_frame_param
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += stride) {
         if (st.lost[i] || (uint64_t)offsets[i] + counts[i] > RTC8X_TRAIN_BUFFER_SIZE) continue;
         const uint32_t prev = nb.train_suffix_terminals[i].prev_vertex_index;
-        if (prev != RTC8X_INVALID_VERTEX_INDEX) nb.train_suffix_terminals[i].prev_vertex_index = fmap[prev];
+        if (prev != RTC8X_INVALID_VERTEX_INDEX) nb.train_suffix_terminals[i].prev_vertex_index = staged_slot(st, offsets, prev);
         if (p.path_vertices) p.path_vertices[i] += st.path_length[i];
         sum[0] += 1;
         sum[1] += st.path_length[i];
@@ -1682,7 +1680,6 @@ try {
     stage.lookups = reinterpret_cast<uint32_t*>(sb + lay.lookups);
     stage.scatters = reinterpret_cast<uint32_t*>(sb + lay.scatters);
     stage.lost = reinterpret_cast<uint32_t*>(sb + lay.lost);
-    uint32_t* fmap = reinterpret_cast<uint32_t*>(sb + lay.fmap);
     hipLaunchKernelGGL(dev::reset_value_kernel, dim3(1), dim3(1), 0, s, stage.counter);
     RTC8X_HIP(hipGetLastError());
     RTC8X_HIP((dev::launch_tracking<RTC8X_ENTRY_GENERATE_TRAINING_DATA>(p, 0, 0, 1, nullptr, counts, s, stage)));
@@ -1696,9 +1693,7 @@ try {
                            (const uint64_t*)tile_sums, offsets, total);
         RTC8X_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(dev::stage_map_kernel, dim3(kCompactBlocks), dim3(256), 0, s, total, stage.tag, offsets, fmap);
-    RTC8X_HIP(hipGetLastError());
-    hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, p, total, stage, fmap, offsets,
+    hipLaunchKernelGGL(dev::stage_copy_kernel, dim3(kCompactBlocks), dim3(256), 0, s, p, total, stage, offsets,
                        counts);
     RTC8X_HIP(hipGetLastError());
     dev::TrainStage redo{};

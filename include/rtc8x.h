@@ -728,6 +728,14 @@ rtc8x_status rtc8x_nrc_infer(rtc8x_nrc* nrc, void* stream, const float* in, uint
  * w x h is the current buffer's training size as prepareNRC left it on the device.  The count is read on
  * the device, so unlike the reference the host does not wait for it. */
 rtc8x_status rtc8x_nrc_infer_training(rtc8x_nrc* nrc, void* stream, const struct rtc8x_frame_params* p);
+/* The render inference (renderer_main.cpp:1626-1632: infer() over round_up(W H, 128) queries before
+ * accumulateInferredRadianceValues), for the p->nrc.num_samples slices of a PATH_TRACE_WITH_NRC launch,
+ * evaluated only at the slots whose p->nrc.inference_terminals entry has a query: the only inferred values
+ * rtc8x_nrc_accumulate reads.  Those equal rtc8x_nrc_infer's over the whole batch (FP16, BF16, FP32; with
+ * E4M3 inference the per-tile activation scales come from the queries alone); the other slots of
+ * p->nrc.inferred_radiance are left as they are.  An escaped path has no query: on a sparse cloud most
+ * slots are skipped. */
+rtc8x_status rtc8x_nrc_infer_render(rtc8x_nrc* nrc, void* stream, const struct rtc8x_frame_params* p);
 /* train() (network_interface.cu:151-159): one optimiser step on n samples (n <= 65536); if loss_on_cpu
  * is not NULL the mean loss is copied to it (synchronises the stream). */
 rtc8x_status rtc8x_nrc_train(rtc8x_nrc* nrc, void* stream, const float* in, const float* target, uint32_t n,

@@ -421,6 +421,7 @@ PROTOTYPES = {
     "rtc8x_nrc_destroy": (_S, [_P]),
     "rtc8x_nrc_infer": (_S, [_P, _P, _P, C.c_uint32, _P]),
     "rtc8x_nrc_infer_training": (_S, [_P, _P, C.POINTER(FrameParams)]),
+    "rtc8x_nrc_infer_render": (_S, [_P, _P, C.POINTER(FrameParams)]),
     "rtc8x_nrc_train": (_S, [_P, _P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
     "rtc8x_nrc_get_loss_scale": (_S, [C.c_void_p, C.POINTER(C.c_float)]),
     "rtc8x_nrc_num_params": (C.c_uint32, [_P]),

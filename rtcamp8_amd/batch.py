@@ -57,7 +57,7 @@ def render_nrc(renderers: Sequence[NrcRenderer], samples: Sequence[int], shares=
     _launch(params, samples, _abi.ENTRY_PATH_TRACE_WITH_NRC, shares, stream)
     for r, p, k in zip(renderers, params, samples):
         nq = round_up(r.width * r.height, 128) * int(k)
-        r.cache.infer(stream, r.buffers.inference_queries.ptr, nq, r.buffers.inferred_radiance.ptr)
+        r.cache.infer_render(stream, p)
         call("rtc8x_nrc_accumulate", C.c_void_p(stream), C.byref(p))
         r.samples += int(k)
         r.stats.inference_queries += nq

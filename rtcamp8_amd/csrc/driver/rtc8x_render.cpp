@@ -474,8 +474,7 @@ int render(const Scene& s, const Options& o, const std::string& data_dir)
                 p.pixel_cost = cost.as<uint32_t>();
                 have_cost = true;
                 check(rtc8x_trace(stream, &p, RTC8X_ENTRY_PATH_TRACE_WITH_NRC, k), "trace with NRC");
-                check(rtc8x_nrc_infer(cache, stream, (const float*)nb->inference_queries.ptr,
-                                      (uint32_t)(round_up(npx, 128) * k), nb->inferred_radiance.as<float>()), "infer");
+                check(rtc8x_nrc_infer_render(cache, stream, &p), "infer");
                 check(rtc8x_nrc_accumulate(stream, &p), "accumulate");
                 done += k;
             }

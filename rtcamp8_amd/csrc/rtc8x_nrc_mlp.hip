@@ -310,6 +310,20 @@ __device__ __forceinline__ uint32_t device_count(uint32_t n, const int32_t* coun
 
 constexpr float kHalfMax = 65504.0f;  // the largest finite FP16 value
 
+// The slots of an inference that hold a query.  With terms, the layout of the NRC render entry's
+// buffers (sample k of pixel i: query slot k * q_stride + i, terminal k * npx + i) and the terminals'
+// query flags; without, every slot below the count.
+struct QueryMask {
+    const rtc8x_terminal_info* terms;
+    uint32_t npx, q_stride;
+};
+__device__ __forceinline__ bool slot_has_query(const QueryMask& m, uint32_t s)
+{
+    if (!m.terms) return true;
+    const uint32_t k = s / m.q_stride, i = s - k * m.q_stride;
+    return i < m.npx && (m.terms[(size_t)k * m.npx + i].has_query & 1u) != 0u;
+}
+
 // Training batches up to this size run the training kernels with one wave (64 samples) per workgroup,
 // so that a 16,384-sample step spreads over 256 workgroups instead of 64; larger ones with four.
 constexpr uint32_t kSmallBatch = 32768;
@@ -348,7 +362,7 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 1 : 3) void forward_kernel(con
                                                       T* __restrict__ hbuf, T* __restrict__ dout,
                                                       float* __restrict__ loss_part, const float* __restrict__ tables,
                                                       const HashGeom geom, const int32_t* __restrict__ count_dims,
-                                                      const float* __restrict__ loss_scale)
+                                                      const float* __restrict__ loss_scale, const QueryMask mask)
 {
     using V = typename Vec8<T>::type;
     n = device_count(n, count_dims);
@@ -358,13 +372,17 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 1 : 3) void forward_kernel(con
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t s0 = (blockIdx.x * WPB + wave) * 64;
     const uint32_t s = s0 + lane;
+    // inference: slots without a query are neither encoded nor written, and a wavefront without any
+    // query ends here (the wavefronts are independent: no workgroup barrier follows)
+    const bool live = s < n && (TRAIN || slot_has_query(mask, s));
+    if (!TRAIN && __ballot(live) == 0) return;
     T* A = act[wave];
     if constexpr (ENC == kEncHash) {
-        encode_hash_row<T, WPB == 1>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
+        encode_hash_row<T, WPB == 1>(live ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
         if (TRAIN && s0 < n) save_tile(A, lane, hbuf + saved_tile(0, n, s0 / 64));
     } else {
         float e[64];
-        if (s < n) {
+        if (live) {
             encode_any<ENC>(in[s], tables, geom, e);
         } else {
 #pragma unroll
@@ -426,7 +444,7 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 1 : 3) void forward_kernel(con
         }
     }
     if (!TRAIN) {
-        if (s < n) {
+        if (live) {
             out[3 * (size_t)s] = pred[wave][lane][0];
             out[3 * (size_t)s + 1] = pred[wave][lane][1];
             out[3 * (size_t)s + 2] = pred[wave][lane][2];
@@ -519,7 +537,7 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
                                                           uint32_t L, const rtc8x_radiance_query* __restrict__ in,
                                                           uint32_t n, float* __restrict__ out,
                                                           const float* __restrict__ tables, const HashGeom geom,
-                                                          const int32_t* __restrict__ count_dims)
+                                                          const int32_t* __restrict__ count_dims, const QueryMask mask)
 {
     using T = __bf16;
     using V = typename Vec8<T>::type;
@@ -529,12 +547,16 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
     __shared__ float pred[4][64][3];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t s = (blockIdx.x * 4 + wave) * 64 + lane;
+    // as in forward_kernel: slots without a query are skipped (their zero rows also leave the tile's
+    // activation scales to the queries)
+    const bool live = s < n && slot_has_query(mask, s);
+    if (__ballot(live) == 0) return;
     T* A = act[wave];
     if constexpr (ENC == kEncHash) {
-        encode_hash_row<T, false>(s < n ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
+        encode_hash_row<T, false>(live ? &in[s] : nullptr, tables, geom, &A[lane * kActStride]);
     } else {
         float e[64];
-        if (s < n) {
+        if (live) {
             encode(in[s], e);
         } else {
 #pragma unroll
@@ -639,7 +661,7 @@ __global__ __launch_bounds__(256, 3) void forward_fp8_kernel(const uint8_t* __re
                 for (int i = 0; i < 3; ++i) pred[wave][mt * 16 + r16][i] = acc[mt][0][i] * unscale;
         }
     }
-    if (s < n) {
+    if (live) {
         out[3 * (size_t)s] = pred[wave][lane][0];
         out[3 * (size_t)s + 1] = pred[wave][lane][1];
         out[3 * (size_t)s + 2] = pred[wave][lane][2];
@@ -1047,33 +1069,35 @@ uint32_t scatter_slices(uint32_t n)
 
 template <typename T, int ENC>
 hipError_t launch_forward_enc(rtc8x_nrc* h, bool train, const void* w, const float* tables, hipStream_t s, const float* in,
-                              uint32_t n, float* out, const float* target, const int32_t* count_dims)
+                              uint32_t n, float* out, const float* target, const int32_t* count_dims,
+                              const nrc::QueryMask& mask)
 {
     if (train && train_waves_per_block(n) == 1)
         hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC, 1>), dim3(train_blocks(n)), dim3(64), 0, s, (const T*)w,
                            h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part,
-                           tables, h->geom, (const int32_t*)nullptr, scale_state(h));
+                           tables, h->geom, (const int32_t*)nullptr, scale_state(h), nrc::QueryMask{});
     else if (train)
         hipLaunchKernelGGL((nrc::forward_kernel<T, true, ENC, 4>), dim3(train_blocks(n)), dim3(256), 0, s, (const T*)w,
                            h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)h->hbuf, (T*)h->dout, h->loss_part,
-                           tables, h->geom, (const int32_t*)nullptr, scale_state(h));
+                           tables, h->geom, (const int32_t*)nullptr, scale_state(h), nrc::QueryMask{});
     else
         hipLaunchKernelGGL((nrc::forward_kernel<T, false, ENC, 4>), dim3((n + 255) / 256), dim3(256), 0, s, (const T*)w,
                            h->L, (const rtc8x_radiance_query*)in, n, out, target, (T*)nullptr, (T*)nullptr,
-                           (float*)nullptr, tables, h->geom, count_dims, scale_state(h));
+                           (float*)nullptr, tables, h->geom, count_dims, scale_state(h), mask);
     return hipGetLastError();
 }
 
 // train: the working weights and the FP32 master tables; inference: the EMA weights and tables
 template <typename T>
 hipError_t launch_forward(rtc8x_nrc* h, bool train, hipStream_t s, const float* in, uint32_t n, float* out,
-                          const float* target, const int32_t* count_dims = nullptr)
+                          const float* target, const int32_t* count_dims = nullptr,
+                          const nrc::QueryMask& mask = nrc::QueryMask{})
 {
     const void* w = train ? h->w : h->winf;
     const float* tables = train ? h->master + h->P_mlp : h->grid_inf;
     if (h->enc == RTC8X_NRC_HASH_GRID)
-        return launch_forward_enc<T, RTC8X_NRC_HASH_GRID>(h, train, w, tables, s, in, n, out, target, count_dims);
-    return launch_forward_enc<T, RTC8X_NRC_TRIANGLE_WAVE>(h, train, w, tables, s, in, n, out, target, count_dims);
+        return launch_forward_enc<T, RTC8X_NRC_HASH_GRID>(h, train, w, tables, s, in, n, out, target, count_dims, mask);
+    return launch_forward_enc<T, RTC8X_NRC_TRIANGLE_WAVE>(h, train, w, tables, s, in, n, out, target, count_dims, mask);
 }
 
 template <typename T>
@@ -1138,17 +1162,19 @@ hipError_t with_type(const rtc8x_nrc* h, F&& f)
 bool is_fp8(const rtc8x_nrc* h) { return h->desc.precision == RTC8X_NRC_BF16_FP8; }
 
 hipError_t launch_forward_fp8(rtc8x_nrc* h, hipStream_t s, const float* in, uint32_t n, float* out,
-                              const int32_t* count_dims = nullptr)
+                              const int32_t* count_dims = nullptr, const nrc::QueryMask& mask = nrc::QueryMask{})
 {
     hipLaunchKernelGGL(nrc::quantize_kernel, dim3(h->L + 1), dim3(256), 0, s, (const __bf16*)h->winf, h->L, h->w8,
                        h->w8_inv);
     const uint32_t blocks = (n + 255) / 256;
     if (h->enc == RTC8X_NRC_HASH_GRID)
         hipLaunchKernelGGL((nrc::forward_fp8_kernel<RTC8X_NRC_HASH_GRID>), dim3(blocks), dim3(256), 0, s, h->w8,
-                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom, count_dims);
+                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom, count_dims,
+                           mask);
     else
         hipLaunchKernelGGL((nrc::forward_fp8_kernel<RTC8X_NRC_TRIANGLE_WAVE>), dim3(blocks), dim3(256), 0, s, h->w8,
-                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom, count_dims);
+                           h->w8_inv, h->L, (const rtc8x_radiance_query*)in, n, out, h->grid_inf, h->geom, count_dims,
+                           mask);
     return hipGetLastError();
 }
 
@@ -1341,6 +1367,28 @@ try {
     return RTC8X_OK;
 }
 RTC8X_CATCH_ALL("rtc8x_nrc_infer_training")
+
+extern "C" rtc8x_status rtc8x_nrc_infer_render(rtc8x_nrc* h, void* stream, const rtc8x_frame_params* p)
+try {
+    RTC8X_REQUIRE(h && p, "rtc8x_nrc_infer_render: NULL argument");
+    const rtc8x_nrc_buffers& nb = p->nrc;
+    RTC8X_REQUIRE(nb.inference_queries && nb.inferred_radiance && nb.inference_terminals && p->image_width > 0 &&
+                      p->image_height > 0,
+                  "rtc8x_nrc_infer_render: NRC buffers missing");
+    const uint32_t npx = p->image_width * p->image_height;
+    const uint32_t q_stride = (npx + 127u) & ~127u, k = nb.num_samples ? nb.num_samples : 1u;
+    RTC8X_REQUIRE((uint64_t)q_stride * k <= 0xFFFFFFFFull, "rtc8x_nrc_infer_render: too many queries");
+    const nrc::QueryMask mask{nb.inference_terminals, npx, q_stride};
+    hipStream_t s = (hipStream_t)stream;
+    const float* in = reinterpret_cast<const float*>(nb.inference_queries);
+    if (is_fp8(h))
+        RTC8X_HIP(launch_forward_fp8(h, s, in, q_stride * k, nb.inferred_radiance, nullptr, mask));
+    else
+        RTC8X_HIP(RTC8X_WITH_TYPE(h, launch_forward<T>(h, false, s, in, q_stride * k, nb.inferred_radiance, nullptr,
+                                                       nullptr, mask)));
+    return RTC8X_OK;
+}
+RTC8X_CATCH_ALL("rtc8x_nrc_infer_render")
 
 extern "C" rtc8x_status rtc8x_nrc_gradient(rtc8x_nrc* h, void* stream, const float* in, const float* target, uint32_t n,
                                            float* grad)

@@ -76,6 +76,10 @@ class NeuralRadianceCache:
         call("rtc8x_nrc_infer", C.c_void_p(self.handle), C.c_void_p(stream), C.c_void_p(input_ptr), int(num),
              C.c_void_p(output_ptr))
 
+    def infer_render(self, stream, params: _abi.FrameParams) -> None:
+        """The render inference of a PATH_TRACE_WITH_NRC launch's samples, at the slots with a query."""
+        call("rtc8x_nrc_infer_render", C.c_void_p(self.handle), C.c_void_p(stream), C.byref(params))
+
     def infer_training(self, stream, params: _abi.FrameParams) -> None:
         """Inference of a training iteration's suffix queries, sized on the device (no host wait)."""
         call("rtc8x_nrc_infer_training", C.c_void_p(self.handle), C.c_void_p(stream), C.byref(params))
@@ -331,7 +335,7 @@ class NrcRenderer:
 
         self._trace(after_gate, trace)
         nq = round_up(self.width * self.height, 128) * k
-        self.cache.infer(self.stream, self.buffers.inference_queries.ptr, nq, self.buffers.inferred_radiance.ptr)
+        self.cache.infer_render(self.stream, p)
         call("rtc8x_nrc_accumulate", s, C.byref(p))
         self.stats.inference_queries += nq
         self.samples += k

@@ -586,3 +586,55 @@ def test_render_with_nrc_several_samples_per_launch():
     assert np.array_equal(r.rng.download(np.uint64, n), gpu_rng)
     r.close()
     ds.close()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16_fp8"])
+def test_render_inference_at_query_slots_matches_the_full_batch(precision):
+    """rtc8x_nrc_infer_render (two samples per launch, dense 128^3 cloud at 96 x 64: about a fifth of the
+    slots hold a query) evaluates the slots with a query and leaves the others as they are (a byte
+    pattern written before the call).  At those slots it equals rtc8x_nrc_infer over the whole batch: bit
+    for bit in FP16;
+    with E4M3 inference, whose per-tile activation scales then come from the queries alone, within
+    0.05 relative RMS."""
+    from rtcamp8_amd import _abi
+    from rtcamp8_amd import scene as sc
+    from rtcamp8_amd.device import DeviceBuffer
+    from rtcamp8_amd.nrc import NrcRenderer, round_up
+
+    prec = {"fp16": _abi.NRC_FP16, "bf16_fp8": _abi.NRC_BF16_FP8}[precision]
+    cfg = sc.CONFIGS["dense128"].with_(width=96, height=64)
+    print(f"[infer render] {precision} {cfg.width}x{cfg.height}", flush=True)
+    ds = sc.DeviceScene(cfg)
+    r = NrcRenderer(ds, num_hidden_layers=2, precision=prec, encoding=_abi.NRC_HASH_GRID, samples_per_launch=2)
+    full = None
+    try:
+        r.reset(7)
+        r.render_frame(spp=2)
+        r.image()
+        b = r.buffers
+        n, q = cfg.width * cfg.height, round_up(cfg.width * cfg.height, 128)
+        b.inferred_radiance.fill(0xA5)
+        p = r.params()
+        p.nrc.num_samples = 2
+        r.cache.infer_render(None, p)
+        got = b.inferred_radiance.download(np.float32, 3 * 2 * q).reshape(-1, 3)
+        full = DeviceBuffer(12 * 2 * q)
+        r.cache.infer(None, b.inference_queries.ptr, 2 * q, full.ptr)
+        want = full.download(np.float32, 3 * 2 * q).reshape(-1, 3)
+        has = (b.inference_terminals.download(np.uint32, 4 * 2 * n).reshape(2, n, 4)[..., 3] & 1) == 1
+    finally:
+        if full is not None:
+            full.free()
+        r.close()
+        ds.close()
+    slots = (np.arange(2)[:, None] * q + np.arange(n)[None, :])
+    live, dead = slots[has], np.setdiff1d(np.arange(2 * q), slots[has])
+    print(f"[infer render] {has.mean():.3f} of the slots hold a query", flush=True)
+    assert 0.05 < has.mean() < 0.95
+    assert np.all(got[dead].view(np.uint32) == 0xA5A5A5A5)
+    if precision == "fp16":
+        assert np.array_equal(got[live].view(np.uint32), want[live].view(np.uint32))
+    else:
+        rel = np.sqrt(np.mean((got[live] - want[live]) ** 2) / np.mean(want[live] ** 2))
+        print(f"[infer render] E4M3: relative RMS difference {rel:.4f}", flush=True)
+        assert rel < 0.05
